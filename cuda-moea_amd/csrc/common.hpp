@@ -1,0 +1,384 @@
+// Shared device/host helpers of the MI355X MOEA engine.
+//
+// Arithmetic contract: every translation unit is compiled with
+// -ffp-contract=off, so each fused multiply-add is an explicit fmaf()/fma()
+// placed where the reference's nvcc build contracts one (nvcc's default
+// -fmad=true, CMakeLists.txt:115-123).  fp32 division and sqrt are IEEE
+// correctly rounded (-fhip-fp32-correctly-rounded-divide-sqrt).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <utility>
+
+namespace moea {
+
+// ----------------------------------------------------------------------------
+// Errors: C-ABI entry points return a status and record a message.
+// ----------------------------------------------------------------------------
+enum Status : int {
+    kOk = 0,
+    kInvalidArgument = -1,
+    kHipError = -2,
+    kWorkspaceTooSmall = -3,
+    kUnsupported = -4,
+    kDeviceFault = -5,
+};
+
+void set_last_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t err, const char* what);
+
+#define MOEA_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t _e = (call);                                               \
+        if (_e != hipSuccess) return ::moea::hip_fail(_e, #call);             \
+    } while (0)
+
+#define MOEA_CHECK(status_expr)                                               \
+    do {                                                                      \
+        int _s = (status_expr);                                               \
+        if (_s != ::moea::kOk) return _s;                                     \
+    } while (0)
+
+// Launch kern<M> for the runtime objective count m (grid/block must be
+// single tokens, e.g. dim3 variables).
+#define MOEA_LAUNCH_M(m, kern, grid, block, shmem, stream, ...)                                    \
+    switch (m) {                                                                                \
+        case 1: hipLaunchKernelGGL(kern<1>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 2: hipLaunchKernelGGL(kern<2>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 3: hipLaunchKernelGGL(kern<3>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 4: hipLaunchKernelGGL(kern<4>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 5: hipLaunchKernelGGL(kern<5>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 6: hipLaunchKernelGGL(kern<6>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 7: hipLaunchKernelGGL(kern<7>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 8: hipLaunchKernelGGL(kern<8>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 9: hipLaunchKernelGGL(kern<9>, grid, block, shmem, stream, __VA_ARGS__); break;     \
+        case 10: hipLaunchKernelGGL(kern<10>, grid, block, shmem, stream, __VA_ARGS__); break;   \
+        default: return ::moea::fail(::moea::kUnsupported, "unsupported objective count");      \
+    }
+
+constexpr int kMaxM = 10;          // objectives supported (reference MAX_M = 8)
+constexpr int kWave = 64;          // CDNA wavefront width
+
+inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ----------------------------------------------------------------------------
+// Counter-based draws.  Philox4x32-10 (Salmon et al., SC'11) keyed by the run
+// seed; the counter is (index lo, index hi, generation, stream).
+// ----------------------------------------------------------------------------
+enum DrawStream : uint32_t { kStInit = 1, kStMate = 2, kStSbx = 3, kStPm = 4 };
+
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = static_cast<uint32_t>(p1);
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = static_cast<uint32_t>(p0);
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {c0, c1, c2, c3};
+}
+
+__host__ __device__ inline U4 draw4(uint64_t seed, uint64_t index, uint32_t gen, uint32_t stream)
+{
+    return philox4x32_10(static_cast<uint32_t>(index), static_cast<uint32_t>(index >> 32), gen,
+                         stream, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+}
+
+// 24-bit uniform in [0, 1); never 1.0.
+__host__ __device__ inline float u01(uint32_t u) { return static_cast<float>(u >> 8) * 5.9604644775390625e-8f; }
+
+// The reference's niching draw: SplitMix64 finalizer of (counter ^ seed), low
+// 32 bits (NS/core/cuda/cuda_random.cu:1606-1614).
+__host__ __device__ inline uint32_t splitmix32(uint64_t counter, uint64_t seed)
+{
+    uint64_t z = counter ^ seed;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return static_cast<uint32_t>(z);
+}
+
+// Niching counters: stream a of generation g starts at g<<33, stream b at
+// (g<<33) | (1<<32).
+__host__ __device__ inline uint64_t niche_base(uint32_t gen, int which)
+{
+    return (static_cast<uint64_t>(gen) << 33) | (static_cast<uint64_t>(which & 1) << 32);
+}
+
+// ----------------------------------------------------------------------------
+// Deterministic pow / acos: evaluated in double with a fixed operation
+// sequence and rounded once, identical on host and device.  They stand in for
+// CUDA's powf / acosf, whose last-bit behaviour cannot be reproduced.
+// ----------------------------------------------------------------------------
+__host__ __device__ inline double log2_d(double x)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) { m = m * 2.0; e -= 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = fma(p, s2, 1.0 / 21.0);
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    p = fma(p, s2, 1.0);
+    const double ln_m = 2.0 * s * p;
+    return static_cast<double>(e) + ln_m * 1.44269504088896338700e+00;
+}
+
+__host__ __device__ inline double exp2_d(double t)
+{
+    if (t > 1100.0) return static_cast<double>(INFINITY);
+    if (t < -1100.0) return 0.0;
+    const double k = nearbyint(t);
+    const double r = t - k;
+    const double y = fma(r, 6.93147180369123816490e-01, r * 1.90821492927058770002e-10);
+    double p = 1.0 / 87178291200.0;
+    p = fma(p, y, 1.0 / 6227020800.0);
+    p = fma(p, y, 1.0 / 479001600.0);
+    p = fma(p, y, 1.0 / 39916800.0);
+    p = fma(p, y, 1.0 / 3628800.0);
+    p = fma(p, y, 1.0 / 362880.0);
+    p = fma(p, y, 1.0 / 40320.0);
+    p = fma(p, y, 1.0 / 5040.0);
+    p = fma(p, y, 1.0 / 720.0);
+    p = fma(p, y, 1.0 / 120.0);
+    p = fma(p, y, 1.0 / 24.0);
+    p = fma(p, y, 1.0 / 6.0);
+    p = fma(p, y, 0.5);
+    p = fma(p, y, 1.0);
+    p = fma(p, y, 1.0);
+    return ldexp(p, static_cast<int>(k));
+}
+
+__host__ __device__ inline float powf_det(float x, float y)
+{
+    if (x != x || y != y) return NAN;
+    if (x < 0.0f) return NAN;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    if (x == 1.0f || y == 0.0f) return 1.0f;
+    if (isinf(x)) return y > 0.0f ? INFINITY : 0.0f;
+    return static_cast<float>(exp2_d(static_cast<double>(y) * log2_d(static_cast<double>(x))));
+}
+
+__host__ __device__ inline double atan01_d(double t)
+{
+    const int i = static_cast<int>(nearbyint(t * 4.0));
+    double c0;
+    switch (i) {
+        case 0: c0 = 0.0; break;
+        case 1: c0 = 2.44978663126864143473e-01; break;
+        case 2: c0 = 4.63647609000806093515e-01; break;
+        case 3: c0 = 6.43501108793284370968e-01; break;
+        default: c0 = 7.85398163397448278999e-01; break;
+    }
+    const double c = 0.25 * static_cast<double>(i);
+    const double r = (t - c) / fma(t, c, 1.0);
+    const double r2 = r * r;
+    double p = -1.0 / 19.0;
+    p = fma(p, r2, 1.0 / 17.0);
+    p = fma(p, r2, -1.0 / 15.0);
+    p = fma(p, r2, 1.0 / 13.0);
+    p = fma(p, r2, -1.0 / 11.0);
+    p = fma(p, r2, 1.0 / 9.0);
+    p = fma(p, r2, -1.0 / 7.0);
+    p = fma(p, r2, 1.0 / 5.0);
+    p = fma(p, r2, -1.0 / 3.0);
+    p = fma(p, r2, 1.0);
+    return c0 + r * p;
+}
+
+// acos on [0, 1] (callers clamp the cosine first, as the reference does).
+__host__ __device__ inline float acosf_det(float xf)
+{
+    if (xf != xf) return NAN;
+    double x = static_cast<double>(xf);
+    if (x >= 1.0) return 0.0f;
+    if (x <= 0.0) x = 0.0;
+    const double t = sqrt((1.0 - x) / (1.0 + x));
+    return static_cast<float>(2.0 * atan01_d(t));
+}
+
+// ----------------------------------------------------------------------------
+// M x M hyperplane solve in FP64.  One-sided Jacobi SVD of A (row i = extreme
+// point i), then a = pinv(A) * 1 with singular values <= 1e-5 dropped: the
+// reference's cusolverDnSgesvd + cublasSgemv pipeline
+// (selection/nsga3/kernels/normalization.cu:1302-1394) evaluated in double.
+// Returns 1 when the reference takes its row-max fallback: non-finite input or
+// singular values, sigma_min < 1e-10, sigma_max / sigma_min > 1e6
+// (normalization.cu:739-771) or no convergence (normalization.cu:1340-1348).
+// The operation sequence matches the oracle's orc_hyperplane_fp64.
+// ----------------------------------------------------------------------------
+__host__ __device__ inline int hyperplane_fp64(const double* a_in, int m, double* a_out)
+{
+    double A[kMaxM * kMaxM], V[kMaxM * kMaxM];
+    if (m > kMaxM) return 1;
+    for (int i = 0; i < m * m; ++i) {
+        A[i] = a_in[i];
+        if (!isfinite(A[i])) return 1;
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) V[i * m + j] = (i == j) ? 1.0 : 0.0;
+    int converged = 0;
+    for (int sweep = 0; sweep < 60 && !converged; ++sweep) {
+        int rotated = 0;
+        for (int p = 0; p < m - 1; ++p) {
+            for (int q = p + 1; q < m; ++q) {
+                double alpha = 0.0, beta = 0.0, gamma = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    const double ap = A[i * m + p], aq = A[i * m + q];
+                    alpha = alpha + ap * ap;
+                    beta = beta + aq * aq;
+                    gamma = gamma + ap * aq;
+                }
+                if (gamma == 0.0 || fabs(gamma) <= 1e-15 * sqrt(alpha * beta)) continue;
+                rotated = 1;
+                const double zeta = (beta - alpha) / (2.0 * gamma);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double s = c * t;
+                for (int i = 0; i < m; ++i) {
+                    const double ap = A[i * m + p], aq = A[i * m + q];
+                    A[i * m + p] = c * ap - s * aq;
+                    A[i * m + q] = s * ap + c * aq;
+                    const double vp = V[i * m + p], vq = V[i * m + q];
+                    V[i * m + p] = c * vp - s * vq;
+                    V[i * m + q] = s * vp + c * vq;
+                }
+            }
+        }
+        if (!rotated) converged = 1;
+    }
+    if (!converged) return 1;
+    double sig[kMaxM];
+    double smax = 0.0, smin = static_cast<double>(INFINITY);
+    for (int j = 0; j < m; ++j) {
+        double ss = 0.0;
+        for (int i = 0; i < m; ++i) ss = ss + A[i * m + j] * A[i * m + j];
+        sig[j] = sqrt(ss);
+        if (!isfinite(sig[j])) return 1;
+        if (sig[j] > smax) smax = sig[j];
+        if (sig[j] < smin) smin = sig[j];
+    }
+    if (smin < 1e-10) return 1;
+    if (smax / smin > 1e6) return 1;
+    for (int r = 0; r < m; ++r) a_out[r] = 0.0;
+    for (int j = 0; j < m; ++j) {
+        if (!(sig[j] > 1e-5)) continue;
+        double usum = 0.0;
+        for (int i = 0; i < m; ++i) usum = usum + A[i * m + j] / sig[j];
+        const double w = usum / sig[j];
+        for (int r = 0; r < m; ++r) a_out[r] = a_out[r] + V[r * m + j] * w;
+    }
+    return 0;
+}
+
+// ----------------------------------------------------------------------------
+// Device-scope atomics and a bounded grid barrier for cooperative launches.
+// ----------------------------------------------------------------------------
+__device__ inline uint32_t ld_acquire_u32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int32_t ld_relaxed_i32(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_relaxed_u64(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_relaxed_u64(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier state: [0] arrival count, [1] generation, [2] error flag.
+// Every block of the cooperative grid must call it the same number of times.
+// Thread 0 of each block publishes the block's writes with an agent-scope
+// release, arrives, waits for the generation to advance and acquires; the
+// workgroup barriers around it order the other threads of the block.
+// The spin is bounded: after kSpinLimit polls the block records an error and
+// returns false, so a broken launch ends instead of hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+__device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t& phase)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const uint32_t target = phase + 1;
+        const uint32_t arrived =
+            __hip_atomic_fetch_add(&state[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&state[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&state[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (ld_acquire_u32(&state[1]) != target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_store(&state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (ld_acquire_u32(&state[2]) != 0) ok = 0;
+        s_ok = ok;
+        phase = target;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// ----------------------------------------------------------------------------
+// Wave helpers
+// ----------------------------------------------------------------------------
+__device__ inline uint64_t ballot(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
+
+// v_writelane_b32 with a compile-time lane: lane J of the result takes the
+// wave-uniform `value` (an inline-constant lane keeps the instruction within
+// the constant-bus limit).
+template <int J>
+__device__ inline uint32_t writelane(uint32_t old, uint32_t value)
+{
+    uint32_t r = old;
+    const uint32_t v = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(value)));
+    asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(r) : "s"(v), "n"(J));
+    return r;
+}
+
+// Compile-time loop: f(std::integral_constant<int, J>) for J in Js.
+template <typename F, int... Js>
+__device__ inline void static_for(F&& f, std::integer_sequence<int, Js...>)
+{
+    (f(std::integral_constant<int, Js>{}), ...);
+}
+
+__device__ inline int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+}  // namespace moea

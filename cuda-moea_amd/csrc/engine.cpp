@@ -1,0 +1,197 @@
+// Native generation loop: the role of EvolutionaryAlgorithm::initialize / step /
+// run_device (NS/algorithm/algorithm.cu:104-329) for the built-in operators.
+//
+// One generation is reproduce -> evaluate -> select (-> RVEA direction
+// adaptation), a fixed sequence of stream-ordered launches with no host
+// synchronization and no allocation, so it can also be captured into a graph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "moea_internal.hpp"
+
+namespace moea {
+
+namespace {
+
+__global__ void set_i32(int32_t* p, int v) { p[0] = v; }
+
+int evaluate(const moea_run_args& a, const moea_population& p, hipStream_t st, int generation)
+{
+    const moea_problem_desc& pr = a.problem;
+    switch (pr.kind) {
+        case MOEA_PROBLEM_QUADRATIC:
+            return eval_quadratic(p.x, a.n, a.d, pr.centers, a.m, p.f, a.n, p.cv, st);
+        case MOEA_PROBLEM_MLP:
+            return eval_mlp(p.x, a.n, a.d, pr.w1, pr.hidden, pr.w2, a.m, p.f, a.n, p.cv, pr.hwork, st);
+        case MOEA_PROBLEM_CALLBACK:
+            if (!pr.callback) return fail(kInvalidArgument, "moea_run: callback problem without callback");
+            return pr.callback(pr.user, p.x, a.n, a.d, a.m, p.f, p.cv, generation, a.max_generations, st);
+        default:
+            return fail(kInvalidArgument, "moea_run: unknown problem kind");
+    }
+}
+
+int check(const moea_run_args& a)
+{
+    if (a.n <= 0 || a.m <= 0 || a.m > kMaxM || a.d <= 0 || a.k <= 0 || a.max_generations <= 0)
+        return fail(kInvalidArgument, "Algorithm requires positive population_size and max_generations");
+    if (a.algorithm != MOEA_ALGORITHM_NSGA3 && a.algorithm != MOEA_ALGORITHM_RVEA)
+        return fail(kInvalidArgument, "algorithm must be NSGA3 or RVEA");
+    if (a.algorithm == MOEA_ALGORITHM_NSGA3 && (a.n % 2) != 0)
+        return fail(kInvalidArgument, "The current NSGA-III backend requires an even population size");
+    for (int i = 0; i < 3; ++i)
+        if (!a.pops[i].x || !a.pops[i].f || !a.pops[i].cv)
+            return fail(kInvalidArgument, "moea_run: population buffers missing");
+    if (!a.bounds || !a.directions || !a.parent_indices || !a.workspace || a.current < 0 || a.current > 2)
+        return fail(kInvalidArgument, "moea_run: missing buffers");
+    if (a.algorithm == MOEA_ALGORITHM_NSGA3 && (!a.fronts[0] || !a.fronts[1] || !a.ideal))
+        return fail(kInvalidArgument, "moea_run: NSGA-III needs fronts and ideal buffers");
+    if (a.algorithm == MOEA_ALGORITHM_RVEA &&
+        (!a.gamma || !a.directions0 || !a.active_device || !a.adapt_scratch))
+        return fail(kInvalidArgument, "moea_run: RVEA needs gamma, initial directions and active buffers");
+    return kOk;
+}
+
+}  // namespace
+
+size_t run_workspace_bytes(int algorithm, int n, int m, int k)
+{
+    if (algorithm == MOEA_ALGORITHM_NSGA3) {
+        const size_t a = nsga3_workspace_bytes(n, m, k);
+        const size_t b = nds_workspace_bytes(n);
+        return a > b ? a : b;
+    }
+    return rvea_workspace_bytes(n, m, k);
+}
+
+int run_init(moea_run_args& a)
+{
+    MOEA_CHECK(check(a));
+    hipStream_t st = static_cast<hipStream_t>(a.stream);
+    moea_population& par = a.pops[a.current];
+    if (!a.initial_x_given) MOEA_CHECK(init_population(par.x, a.n, a.d, a.bounds, a.seed, st));
+    MOEA_CHECK(evaluate(a, par, st, -1));
+    if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
+        // NSGA3EnvironmentSelector::prepare: parent fronts over n with stop at n,
+        // and the initial ideal point (nsga3_environment_selector.cu:236-287)
+        MOEA_CHECK(nds_run(par.f, a.n, par.cv, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init,
+                           a.workspace, a.workspace_bytes, st));
+        MOEA_CHECK(ideal_point(par.f, a.n, a.n, a.m, a.ideal, st));
+    } else {
+        MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
+        // active = n (RVEAEnvironmentSelector::reset / allocate)
+        hipLaunchKernelGGL(set_i32, dim3(1), dim3(1), 0, st, a.active_device, a.n);
+        MOEA_HIP(hipGetLastError());
+    }
+    a.generation = 0;
+    return kOk;
+}
+
+int run_step(moea_run_args& a, int generations)
+{
+    MOEA_CHECK(check(a));
+    hipStream_t st = static_cast<hipStream_t>(a.stream);
+    for (int it = 0; it < generations && a.generation < a.max_generations; ++it) {
+        const int g = a.generation;
+        const int off = (a.current + 1) % 3, nxt = (a.current + 2) % 3;
+        moea_population& par = a.pops[a.current];
+        moea_population& ofs = a.pops[off];
+        moea_population& nx = a.pops[nxt];
+        moea_reproduce_args r{};
+        r.n = a.n; r.d = a.d;
+        r.active = a.n;
+        r.active_device = (a.algorithm == MOEA_ALGORITHM_RVEA) ? a.active_device : nullptr;
+        r.mating = a.mating;
+        r.parents_x = par.x;
+        r.parents_rank = (a.algorithm == MOEA_ALGORITHM_NSGA3) ? a.fronts[a.fronts_current] : nullptr;
+        r.parents_cv = par.cv;
+        r.bounds = a.bounds;
+        r.sbx_eta = annealed(a.sbx_eta[0], a.sbx_eta[1], g, a.max_generations);
+        r.sbx_prob = a.sbx_prob;
+        r.sbx_copy_prob = a.sbx_copy_prob;
+        r.mutation = a.mutation;
+        r.pm_eta = annealed(a.pm_eta[0], a.pm_eta[1], g, a.max_generations);
+        r.pm_prob = a.pm_prob;
+        r.draws = nullptr;
+        r.seed = a.seed;
+        r.generation = g;
+        r.offspring_x = ofs.x;
+        r.parent_indices = a.parent_indices;
+        r.stream = a.stream;
+        if (r.mating == MOEA_MATING_TOURNAMENT && !r.parents_rank)
+            return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
+        MOEA_CHECK(reproduce(r));
+        MOEA_CHECK(evaluate(a, ofs, st, g));
+        if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
+            moea_nsga3_select_args s{};
+            s.n = a.n; s.m = a.m; s.d = a.d; s.k = a.k;
+            s.parents = par; s.offspring = ofs; s.next = nx;
+            s.next_fronts = a.fronts[1 - a.fronts_current];
+            s.directions = a.directions;
+            s.ideal = a.ideal;
+            s.draws = nullptr;
+            s.seed = a.seed;
+            s.generation = g;
+            s.stats = a.stats;
+            s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
+            s.stream = a.stream;
+            MOEA_CHECK(nsga3_select(s));
+            a.fronts_current = 1 - a.fronts_current;
+        } else {
+            moea_rvea_select_args s{};
+            s.n = a.n; s.m = a.m; s.d = a.d; s.k = a.k;
+            s.active = a.n;
+            s.active_device = a.active_device;
+            s.parents = par; s.offspring = ofs; s.next = nx;
+            s.directions = a.directions;
+            s.gamma = a.gamma;
+            // EvoX-style t = generation + 1 (rvea_environment_selector.cu:236-239)
+            float total = static_cast<float>(a.max_generations);
+            float progress = static_cast<float>(g + 1) / (total < 1.0f ? 1.0f : total);
+            progress = progress < 0.0f ? 0.0f : (progress > 1.0f ? 1.0f : progress);
+            s.penalty = static_cast<float>(a.m) *
+                        static_cast<float>(std::pow(static_cast<double>(progress), static_cast<double>(a.rvea_alpha)));
+            s.out_active = a.active_device;
+            s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
+            s.stream = a.stream;
+            MOEA_CHECK(rvea_select(s));
+            // AdaptiveRVEADirections::update (reference_directions.cu:132-162)
+            int interval = static_cast<int>(std::lround(static_cast<double>(a.rvea_frequency) * a.max_generations));
+            if (interval < 1) interval = 1;
+            if ((g + 1) % interval == 0) {
+                MOEA_CHECK(rvea_adapt(a.directions0, nx.f, a.n, a.active_device, a.m, a.k,
+                                      const_cast<float*>(a.directions), a.adapt_scratch, st));
+                MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
+            }
+        }
+        a.current = nxt;
+        a.generation = g + 1;
+    }
+    return kOk;
+}
+
+}  // namespace moea
+
+extern "C" {
+
+size_t moea_run_workspace_bytes(int algorithm, int n, int m, int k)
+{
+    if (n <= 0 || m <= 0 || k <= 0) return 0;
+    return moea::run_workspace_bytes(algorithm, n, m, k);
+}
+
+int moea_run_init(moea_run_args* args)
+{
+    if (!args) return moea::fail(moea::kInvalidArgument, "moea_run_init: null args");
+    return moea::run_init(*args);
+}
+
+int moea_run_step(moea_run_args* args, int generations)
+{
+    if (!args) return moea::fail(moea::kInvalidArgument, "moea_run_step: null args");
+    return moea::run_step(*args, generations);
+}
+
+}  // extern "C"

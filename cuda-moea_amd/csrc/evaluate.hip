@@ -1,0 +1,175 @@
+// Built-in objective kernels (the IProblemEvaluator::evaluate role,
+// NI/problem/problem_evaluator.cuh:10-31).
+//
+// Quadratic-center: f_c(x) = sum_j (x_j - c_cj)^2.  HBM-bound: one wave reads a
+// row once and produces all m objectives.  Fixed summation order (shared with
+// the oracle): lane l accumulates j = l, l+64, ... ascending with one fma per
+// element, then a pairwise tree over lanes (offsets 32, 16, ..., 1).
+#include "common.hpp"
+#include "moea_internal.hpp"
+
+namespace moea {
+
+namespace {
+
+template <int M>
+__global__ __launch_bounds__(256) void quadratic_kernel(const float* __restrict__ x, int n, int d,
+                                                        const float* __restrict__ c,
+                                                        float* __restrict__ f, int ldf,
+                                                        float* __restrict__ cv)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float* xr = x + static_cast<size_t>(row) * d;
+    float acc[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) acc[k] = 0.0f;
+    for (int j = lane; j < d; j += 64) {
+        const float v = xr[j];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float t = v - c[static_cast<size_t>(k) * d + j];
+            acc[k] = fmaf(t, t, acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        float s = acc[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s = s + __shfl_xor(s, o);
+        acc[k] = s;
+    }
+    if (lane < M) {
+        float out = acc[0];
+#pragma unroll
+        for (int k = 1; k < M; ++k) if (lane == k) out = acc[k];
+        f[static_cast<size_t>(lane) * ldf + row] = out;
+    }
+    if (lane == 0 && cv) cv[row] = 0.0f;
+}
+
+// MLP hidden layer H = tanh(X W1^T): (n, d) x (hidden, d)^T -> (n, hidden).
+// 64 x 64 output tile per 256-thread block, 16-wide k slices staged in LDS,
+// each thread owns a 4 x 4 register sub-tile.
+constexpr int kGT = 64, kGK = 16;
+
+__global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict__ x, int n, int d,
+                                                         const float* __restrict__ w1, int hidden,
+                                                         float* __restrict__ h)
+{
+    __shared__ float s_x[kGK][kGT + 1];
+    __shared__ float s_w[kGK][kGT + 1];
+    const int r0 = blockIdx.x * kGT, c0 = blockIdx.y * kGT;
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < d; k0 += kGK) {
+        for (int e = threadIdx.x; e < kGT * kGK; e += 256) {
+            const int rr = e / kGK, kk = e % kGK;
+            const int gr = r0 + rr, gk = k0 + kk;
+            s_x[kk][rr] = (gr < n && gk < d) ? x[static_cast<size_t>(gr) * d + gk] : 0.0f;
+            const int gc = c0 + rr;
+            s_w[kk][rr] = (gc < hidden && gk < d) ? w1[static_cast<size_t>(gc) * d + gk] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = s_x[kk][tr * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = s_w[kk][tc * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gr = r0 + tr * 4 + i, gc = c0 + tc * 4 + j;
+            if (gr < n && gc < hidden) h[static_cast<size_t>(gr) * hidden + gc] = tanhf(acc[i][j]);
+        }
+}
+
+// Output head: y = W2 h, f_c = y_c^2 + 0.01 * ||x - 0.5||^2; one wave per row.
+template <int M>
+__global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__ x, int n, int d,
+                                                       const float* __restrict__ h, int hidden,
+                                                       const float* __restrict__ w2,
+                                                       float* __restrict__ f, int ldf,
+                                                       float* __restrict__ cv)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    float y[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) y[k] = 0.0f;
+    for (int j = lane; j < hidden; j += 64) {
+        const float hv = h[static_cast<size_t>(row) * hidden + j];
+#pragma unroll
+        for (int k = 0; k < M; ++k) y[k] = fmaf(w2[static_cast<size_t>(k) * hidden + j], hv, y[k]);
+    }
+    float s = 0.0f;
+    for (int j = lane; j < d; j += 64) {
+        const float t = x[static_cast<size_t>(row) * d + j] - 0.5f;
+        s = fmaf(t, t, s);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        s = s + __shfl_xor(s, o);
+#pragma unroll
+        for (int k = 0; k < M; ++k) y[k] = y[k] + __shfl_xor(y[k], o);
+    }
+    if (lane < M) {
+        float yk = y[0];
+#pragma unroll
+        for (int k = 1; k < M; ++k) if (lane == k) yk = y[k];
+        f[static_cast<size_t>(lane) * ldf + row] = fmaf(yk, yk, 0.01f * s);
+    }
+    if (lane == 0 && cv) cv[row] = 0.0f;
+}
+
+}  // namespace
+
+int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2, int m,
+             float* f, int ldf, float* cv, float* hwork, hipStream_t stream)
+{
+    if (!x || !w1 || !w2 || !f || !hwork || n <= 0 || d <= 0 || hidden <= 0 || m <= 0 || m > kMaxM ||
+        ldf < n)
+        return fail(kInvalidArgument, "eval_mlp: invalid arguments");
+    hipLaunchKernelGGL(mlp_hidden_kernel, dim3(ceil_div(n, kGT), ceil_div(hidden, kGT)), dim3(256), 0,
+                       stream, x, n, d, w1, hidden, hwork);
+    MOEA_HIP(hipGetLastError());
+    const dim3 grid(ceil_div(n, 4)), block(256);
+    switch (m) {
+#define MOEA_H(MM) case MM: hipLaunchKernelGGL(mlp_head_kernel<MM>, grid, block, 0, stream, x, n, d, hwork, hidden, w2, f, ldf, cv); break;
+        MOEA_H(1) MOEA_H(2) MOEA_H(3) MOEA_H(4) MOEA_H(5) MOEA_H(6) MOEA_H(7) MOEA_H(8) MOEA_H(9) MOEA_H(10)
+#undef MOEA_H
+        default: return fail(kUnsupported, "eval_mlp: unsupported objective count");
+    }
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f, int ldf,
+                   float* cv, hipStream_t stream)
+{
+    if (!x || !centers || !f || n <= 0 || d <= 0 || m <= 0 || m > kMaxM || ldf < n)
+        return fail(kInvalidArgument, "eval_quadratic: invalid arguments");
+    const dim3 grid(ceil_div(n, 4)), block(256);
+    switch (m) {
+#define MOEA_Q(MM) case MM: hipLaunchKernelGGL(quadratic_kernel<MM>, grid, block, 0, stream, x, n, d, centers, f, ldf, cv); break;
+        MOEA_Q(1) MOEA_Q(2) MOEA_Q(3) MOEA_Q(4) MOEA_Q(5) MOEA_Q(6) MOEA_Q(7) MOEA_Q(8) MOEA_Q(9) MOEA_Q(10)
+#undef MOEA_Q
+        default: return fail(kUnsupported, "eval_quadratic: unsupported objective count");
+    }
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace moea

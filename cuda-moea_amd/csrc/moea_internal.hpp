@@ -1,0 +1,79 @@
+// Internal interfaces between the HIP modules of libmoea_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/moea_hip.h"
+
+namespace moea {
+
+// nds.hip
+size_t nds_workspace_bytes(int n);
+int nds_max_n();
+int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
+            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream);
+int peel_grid_size(int nblk);
+
+// nsga3.hip
+size_t nsga3_workspace_bytes(int n, int m, int k);
+int nsga3_select(const moea_nsga3_select_args& a);
+int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream);
+
+// reproduce.hip
+int reproduce(const moea_reproduce_args& a);
+int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, hipStream_t stream);
+
+// evaluate.hip
+int eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f, int ldf,
+                   float* cv, hipStream_t stream);
+int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2, int m,
+             float* f, int ldf, float* cv, float* hwork, hipStream_t stream);
+
+// rvea.hip
+size_t rvea_workspace_bytes(int n, int m, int k);
+int rvea_select(const moea_rvea_select_args& a);
+int rvea_gamma(const float* v, int m, int k, float* gamma, hipStream_t stream);
+int rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_active, int m, int k,
+               float* v_out, float* scratch, hipStream_t stream);
+
+// sort.hip: stable LSD radix sort of (u64 key, i32 value) pairs, n fixed on the host
+size_t sort_pairs_temp_bytes(int n);
+int sort_pairs(void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+               const int32_t* vals_in, int32_t* vals_out, int n, hipStream_t stream);
+
+// host helpers
+float annealed(float begin, float end, int generation, int total);
+
+// Bump allocator over a caller-provided workspace.
+struct Carve {
+    char* base;
+    size_t used = 0;
+    size_t cap;
+    Carve(void* p, size_t c) : base(static_cast<char*>(p)), cap(c) {}
+    template <typename T>
+    T* take(size_t count)
+    {
+        const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+        T* out = reinterpret_cast<T*>(base ? base + used : nullptr);
+        used += bytes;
+        return out;
+    }
+    void* take_bytes(size_t bytes)
+    {
+        void* out = base ? base + used : nullptr;
+        used += (bytes + 255) / 256 * 256;
+        return out;
+    }
+    bool ok() const { return used <= cap; }
+};
+
+}  // namespace moea
+
+namespace moea {
+size_t run_workspace_bytes(int algorithm, int n, int m, int k);
+int run_init(moea_run_args& a);
+int run_step(moea_run_args& a, int generations);
+}  // namespace moea

@@ -1,0 +1,611 @@
+// NSGA-III environmental selection on CDNA4.
+//
+// Replaces NSGA3EnvironmentSelector::select
+// (NS/selection/nsga3/nsga3_environment_selector.cu:289-538).  Every size that
+// the reference reads back to the host (pair count, front sizes, SVD status,
+// N_prior / N_last) stays on the device: kernels are launched for the largest
+// possible extent and read the live sizes from a small metadata array, so one
+// generation is a fixed launch sequence that a HIP graph can capture.
+//
+// Merged index space: s in [0, n) is parent s, s in [n, 2n) is offspring s-n.
+// Variables are never merged; the survivor gather reads them in place.
+#include "common.hpp"
+#include "moea_internal.hpp"
+
+namespace moea {
+
+namespace {
+
+// meta layout (int32, device)
+enum Meta : int { kMaxFront = 0, kNPrior = 1, kNLast = 2, kIll = 3, kNRem = 4, kMetaLen = 8 };
+
+constexpr int kScanThreads = 1024;
+
+// Exclusive scan of one int per thread across a 1024-thread block.
+__device__ inline int block_exclusive_scan(int v, int* s_wave, int& total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const int nw = blockDim.x >> 6;
+        int w = lane < nw ? s_wave[lane] : 0;
+        int wi = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(wi, o);
+            if (lane >= o) wi += t;
+        }
+        if (lane < nw) s_wave[lane] = wi - w;
+        if (lane == nw - 1) s_wave[63] = wi;
+    }
+    __syncthreads();
+    const int res = s_wave[wave] + incl - v;
+    total = s_wave[63];
+    __syncthreads();
+    return res;
+}
+
+// f (m, 2n) and cv (2n) of the merged population (nsga3_population_update.cu:7-115
+// copies variables too; here only the O(M n) objective columns move).
+__global__ void merge_objectives(const float* __restrict__ pf, const float* __restrict__ of,
+                                 const float* __restrict__ pcv, const float* __restrict__ ocv,
+                                 int n, int m, float* __restrict__ fm, float* __restrict__ cvm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nm = 2 * n;
+    if (i >= nm) return;
+    const bool par = i < n;
+    const int s = par ? i : i - n;
+    for (int k = 0; k < m; ++k)
+        fm[static_cast<size_t>(k) * nm + i] = par ? pf[static_cast<size_t>(k) * n + s]
+                                                  : of[static_cast<size_t>(k) * n + s];
+    cvm[i] = par ? pcv[s] : ocv[s];
+}
+
+// prior = {front < max_front}, last = {front == max_front}, each ascending
+// (nds_frontmask_kernel + copy_if, non_dominated_sort.cu:1580-1629);
+// gidx = [prior; last].  Single 1024-thread block.
+__global__ __launch_bounds__(kScanThreads) void split_fronts(const int32_t* __restrict__ fronts,
+                                                             int nm, int n, int32_t* __restrict__ gidx,
+                                                             int32_t* __restrict__ meta)
+{
+    __shared__ int s_wave[64];
+    const int mf = meta[kMaxFront];
+    const int chunk = (nm + kScanThreads - 1) / kScanThreads;
+    const int b = threadIdx.x * chunk, e = min(nm, b + chunk);
+    int cp = 0, cl = 0;
+    for (int i = b; i < e; ++i) {
+        const int fr = fronts[i];
+        cp += (fr >= 0 && fr < mf);
+        cl += (fr == mf);
+    }
+    int tp, tl;
+    int op = block_exclusive_scan(cp, s_wave, tp);
+    int ol = block_exclusive_scan(cl, s_wave, tl);
+    for (int i = b; i < e; ++i) {
+        const int fr = fronts[i];
+        if (fr >= 0 && fr < mf) gidx[op++] = i;
+        else if (fr == mf) gidx[tp + ol++] = i;
+    }
+    if (threadIdx.x == 0) {
+        meta[kNPrior] = tp;
+        meta[kNLast] = tl;
+        meta[kNRem] = n - tp;
+        meta[kIll] = -1;
+    }
+}
+
+// Running ideal point: ideal = fminf(ideal, min over offspring) (ideal_point.cu:25-165).
+__global__ void ideal_update(const float* __restrict__ f, int ldf, int col0, int count, int m,
+                             float* __restrict__ ideal, int init)
+{
+    __shared__ float s_min[1024 / 64];
+    const int k = blockIdx.x;
+    float z = INFINITY;
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        z = fminf(z, f[static_cast<size_t>(k) * ldf + col0 + i]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) z = fminf(z, __shfl_xor(z, o));
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = INFINITY;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) r = fminf(r, s_min[w]);
+        ideal[k] = init ? r : fminf(ideal[k], r);
+    }
+}
+
+// Translate, clamp and reduce the extreme-point / row-max candidates per block
+// (translated_ndsfit_kernel, compute_asf_legacy_kernel, find_intercepts_legacy_kernel:
+// normalization.cu:370-444, 647-729, 802-856).
+template <int M>
+__global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, int nm,
+                                                 const int32_t* __restrict__ gidx,
+                                                 const float* __restrict__ ideal,
+                                                 const int32_t* __restrict__ meta,
+                                                 float* __restrict__ fn, float* __restrict__ asf_val,
+                                                 int32_t* __restrict__ asf_idx,
+                                                 float* __restrict__ row_max)
+{
+    __shared__ float s_val[4][M];
+    __shared__ int s_idx[4][M];
+    __shared__ float s_max[4][M];
+    const int n_nds = meta[kNPrior] + meta[kNLast];
+    if (meta[kNRem] <= 0) return;
+    if (blockIdx.x * 256 >= n_nds) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n_nds;
+    float v[M];
+    if (valid) {
+        const int s = gidx[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            float t = fm[static_cast<size_t>(k) * nm + s] - ideal[k];
+            if (isfinite(t)) t = fmaxf(t, 0.0f);
+            v[k] = t;
+            fn[static_cast<size_t>(k) * nm + i] = t;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        float best = INFINITY;
+        int bi = -1;
+        if (valid) {
+            float mx = -INFINITY;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const float t = (v[k] != v[k]) ? INFINITY : v[k];
+                mx = fmaxf(mx, t * ((k == j) ? 1.0f : 1e6f));
+            }
+            if (mx < best) { best = mx; bi = i; }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float ov = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ov < best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        float rm = -INFINITY;
+        if (valid) rm = (v[j] != v[j]) ? -INFINITY : v[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) rm = fmaxf(rm, __shfl_xor(rm, o));
+        if (lane == 0) { s_val[wave][j] = best; s_idx[wave][j] = bi; s_max[wave][j] = rm; }
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        const int j = threadIdx.x;
+        float best = INFINITY;
+        int bi = -1;
+        float rm = -INFINITY;
+        for (int w = 0; w < 4; ++w) {
+            const float ov = s_val[w][j];
+            const int oi = s_idx[w][j];
+            if (ov < best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            rm = fmaxf(rm, s_max[w][j]);
+        }
+        asf_val[blockIdx.x * M + j] = best;
+        asf_idx[blockIdx.x * M + j] = bi;
+        row_max[blockIdx.x * M + j] = rm;
+    }
+}
+
+// Extreme points, FP64 hyperplane or fallback, sanitize (normalization.cu:453-540,
+// 572-583, 1593-1636).  One thread.
+__global__ void norm_solve(const float* __restrict__ fn, int nm, int m,
+                           const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
+                           const float* __restrict__ row_max, int32_t* __restrict__ meta,
+                           float* __restrict__ a_out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (meta[kNRem] <= 0) return;
+    const int n_nds = meta[kNPrior] + meta[kNLast];
+    const int nb = (n_nds + 255) / 256;
+    int ext[kMaxM];
+    for (int j = 0; j < m; ++j) {
+        float best = INFINITY;
+        int bi = -1;
+        for (int b = 0; b < nb; ++b) {
+            float v = asf_val[b * m + j];
+            const int idx = asf_idx[b * m + j];
+            v = (v != v) ? INFINITY : v;
+            if (v < best || (v == best && idx < bi)) { best = v; bi = idx; }
+        }
+        ext[j] = bi;
+    }
+    double E[kMaxM * kMaxM], a[kMaxM];
+    for (int i = 0; i < m; ++i)
+        for (int k = 0; k < m; ++k)
+            E[i * m + k] = (ext[i] >= 0 && ext[i] < n_nds)
+                               ? static_cast<double>(fn[static_cast<size_t>(k) * nm + ext[i]])
+                               : static_cast<double>(INFINITY);
+    const int ill = hyperplane_fp64(E, m, a);
+    if (ill) {
+        for (int k = 0; k < m; ++k) {
+            float mx = -INFINITY;
+            for (int b = 0; b < nb; ++b) mx = fmaxf(mx, row_max[b * m + k]);
+            a[k] = (mx > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(mx, 1e-6f)) : 1.0;
+        }
+    }
+    for (int k = 0; k < m; ++k) {
+        float af = static_cast<float>(a[k]);
+        if (!isfinite(af) || af <= 0.0f) af = 1.0f;
+        a_out[k] = af;
+    }
+    meta[kIll] = ill;
+}
+
+// Perpendicular-distance association, one individual per thread against a
+// chunk of directions staged in LDS (association.cu:85-188): dot by an fmaf
+// chain, residual x - dot*w as one fma, d2 by an fmaf chain, first minimum
+// wins.  Grid: (ceil(2n/256), ceil(K/kAssocChunk)).
+constexpr int kAssocChunk = 1024;
+
+template <int M>
+__global__ __launch_bounds__(256) void associate_partial(const float* __restrict__ fn, int nm,
+                                                         const float* __restrict__ a,
+                                                         const float* __restrict__ w, int k,
+                                                         const int32_t* __restrict__ meta,
+                                                         float* __restrict__ part_d2,
+                                                         int32_t* __restrict__ part_idx)
+{
+    __shared__ float s_w[M][kAssocChunk];
+    if (meta[kNRem] <= 0) return;
+    const int n_nds = meta[kNPrior] + meta[kNLast];
+    if (blockIdx.x * 256 >= n_nds) return;
+    const int j0 = blockIdx.y * kAssocChunk;
+    const int jn = min(kAssocChunk, k - j0);
+    for (int e = threadIdx.x; e < M * kAssocChunk; e += 256) {
+        const int c = e / kAssocChunk, j = e % kAssocChunk;
+        s_w[c][j] = (j < jn) ? w[static_cast<size_t>(c) * k + j0 + j] : 0.0f;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_nds) return;
+    float x[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) x[c] = fn[static_cast<size_t>(c) * nm + i] * a[c];
+    float best = INFINITY;
+    int bj = -1;
+    for (int j = 0; j < jn; ++j) {
+        float dot = 0.0f;
+#pragma unroll
+        for (int c = 0; c < M; ++c) dot = fmaf(x[c], s_w[c][j], dot);
+        float d2 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const float r = fmaf(-dot, s_w[c][j], x[c]);
+            d2 = fmaf(r, r, d2);
+        }
+        if (d2 < best) { best = d2; bj = j0 + j; }
+    }
+    part_d2[static_cast<size_t>(blockIdx.y) * nm + i] = best;
+    part_idx[static_cast<size_t>(blockIdx.y) * nm + i] = bj;
+}
+
+// Reduce the chunk partials in chunk order (association_phase2_kernel, :397-447).
+__global__ void associate_final(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
+                                int nchunks, int nm, const int32_t* __restrict__ meta,
+                                float* __restrict__ dist, int32_t* __restrict__ ref)
+{
+    if (meta[kNRem] <= 0) return;
+    const int n_nds = meta[kNPrior] + meta[kNLast];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nds) return;
+    float best = INFINITY;
+    int bj = -1;
+    for (int c = 0; c < nchunks; ++c) {
+        const float d = part_d2[static_cast<size_t>(c) * nm + i];
+        const int idx = part_idx[static_cast<size_t>(c) * nm + i];
+        if (d < best || (d == best && idx >= 0 && idx < bj)) { best = d; bj = idx; }
+    }
+    dist[i] = (best != INFINITY) ? sqrtf(best) : INFINITY;
+    ref[i] = bj;
+}
+
+// rho_prior histogram (prior_histogram_kernel, niching.cu:21-33).  Directions
+// with no association (ref -1, from non-finite normalized objectives) count in
+// no niche; the reference would index rho[-1] there.
+__global__ void rho_clear(int32_t* __restrict__ rho, int k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) rho[i] = 0;
+}
+__global__ void rho_hist(const int32_t* __restrict__ ref, const int32_t* __restrict__ meta, int k,
+                         int32_t* __restrict__ rho)
+{
+    if (meta[kNRem] <= 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= meta[kNPrior]) return;
+    const int r = ref[i];
+    if (r >= 0 && r < k) atomicAdd(&rho[r], 1);
+}
+
+// Phase-2 keys: ref << 32 | bits(dist), value = last-front local index
+// (build_lf_asrpts_asdist_key_kernel, niching.cu:68-92).  Slots past N_last
+// get the all-ones key; a stable sort keeps them after every real entry.
+__global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __restrict__ dist,
+                            const int32_t* __restrict__ meta, int nm, uint64_t* __restrict__ key,
+                            int32_t* __restrict__ val)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm) return;
+    const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
+    const int n_prior = meta[kNPrior];
+    uint64_t kk = ~0ull;
+    if (t < n_last) {
+        const uint32_t r = static_cast<uint32_t>(ref[n_prior + t]);
+        kk = (static_cast<uint64_t>(r) << 32) | __float_as_uint(dist[n_prior + t]);
+    }
+    key[t] = kk;
+    val[t] = t;
+}
+
+// First position of the run of equal high words containing t (keys sorted).
+__device__ inline int segment_head(const uint64_t* __restrict__ key, int t)
+{
+    const uint32_t hi = static_cast<uint32_t>(key[t] >> 32);
+    int lo = 0, up = t;
+    while (lo < up) {
+        const int mid = (lo + up) >> 1;
+        if (static_cast<uint32_t>(key[mid] >> 32) < hi) lo = mid + 1;
+        else up = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t niche_draw(const uint32_t* injected, int which, int t, uint64_t seed,
+                                      uint32_t gen)
+{
+    return injected ? injected[t] : splitmix32(niche_base(gen, which) + static_cast<uint64_t>(t), seed);
+}
+
+// Phase-4 rank keys: ref << 32 | (rho == 0 && local == 0 ? 0 : draw_a | 1)
+// (compute_rank_keys_kernel, niching.cu:114-139).
+__global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
+                            const int32_t* __restrict__ rho, int k, int nm,
+                            const uint32_t* __restrict__ inj_a, uint64_t seed, uint32_t gen,
+                            uint64_t* __restrict__ key2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm) return;
+    const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
+    uint64_t kk = ~0ull;
+    if (t < n_last) {
+        const uint32_t r = static_cast<uint32_t>(key1s[t] >> 32);
+        const int local = t - segment_head(key1s, t);
+        const int ri = static_cast<int>(r);
+        const int rh = (ri >= 0 && ri < k) ? rho[ri] : 0;
+        const uint32_t low = (rh == 0 && local == 0) ? 0u : (niche_draw(inj_a, 0, t, seed, gen) | 1u);
+        kk = (static_cast<uint64_t>(r) << 32) | low;
+    }
+    key2[t] = kk;
+}
+
+// Phase-5 keys: (rho + rank) << 32 | draw_b (compute_niching_keys_kernel, niching.cu:157-178).
+__global__ void niche_keys3(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
+                            const int32_t* __restrict__ rho, int k, int nm,
+                            const uint32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
+                            uint64_t* __restrict__ key3)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm) return;
+    const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
+    uint64_t kk = ~0ull;
+    if (t < n_last) {
+        const int ri = static_cast<int>(static_cast<uint32_t>(key2s[t] >> 32));
+        const int rank = t - segment_head(key2s, t);
+        const int rh = (ri >= 0 && ri < k) ? rho[ri] : 0;
+        kk = (static_cast<uint64_t>(static_cast<uint32_t>(rh + rank)) << 32) |
+             niche_draw(inj_b, 1, t, seed, gen);
+    }
+    key3[t] = kk;
+}
+
+// Survivor flags: all prior-front members plus the first N_rem niching picks
+// (gather_selected_kernel, niching.cu:192-206; prior copy at :303-312).
+__global__ void mark_chosen(const int32_t* __restrict__ fronts, const int32_t* __restrict__ meta,
+                            int nm, uint8_t* __restrict__ flag)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nm) return;
+    const int fr = fronts[s];
+    flag[s] = (fr >= 0 && fr < meta[kMaxFront]) ? 1 : 0;
+}
+__global__ void mark_selected(const int32_t* __restrict__ val3s, const int32_t* __restrict__ gidx,
+                              const int32_t* __restrict__ meta, uint8_t* __restrict__ flag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_rem = meta[kNRem];
+    if (n_rem <= 0 || meta[kNLast] <= 0) return;
+    if (t >= n_rem) return;
+    flag[gidx[meta[kNPrior] + val3s[t]]] = 1;
+}
+
+// Chosen indices in ascending merged order (thrust::sort of the selection,
+// nsga3_population_update.cu:269).  Single 1024-thread block.
+__global__ __launch_bounds__(kScanThreads) void compact_flags(const uint8_t* __restrict__ flag, int nm,
+                                                              int n, int32_t* __restrict__ chosen,
+                                                              int32_t* __restrict__ meta)
+{
+    __shared__ int s_wave[64];
+    const int chunk = (nm + kScanThreads - 1) / kScanThreads;
+    const int b = threadIdx.x * chunk, e = min(nm, b + chunk);
+    int c = 0;
+    for (int i = b; i < e; ++i) c += flag[i];
+    int tot;
+    int o = block_exclusive_scan(c, s_wave, tot);
+    for (int i = b; i < e; ++i)
+        if (flag[i] && o < n) chosen[o++] = i;
+    if (threadIdx.x == 0) meta[5] = tot;
+}
+
+// Survivor gather (pop_update_*_kernel + fit_and_fronts_update_kernel,
+// nsga3_population_update.cu:119-225); one wave per row.
+__global__ __launch_bounds__(256) void gather_next(const int32_t* __restrict__ chosen, int n, int d,
+                                                   int m, const float* __restrict__ px,
+                                                   const float* __restrict__ ox,
+                                                   const float* __restrict__ fm,
+                                                   const float* __restrict__ cvm,
+                                                   const int32_t* __restrict__ fronts_mix,
+                                                   float* __restrict__ nx, float* __restrict__ nf,
+                                                   float* __restrict__ ncv,
+                                                   int32_t* __restrict__ nfronts)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    int s = chosen[row];
+    if (s < 0 || s >= 2 * n) s = 0;  // unreachable: exactly n survivors are flagged
+    const float* src = (s < n) ? px + static_cast<size_t>(s) * d : ox + static_cast<size_t>(s - n) * d;
+    float* dst = nx + static_cast<size_t>(row) * d;
+    if ((d & 3) == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        for (int j = lane; j < (d >> 2); j += 64) d4[j] = s4[j];
+    } else {
+        for (int j = lane; j < d; j += 64) dst[j] = src[j];
+    }
+    const int nm = 2 * n;
+    if (lane < m) nf[static_cast<size_t>(lane) * n + row] = fm[static_cast<size_t>(lane) * nm + s];
+    if (lane == 0) {
+        ncv[row] = cvm[s];
+        nfronts[row] = fronts_mix[s];
+    }
+}
+
+__global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restrict__ stats)
+{
+    if (threadIdx.x < kMetaLen) stats[threadIdx.x] = meta[threadIdx.x];
+}
+
+struct Layout {
+    float* fm; float* cvm; int32_t* fronts; int32_t* meta; int32_t* gidx; float* fn;
+    float* asf_val; int32_t* asf_idx; float* row_max; float* a;
+    float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho;
+    uint64_t* k0; uint64_t* k1; int32_t* v0; int32_t* v1; void* sort_tmp; size_t sort_bytes;
+    uint8_t* flag; int32_t* chosen; void* nds_ws; size_t nds_bytes;
+};
+
+Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
+{
+    const int nm = 2 * n;
+    const int nb = ceil_div(nm, 256);
+    const int nchunks = ceil_div(k, kAssocChunk);
+    Layout L{};
+    L.fm = c.take<float>(static_cast<size_t>(m) * nm);
+    L.cvm = c.take<float>(nm);
+    L.fronts = c.take<int32_t>(nm);
+    L.meta = c.take<int32_t>(kMetaLen);
+    L.gidx = c.take<int32_t>(nm);
+    L.fn = c.take<float>(static_cast<size_t>(m) * nm);
+    L.asf_val = c.take<float>(static_cast<size_t>(nb) * m);
+    L.asf_idx = c.take<int32_t>(static_cast<size_t>(nb) * m);
+    L.row_max = c.take<float>(static_cast<size_t>(nb) * m);
+    L.a = c.take<float>(kMaxM);
+    L.part_d2 = c.take<float>(static_cast<size_t>(nchunks) * nm);
+    L.part_idx = c.take<int32_t>(static_cast<size_t>(nchunks) * nm);
+    L.dist = c.take<float>(nm);
+    L.ref = c.take<int32_t>(nm);
+    L.rho = c.take<int32_t>(k);
+    L.k0 = c.take<uint64_t>(nm);
+    L.k1 = c.take<uint64_t>(nm);
+    L.v0 = c.take<int32_t>(nm);
+    L.v1 = c.take<int32_t>(nm);
+    L.sort_bytes = sort_pairs_temp_bytes(nm);
+    L.sort_tmp = c.take_bytes(L.sort_bytes);
+    L.flag = c.take<uint8_t>(nm);
+    L.chosen = c.take<int32_t>(n);
+    L.nds_bytes = nds_workspace_bytes(nm);
+    L.nds_ws = c.take_bytes(L.nds_bytes);
+    (void)ws; (void)cap;
+    return L;
+}
+
+}  // namespace
+
+size_t nsga3_workspace_bytes(int n, int m, int k)
+{
+    Carve c(nullptr, 0);
+    carve(nullptr, 0, n, m, k, c);
+    return c.used;
+}
+
+int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream)
+{
+    if (!f || !ideal || n <= 0 || m <= 0 || m > kMaxM) return fail(kInvalidArgument, "ideal_point: invalid arguments");
+    hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, stream, f, ldf, 0, n, m, ideal, 1);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int nsga3_select(const moea_nsga3_select_args& A)
+{
+    const int n = A.n, m = A.m, d = A.d, k = A.k;
+    if (n <= 0 || (n & 1) || m <= 0 || m > kMaxM || d <= 0 || k <= 0)
+        return fail(kInvalidArgument, "nsga3_select: invalid sizes (n must be even and positive)");
+    if (!A.parents.x || !A.parents.f || !A.parents.cv || !A.offspring.x || !A.offspring.f ||
+        !A.offspring.cv || !A.next.x || !A.next.f || !A.next.cv || !A.next_fronts ||
+        !A.directions || !A.ideal || !A.workspace)
+        return fail(kInvalidArgument, "nsga3_select: null pointer");
+    if (A.next.x == A.parents.x || A.next.x == A.offspring.x)
+        return fail(kInvalidArgument, "nsga3_select: next must not alias parents or offspring");
+    if (2 * n > nds_max_n()) return fail(kUnsupported, "nsga3_select: population too large");
+    Carve c(A.workspace, A.workspace_bytes);
+    const Layout L = carve(A.workspace, A.workspace_bytes, n, m, k, c);
+    if (!c.ok()) return fail(kWorkspaceTooSmall, "nsga3_select: workspace too small");
+    hipStream_t st = static_cast<hipStream_t>(A.stream);
+    const int nm = 2 * n;
+    const int nb = ceil_div(nm, 256);
+    const int nchunks = ceil_div(k, kAssocChunk);
+    const uint32_t gen = static_cast<uint32_t>(A.generation);
+    const moea_draws* dr = A.draws;
+
+    hipLaunchKernelGGL(merge_objectives, dim3(nb), dim3(256), 0, st, A.parents.f, A.offspring.f,
+                       A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(nds_run(L.fm, nm, L.cvm, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
+    hipLaunchKernelGGL(split_fronts, dim3(1), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.gidx, L.meta);
+    hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, st, L.fm, nm, n, n, m, A.ideal, 0);
+    const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
+    MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, A.ideal, L.meta, L.fn,
+                  L.asf_val, L.asf_idx, L.row_max);
+    hipLaunchKernelGGL(norm_solve, dim3(1), dim3(64), 0, st, L.fn, nm, m, L.asf_val, L.asf_idx,
+                       L.row_max, L.meta, L.a);
+    MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
+                  L.meta, L.part_d2, L.part_idx);
+    hipLaunchKernelGGL(associate_final, dim3(nb), dim3(256), 0, st, L.part_d2, L.part_idx, nchunks,
+                       nm, L.meta, L.dist, L.ref);
+    hipLaunchKernelGGL(rho_clear, dim3(ceil_div(k, 256)), dim3(256), 0, st, L.rho, k);
+    hipLaunchKernelGGL(rho_hist, dim3(nb), dim3(256), 0, st, L.ref, L.meta, k, L.rho);
+    hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, L.k0, L.v0);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st));
+    hipLaunchKernelGGL(niche_keys2, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
+                       dr ? dr->niche_a : nullptr, A.seed, gen, L.k0);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v1, L.v0, nm, st));
+    hipLaunchKernelGGL(niche_keys3, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
+                       dr ? dr->niche_b : nullptr, A.seed, gen, L.k0);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st));
+    hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
+    hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
+                       L.flag);
+    hipLaunchKernelGGL(compact_flags, dim3(1), dim3(kScanThreads), 0, st, L.flag, nm, n, L.chosen, L.meta);
+    hipLaunchKernelGGL(gather_next, dim3(ceil_div(n, 4)), dim3(256), 0, st, L.chosen, n, d, m,
+                       A.parents.x, A.offspring.x, L.fm, L.cvm, L.fronts, A.next.x, A.next.f,
+                       A.next.cv, A.next_fronts);
+    if (A.stats) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace moea

@@ -1,0 +1,232 @@
+// Mating -> SBX -> polynomial mutation, fused into two coalesced kernels.
+//
+// Replaces ReproductionPipeline::generate (NS/reproduction/reproduction_pipeline.cu:37-56):
+// TournamentMating / RandomMating (mating_selection.cu:11-108), SBX
+// (sbx_crossover.cu:20-190) and PM (polynomial_mutation.cu:20-124).  The
+// reference materializes every draw array with separate cuRAND kernels; here
+// draws are either injected or generated in registers (Philox4x32-10), and the
+// crossover and mutation of a pair happen in one pass over its two rows.
+#include "common.hpp"
+#include "moea_internal.hpp"
+
+namespace moea {
+
+namespace {
+
+__global__ void mating_kernel(int n, int active_host, const int32_t* __restrict__ active_dev, int mating,
+                              const int32_t* __restrict__ rank,
+                              const float* __restrict__ cv, const int32_t* __restrict__ inj_a,
+                              const int32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
+                              int32_t* __restrict__ winners)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int active = active_dev ? active_dev[0] : active_host;
+    active = active < 1 ? 1 : (active > n ? n : active);
+    int a, b;
+    if (inj_a) {
+        // injected candidates are clamped into [0, n) so a bad draw cannot read out of bounds
+        a = min(max(inj_a[t], 0), n - 1);
+        b = inj_b ? min(max(inj_b[t], 0), n - 1) : a;
+    } else {
+        const U4 r = draw4(seed, static_cast<uint64_t>(t), gen, kStMate);
+        a = static_cast<int>(r.x % static_cast<uint32_t>(active));
+        b = static_cast<int>(r.y % static_cast<uint32_t>(active));
+    }
+    if (mating == MOEA_MATING_RANDOM) { winners[t] = a; return; }
+    // tournament_winner_kernel (mating_selection.cu:11-35)
+    const float ca = cv[a], cb = cv[b];
+    int w;
+    if (ca == 0.0f && cb != 0.0f) w = a;
+    else if (ca != 0.0f && cb == 0.0f) w = b;
+    else if (ca != 0.0f && cb != 0.0f) w = (ca <= cb) ? a : b;
+    else w = (rank[a] <= rank[b]) ? a : b;
+    winners[t] = w;
+}
+
+// SBX of one (pair, variable) element.  In midpoint +/- delta both operands are
+// products; the NVPTX combiner fuses the first (0.5f * (x1 + x2), an exact
+// product), so the sums evaluate unfused as written.
+__device__ inline void sbx_one(float x1, float x2, float ub, float up, int sign, float uc, float lo,
+                               float hi, float eta, float prob, float copy_prob, float& ca, float& cb)
+{
+    if (up >= prob || uc < copy_prob) { ca = x1; cb = x2; return; }
+    const float power = 1.0f / (1.0f + eta);
+    float beta = (ub < 0.5f) ? powf_det(2.0f * ub, power) : powf_det(1.0f / (2.0f - 2.0f * ub), power);
+    beta = beta * static_cast<float>(2 * sign - 1);
+    const float mid = 0.5f * (x1 + x2);
+    const float delta = 0.5f * beta * (x1 - x2);
+    ca = fminf(fmaxf(mid + delta, lo), hi);
+    cb = fminf(fmaxf(mid - delta, lo), hi);
+}
+
+// Deb's polynomial mutation of one element; value + delta * range is fused.
+__device__ inline float pm_one(float value, float ug, float ud, float lo, float hi, float eta, float rate)
+{
+    if (ug >= rate) return value;
+    const float range = hi - lo;
+    const float normalized = (value - lo) / range;
+    const float exponent = eta + 1.0f;
+    float delta;
+    if (ud <= 0.5f) {
+        const float xy = 1.0f - normalized;
+        const float vm = 2.0f * ud + (1.0f - 2.0f * ud) * powf_det(xy, exponent);
+        delta = powf_det(vm, 1.0f / exponent) - 1.0f;
+    } else {
+        const float xy = normalized;
+        const float vm = 2.0f * (1.0f - ud) + 2.0f * (ud - 0.5f) * powf_det(xy, exponent);
+        delta = 1.0f - powf_det(vm, 1.0f / exponent);
+    }
+    return fminf(fmaxf(fmaf(delta, range, value), lo), hi);
+}
+
+struct RepArgs {
+    int n, d, pairs;
+    const float* px;
+    const int32_t* winners;
+    const float* bounds;
+    float eta_c, prob_c, copy_c;
+    int mutate;
+    float eta_m, rate_m;
+    const float* sbx_u; const float* sbx_p; const int32_t* sbx_sign; const float* sbx_copy;
+    const float* pm_gate; const float* pm_delta;
+    uint64_t seed;
+    uint32_t gen;
+    float* ox;
+};
+
+// One thread per (pair, variable); the odd tail row (n odd) copies its parent
+// (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
+__global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
+{
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long total_pair = static_cast<long long>(R.pairs) * R.d;
+    const bool tail = (R.n & 1) != 0;
+    if (e >= total_pair + (tail ? R.d : 0)) return;
+    if (e < total_pair) {
+        const int pair = static_cast<int>(e / R.d);
+        const int j = static_cast<int>(e - static_cast<long long>(pair) * R.d);
+        const int w0 = R.winners[2 * pair], w1 = R.winners[2 * pair + 1];
+        const float x1 = R.px[static_cast<size_t>(w0) * R.d + j];
+        const float x2 = R.px[static_cast<size_t>(w1) * R.d + j];
+        const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
+        float ub, up, uc;
+        int sg;
+        if (R.sbx_u) {
+            ub = R.sbx_u[e]; up = R.sbx_p[e]; sg = R.sbx_sign[e]; uc = R.sbx_copy[e];
+        } else {
+            const U4 r = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
+            ub = u01(r.x); up = u01(r.y); sg = static_cast<int>(r.z >> 31); uc = u01(r.w);
+        }
+        float ca, cb;
+        sbx_one(x1, x2, ub, up, sg, uc, lo, hi, R.eta_c, R.prob_c, R.copy_c, ca, cb);
+        if (R.mutate) {
+            const size_t ra = static_cast<size_t>(2 * pair) * R.d + j;
+            const size_t rb = ra + R.d;
+            float ga, da, gb, db;
+            if (R.pm_gate) {
+                ga = R.pm_gate[ra]; da = R.pm_delta[ra]; gb = R.pm_gate[rb]; db = R.pm_delta[rb];
+            } else {
+                const U4 r = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm);
+                ga = u01(r.x); da = u01(r.y); gb = u01(r.z); db = u01(r.w);
+            }
+            ca = pm_one(ca, ga, da, lo, hi, R.eta_m, R.rate_m);
+            cb = pm_one(cb, gb, db, lo, hi, R.eta_m, R.rate_m);
+        }
+        R.ox[static_cast<size_t>(2 * pair) * R.d + j] = ca;
+        R.ox[static_cast<size_t>(2 * pair + 1) * R.d + j] = cb;
+    } else {
+        const int j = static_cast<int>(e - total_pair);
+        const int row = R.n - 1;
+        const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
+        float v = R.px[static_cast<size_t>(R.winners[row]) * R.d + j];
+        if (R.mutate) {
+            const size_t ra = static_cast<size_t>(row) * R.d + j;
+            float g, dd;
+            if (R.pm_gate) {
+                g = R.pm_gate[ra]; dd = R.pm_delta[ra];
+            } else {
+                const U4 r = draw4(R.seed, static_cast<uint64_t>(row >> 1) * R.d + j, R.gen, kStPm);
+                g = u01(r.x); dd = u01(r.y);
+            }
+            v = pm_one(v, g, dd, lo, hi, R.eta_m, R.rate_m);
+        }
+        R.ox[static_cast<size_t>(row) * R.d + j] = v;
+    }
+}
+
+// Population::initialize_random (core/population.cu:97-115):
+// lower + u * (upper - lower), contracted to one fma by nvcc.
+__global__ void init_kernel(float* __restrict__ x, long long count, int d,
+                            const float* __restrict__ bounds, uint64_t seed)
+{
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const int j = static_cast<int>(e % d);
+    const U4 r = draw4(seed, static_cast<uint64_t>(e), 0u, kStInit);
+    const float lo = bounds[2 * j], hi = bounds[2 * j + 1];
+    x[e] = fmaf(u01(r.x), hi - lo, lo);
+}
+
+}  // namespace
+
+float annealed(float begin, float end, int generation, int total)
+{
+    float progress = 1.0f;
+    if (total > 0) {
+        progress = static_cast<float>(generation) / static_cast<float>(total);
+        if (progress < 0.0f) progress = 0.0f;
+        if (progress > 1.0f) progress = 1.0f;
+    }
+    return begin + (end - begin) * progress;
+}
+
+int reproduce(const moea_reproduce_args& A)
+{
+    if (A.n <= 0 || A.d <= 0 || A.active <= 0 || A.active > A.n)
+        return fail(kInvalidArgument, "reproduce: invalid sizes or active count");
+    if (!A.parents_x || !A.bounds || !A.offspring_x || !A.parent_indices)
+        return fail(kInvalidArgument, "reproduce: null pointer");
+    if (A.mating == MOEA_MATING_TOURNAMENT && (!A.parents_rank || !A.parents_cv))
+        return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
+    if (A.mating != MOEA_MATING_TOURNAMENT && A.mating != MOEA_MATING_RANDOM)
+        return fail(kInvalidArgument, "reproduce: unknown mating kind");
+    hipStream_t st = static_cast<hipStream_t>(A.stream);
+    const moea_draws* dr = A.draws;
+    const uint32_t gen = static_cast<uint32_t>(A.generation);
+    hipLaunchKernelGGL(mating_kernel, dim3(ceil_div(A.n, 256)), dim3(256), 0, st, A.n, A.active,
+                       A.active_device, A.mating, A.parents_rank, A.parents_cv, dr ? dr->mate_a : nullptr,
+                       dr ? dr->mate_b : nullptr, A.seed, gen, A.parent_indices);
+    MOEA_HIP(hipGetLastError());
+    RepArgs R{};
+    R.n = A.n; R.d = A.d; R.pairs = A.n / 2;
+    R.px = A.parents_x; R.winners = A.parent_indices; R.bounds = A.bounds;
+    R.eta_c = A.sbx_eta; R.prob_c = A.sbx_prob; R.copy_c = A.sbx_copy_prob;
+    R.mutate = (A.mutation == MOEA_MUTATION_POLYNOMIAL) ? 1 : 0;
+    R.eta_m = A.pm_eta; R.rate_m = A.pm_prob / static_cast<float>(A.d);
+    if (dr) {
+        R.sbx_u = dr->sbx_u; R.sbx_p = dr->sbx_p; R.sbx_sign = dr->sbx_sign; R.sbx_copy = dr->sbx_copy;
+        R.pm_gate = dr->pm_gate; R.pm_delta = dr->pm_delta;
+        if ((R.sbx_u == nullptr) != (R.sbx_p == nullptr) || (R.sbx_u == nullptr) != (R.sbx_sign == nullptr) ||
+            (R.sbx_u == nullptr) != (R.sbx_copy == nullptr) || (R.pm_gate == nullptr) != (R.pm_delta == nullptr))
+            return fail(kInvalidArgument, "reproduce: partially injected draw set");
+    }
+    R.seed = A.seed; R.gen = gen; R.ox = A.offspring_x;
+    const long long total = static_cast<long long>(R.pairs) * A.d + ((A.n & 1) ? A.d : 0);
+    hipLaunchKernelGGL(sbx_pm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                       st, R);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, hipStream_t stream)
+{
+    if (!x || !bounds || n <= 0 || d <= 0) return fail(kInvalidArgument, "init_population: invalid arguments");
+    const long long count = static_cast<long long>(n) * d;
+    hipLaunchKernelGGL(init_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, stream,
+                       x, count, d, bounds, seed);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace moea

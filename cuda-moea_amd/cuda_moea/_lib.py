@@ -1,0 +1,152 @@
+"""ctypes binding of libmoea_hip.so (C ABI declared in include/moea_hip.h).
+
+The library is the only compute path: if it is missing this module raises at
+import time instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_float, c_int, c_int32, c_size_t, c_uint64, c_void_p
+from pathlib import Path
+
+_PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = _PKG_ROOT / "lib" / "libmoea_hip.so"
+
+MOEA_OK = 0
+MATING_TOURNAMENT, MATING_RANDOM = 0, 1
+MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
+ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
+PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK = 0, 1, 2
+MAX_OBJECTIVES = 10
+
+
+class Draws(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in (
+        "mate_a", "mate_b", "sbx_u", "sbx_p", "sbx_sign", "sbx_copy", "pm_gate", "pm_delta",
+        "niche_a", "niche_b")]
+
+
+class PopulationC(ctypes.Structure):
+    _fields_ = [("x", c_void_p), ("f", c_void_p), ("cv", c_void_p)]
+
+
+class NSGA3SelectArgs(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int),
+        ("parents", PopulationC), ("offspring", PopulationC), ("next", PopulationC),
+        ("next_fronts", c_void_p), ("directions", c_void_p), ("ideal", c_void_p),
+        ("draws", POINTER(Draws)), ("seed", c_uint64), ("generation", c_int),
+        ("stats", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
+        ("stream", c_void_p),
+    ]
+
+
+class ReproduceArgs(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("d", c_int), ("active", c_int), ("active_device", c_void_p),
+        ("mating", c_int), ("parents_x", c_void_p), ("parents_rank", c_void_p),
+        ("parents_cv", c_void_p), ("bounds", c_void_p), ("sbx_eta", c_float),
+        ("sbx_prob", c_float), ("sbx_copy_prob", c_float), ("mutation", c_int),
+        ("pm_eta", c_float), ("pm_prob", c_float), ("draws", POINTER(Draws)),
+        ("seed", c_uint64), ("generation", c_int), ("offspring_x", c_void_p),
+        ("parent_indices", c_void_p), ("stream", c_void_p),
+    ]
+
+
+class RVEASelectArgs(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int), ("active", c_int),
+        ("active_device", c_void_p),
+        ("parents", PopulationC), ("offspring", PopulationC), ("next", PopulationC),
+        ("directions", c_void_p), ("gamma", c_void_p), ("penalty", c_float),
+        ("out_active", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
+        ("stream", c_void_p),
+    ]
+
+
+EVALUATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                               c_void_p, c_int, c_int, c_void_p)
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_int), ("centers", c_void_p), ("w1", c_void_p), ("w2", c_void_p),
+        ("hidden", c_int), ("hwork", c_void_p), ("callback", EVALUATE_FN), ("user", c_void_p),
+    ]
+
+
+class RunArgs(ctypes.Structure):
+    _fields_ = [
+        ("algorithm", c_int), ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int),
+        ("max_generations", c_int), ("pops", PopulationC * 3), ("current", c_int),
+        ("fronts", c_void_p * 2), ("fronts_current", c_int), ("parent_indices", c_void_p),
+        ("bounds", c_void_p), ("directions", c_void_p), ("directions0", c_void_p),
+        ("gamma", c_void_p), ("ideal", c_void_p), ("active_device", c_void_p),
+        ("adapt_scratch", c_void_p), ("stats", c_void_p), ("stats_init", c_void_p),
+        ("mating", c_int), ("mutation", c_int), ("sbx_eta", c_float * 2), ("sbx_prob", c_float),
+        ("sbx_copy_prob", c_float), ("pm_eta", c_float * 2), ("pm_prob", c_float),
+        ("rvea_alpha", c_float), ("rvea_frequency", c_float), ("seed", c_uint64),
+        ("initial_x_given", c_int), ("problem", ProblemDesc), ("generation", c_int),
+        ("workspace", c_void_p), ("workspace_bytes", c_size_t), ("stream", c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/moea_hip.h declares
+SIGNATURES = {
+    "moea_abi_version": (c_int, []),
+    "moea_last_error": (ctypes.c_char_p, []),
+    "moea_nds_workspace_bytes": (c_size_t, [c_int]),
+    "moea_nds_rank": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                              c_void_p, c_size_t, c_void_p]),
+    "moea_das_dennis": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "moea_nsga3_select": (c_int, [POINTER(NSGA3SelectArgs)]),
+    "moea_ideal_point": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "moea_reproduce": (c_int, [POINTER(ReproduceArgs)]),
+    "moea_annealed": (c_float, [c_float, c_float, c_int, c_int]),
+    "moea_init_population": (c_int, [c_void_p, c_int, c_int, c_void_p, c_uint64, c_void_p]),
+    "moea_eval_quadratic": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
+                                    c_void_p, c_void_p]),
+    "moea_eval_mlp": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                              c_int, c_void_p, c_void_p, c_void_p]),
+    "moea_rvea_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "moea_rvea_select": (c_int, [POINTER(RVEASelectArgs)]),
+    "moea_rvea_penalty": (c_float, [c_int, c_int, c_int, c_float]),
+    "moea_rvea_gamma": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "moea_rvea_adapt": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p,
+                                c_void_p, c_void_p]),
+    "moea_run_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "moea_run_init": (c_int, [POINTER(RunArgs)]),
+    "moea_run_step": (c_int, [POINTER(RunArgs), c_int]),
+}
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"libmoea_hip.so not found at {LIB_PATH}; build it with "
+            "`make -C cuda-moea_amd` (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class MoeaError(RuntimeError):
+    """A libmoea_hip call failed on the device side."""
+
+
+def check(status: int) -> None:
+    """Raise the exception type the reference raises for the same failure:
+    invalid configurations -> ValueError, device problems -> RuntimeError."""
+    if status == MOEA_OK:
+        return
+    msg = (lib.moea_last_error() or b"").decode(errors="replace")
+    if status in (-1, -4):
+        raise ValueError(msg)
+    raise MoeaError(f"libmoea_hip error {status}: {msg}")

@@ -1,0 +1,451 @@
+"""Python facade with the reference's API (python/cuda_moea/algorithm.py:16-165).
+
+The generation loop itself is native: ``moea_run_init`` / ``moea_run_step`` in
+libmoea_hip.so run EvolutionaryAlgorithm::initialize / step
+(NS/algorithm/algorithm.cu:104-286) as stream-ordered launches with no host
+synchronization.  Python owns the device buffers (torch tensors) and the
+configuration.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from ._specs import (
+    AdaptiveRVEADirections, CudaConfig, DasDennisDirections, NoMutation,
+    NSGA3EnvironmentSelector, PolynomialMutation, PythonProblem, RandomMating,
+    RVEAEnvironmentSelector, SBX, TournamentMating, UserDefinedDirections,
+)
+from . import kernels as K
+from .problem import DTLZ2, QuadraticCenters, SeededMLP
+
+
+@dataclass(frozen=True)
+class Population:
+    variables: Any
+    objectives: Any
+    constraints: Any
+
+
+@dataclass(frozen=True)
+class Result(Population):
+    auxiliary_indices: Any
+    active_count: int
+    total_ms: float
+
+
+def _unsupported(obj: Any, what: str) -> NotImplementedError:
+    return NotImplementedError(f"{type(obj).__name__} is not supported as {what} by this engine")
+
+
+class Algorithm:
+    def __init__(self, algorithm: str = "NSGA3", *, population_size: int = 1024,
+                 max_generations: int = 100, problem: Any | None = None,
+                 mating: Any | None = None, crossover: Any | None = None,
+                 mutation: Any | None = None, reference_directions: Any | None = None,
+                 environment_selector: Any | None = None, initial_population: Any | None = None,
+                 cuda: CudaConfig | None = None, device: int | str | None = None,
+                 seed: int | None = None, enable_warmup: bool | None = None,
+                 progress_interval: int = 100, print_progress: bool = True,
+                 save_data: str | Path | None = None, save_interval: int = 1) -> None:
+        kind = algorithm.upper().replace("-", "")
+        if kind not in {"NSGA3", "RVEA"}:
+            raise ValueError("algorithm must be 'NSGA3' or 'RVEA'")
+        cuda = CudaConfig() if cuda is None else cuda
+        if device is not None:
+            cuda.device_id = _device_index(device)
+        if seed is not None:
+            cuda.seed = seed
+        if enable_warmup is not None:
+            cuda.enable_warmup = enable_warmup
+        if save_data is not None:
+            raise NotImplementedError("snapshot recording (save_data) is not supported by this engine")
+        self.kind = kind
+        self.cuda = cuda
+        self.population_size = int(population_size)
+        self.max_generations = int(max_generations)
+        self.problem = DTLZ2() if problem is None else problem
+        self.mating = (TournamentMating() if kind == "NSGA3" else RandomMating()) if mating is None else mating
+        self.crossover = SBX() if crossover is None else crossover
+        self.mutation = PolynomialMutation() if mutation is None else mutation
+        self.reference_directions = ((DasDennisDirections() if kind == "NSGA3" else AdaptiveRVEADirections())
+                                     if reference_directions is None else reference_directions)
+        self.environment_selector = ((NSGA3EnvironmentSelector() if kind == "NSGA3" else RVEAEnvironmentSelector())
+                                     if environment_selector is None else environment_selector)
+        self.initial_population = initial_population
+        self.progress_interval = progress_interval
+        self.print_progress = print_progress
+        self._validate_strategies()
+        self._initialized = False
+        self._total_ms = 0.0
+        self._resets = 0
+        self._error: BaseException | None = None
+
+    # ------------------------------------------------------------ config ----
+    def _validate_strategies(self) -> None:
+        if not isinstance(self.mating, (TournamentMating, RandomMating)):
+            raise _unsupported(self.mating, "mating")
+        if not isinstance(self.crossover, SBX):
+            raise _unsupported(self.crossover, "crossover")
+        self.crossover.validate()
+        if isinstance(self.mutation, PolynomialMutation):
+            self.mutation.validate()
+        elif not isinstance(self.mutation, NoMutation):
+            raise _unsupported(self.mutation, "mutation")
+        rd = self.reference_directions
+        if not isinstance(rd, (DasDennisDirections, AdaptiveRVEADirections, UserDefinedDirections)):
+            raise _unsupported(rd, "reference directions")
+        if isinstance(rd, DasDennisDirections) and rd.partitions < 0:
+            raise ValueError("Das-Dennis partitions cannot be negative")
+        sel = self.environment_selector
+        if self.kind == "NSGA3":
+            if not isinstance(sel, NSGA3EnvironmentSelector):
+                raise _unsupported(sel, "NSGA-III environment selector")
+            if not sel.sparse_ratio > 0.0:
+                raise ValueError("NSGA-III sparse_ratio must be positive")
+            if sel.cv_bins > 0:
+                raise NotImplementedError("NSGA-III CV quantization (cv_bins > 0) is not supported")
+            if isinstance(rd, AdaptiveRVEADirections):
+                raise ValueError("NSGA-III takes Das-Dennis or user-defined directions")
+        else:
+            if not isinstance(sel, RVEAEnvironmentSelector):
+                raise _unsupported(sel, "RVEA environment selector")
+            if not sel.alpha > 0.0:
+                raise ValueError("RVEA alpha must be positive")
+        if not isinstance(self.problem, (QuadraticCenters, SeededMLP, PythonProblem)):
+            raise _unsupported(self.problem, "problem")
+
+    # ------------------------------------------------------------ device ----
+    def initialize(self) -> None:
+        if self._initialized:
+            return
+        if self.population_size <= 0 or self.max_generations <= 0:
+            raise ValueError("Algorithm requires positive population_size and max_generations")
+        if not torch.cuda.is_available():
+            raise RuntimeError("cuda_moea requires a ROCm GPU")
+        p = self.problem
+        n, d, m = self.population_size, int(p.dimension), int(p.objectives)
+        if d <= 0 or m <= 0 or len(p.lower_bounds) != d or len(p.upper_bounds) != d:
+            raise ValueError("Problem must provide positive dimensions and one lower/upper bound per decision variable")
+        if m > _lib.MAX_OBJECTIVES:
+            raise ValueError(f"this engine supports at most {_lib.MAX_OBJECTIVES} objectives")
+        if self.kind == "NSGA3" and n % 2 != 0:
+            raise ValueError("The current NSGA-III backend requires an even population size")
+        dev = torch.device("cuda", self.cuda.device_id)
+        self.device = dev
+        self.n, self.d, self.m = n, d, m
+        lb = np.asarray(p.lower_bounds, np.float32)
+        ub = np.asarray(p.upper_bounds, np.float32)
+        bounds = np.empty(2 * d, np.float32)
+        bounds[0::2], bounds[1::2] = lb, ub
+        self._bounds = torch.from_numpy(bounds).to(dev)
+        self._lb, self._ub = lb, ub
+        v = self._directions_host(n, m)
+        k = v.shape[1]
+        self.k = k
+        self._directions = torch.from_numpy(v).to(dev)
+        self._directions0 = self._directions.clone()
+        self._pops = [K.Population.empty(n, d, m, dev) for _ in range(3)]
+        self._fronts = [torch.full((n,), -1, dtype=torch.int32, device=dev) for _ in range(2)]
+        self._parent_idx = torch.empty(n, dtype=torch.int32, device=dev)
+        self._ideal = torch.empty(m, dtype=torch.float32, device=dev)
+        self._gamma = torch.empty(k, dtype=torch.float32, device=dev)
+        self._active = torch.full((1,), n, dtype=torch.int32, device=dev)
+        self._scratch = torch.empty(max(m, 16), dtype=torch.float32, device=dev)
+        self._stats = torch.zeros(8, dtype=torch.int32, device=dev)
+        self._stats_init = torch.zeros(8, dtype=torch.int32, device=dev)
+        alg = _lib.ALGORITHM_NSGA3 if self.kind == "NSGA3" else _lib.ALGORITHM_RVEA
+        self._ws = K.workspace(lib.moea_run_workspace_bytes(alg, n, m, k), dev)
+        a = _lib.RunArgs()
+        a.algorithm, a.n, a.m, a.d, a.k, a.max_generations = alg, n, m, d, k, self.max_generations
+        for i, pop in enumerate(self._pops):
+            a.pops[i] = pop.c()
+        a.current = 0
+        a.fronts[0], a.fronts[1] = self._fronts[0].data_ptr(), self._fronts[1].data_ptr()
+        a.fronts_current = 0
+        a.parent_indices = self._parent_idx.data_ptr()
+        a.bounds = self._bounds.data_ptr()
+        a.directions, a.directions0 = self._directions.data_ptr(), self._directions0.data_ptr()
+        a.gamma, a.ideal = self._gamma.data_ptr(), self._ideal.data_ptr()
+        a.active_device, a.adapt_scratch = self._active.data_ptr(), self._scratch.data_ptr()
+        a.stats, a.stats_init = self._stats.data_ptr(), self._stats_init.data_ptr()
+        a.mating = _lib.MATING_TOURNAMENT if isinstance(self.mating, TournamentMating) else _lib.MATING_RANDOM
+        if isinstance(self.mutation, PolynomialMutation):
+            a.mutation = _lib.MUTATION_POLYNOMIAL
+            a.pm_eta[0], a.pm_eta[1] = self.mutation.eta_initial, self.mutation.eta_final
+            a.pm_prob = self.mutation.probability
+        else:
+            a.mutation = _lib.MUTATION_NONE
+        c = self.crossover
+        a.sbx_eta[0], a.sbx_eta[1] = c.eta_initial, c.eta_final
+        a.sbx_prob, a.sbx_copy_prob = c.probability, c.variable_copy_probability
+        if self.kind == "RVEA":
+            a.rvea_alpha = self.environment_selector.alpha
+            freq = getattr(self.reference_directions, "frequency", 1.0)
+            if isinstance(self.reference_directions, AdaptiveRVEADirections) and not 0.0 < freq <= 1.0:
+                raise ValueError("RVEA reference adaptation frequency must be in (0, 1]")
+            # fixed (user-defined) directions never adapt: an interval past the run
+            a.rvea_frequency = freq if isinstance(self.reference_directions, AdaptiveRVEADirections) else 1e9
+        a.seed = self._run_seed()
+        self._setup_problem(a)
+        if self.initial_population is not None:
+            init = torch.as_tensor(self.initial_population, dtype=torch.float32)
+            if tuple(init.shape) != (n, d) and init.numel() != n * d:
+                raise ValueError("initial_population must have shape (population_size, dimension)")
+            init = init.reshape(n, d)
+            if (not torch.isfinite(init).all() or (init < torch.from_numpy(lb)).any()
+                    or (init > torch.from_numpy(ub)).any()):
+                raise ValueError("initial_population contains a non-finite or out-of-bounds value")
+            self._pops[0].x.copy_(init.to(dev))
+            a.initial_x_given = 1
+        a.workspace, a.workspace_bytes = self._ws.data_ptr(), self._ws.numel()
+        a.stream = torch.cuda.current_stream(dev).cuda_stream
+        self._args = a
+        self._call(lib.moea_run_init)
+        self._initialized = True
+        self._total_ms = 0.0
+
+    def _run_seed(self) -> int:
+        s = int(self.cuda.seed) & (2**64 - 1)
+        return (s + self._resets * 0x9E3779B97F4A7C15) & (2**64 - 1)
+
+    def _directions_host(self, n: int, m: int) -> np.ndarray:
+        rd = self.reference_directions
+        if isinstance(rd, UserDefinedDirections):
+            vals = np.asarray(rd.values, dtype=np.float32)
+            if vals.ndim != 2 or vals.shape[0] != m or vals.size == 0:
+                raise ValueError("User directions must have shape (objective_count, count)")
+            vals = np.array(vals, dtype=np.float32, copy=True)
+            if rd.normalize:
+                for j in range(vals.shape[1]):
+                    ss = np.float32(0.0)
+                    for c in range(m):
+                        ss = np.float32(ss + vals[c, j] * vals[c, j])
+                    if ss <= 0:
+                        raise ValueError("Reference directions must be non-zero")
+                    inv = np.float32(1.0) / np.sqrt(ss)
+                    vals[:, j] = vals[:, j] * inv
+            return np.ascontiguousarray(vals)
+        if isinstance(rd, AdaptiveRVEADirections):
+            return K.das_dennis(n, m, 0, kind=2)
+        return K.das_dennis(n, m, rd.partitions)
+
+    def _setup_problem(self, a: _lib.RunArgs) -> None:
+        p = self.problem
+        pd = _lib.ProblemDesc()
+        if isinstance(p, QuadraticCenters):
+            self._centers = p.centers.to(self.device).contiguous()
+            pd.kind, pd.centers = _lib.PROBLEM_QUADRATIC, self._centers.data_ptr()
+        elif isinstance(p, SeededMLP):
+            self._w1 = p.w1.to(self.device).contiguous()
+            self._w2 = p.w2.to(self.device).contiguous()
+            self._hwork = torch.empty(self.n, p.hidden, dtype=torch.float32, device=self.device)
+            pd.kind, pd.w1, pd.w2 = _lib.PROBLEM_MLP, self._w1.data_ptr(), self._w2.data_ptr()
+            pd.hidden, pd.hwork = p.hidden, self._hwork.data_ptr()
+        else:
+            self._by_ptr = {}
+            for pop in self._pops:
+                self._by_ptr[pop.x.data_ptr()] = pop
+            if hasattr(p, "initialize"):
+                p.initialize({"device": self.cuda.device_id})
+            self._callback = _lib.EVALUATE_FN(self._evaluate_python)
+            pd.kind, pd.callback = _lib.PROBLEM_CALLBACK, self._callback
+        a.problem = pd
+
+    def _evaluate_python(self, user, x, n, d, m, f, cv, generation, max_generations, stream) -> int:
+        # adapters.cpp:115-173: evaluate on the run's stream, copy (n, m) -> (m, n)
+        try:
+            pop = self._by_ptr[x]
+            out = self.problem.evaluate(pop.x, {"generation": generation,
+                                                "max_generations": max_generations,
+                                                "device": self.cuda.device_id})
+            cons = None
+            if isinstance(out, dict):
+                obj, cons = out["objectives"], out.get("constraints")
+            elif isinstance(out, tuple):
+                obj = out[0]
+                cons = out[1] if len(out) > 1 else None
+            else:
+                obj = out
+            if not isinstance(obj, torch.Tensor) or not obj.is_cuda:
+                raise ValueError("objectives must be a torch.Tensor on the algorithm CUDA device")
+            if tuple(obj.shape) != (n, m):
+                raise ValueError("objectives must have shape (population_size, objective_count)")
+            pop.f.copy_(obj.to(torch.float32).t())
+            if cons is not None:
+                cons = torch.as_tensor(cons, device=self.device, dtype=torch.float32)
+                if cons.numel() != n:
+                    raise ValueError("constraints must contain population_size values")
+                pop.cv.copy_(cons.reshape(n))
+            else:
+                pop.cv.zero_()
+            return 0
+        except BaseException as exc:  # surfaced after the native call returns
+            self._error = exc
+            return -1
+
+    def _call(self, fn, *extra) -> None:
+        self._error = None
+        status = fn(ctypes.byref(self._args), *extra)
+        if self._error is not None:
+            err, self._error = self._error, None
+            raise err
+        check(status)
+
+    # --------------------------------------------------------- lifecycle ----
+    def step(self) -> None:
+        if not self._initialized:
+            self.initialize()
+        if self.finished:
+            return
+        self._call(lib.moea_run_step, 1)
+        self._progress()
+
+    def _progress(self) -> None:
+        g = self.generation
+        if self.print_progress and (g == self.max_generations or
+                                    (self.progress_interval > 0 and g % self.progress_interval == 0)):
+            print(f"Generation {g} / {self.max_generations} completed")
+
+    def run(self, *, copy: bool = True) -> Result:
+        if not self._initialized:
+            self.initialize()
+        stream = torch.cuda.current_stream(self.device)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        while not self.finished:
+            if self.print_progress and self.progress_interval > 0:
+                todo = self.progress_interval - self.generation % self.progress_interval
+            else:
+                todo = self.max_generations - self.generation
+            self._call(lib.moea_run_step, int(todo))
+            self._progress()
+        stop.record(stream)
+        stop.synchronize()
+        self._total_ms = float(start.elapsed_time(stop))
+        return self.result(copy=copy)
+
+    def _current(self) -> K.Population:
+        return self._pops[self._args.current]
+
+    def result(self, *, copy: bool = True) -> Result:
+        if not self._initialized:
+            raise RuntimeError("Algorithm has not been initialized")
+        pop = self._current()
+        torch.cuda.current_stream(self.device).synchronize()
+        if self.kind == "NSGA3":
+            aux = self._fronts[self._args.fronts_current]
+            active = self.n
+        else:
+            active = int(self._active.item())
+            fr, _ = K.nds_rank(pop.f[:, :active].contiguous(), pop.cv[:active].contiguous())
+            aux = torch.nonzero(fr == 0).flatten().to(torch.int32)
+        x, f, cv = pop.x, pop.f.t(), pop.cv
+        if copy:
+            x, f, cv, aux = x.clone(), f.contiguous(), cv.clone(), aux.clone()
+        return Result(x, f, cv, aux, active, self._total_ms)
+
+    def reset(self) -> None:
+        if not self._initialized:
+            return
+        self.synchronize()
+        self._resets += 1
+        self._directions.copy_(self._directions0)
+        self._args.seed = self._run_seed()
+        self._args.current = 0
+        self._args.fronts_current = 0
+        self._args.initial_x_given = 0
+        self._call(lib.moea_run_init)
+        self._total_ms = 0.0
+
+    def synchronize(self) -> None:
+        if self._initialized:
+            torch.cuda.current_stream(self.device).synchronize()
+
+    @property
+    def population(self) -> Population:
+        if not self._initialized:
+            raise RuntimeError("Algorithm has not been initialized")
+        pop = self._current()
+        return Population(pop.x, pop.f.t(), pop.cv)
+
+    @property
+    def initialized(self) -> bool:
+        return self._initialized
+
+    @property
+    def finished(self) -> bool:
+        return self._initialized and self._args.generation >= self.max_generations
+
+    @property
+    def generation(self) -> int:
+        return int(self._args.generation) if self._initialized else 0
+
+    @property
+    def directions(self) -> torch.Tensor:
+        return self._directions
+
+
+class NSGA3(Algorithm):
+    def __init__(self, **kwargs: Any) -> None:
+        super().__init__("NSGA3", **kwargs)
+
+
+class RVEA(Algorithm):
+    def __init__(self, **kwargs: Any) -> None:
+        super().__init__("RVEA", **kwargs)
+
+
+class AlgorithmBuilder:
+    def __init__(self, algorithm: str) -> None:
+        self.algorithm = algorithm
+        self.options: dict[str, Any] = {}
+
+    def set(self, **options: Any) -> "AlgorithmBuilder":
+        self.options.update(options)
+        return self
+
+    def population_size(self, value: int) -> "AlgorithmBuilder":
+        return self.set(population_size=value)
+
+    def max_generations(self, value: int) -> "AlgorithmBuilder":
+        return self.set(max_generations=value)
+
+    def problem(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(problem=value)
+
+    def mating(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(mating=value)
+
+    def crossover(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(crossover=value)
+
+    def mutation(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(mutation=value)
+
+    def reference_directions(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(reference_directions=value)
+
+    def environment_selector(self, value: Any) -> "AlgorithmBuilder":
+        return self.set(environment_selector=value)
+
+    def build(self) -> Algorithm:
+        return Algorithm(self.algorithm, **self.options)
+
+
+def _device_index(device: int | str) -> int:
+    if isinstance(device, int):
+        return device
+    if device == "cuda":
+        return 0
+    if device.startswith("cuda:"):
+        return int(device.split(":", 1)[1])
+    raise ValueError("cuda_moea only supports GPU devices")

@@ -1,0 +1,325 @@
+/*
+ * moea_hip.h -- C ABI of the MI355X-native MOEA engine (libmoea_hip.so).
+ *
+ * One NSGA-III / RVEA generation on the GPU: constrained non-dominated sorting,
+ * ideal point and FP64 hyperplane normalization, perpendicular-distance
+ * association, niching, RVEA angle-penalized selection, tournament / random
+ * mating, SBX + polynomial mutation and the built-in objectives, plus a batched
+ * engine that advances many independent populations in one launch sequence.
+ *
+ * Conventions (those of the reference, python/csrc/native/include/cuda_moea/
+ * core/types.cuh:21-51):
+ *   - variables  (n, d)  row-major float32
+ *   - objectives (m, ld) objective-major float32 (column i = individual i)
+ *   - constraints (n,)   float32, feasible means exactly 0.0f
+ *   - bounds (2d,)       interleaved [lb0, ub0, lb1, ub1, ...]
+ *   - directions (m, k)  objective-major, unit length
+ * All pointers are device pointers unless stated; `stream` is a hipStream_t.
+ * The caller owns every buffer, including the workspace whose size the
+ * *_workspace_bytes functions report; nothing here allocates device memory,
+ * so every call can be captured into a HIP graph.
+ * Every function returns MOEA_OK (0) or a negative MOEA_E_* code; the message
+ * of the last failure on the calling thread is moea_last_error().
+ */
+#ifndef MOEA_HIP_H
+#define MOEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOEA_OK 0
+#define MOEA_E_INVALID (-1)
+#define MOEA_E_HIP (-2)
+#define MOEA_E_WORKSPACE (-3)
+#define MOEA_E_UNSUPPORTED (-4)
+#define MOEA_E_DEVICE (-5)
+
+#define MOEA_MAX_OBJECTIVES 10
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* ABI version of this header (bumped on any signature change). */
+int moea_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* moea_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Draws                                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Injected draws for one generation.  Replaces the cuRAND-backed draw kernels
+ * the reference launches per operator (core/cuda/cuda_random.cu:1393-1702),
+ * whose stream layout depends on device occupancy and cannot be reproduced.
+ * Any NULL array is generated in-kernel from (seed, generation) instead:
+ * Philox4x32-10 for mating / SBX / PM, and the reference's own SplitMix64
+ * counter hash (cuda_random.cu:1606-1644) for the niching tie-breaks.
+ *   mate_a, mate_b : (n) tournament candidates (mate_b unused by random mating)
+ *   sbx_u, sbx_p, sbx_copy : (n/2 * d) floats in [0,1), index pair*d + j
+ *   sbx_sign       : (n/2 * d) int32 0/1
+ *   pm_gate, pm_delta : (n * d) floats in [0,1), index row*d + j
+ *   niche_a, niche_b  : (2n) uint32, consumed by sorted position
+ */
+typedef struct moea_draws {
+    const int32_t* mate_a;
+    const int32_t* mate_b;
+    const float* sbx_u;
+    const float* sbx_p;
+    const int32_t* sbx_sign;
+    const float* sbx_copy;
+    const float* pm_gate;
+    const float* pm_delta;
+    const uint32_t* niche_a;
+    const uint32_t* niche_b;
+} moea_draws;
+
+/* ------------------------------------------------------------------------ */
+/* Non-dominated sorting                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes for moea_nds_rank on n individuals. */
+size_t moea_nds_workspace_bytes(int n);
+
+/* Constrained non-dominated sorting of n individuals.  Replaces run_ndsort /
+ * run_ndsort_init (selection/nsga3/kernels/non_dominated_sort.cuh, defined at
+ * non_dominated_sort.cu:1333-1574).  Fronts are peeled until at least stop_at
+ * individuals are assigned; the others get front -1.  Writes fronts (n,) and
+ * max_front (1,) = index of the last peeled front, both on the device.
+ * f: (m, ldf) objectives, cv: (n,) constraint violations. */
+int moea_nds_rank(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
+                  int32_t* fronts, int32_t* max_front, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Reference directions (host)                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Das-Dennis directions.  kind 0: sized from requested_count
+ * (reference/nsga3_reference_generator.cu:62-179); kind 1: from partitions
+ * (:182-221); kind 2: RVEA variant, values clamped to >= 1e-6 before
+ * normalizing (reference/rvea_reference_generator.cu:185-217).
+ * Returns K; when out_host is non-NULL it receives (m, K) unit directions and
+ * raw_host (optional) the unnormalized lattice points. */
+int moea_das_dennis(int requested_count, int m, int partitions, int kind, float* out_host,
+                    float* raw_host);
+
+/* ------------------------------------------------------------------------ */
+/* NSGA-III environmental selection                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct moea_population {
+    float* x;        /* (n, d) */
+    float* f;        /* (m, n) */
+    float* cv;       /* (n,)   */
+} moea_population;
+
+typedef struct moea_nsga3_select_args {
+    int n, m, d, k;
+    moea_population parents;    /* input */
+    moea_population offspring;  /* input */
+    moea_population next;       /* output; must not alias parents/offspring */
+    int32_t* next_fronts;       /* (n,) output: fronts of the survivors */
+    const float* directions;    /* (m, k) */
+    float* ideal;               /* (m,) running ideal point, updated */
+    const moea_draws* draws;    /* NULL: all in-kernel */
+    uint64_t seed;
+    int generation;
+    int32_t* stats;             /* optional (8,) device: max_front, n_prior, n_last,
+                                   fallback flag, n_rem */
+    void* workspace;
+    size_t workspace_bytes;
+    void* stream;
+} moea_nsga3_select_args;
+
+size_t moea_nsga3_workspace_bytes(int n, int m, int k);
+
+/* One NSGA-III selection over the 2n merged individuals.  Replaces
+ * IEnvironmentSelector::select for NSGA3EnvironmentSelector
+ * (selection/nsga3/nsga3_environment_selector.cu:289-538): constrained NDS,
+ * prior/last front split, ideal point, extreme points with the M x M
+ * hyperplane solve in FP64 and its row-max fallback, association, niching
+ * and the survivor gather in ascending merged-index order. */
+int moea_nsga3_select(const moea_nsga3_select_args* args);
+
+/* Running ideal point over one population (ideal_point.cu:97-133); used by
+ * NSGA3EnvironmentSelector::prepare on the initial parents. */
+int moea_ideal_point(const float* f, int ldf, int n, int m, float* ideal, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Reproduction                                                              */
+/* ------------------------------------------------------------------------ */
+
+#define MOEA_MATING_TOURNAMENT 0
+#define MOEA_MATING_RANDOM 1
+#define MOEA_MUTATION_NONE 0
+#define MOEA_MUTATION_POLYNOMIAL 1
+
+typedef struct moea_reproduce_args {
+    int n, d;
+    int active;                  /* mating draws candidates from [0, active) */
+    const int32_t* active_device;/* optional (1,) device count overriding `active` */
+    int mating;                  /* MOEA_MATING_* */
+    const float* parents_x;      /* (n, d) */
+    const int32_t* parents_rank; /* (n,) tournament only */
+    const float* parents_cv;     /* (n,) tournament only */
+    const float* bounds;         /* (2d,) */
+    float sbx_eta, sbx_prob, sbx_copy_prob;   /* annealed eta for this generation */
+    int mutation;                /* MOEA_MUTATION_* */
+    float pm_eta, pm_prob;
+    const moea_draws* draws;     /* NULL: in-kernel */
+    uint64_t seed;
+    int generation;
+    float* offspring_x;          /* (n, d) output */
+    int32_t* parent_indices;     /* (n,) output: mating winners */
+    void* stream;
+} moea_reproduce_args;
+
+/* Mating -> SBX -> PM.  Replaces ReproductionPipeline::generate
+ * (reproduction/reproduction_pipeline.cu:37-56) with TournamentMating /
+ * RandomMating (mating_selection.cu:11-108), SimulatedBinaryCrossover
+ * (sbx_crossover.cu:20-190) and PolynomialMutation (polynomial_mutation.cu:20-124). */
+int moea_reproduce(const moea_reproduce_args* args);
+
+/* Linear eta annealing (sbx_crossover.cu:12-18), host helper. */
+float moea_annealed(float begin, float end, int generation, int total);
+
+/* Uniform initial population lb + u * (ub - lb) from the in-kernel generator
+ * (Population::initialize_random, core/population.cu:97-115). */
+int moea_init_population(float* x, int n, int d, const float* bounds, uint64_t seed, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Built-in objectives (IProblemEvaluator::evaluate,                          */
+/* problem/problem_evaluator.cuh:10-31)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* f_c(x) = sum_j (x_j - centers[c, j])^2, c < m; cv = 0.  f is (m, ldf). */
+int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f,
+                        int ldf, float* cv, void* stream);
+
+/* Seeded MLP objective: h = tanh(W1 x) (hidden units), y = W2 h (m outputs),
+ * f_c = y_c^2 + 0.01 * ||x - 0.5||^2; cv = 0.  w1: (hidden, d), w2: (m, hidden).
+ * hwork: (n, hidden) scratch. */
+int moea_eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2,
+                  int m, float* f, int ldf, float* cv, float* hwork, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* RVEA                                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct moea_rvea_select_args {
+    int n, m, d, k;
+    int active;                 /* live parent slots [0, active) */
+    const int32_t* active_device;/* optional (1,) device count overriding `active` */
+    moea_population parents, offspring, next;
+    const float* directions;    /* (m, k) current (adapted) directions */
+    const float* gamma;         /* (k,) */
+    float penalty;              /* m * (t / G)^alpha, host-computed */
+    int32_t* out_active;        /* (1,) device: new active count */
+    void* workspace;
+    size_t workspace_bytes;
+    void* stream;
+} moea_rvea_select_args;
+
+size_t moea_rvea_workspace_bytes(int n, int m, int k);
+
+/* RVEA environmental selection.  Replaces RVEAEnvironmentSelector::select
+ * (selection/rvea/rvea_environment_selector.cu:163-287): z-min translation,
+ * cosine partition, APD, one elitist winner per direction, truncation, and
+ * the gather with NaN / sentinel dead rows. */
+int moea_rvea_select(const moea_rvea_select_args* args);
+
+/* APD penalty scale m * (t / G)^alpha (angle_penalized_distance.cu:159-184). */
+float moea_rvea_penalty(int m, int t, int max_generations, float alpha);
+
+/* gamma_j = acos(max_{k != j} v_j . v_k) (angle_penalized_distance.cu:387-417). */
+int moea_rvea_gamma(const float* directions, int m, int k, float* gamma, void* stream);
+
+/* V = normalize(V0 * (zmax - zmin)) over the first n_active columns of f
+ * (rvea_reference_adaptation.cu:279-317).  n_active is read on the device;
+ * scratch holds at least m floats. */
+int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_active, int m,
+                    int k, float* v_out, float* scratch, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Native run loop                                                           */
+/* ------------------------------------------------------------------------ */
+
+#define MOEA_ALGORITHM_NSGA3 0
+#define MOEA_ALGORITHM_RVEA 1
+
+#define MOEA_PROBLEM_QUADRATIC 0
+#define MOEA_PROBLEM_MLP 1
+#define MOEA_PROBLEM_CALLBACK 2
+
+/* Problem evaluator callback: the C form of IProblemEvaluator::evaluate
+ * (problem/problem_evaluator.cuh:10-31).  Must write f (m, n) and cv (n) for
+ * the n rows of x (n, d), enqueued on `stream`; generation is -1 for the
+ * initial population (algorithm.cu:180-184).  Returns MOEA_OK or an error. */
+typedef int (*moea_evaluate_fn)(void* user, const float* x, int n, int d, int m, float* f,
+                                float* cv, int generation, int max_generations, void* stream);
+
+typedef struct moea_problem_desc {
+    int kind;                 /* MOEA_PROBLEM_* */
+    const float* centers;     /* quadratic: (m, d) */
+    const float* w1;          /* mlp: (hidden, d) */
+    const float* w2;          /* mlp: (m, hidden) */
+    int hidden;
+    float* hwork;             /* mlp: (n, hidden) scratch */
+    moea_evaluate_fn callback;
+    void* user;
+} moea_problem_desc;
+
+/* State of one run.  The caller owns every buffer; moea_run_* update the
+ * integer fields (current, fronts_current, generation). */
+typedef struct moea_run_args {
+    int algorithm;             /* MOEA_ALGORITHM_* */
+    int n, m, d, k, max_generations;
+    moea_population pops[3];   /* rotating parents / offspring / next */
+    int current;               /* index of the parents in pops */
+    int32_t* fronts[2];        /* NSGA-III parent fronts, ping-pong */
+    int fronts_current;
+    int32_t* parent_indices;   /* (n) mating winners */
+    const float* bounds;       /* (2d) */
+    const float* directions;   /* (m, k); RVEA adapts it in place */
+    const float* directions0;  /* RVEA initial directions (m, k) */
+    float* gamma;              /* RVEA (k) */
+    float* ideal;              /* NSGA-III (m) */
+    int32_t* active_device;    /* RVEA (1) */
+    float* adapt_scratch;      /* RVEA (m) */
+    int32_t* stats;            /* NSGA-III optional (8) per-generation selection stats */
+    int32_t* stats_init;       /* NSGA-III (8) */
+    int mating, mutation;
+    float sbx_eta[2], sbx_prob, sbx_copy_prob;
+    float pm_eta[2], pm_prob;
+    float rvea_alpha, rvea_frequency;
+    uint64_t seed;
+    int initial_x_given;       /* 1: pops[current].x already holds the initial population */
+    moea_problem_desc problem;
+    int generation;
+    void* workspace;
+    size_t workspace_bytes;
+    void* stream;
+} moea_run_args;
+
+size_t moea_run_workspace_bytes(int algorithm, int n, int m, int k);
+
+/* EvolutionaryAlgorithm::initialize (algorithm.cu:104-217): initial population
+ * (unless given), evaluation at generation -1, NSGA-III parent fronts and ideal
+ * point or RVEA gamma and active count. */
+int moea_run_init(moea_run_args* args);
+
+/* Up to `generations` calls of EvolutionaryAlgorithm::step (algorithm.cu:219-286),
+ * stopping at max_generations; no host synchronization. */
+int moea_run_step(moea_run_args* args, int generations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOEA_HIP_H */

@@ -1,0 +1,325 @@
+"""ctypes binding of the CPU oracle (``liboracle.so``, built from moea_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+(``cuda-moea_amd``) never imports it.
+
+All arrays are numpy arrays in the reference's layouts: objectives (M, n),
+variables (n, D), constraints (n,), bounds interleaved (2D,), directions (M, K).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "liboracle.so"
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+c_int, c_float, c_double, c_u32, c_u64 = (ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                                         ctypes.c_uint32, ctypes.c_uint64)
+vp = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIB_PATH))
+        sig = {
+            "orc_philox4x32_10": (None, [u32p, u32p, u32p]),
+            "orc_splitmix32": (c_u32, [c_u64, c_u64]),
+            "orc_splitmix64_mix": (c_u64, [c_u64]),
+            "orc_u01": (c_float, [c_u32]),
+            "orc_log2_d": (c_double, [c_double]),
+            "orc_exp2_d": (c_double, [c_double]),
+            "orc_powf": (c_float, [c_float, c_float]),
+            "orc_acosf": (c_float, [c_float]),
+            "orc_das_dennis": (c_int, [c_int, c_int, c_int, c_int, vp, vp]),
+            "orc_normalize_directions": (c_int, [f32p, c_int, c_int]),
+            "orc_dominates": (c_int, [f32p, c_int, c_int, f32p, c_int, c_int]),
+            "orc_nds": (c_int, [f32p, f32p, c_int, c_int, c_int, i32p]),
+            "orc_ideal": (None, [f32p, c_int, c_int, c_int, c_int, f32p]),
+            "orc_hyperplane_fp64": (c_int, [f64p, c_int, f64p, vp]),
+            "orc_normalize": (c_int, [f32p, c_int, c_int, f32p, f32p, i32p, f64p]),
+            "orc_associate": (None, [f32p, c_int, c_int, f32p, c_int, f32p, i32p]),
+            "orc_niching": (None, [i32p, f32p, c_int, c_int, c_int, c_int, u32p, u32p, i32p, vp]),
+            "orc_nsga3_select": (c_int, [f32p, f32p, c_int, c_int, f32p, c_int, f32p, u32p,
+                                         u32p, i32p, i32p, i32p, vp]),
+            "orc_annealed": (c_float, [c_float, c_float, c_int, c_int]),
+            "orc_tournament": (None, [i32p, i32p, c_int, i32p, f32p, i32p]),
+            "orc_sbx": (None, [f32p, i32p, f32p, f32p, f32p, i32p, f32p, f32p, c_float,
+                               c_float, c_float, c_int, c_int]),
+            "orc_pm": (None, [f32p, f32p, f32p, f32p, c_float, c_float, c_int, c_int]),
+            "orc_eval_quadratic": (None, [f32p, c_int, c_int, f32p, c_int, f32p, c_int, vp]),
+            "orc_draw_init": (None, [c_u64, c_int, c_int, f32p, f32p]),
+            "orc_draw_mating": (None, [c_u64, c_u32, c_int, c_int, i32p, vp]),
+            "orc_draw_sbx": (None, [c_u64, c_u32, c_int, c_int, f32p, f32p, i32p, f32p]),
+            "orc_draw_pm": (None, [c_u64, c_u32, c_int, c_int, f32p, f32p]),
+            "orc_draw_niche": (None, [c_u64, c_u32, c_int, u32p, u32p]),
+            "orc_nsga3_run": (c_double, [c_int, c_int, c_int, c_int, c_int, c_int, c_u64, f32p,
+                                         f32p, c_float, c_float, c_float, c_float, c_float,
+                                         c_float, c_float, vp, f32p, f32p, f32p, i32p, c_int]),
+            "orc_rvea_gamma": (None, [f32p, c_int, c_int, f32p]),
+            "orc_rvea_penalty": (c_float, [c_int, c_int, c_int, c_float]),
+            "orc_rvea_select": (c_int, [f32p, f32p, c_int, c_int, f32p, f32p, c_int, c_int,
+                                        c_float, i32p, vp, vp, vp]),
+            "orc_rvea_adapt": (None, [f32p, f32p, c_int, c_int, c_int, c_int, f32p]),
+            "orc_num_threads": (c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- draws ----
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_c(ctr, np.uint32), _c(key, np.uint32), out)
+    return out
+
+
+def splitmix32(counter: int, seed: int) -> int:
+    return int(lib().orc_splitmix32(counter, seed))
+
+
+def splitmix64_mix(z: int) -> int:
+    return int(lib().orc_splitmix64_mix(z))
+
+
+def u01(u: int) -> float:
+    return float(lib().orc_u01(u))
+
+
+def draw_init(seed, n, d, bounds):
+    x = np.zeros((n, d), np.float32)
+    lib().orc_draw_init(seed, n, d, _c(bounds, np.float32), x)
+    return x
+
+
+def draw_mating(seed, gen, count, active, pair=True):
+    a = np.zeros(count, np.int32)
+    b = np.zeros(count, np.int32) if pair else None
+    lib().orc_draw_mating(seed, gen, count, active, a, _ptr(b))
+    return (a, b) if pair else a
+
+
+def draw_sbx(seed, gen, pairs, d):
+    ub = np.zeros(pairs * d, np.float32); up = np.zeros(pairs * d, np.float32)
+    sg = np.zeros(pairs * d, np.int32); uc = np.zeros(pairs * d, np.float32)
+    lib().orc_draw_sbx(seed, gen, pairs, d, ub, up, sg, uc)
+    return ub, up, sg, uc
+
+
+def draw_pm(seed, gen, n, d):
+    ug = np.zeros(n * d, np.float32); ud = np.zeros(n * d, np.float32)
+    lib().orc_draw_pm(seed, gen, n, d, ug, ud)
+    return ug, ud
+
+
+def draw_niche(seed, gen, count):
+    ua = np.zeros(max(count, 1), np.uint32); ub = np.zeros(max(count, 1), np.uint32)
+    lib().orc_draw_niche(seed, gen, count, ua, ub)
+    return ua[:count], ub[:count]
+
+
+# ------------------------------------------------------------------ math ----
+def powf(x, y):
+    return float(lib().orc_powf(x, y))
+
+
+def acosf(x):
+    return float(lib().orc_acosf(x))
+
+
+# ------------------------------------------------------------ directions ----
+def das_dennis(n_req, m, partitions=0, kind=None, raw=False):
+    """kind: 0 NSGA-III by count, 1 by partitions, 2 RVEA by count."""
+    if kind is None:
+        kind = 1 if partitions > 0 else 0
+    k = lib().orc_das_dennis(n_req, m, partitions, kind, None, None)
+    if k <= 0:
+        raise ValueError("invalid Das-Dennis request")
+    out = np.zeros((m, k), np.float32)
+    rw = np.zeros((m, k), np.float32)
+    lib().orc_das_dennis(n_req, m, partitions, kind, _ptr(out), _ptr(rw))
+    return (out, rw) if raw else out
+
+
+# ------------------------------------------------------------- selection ----
+def nds(f, cv, stop_at=None):
+    f = _c(f, np.float32); cv = _c(cv, np.float32)
+    m, n = f.shape
+    fr = np.zeros(n, np.int32)
+    mf = lib().orc_nds(f, cv, n, m, n if stop_at is None else stop_at, fr)
+    return fr, int(mf)
+
+
+def dominates(f, cv, q, p):
+    f = _c(f, np.float32); cv = _c(cv, np.float32)
+    m, n = f.shape
+    return bool(lib().orc_dominates(f, n, m, cv, q, p))
+
+
+def hyperplane_fp64(E):
+    E = _c(E, np.float64)
+    m = E.shape[0]
+    a = np.zeros(m, np.float64)
+    ill = lib().orc_hyperplane_fp64(E, m, a, None)
+    return int(ill), a
+
+
+def normalize(f, ideal):
+    fn = np.array(f, dtype=np.float32, order="C", copy=True)
+    m, n = fn.shape
+    a = np.zeros(m, np.float32); ext = np.zeros(m, np.int32); a64 = np.zeros(m, np.float64)
+    ill = lib().orc_normalize(fn, n, m, _c(ideal, np.float32), a, ext, a64)
+    return fn, a, ext, a64, int(ill)
+
+
+def associate(fn, w):
+    fn = _c(fn, np.float32); w = _c(w, np.float32)
+    m, n = fn.shape
+    k = w.shape[1]
+    dist = np.zeros(n, np.float32); ref = np.zeros(n, np.int32)
+    lib().orc_associate(fn, n, m, w, k, dist, ref)
+    return dist, ref
+
+
+def niching(ref, dist, n_prior, n_last, k, n_rem, ua, ub):
+    out = np.zeros(max(n_rem, 1), np.int32)
+    lib().orc_niching(_c(ref, np.int32), _c(dist, np.float32), n_prior, n_last, k, n_rem,
+                      _c(ua, np.uint32), _c(ub, np.uint32), out, None)
+    return out[:n_rem]
+
+
+def nsga3_select(f_mix, cv_mix, w, ideal_last, ua, ub):
+    """Returns (chosen sorted (n,), fronts_mix (2n,), stats, ideal_current)."""
+    f_mix = _c(f_mix, np.float32); cv_mix = _c(cv_mix, np.float32); w = _c(w, np.float32)
+    m, nm = f_mix.shape
+    n = nm // 2
+    ideal = np.array(ideal_last, dtype=np.float32, copy=True)
+    chosen = np.zeros(n, np.int32); fr = np.zeros(nm, np.int32); st = np.zeros(4, np.int32)
+    ua = _c(ua, np.uint32); ub = _c(ub, np.uint32)
+    if ua.size == 0:
+        ua = np.zeros(1, np.uint32); ub = np.zeros(1, np.uint32)
+    lib().orc_nsga3_select(f_mix, cv_mix, n, m, w, w.shape[1], ideal, ua, ub, chosen, fr, st,
+                           None)
+    return chosen, fr, st, ideal
+
+
+# ---------------------------------------------------------- reproduction ----
+def annealed(b, e, g, t):
+    return float(lib().orc_annealed(b, e, g, t))
+
+
+def tournament(ca, cb, rank, cv):
+    w = np.zeros(len(ca), np.int32)
+    lib().orc_tournament(_c(ca, np.int32), _c(cb, np.int32), len(ca), _c(rank, np.int32),
+                         _c(cv, np.float32), w)
+    return w
+
+
+def sbx(parents, idx, bounds, ub, up, sg, uc, eta, prob, copy_prob, n_off):
+    parents = _c(parents, np.float32)
+    d = parents.shape[1]
+    off = np.zeros((n_off, d), np.float32)
+    lib().orc_sbx(parents, _c(idx, np.int32), _c(bounds, np.float32), _c(ub, np.float32),
+                  _c(up, np.float32), _c(sg, np.int32), _c(uc, np.float32), off, eta, prob,
+                  copy_prob, n_off, d)
+    return off
+
+
+def pm(off, bounds, ug, ud, eta, prob):
+    off = np.array(off, dtype=np.float32, order="C", copy=True)
+    n, d = off.shape
+    lib().orc_pm(_c(ug, np.float32), _c(ud, np.float32), off, _c(bounds, np.float32), eta, prob,
+                 n, d)
+    return off
+
+
+def eval_quadratic(x, centers):
+    x = _c(x, np.float32); centers = _c(centers, np.float32)
+    n, d = x.shape
+    m = centers.shape[0]
+    f = np.zeros((m, n), np.float32)
+    lib().orc_eval_quadratic(x, n, d, centers, m, f, n, None)
+    return f
+
+
+def nsga3_run(n, m, d, gens, seed, bounds, centers, k_req=None, partitions=0,
+              sbx=(30.0, 30.0, 1.0, 0.0), pm=(20.0, 20.0, 1.0), init_x=None, limit=0):
+    """Full NSGA-III run with the in-kernel draw generator.  Returns
+    (x, f, cv, fronts, loop_ms)."""
+    x = np.zeros((n, d), np.float32); f = np.zeros((m, n), np.float32)
+    cv = np.zeros(n, np.float32); fr = np.zeros(n, np.int32)
+    ix = None if init_x is None else _c(init_x, np.float32)
+    ms = lib().orc_nsga3_run(n, m, d, n if k_req is None else k_req, partitions, gens, seed,
+                             _c(bounds, np.float32), _c(centers, np.float32), sbx[0], sbx[1],
+                             sbx[2], sbx[3], pm[0], pm[1], pm[2], _ptr(ix), x, f, cv, fr, limit)
+    return x, f, cv, fr, float(ms)
+
+
+# ------------------------------------------------------------------ RVEA ----
+def rvea_gamma(v):
+    v = _c(v, np.float32)
+    m, k = v.shape
+    g = np.zeros(k, np.float32)
+    lib().orc_rvea_gamma(v, m, k, g)
+    return g
+
+
+def rvea_penalty(m, t, max_gen, alpha):
+    return float(lib().orc_rvea_penalty(m, t, max_gen, alpha))
+
+
+def rvea_select(f_mix, cv_mix, v, gamma, active, penalty):
+    f_mix = _c(f_mix, np.float32); v = _c(v, np.float32)
+    m, nm = f_mix.shape
+    n = nm // 2
+    sel = np.zeros(n, np.int32); ref = np.zeros(nm, np.int32)
+    th = np.zeros(nm, np.float32); apd = np.zeros(nm, np.float32)
+    na = lib().orc_rvea_select(f_mix, _c(cv_mix, np.float32), n, m, v, _c(gamma, np.float32),
+                               v.shape[1], active, penalty, sel, _ptr(ref), _ptr(th), _ptr(apd))
+    return sel[:na], int(na), ref, th, apd
+
+
+def rvea_adapt(v0, f, n_active):
+    v0 = _c(v0, np.float32); f = _c(f, np.float32)
+    m, k = v0.shape
+    out = np.zeros((m, k), np.float32)
+    lib().orc_rvea_adapt(v0, f, n_active, f.shape[1], m, k, out)
+    return out
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(n)

@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every function include/moea_hip.h
+declares (no compute calls; CPU only)."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "moea_hip.h"
+LIB = ROOT / "cuda-moea_amd" / "lib" / "libmoea_hip.so"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(moea_[a-z0-9_]+)\s*\(", text)) - {"moea_evaluate_fn"})
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("moea_nds_rank", "moea_nsga3_select", "moea_reproduce", "moea_rvea_select",
+                 "moea_eval_quadratic", "moea_run_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not LIB.exists():
+        subprocess.run(["make", "-s", "-j8", "-C", str(ROOT / "cuda-moea_amd")], check=True)
+    lib = ctypes.CDLL(str(LIB))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    import cuda_moea._lib as L
+    assert set(declared_functions()) <= set(L.SIGNATURES)
+
+
+def test_host_side_calls():
+    import cuda_moea._lib as L
+    assert L.lib.moea_abi_version() == 1
+    assert L.lib.moea_das_dennis(92, 3, 0, 0, None, None) == 91
+    assert L.lib.moea_das_dennis(32768, 3, 0, 0, None, None) == 32640
+    assert abs(L.lib.moea_annealed(30.0, 10.0, 50, 100) - 20.0) < 1e-6
+    assert abs(L.lib.moea_rvea_penalty(5, 50, 100, 2.0) - 1.25) < 1e-6
+    assert L.lib.moea_das_dennis(92, 11, 0, 0, None, None) < 0
+    assert b"objective" in L.lib.moea_last_error()
+
+
+def test_das_dennis_matches_oracle():
+    import numpy as np
+    import oracle as O
+    from cuda_moea import kernels as K
+    for n, m, p, k in [(92, 3, 0, 0), (210, 5, 0, 0), (16384, 5, 0, 2), (100, 8, 0, 0),
+                       (0, 8, 3, 1), (0, 3, 12, 1), (32768, 3, 0, 0)]:
+        np.testing.assert_array_equal(K.das_dennis(n, m, p, kind=k), O.das_dennis(n, m, p, kind=k))
+
+
+def test_api_validation_errors():
+    import pytest
+    import cuda_moea as cm
+    with pytest.raises(ValueError):
+        cm.Algorithm("MOEAD")
+    with pytest.raises(ValueError):
+        cm.SBX(eta_initial=-1.0).validate()
+    with pytest.raises(ValueError):
+        cm.NSGA3(crossover=cm.SBX(probability=2.0))
+    with pytest.raises(ValueError):
+        cm.RVEA(environment_selector=cm.RVEAEnvironmentSelector(alpha=0.0))
+    with pytest.raises(NotImplementedError):
+        cm.NSGA3(mating=cm.PythonMating())
+    b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(
+        cm.QuadraticCenters(12, 3))
+    alg = b.build()
+    assert isinstance(alg, cm.Algorithm) and alg.population_size == 92 and not alg.initialized
