@@ -1,0 +1,142 @@
+"""Headline benchmark: NSGA-III generations per second at N=32768, M=3, D=1024
+(quadratic-center objectives), one MI355X.
+
+A step is one generation (mating -> SBX -> PM -> objective evaluation ->
+NSGA-III selection) of the native run loop, inputs resident in HBM.  W warmup
+generations, then K timed generations between two synchronized HIP events on
+the run stream.  A second pass of K generations with per-stage HIP events
+gives the dominance kernel's average launch time for the roofline.  The CPU
+baseline times one generation of the oracle (C restatement of the reference
+algorithm) on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cuda-moea_amd"))
+
+N, M, D = 32768, 3, 1024
+VALU_PEAK_CMP = 256 * 64 * 2.4e9          # compare lanes / s (256 CUs x 64 lanes x 2.4 GHz)
+HBM_PEAK = 8.0e12
+
+
+def _traffic_from_profiles() -> float | None:
+    """HBM bytes per dominance launch from the committed rocprofv3 PMC summary."""
+    p = ROOT / "profiles" / "nds_dominance_n65536_m3_pmc.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(threads: int) -> dict:
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import torch
+    import oracle as O
+    g = torch.Generator().manual_seed(0)
+    centers = torch.rand(M, D, generator=g).numpy()
+    bounds = np.tile(np.array([0.0, 1.0], np.float32), D)
+    # one generation of the full NSGA-III loop at the benchmark shape
+    _, _, _, _, ms = O.nsga3_run(N, M, D, 1, 0, bounds, centers, limit=1)
+    return {"value": 1000.0 / ms, "unit": "gens/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 generation of the oracle NSGA-III loop at N={N} M={M} D={D} "
+                      f"(quadratic-center, K=32640), {O.num_threads()} OpenMP threads, "
+                      f"{ms / 1000:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build runs each population on a single GPU "
+              "and has no multi-GPU path (see DESIGN.md for how it would shard); use --gpus 1",
+              file=sys.stderr)
+        return 2
+    import torch
+    import cuda_moea as cm
+    from cuda_moea import _lib
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    K, W = args.steps, args.warmup
+    alg = cm.NSGA3(population_size=N, max_generations=W + 2 * K,
+                   problem=cm.QuadraticCenters(D, M, seed=0), seed=0, print_progress=False)
+    alg.initialize()
+    stream = torch.cuda.current_stream(dev)
+    if W:
+        alg._call(_lib.lib.moea_run_step, W)
+    stream.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    alg._call(_lib.lib.moea_run_step, K)
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_time(t1)
+    ms_per_gen = ms / K
+    max_front = int(alg._stats[0].item())
+
+    # pass 2: same work with per-stage events for the kernel roofline
+    _lib.profile_enable(True)
+    alg._call(_lib.lib.moea_run_step, K)
+    stream.synchronize()
+    stages = _lib.profile_read()
+    _lib.profile_enable(False)
+    dom_ms, dom_n = stages["nds_dominance"]
+    dom_avg_s = dom_ms / max(dom_n, 1) / 1e3
+    nm = 2 * N
+    work = M * nm * (nm - 1)                      # element compares per launch
+    achieved = work / dom_avg_s / 1e9             # Gcmp/s
+    peak = VALU_PEAK_CMP / 1e9
+    stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_threads)
+
+    line = {
+        "metric": "gens/sec NSGA-III N=32768 M=3; batched runs/sec + p99 latency vs plain",
+        "value": round(1000.0 / ms_per_gen, 3),
+        "unit": "gens/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_gen, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: quadratic-center objectives, centers and population seeded (seed 0)",
+        "config": {"workload": "NSGA-III single population, quadratic-center objectives",
+                   "N": N, "M": M, "D": D, "K_directions": alg.k, "generations_timed": K,
+                   "last_front_index": max_front},
+        "roofline": {"bound": "valu", "kernel": "nds_dominance",
+                     "achieved": round(achieved, 1), "peak": round(peak, 1), "unit": "Gcmp/s",
+                     "frac": round(achieved / peak, 4), "traffic": _traffic_from_profiles(),
+                     "work_per_launch": work, "avg_launch_ms": round(dom_avg_s * 1e3, 4)},
+        "stage_ms_per_generation": stage_ms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -122,8 +122,12 @@ int run_step(moea_run_args& a, int generations)
         r.stream = a.stream;
         if (r.mating == MOEA_MATING_TOURNAMENT && !r.parents_rank)
             return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
+        stage_begin(kStReproduce, st);
         MOEA_CHECK(reproduce(r));
+        stage_end(kStReproduce, st);
+        stage_begin(kStEvaluate, st);
         MOEA_CHECK(evaluate(a, ofs, st, g));
+        stage_end(kStEvaluate, st);
         if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
             moea_nsga3_select_args s{};
             s.n = a.n; s.m = a.m; s.d = a.d; s.k = a.k;
@@ -156,14 +160,18 @@ int run_step(moea_run_args& a, int generations)
             s.out_active = a.active_device;
             s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
             s.stream = a.stream;
+            stage_begin(kStRveaSelect, st);
             MOEA_CHECK(rvea_select(s));
+            stage_end(kStRveaSelect, st);
             // AdaptiveRVEADirections::update (reference_directions.cu:132-162)
             int interval = static_cast<int>(std::lround(static_cast<double>(a.rvea_frequency) * a.max_generations));
             if (interval < 1) interval = 1;
             if ((g + 1) % interval == 0) {
+                stage_begin(kStRveaAdapt, st);
                 MOEA_CHECK(rvea_adapt(a.directions0, nx.f, a.n, a.active_device, a.m, a.k,
                                       const_cast<float*>(a.directions), a.adapt_scratch, st));
                 MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
+                stage_end(kStRveaAdapt, st);
             }
         }
         a.current = nxt;

@@ -77,3 +77,13 @@ size_t run_workspace_bytes(int algorithm, int n, int m, int k);
 int run_init(moea_run_args& a);
 int run_step(moea_run_args& a, int generations);
 }  // namespace moea
+
+namespace moea {
+// profile.cpp: opt-in stage timing
+enum Stage : int {
+    kStReproduce = 0, kStEvaluate, kStDominance, kStPeel, kStSelectPrep, kStAssociation,
+    kStNiching, kStGather, kStRveaSelect, kStRveaAdapt, kNumStages
+};
+void stage_begin(int stage, hipStream_t stream);
+void stage_end(int stage, hipStream_t stream);
+}  // namespace moea

@@ -321,6 +321,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     hipLaunchKernelGGL(nds_block_flags, dim3(W), dim3(64), 0, stream, f, ldf, cv, n, m, flags);
     MOEA_HIP(hipGetLastError());
     const int tri = Wc * (Wc + 1) / 2;
+    stage_begin(kStDominance, stream);
 #define MOEA_DOM_CASE(MM)                                                                   \
     case MM:                                                                               \
         hipLaunchKernelGGL(nds_dominance<MM>, dim3(tri), dim3(kDomThreads), 0, stream, f, ldf, \
@@ -333,6 +334,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     }
 #undef MOEA_DOM_CASE
     MOEA_HIP(hipGetLastError());
+    stage_end(kStDominance, stream);
 
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, stream, state, 64);
     MOEA_HIP(hipGetLastError());
@@ -340,8 +342,10 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     const int nblk = W;
     int grid = peel_grid_size(nblk);
     void* kargs[] = {&args};
+    stage_begin(kStPeel, stream);
     MOEA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(nds_peel), dim3(grid),
                                         dim3(kPeelThreads), kargs, 0, stream));
+    stage_end(kStPeel, stream);
     return kOk;
 }
 

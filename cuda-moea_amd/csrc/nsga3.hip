@@ -572,6 +572,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm);
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(nds_run(L.fm, nm, L.cvm, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
+    stage_begin(kStSelectPrep, st);
     hipLaunchKernelGGL(split_fronts, dim3(1), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.gidx, L.meta);
     hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, st, L.fm, nm, n, n, m, A.ideal, 0);
     const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
@@ -579,10 +580,14 @@ int nsga3_select(const moea_nsga3_select_args& A)
                   L.asf_val, L.asf_idx, L.row_max);
     hipLaunchKernelGGL(norm_solve, dim3(1), dim3(64), 0, st, L.fn, nm, m, L.asf_val, L.asf_idx,
                        L.row_max, L.meta, L.a);
+    stage_end(kStSelectPrep, st);
+    stage_begin(kStAssociation, st);
     MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
                   L.meta, L.part_d2, L.part_idx);
     hipLaunchKernelGGL(associate_final, dim3(nb), dim3(256), 0, st, L.part_d2, L.part_idx, nchunks,
                        nm, L.meta, L.dist, L.ref);
+    stage_end(kStAssociation, st);
+    stage_begin(kStNiching, st);
     hipLaunchKernelGGL(rho_clear, dim3(ceil_div(k, 256)), dim3(256), 0, st, L.rho, k);
     hipLaunchKernelGGL(rho_hist, dim3(nb), dim3(256), 0, st, L.ref, L.meta, k, L.rho);
     hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, L.k0, L.v0);
@@ -600,9 +605,12 @@ int nsga3_select(const moea_nsga3_select_args& A)
     hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
                        L.flag);
     hipLaunchKernelGGL(compact_flags, dim3(1), dim3(kScanThreads), 0, st, L.flag, nm, n, L.chosen, L.meta);
+    stage_end(kStNiching, st);
+    stage_begin(kStGather, st);
     hipLaunchKernelGGL(gather_next, dim3(ceil_div(n, 4)), dim3(256), 0, st, L.chosen, n, d, m,
                        A.parents.x, A.offspring.x, L.fm, L.cvm, L.fronts, A.next.x, A.next.f,
                        A.next.cv, A.next_fronts);
+    stage_end(kStGather, st);
     if (A.stats) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
     MOEA_HIP(hipGetLastError());
     return kOk;

@@ -1,0 +1,135 @@
+// Opt-in per-stage timing with HIP events on the launch stream (the role of
+// the reference's ASAP_NDSORT_FRONT_PROP_TIMING breakdown,
+// non_dominated_sort.cu:101-107, 1132-1276).  Off by default; when enabled,
+// each instrumented stage records an event pair on its stream and
+// moea_profile_read sums the elapsed times.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "moea_internal.hpp"
+
+namespace moea {
+
+namespace {
+
+const char* kStageNames[kNumStages] = {
+    "reproduce", "evaluate", "nds_dominance", "nds_peel", "select_prep",
+    "association", "niching", "gather", "rvea_select", "rvea_adapt",
+};
+
+struct Timer {
+    std::mutex mu;
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    size_t next = 0;
+    struct Rec { int stage; hipEvent_t a, b; };
+    std::vector<Rec> open;
+    std::vector<Rec> done;
+    double ms[kNumStages] = {};
+    long long count[kNumStages] = {};
+};
+
+Timer& timer()
+{
+    static Timer t;
+    return t;
+}
+
+hipEvent_t take(Timer& t)
+{
+    if (t.next >= t.pool.size()) return nullptr;
+    return t.pool[t.next++];
+}
+
+void drain(Timer& t)
+{
+    for (auto& r : t.done) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            t.ms[r.stage] += ms;
+            t.count[r.stage] += 1;
+        }
+    }
+    t.done.clear();
+    t.next = 0;
+}
+
+}  // namespace
+
+void stage_begin(int stage, hipStream_t stream)
+{
+    Timer& t = timer();
+    if (!t.enabled) return;
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (t.next + 2 > t.pool.size()) drain(t);
+    hipEvent_t a = take(t), b = take(t);
+    if (!a || !b) return;
+    hipEventRecord(a, stream);
+    t.open.push_back({stage, a, b});
+}
+
+void stage_end(int stage, hipStream_t stream)
+{
+    Timer& t = timer();
+    if (!t.enabled) return;
+    std::lock_guard<std::mutex> lk(t.mu);
+    for (size_t i = t.open.size(); i-- > 0;) {
+        if (t.open[i].stage == stage) {
+            hipEventRecord(t.open[i].b, stream);
+            t.done.push_back(t.open[i]);
+            t.open.erase(t.open.begin() + static_cast<long>(i));
+            return;
+        }
+    }
+}
+
+}  // namespace moea
+
+using namespace moea;
+
+extern "C" {
+
+int moea_profile_enable(int enable)
+{
+    Timer& t = timer();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (enable && !t.enabled) {
+        t.pool.resize(8192);
+        for (auto& e : t.pool) MOEA_HIP(hipEventCreate(&e));
+        t.next = 0;
+        for (int i = 0; i < kNumStages; ++i) { t.ms[i] = 0.0; t.count[i] = 0; }
+        t.enabled = true;
+    } else if (!enable && t.enabled) {
+        drain(t);
+        for (auto& e : t.pool) hipEventDestroy(e);
+        t.pool.clear();
+        t.open.clear();
+        t.enabled = false;
+    }
+    return kOk;
+}
+
+int moea_profile_read(double* ms_out, long long* count_out, int max_stages)
+{
+    Timer& t = timer();
+    std::lock_guard<std::mutex> lk(t.mu);
+    drain(t);
+    const int n = max_stages < kNumStages ? max_stages : kNumStages;
+    for (int i = 0; i < n; ++i) {
+        if (ms_out) ms_out[i] = t.ms[i];
+        if (count_out) count_out[i] = t.count[i];
+        t.ms[i] = 0.0;
+        t.count[i] = 0;
+    }
+    return kNumStages;
+}
+
+const char* moea_profile_stage_name(int stage)
+{
+    return (stage >= 0 && stage < kNumStages) ? kStageNames[stage] : "";
+}
+
+}  // extern "C"

@@ -150,3 +150,25 @@ def check(status: int) -> None:
     if status in (-1, -4):
         raise ValueError(msg)
     raise MoeaError(f"libmoea_hip error {status}: {msg}")
+
+
+# optional per-stage timing (profiling builds nothing extra; off by default)
+for _name, _res, _args in (("moea_profile_enable", c_int, [c_int]),
+                           ("moea_profile_read", c_int, [c_void_p, c_void_p, c_int]),
+                           ("moea_profile_stage_name", ctypes.c_char_p, [c_int])):
+    _fn = getattr(lib, _name)
+    _fn.restype, _fn.argtypes = _res, _args
+    SIGNATURES[_name] = (_res, _args)
+
+
+def profile_enable(on: bool) -> None:
+    check(lib.moea_profile_enable(1 if on else 0))
+
+
+def profile_read() -> dict:
+    """Per-stage {name: (total_ms, launches)} since the last read (synchronizes)."""
+    import numpy as _np
+    ms = _np.zeros(32, _np.float64)
+    cnt = _np.zeros(32, _np.int64)
+    n = lib.moea_profile_read(ms.ctypes.data_as(c_void_p), cnt.ctypes.data_as(c_void_p), 32)
+    return {lib.moea_profile_stage_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}

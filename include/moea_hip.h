@@ -318,6 +318,21 @@ int moea_run_init(moea_run_args* args);
  * stopping at max_generations; no host synchronization. */
 int moea_run_step(moea_run_args* args, int generations);
 
+/* ------------------------------------------------------------------------ */
+/* Profiling                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Opt-in per-stage timing with HIP events on the launch stream, the role of
+ * the reference's ASAP_NDSORT_FRONT_PROP_TIMING breakdown
+ * (non_dominated_sort.cu:101-107).  Off by default. */
+int moea_profile_enable(int enable);
+
+/* Per-stage total milliseconds and launch counts since the last read; returns
+ * the number of stages.  Synchronizes on the recorded events. */
+int moea_profile_read(double* ms_out, long long* count_out, int max_stages);
+
+const char* moea_profile_stage_name(int stage);
+
 #ifdef __cplusplus
 }
 #endif

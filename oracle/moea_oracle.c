@@ -825,6 +825,7 @@ ORC_API void orc_sbx(const float* parents, const int32_t* idx, const float* boun
                      int n_off, int d)
 {
     const int pairs = n_off / 2;
+#pragma omp parallel for schedule(static)
     for (int pr = 0; pr < pairs; ++pr) {
         const float* x1 = parents + (size_t)idx[2 * pr] * d;
         const float* x2 = parents + (size_t)idx[2 * pr + 1] * d;
@@ -846,6 +847,7 @@ ORC_API void orc_pm(const float* u_gate, const float* u_delta, float* off, const
                     float eta, float prob, int n, int d)
 {
     const float rate = prob / (float)d;
+#pragma omp parallel for schedule(static)
     for (int r = 0; r < n; ++r)
         for (int j = 0; j < d; ++j) {
             const size_t e = (size_t)r * d + j;
@@ -890,6 +892,7 @@ ORC_API void orc_eval_quadratic(const float* x, int n, int d, const float* cente
 
 ORC_API void orc_draw_init(uint64_t seed, int n, int d, const float* bounds, float* x)
 {
+#pragma omp parallel for schedule(static)
     for (int r = 0; r < n; ++r)
         for (int j = 0; j < d; ++j) {
             uint32_t o[4];
@@ -915,6 +918,7 @@ ORC_API void orc_draw_mating(uint64_t seed, uint32_t gen, int count, int active,
 ORC_API void orc_draw_sbx(uint64_t seed, uint32_t gen, int pairs, int d, float* ub, float* up,
                           int32_t* sign, float* uc)
 {
+#pragma omp parallel for schedule(static)
     for (int pr = 0; pr < pairs; ++pr)
         for (int j = 0; j < d; ++j) {
             uint32_t o[4];
@@ -929,6 +933,7 @@ ORC_API void orc_draw_sbx(uint64_t seed, uint32_t gen, int pairs, int d, float* 
 
 ORC_API void orc_draw_pm(uint64_t seed, uint32_t gen, int n, int d, float* ug, float* ud)
 {
+#pragma omp parallel for schedule(static)
     for (int r = 0; r < n; ++r)
         for (int j = 0; j < d; ++j) {
             uint32_t o[4];
