@@ -302,6 +302,10 @@ __device__ inline uint32_t ld_acquire_u32(const uint32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline uint32_t ld_relaxed_u32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline int32_t ld_relaxed_i32(const int32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -315,11 +319,20 @@ __device__ inline void st_relaxed_u64(uint64_t* p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier state: [0] arrival count, [1] generation, [2] error flag.
-// Every block of the cooperative grid must call it the same number of times.
-// Thread 0 of each block publishes the block's writes with an agent-scope
-// release, arrives, waits for the generation to advance and acquires; the
-// workgroup barriers around it order the other threads of the block.
+// Grid barrier for cooperative launches, two-level to spread the arrival
+// atomics: block b arrives on the counter of group b % 8 (blocks are dealt to
+// the 8 XCDs round-robin, so a group is one XCD's blocks); the last arrival of
+// a group arrives on the top counter; the last top arrival advances the
+// generation word.  Waiting blocks poll the generation with relaxed loads and
+// take one acquire fence when it moves (an acquire per poll would invalidate
+// the XCD's L2 on every poll).  Every arrival is an acq_rel RMW, which chains
+// each block's writes before the barrier to every block's reads after it.
+// Words of `state` (each counter on its own 64-byte line), all zero at launch:
+constexpr int kBarGroups = 8;
+constexpr int kBarTop = kBarGroups * 16;      // top-level arrival count
+constexpr int kBarGen = kBarTop + 16;         // generation
+constexpr int kBarError = kBarGen + 16;       // set when a spin gives up
+constexpr int kBarWords = kBarError + 16;     // first word free for the caller
 // The spin is bounded: after kSpinLimit polls the block records an error and
 // returns false, so a broken launch ends instead of hanging the device.
 constexpr uint32_t kSpinLimit = 1u << 24;
@@ -331,23 +344,35 @@ __device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t&
     if (threadIdx.x == 0) {
         int ok = 1;
         const uint32_t target = phase + 1;
+        const uint32_t g = blockIdx.x % kBarGroups;
+        const uint32_t group_size = (nblocks - g + kBarGroups - 1) / kBarGroups;
+        const uint32_t groups = nblocks < kBarGroups ? nblocks : kBarGroups;
+        bool released = false;
         const uint32_t arrived =
-            __hip_atomic_fetch_add(&state[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&state[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&state[1], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+            __hip_atomic_fetch_add(&state[g * 16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == group_size - 1) {
+            __hip_atomic_store(&state[g * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t top =
+                __hip_atomic_fetch_add(&state[kBarTop], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == groups - 1) {
+                __hip_atomic_store(&state[kBarTop], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&state[kBarGen], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+        if (!released) {
             uint32_t spins = 0;
-            while (ld_acquire_u32(&state[1]) != target) {
+            while (__hip_atomic_load(&state[kBarGen], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
-                    __hip_atomic_store(&state[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&state[kBarError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        if (ld_acquire_u32(&state[2]) != 0) ok = 0;
+        if (__hip_atomic_load(&state[kBarError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
         s_ok = ok;
         phase = target;
     }
@@ -380,5 +405,33 @@ __device__ inline void static_for(F&& f, std::integer_sequence<int, Js...>)
 }
 
 __device__ inline int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Marks a value the whole wave agrees on as uniform (kept in an SGPR), so
+// loads indexed by it become scalar loads.
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Lane mask of !(s > v) for a wave-uniform s: one v_cmp_ngt_f32 writing an
+// SGPR pair (true for unordered operands).
+__device__ inline uint64_t mask_not_gt(float s, float v)
+{
+    const float su = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s)));
+    uint64_t m;
+    asm("v_cmp_ngt_f32_e64 %0, %1, %2" : "=s"(m) : "s"(su), "v"(v));
+    return m;
+}
+
+// Per lane: acc * 2 + (bit `lane` of mask).  One v_addc_co_u32 with the
+// wave-uniform lane mask as carry-in, i.e. a bit-serial shift of a ballot
+// into a per-lane word.
+__device__ inline uint32_t shift_in(uint32_t acc, uint64_t mask)
+{
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(mask)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(mask >> 32)));
+    const uint64_t m = (static_cast<uint64_t>(hi) << 32) | lo;
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(acc), "s"(m));
+    return r;
+}
 
 }  // namespace moea

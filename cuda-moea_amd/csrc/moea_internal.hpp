@@ -15,7 +15,6 @@ size_t nds_workspace_bytes(int n);
 int nds_max_n();
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
             int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream);
-int peel_grid_size(int nblk);
 
 // nsga3.hip
 size_t nsga3_workspace_bytes(int n, int m, int k);
@@ -40,9 +39,10 @@ int rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_active
                float* v_out, float* scratch, hipStream_t stream);
 
 // sort.hip: stable LSD radix sort of (u64 key, i32 value) pairs, n fixed on the host
-size_t sort_pairs_temp_bytes(int n);
+size_t sort_pairs_temp_bytes(int n, int end_bit = 64);
 int sort_pairs(void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
-               const int32_t* vals_in, int32_t* vals_out, int n, hipStream_t stream);
+               const int32_t* vals_in, int32_t* vals_out, int n, hipStream_t stream,
+               int end_bit = 64);
 
 // host helpers
 float annealed(float begin, float end, int generation, int total);
@@ -82,7 +82,7 @@ namespace moea {
 // profile.cpp: opt-in stage timing
 enum Stage : int {
     kStReproduce = 0, kStEvaluate, kStDominance, kStPeel, kStSelectPrep, kStAssociation,
-    kStNiching, kStGather, kStRveaSelect, kStRveaAdapt, kNumStages
+    kStNiching, kStGather, kStRveaSelect, kStRveaAdapt, kStNdsSort, kNumStages
 };
 void stage_begin(int stage, hipStream_t stream);
 void stage_end(int stage, hipStream_t stream);

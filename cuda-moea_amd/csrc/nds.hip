@@ -7,12 +7,34 @@
 // assigned; the rest keep front -1; :824-840, 1182-1204), but with no pair
 // buffer, no host round trip per front and no capacity limit.
 //
-// Data layout of the dominator bitmask B (u64 words):
+// Pipeline (all on one stream, no host synchronisation):
+//  1. nds_keys + radix sort: rows are ordered by the key
+//       segment 0  feasible, no NaN objective    -> f0 (with -0 read as +0)
+//       segment 1  feasible, some NaN objective  -> (input order)
+//       segment 2  infeasible                    -> cv (NaN last)
+//     In this order a later row can dominate an earlier row only when both
+//     are in segment 0 with equal f0: a feasible row is never dominated by an
+//     infeasible one, a NaN objective compares false both ways, infeasible
+//     rows dominate by strictly smaller cv, and q <= p in every objective with
+//     f0(q) > f0(p) is impossible.
+//  2. nds_gather: objectives and cv in sorted order; per 64-row block b,
+//     kind[b] (all rows in segment 0?) and hi[b], the last block holding a row
+//     whose key equals the key of block b's last row (b itself unless an f0
+//     tie runs past the block).  Dominators of a row of block b all lie in
+//     blocks <= hi[b].
+//  3. nds_dominance: the dominator bitmask B in sorted order, words (p, w) for
+//     w <= hi[p/64].  Tiles below the diagonal whose blocks are all segment 0
+//     and not tied need only "q_k <= p_k for k >= 1" (f0 is already strictly
+//     smaller for every q of the earlier block): M - 1 compares per pair.
+//  4. nds_peel: front peeling with per-row cursors on B in one cooperative
+//     launch; fronts are written back in input order.
+//
+// Data layout of B (u64 words, sorted row index p):
 //   W  = ceil(n / 64) words per row (bit q%64 of word q/64 = "q dominates p"),
 //   Wc = ceil(W / 8) chunks of 8 words per row,
 //   word (p, w) lives at (((p/64) * Wc + w/8) * 64 + p%64) * 8 + w%8.
-// A 64-row x 8-word chunk is 4 KiB contiguous, so the tile kernel writes whole
-// 64-byte row segments and the peel kernel reads them back coalesced.
+// A 64-row x 8-word chunk is 4 KiB contiguous: the dominance kernel writes it
+// as one 64-byte segment per lane, the peel reads a row's words 8 at a time.
 #include "common.hpp"
 #include "moea_internal.hpp"
 
@@ -20,191 +42,314 @@ namespace moea {
 
 namespace {
 
-constexpr int kTB = 8;                   // tiles per block edge (8 x 8 tiles of 64 x 64)
-constexpr int kDomThreads = 256;         // 4 waves
+constexpr int kDomWaves = 4;             // waves (work units) per dominance workgroup
+constexpr int kDomThreads = 64 * kDomWaves;
+constexpr int kKeyBits = 34;             // segment (2 bits) above a 32-bit order key
+constexpr uint8_t kPlain = 0;            // every row of the block is in segment 0
+constexpr uint8_t kMixed = 1;
 
 __host__ __device__ inline size_t bword_index(int p, int w, int Wc)
 {
     return ((static_cast<size_t>(p >> 6) * Wc + (w >> 3)) * 64 + (p & 63)) * 8 + (w & 7);
 }
 
-// Per 64-row block: 1 when every row is feasible (cv == 0) and has only finite
-// objectives, so the compare-only fast path applies.
-__global__ void nds_block_flags(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
-                                int n, int m, uint8_t* __restrict__ simple)
+// Monotone map float -> u32 (numeric order for non-NaN values; -0 maps like +0).
+__device__ inline uint32_t order_key(float x)
 {
-    const int blk = blockIdx.x;
-    const int p = blk * 64 + threadIdx.x;
-    bool ok = true;
-    if (p < n) {
-        ok = (cv[p] == 0.0f);
-        for (int k = 0; k < m; ++k) ok = ok && isfinite(f[static_cast<size_t>(k) * ldf + p]);
-    }
-    const uint64_t bad = ballot(!ok);
-    if (threadIdx.x == 0) simple[blk] = (bad == 0) ? 1 : 0;
+    const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// One wave processes one 64 x 64 tile (rows p in block pb, lanes q in block qb)
-// and returns, through the out-parameters:
-//   row_word (lane i): bits q of "q dominates p = pb*64+i"      -> word (p, qb)
-//   col_word (lane q): bits j of "p = pb*64+j dominates q"        -> word (q, pb)
-// The reversed-direction word is built bit-serially with acc = 2*acc + bit
-// (one v_addc per row) and bit-reversed at the end.
-template <int M, bool SIMPLE>
-__device__ inline void dom_tile(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
-                                int n, int pb, int qb, bool need_col, uint64_t& row_word,
-                                uint64_t& col_word)
+__global__ void nds_keys(const float* __restrict__ f, int ldf, const float* __restrict__ cv, int n,
+                         int m, uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float c = cv[s];
+    uint64_t key;
+    if (c == 0.0f) {
+        bool nan = false;
+        for (int k = 0; k < m; ++k) nan = nan || isnan(f[static_cast<size_t>(k) * ldf + s]);
+        key = nan ? (1ull << 32) : order_key(f[s]);
+    } else {
+        key = (2ull << 32) | (isnan(c) ? 0xFFFFFFFFu : order_key(c));
+    }
+    keys[s] = key;
+    vals[s] = s;
+}
+
+// Sorted copies of the objectives (m rows of npad) and cv; padding rows are 0.
+__global__ void nds_gather(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
+                           int n, int m, int npad, const int32_t* __restrict__ perm,
+                           float* __restrict__ fs, float* __restrict__ cvs)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npad) return;
+    if (t < n) {
+        const int s = perm[t];
+        for (int k = 0; k < m; ++k) fs[static_cast<size_t>(k) * npad + t] = f[static_cast<size_t>(k) * ldf + s];
+        cvs[t] = cv[s];
+    } else {
+        for (int k = 0; k < m; ++k) fs[static_cast<size_t>(k) * npad + t] = 0.0f;
+        cvs[t] = 0.0f;
+    }
+}
+
+// One wave per 64-row block: kind[b] and hi[b] (see the header comment).
+__global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
+                               int32_t* __restrict__ hi)
+{
+    const int b = blockIdx.x;
+    const int t = b * 64 + static_cast<int>(threadIdx.x);
+    const bool other = (t < n) && (keys[t] >> 32) != 0;
+    const uint64_t any_other = ballot(other);
+    if (threadIdx.x != 0) return;
+    kind[b] = any_other ? kMixed : kPlain;
+    const int last = min(n, b * 64 + 64) - 1;
+    const uint64_t kl = keys[last];
+    int end = last;
+    if ((kl >> 32) == 0) {
+        // first position after `last` whose key differs (keys ascend)
+        int lo = last + 1, hi_ = n;
+        while (lo < hi_) {
+            const int mid = (lo + hi_) >> 1;
+            if (keys[mid] == kl) lo = mid + 1;
+            else hi_ = mid;
+        }
+        end = lo - 1;
+    }
+    hi[b] = end >> 6;
+}
+
+// Tile in which every earlier row q (block qb, wave-uniform) has f0 strictly
+// below every row p (lane): q dominates p iff q_k <= p_k for k = 1..M-1.
+// Returns the row word of lane p over the 64 rows q of the block.
+template <int M>
+__device__ inline uint64_t tile_fast(const float* __restrict__ fs, int npad, int q0,
+                                     const float (&pv)[M])
+{
+    constexpr int G = (M <= 5) ? 16 : 8;   // rows q per group of scalar loads
+    uint32_t acc_hi = 0, acc_lo = 0;
+    static_for([&](auto GC) {
+        constexpr int g0 = 64 - G * (decltype(GC)::value + 1);   // groups in descending order
+        float qv[M > 1 ? M - 1 : 1][G];
+#pragma unroll
+        for (int k = 1; k < M; ++k)
+#pragma unroll
+            for (int i = 0; i < G; ++i) qv[k - 1][i] = fs[static_cast<size_t>(k) * npad + q0 + g0 + i];
+        static_for([&](auto IC) {
+            constexpr int i = G - 1 - decltype(IC)::value;
+            constexpr int j = g0 + i;                  // descending: bit j lands at position j
+            uint64_t dom = ~0ull;
+#pragma unroll
+            for (int k = 1; k < M; ++k) dom &= mask_not_gt(qv[k - 1][i], pv[k]);
+            if (j >= 32) acc_hi = shift_in(acc_hi, dom);
+            else acc_lo = shift_in(acc_lo, dom);
+        }, std::make_integer_sequence<int, G>{});
+    }, std::make_integer_sequence<int, 64 / G>{});
+    return (static_cast<uint64_t>(acc_hi) << 32) | acc_lo;
+}
+
+// Any tile with the full constrained rule.  Row word of lane p over the rows q
+// of block qb (q < n, q != p); when `backward` is set, the words "p dominates
+// q" of the rows q are stored at word pb of their rows (tied tiles only).
+template <int M>
+__device__ inline uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
+                                        int npad, int n, int qb, int pb, const float (&pv)[M],
+                                        float cp, bool backward, uint64_t* __restrict__ B, int Wc)
 {
     const int lane = lane_id();
-    const int q = qb * 64 + lane;
-    const bool qvalid = q < n;
-    float qv[M];
-#pragma unroll
-    for (int k = 0; k < M; ++k) qv[k] = qvalid ? f[static_cast<size_t>(k) * ldf + q] : 0.0f;
-    const float cq = qvalid ? cv[q] : 0.0f;
-    const bool qf = (cq == 0.0f);
-    const uint64_t valid = ballot(qvalid);
-
-    uint32_t rw_lo = 0, rw_hi = 0;   // row words assembled with writelane
-    uint32_t acc_lo = 0, acc_hi = 0; // bit-serial column word
-    const int p0 = pb * 64;
-    const int pcount = min(64, n - p0);
+    const int p = pb * 64 + lane;
+    const bool pvalid = p < n;
+    const bool pf = (cp == 0.0f);
+    const int q0 = qb * 64;
+    uint32_t acc_hi = 0, acc_lo = 0;
+    uint32_t col_lo = 0, col_hi = 0;
     static_for([&](auto JC) {
-        constexpr int j = decltype(JC)::value;
-        uint64_t q_dom_p = 0;
-        bool pdq_lane = false;
-        if (j < pcount) {
-            const int p = p0 + j;
-            float pv[M];
+        constexpr int j = 63 - decltype(JC)::value;
+        const int q = q0 + j;
+        uint64_t qdp_m = 0, pdq_m = 0;
+        if (q < n) {
+            const float cq = cvs[q];
+            const bool qf = (cq == 0.0f);
+            bool le_qp = true, lt_qp = false, le_pq = true, lt_pq = false;
 #pragma unroll
-            for (int k = 0; k < M; ++k) pv[k] = f[static_cast<size_t>(k) * ldf + p];
-            if (SIMPLE) {
-                bool lt = false, gt = false;
-#pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    lt = lt | (qv[k] < pv[k]);
-                    gt = gt | (qv[k] > pv[k]);
-                }
-                q_dom_p = ballot(lt && !gt) & valid;
-                pdq_lane = gt && !lt && qvalid;
-            } else {
-                const float cp = cv[p];
-                const bool pf = (cp == 0.0f);
-                bool le_qp = true, lt_qp = false, le_pq = true, lt_pq = false;
-#pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    le_qp = le_qp & (qv[k] <= pv[k]);
-                    lt_qp = lt_qp | (qv[k] < pv[k]);
-                    le_pq = le_pq & (pv[k] <= qv[k]);
-                    lt_pq = lt_pq | (pv[k] < qv[k]);
-                }
-                const bool qdp = (qf && !pf) || (!qf && !pf && (cq < cp)) ||
-                                 (qf && pf && le_qp && lt_qp);
-                const bool pdq = (pf && !qf) || (!pf && !qf && (cp < cq)) ||
-                                 (pf && qf && le_pq && lt_pq);
-                q_dom_p = ballot(qdp && q != p) & valid;
-                pdq_lane = pdq && q != p && qvalid;
+            for (int k = 0; k < M; ++k) {
+                const float qk = fs[static_cast<size_t>(k) * npad + q];
+                le_qp = le_qp && (qk <= pv[k]);
+                lt_qp = lt_qp || (qk < pv[k]);
+                le_pq = le_pq && (pv[k] <= qk);
+                lt_pq = lt_pq || (pv[k] < qk);
             }
+            const bool qdp = (qf && !pf) || (!qf && !pf && (cq < cp)) || (qf && pf && le_qp && lt_qp);
+            const bool pdq = (pf && !qf) || (!pf && !qf && (cp < cq)) || (pf && qf && le_pq && lt_pq);
+            qdp_m = ballot(qdp && q != p);
+            pdq_m = ballot(pdq && pvalid && q != p);
         }
-        rw_lo = writelane<j>(rw_lo, static_cast<uint32_t>(q_dom_p));
-        rw_hi = writelane<j>(rw_hi, static_cast<uint32_t>(q_dom_p >> 32));
-        if (need_col) {
-            const uint32_t bit = pdq_lane ? 1u : 0u;
-            if (j < 32) acc_lo = acc_lo + acc_lo + bit;
-            else acc_hi = acc_hi + acc_hi + bit;
+        if (j >= 32) acc_hi = shift_in(acc_hi, qdp_m);
+        else acc_lo = shift_in(acc_lo, qdp_m);
+        if (backward) {
+            col_lo = writelane<j>(col_lo, static_cast<uint32_t>(pdq_m));
+            col_hi = writelane<j>(col_hi, static_cast<uint32_t>(pdq_m >> 32));
         }
     }, std::make_integer_sequence<int, 64>{});
-    row_word = (static_cast<uint64_t>(rw_hi) << 32) | rw_lo;
-    col_word = (static_cast<uint64_t>(__builtin_bitreverse32(acc_hi)) << 32) |
-               __builtin_bitreverse32(acc_lo);
+    if (backward) {
+        const int q = q0 + lane;
+        if (q < n) B[bword_index(q, pb, Wc)] = (static_cast<uint64_t>(col_hi) << 32) | col_lo;
+    }
+    return (static_cast<uint64_t>(acc_hi) << 32) | acc_lo;
 }
 
-// Grid: one workgroup per 8 x 8 block of tiles (a, b) with a <= b in the
-// upper triangle of the block grid.  Results are staged in LDS and written as
-// full 8-word row chunks in both directions.
+// Work unit u -> (row block pb, chunk qc), qc <= pb / 8.  Units are grouped by
+// a = pb / 8: group a holds 8 row blocks x (a + 1) chunks.
+__device__ inline void decode_unit(long long u, int& pb, int& qc)
+{
+    long long a = static_cast<long long>((sqrt(1.0 + static_cast<double>(u)) - 1.0) * 0.5);
+    while (4 * a * (a + 1) > u) --a;
+    while (4 * (a + 1) * (a + 2) <= u) ++a;
+    const long long rem = u - 4 * a * (a + 1);
+    pb = static_cast<int>(8 * a + rem / (a + 1));
+    qc = static_cast<int>(rem % (a + 1));
+}
+
+// One wave per unit (pb, qc): the 8 words (p, 8qc .. 8qc+7) of the 64 rows of
+// block pb, accumulated in registers and written as one 64-byte segment per
+// lane.  In the diagonal chunk only words <= pb are written; words past pb
+// belong to tied tiles, whose backward direction the wave of the later block
+// stores.
 template <int M>
 __global__ __launch_bounds__(kDomThreads) void nds_dominance(
-    const float* __restrict__ f, int ldf, const float* __restrict__ cv, int n, int W, int Wc,
-    const uint8_t* __restrict__ simple, uint64_t* __restrict__ B)
+    const float* __restrict__ fs, const float* __restrict__ cvs, int npad, int n, int W, int Wc,
+    const uint8_t* __restrict__ kind, const int32_t* __restrict__ hi, uint64_t* __restrict__ B,
+    long long units)
 {
-    __shared__ uint64_t s_row[kTB][kTB][64];   // [pb_local][qb_local][lane]  -> rows of P blocks
-    __shared__ uint64_t s_col[kTB][kTB][64];   // [qb_local][pb_local][lane]  -> rows of Q blocks
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x >> 6));
+    const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
+    if (u >= units) return;
+    int pb, qc;
+    decode_unit(u, pb, qc);
+    pb = wave_uniform(pb);
+    qc = wave_uniform(qc);
+    if (pb >= W) return;
+    const int lane = lane_id();
+    const int p = pb * 64 + lane;
+    float pv[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) pv[k] = fs[static_cast<size_t>(k) * npad + p];
+    const float cp = cvs[p];
+    const bool p_plain = kind[pb] == kPlain;
 
-    // decode (a, b), a <= b, from the linear block id of an upper triangle
-    const int nb = Wc;  // block-grid edge in units of 8 tiles
-    int rem = blockIdx.x, a = 0;
-    while (rem >= nb - a) { rem -= nb - a; ++a; }
-    const int b = a + rem;
-    const bool diag = (a == b);
-
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    for (int t = wave; t < kTB * kTB; t += kDomThreads / 64) {
-        const int pl = t / kTB, ql = t % kTB;
-        const int pb = a * kTB + pl, qb = b * kTB + ql;
-        uint64_t rw = 0, cw = 0;
-        const bool in_range = pb < W && qb < W;
-        const bool needed = in_range && (!diag || pl <= ql);
-        if (needed) {
-            const bool need_col = !(diag && pl == ql);
-            const bool s = simple[pb] && simple[qb];
-            if (s) dom_tile<M, true>(f, ldf, cv, n, pb, qb, need_col, rw, cw);
-            else dom_tile<M, false>(f, ldf, cv, n, pb, qb, need_col, rw, cw);
+    uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0, w7 = 0;
+#pragma unroll 1
+    for (int t = 0; t < 8; ++t) {
+        const int qb = qc * 8 + t;
+        uint64_t word = 0;
+        if (qb < pb) {
+            const bool tied = hi[qb] >= pb;
+            if (!tied && p_plain && kind[qb] == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
+            else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc);
+        } else if (qb == pb) {
+            word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc);
         }
-        if (!diag || pl <= ql) s_row[pl][ql][lane] = rw;
-        if (!diag || pl < ql) s_col[ql][pl][lane] = cw;
+        // rotate: after 8 steps w0..w7 hold tiles 0..7
+        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7; w7 = word;
     }
-    __syncthreads();
-    // write: rows of P blocks, chunk b; rows of Q blocks, chunk a
-    for (int e = threadIdx.x; e < kTB * 64 * 8; e += kDomThreads) {
-        const int pl = e / 512, r = (e / 8) % 64, w = e % 8;
-        const int pb = a * kTB + pl;
-        if (pb < W && b * kTB + w < W) {
-            uint64_t v;
-            if (!diag || pl <= w) v = s_row[pl][w][r];
-            else v = s_col[pl][w][r];
-            B[((static_cast<size_t>(pb) * Wc + b) * 64 + r) * 8 + w] = v;
-        }
-        if (!diag) {
-            const int qb = b * kTB + pl;
-            if (qb < W && a * kTB + w < W)
-                B[((static_cast<size_t>(qb) * Wc + a) * 64 + r) * 8 + w] = s_col[pl][w][r];
-        }
+    uint64_t* dst = B + ((static_cast<size_t>(pb) * Wc + qc) * 64 + lane) * 8;
+    if (qc * 8 + 7 < pb) {
+        ulonglong2* d2 = reinterpret_cast<ulonglong2*>(dst);
+        d2[0] = make_ulonglong2(w0, w1);
+        d2[1] = make_ulonglong2(w2, w3);
+        d2[2] = make_ulonglong2(w4, w5);
+        d2[3] = make_ulonglong2(w6, w7);
+    } else {
+        const int last = pb - qc * 8;   // words 0..last of this chunk
+        dst[0] = w0;
+        if (last >= 1) dst[1] = w1;
+        if (last >= 2) dst[2] = w2;
+        if (last >= 3) dst[3] = w3;
+        if (last >= 4) dst[4] = w4;
+        if (last >= 5) dst[5] = w5;
+        if (last >= 6) dst[6] = w6;
+        if (last >= 7) dst[7] = w7;
     }
 }
 
-// Peel state (device): [0..2] barrier, [3] total assigned, [4] max_front,
-// [5] error.  A0 / A1: assigned-set bitsets (W words each), double-buffered
-// across rounds.
+// Peel state (device), u32 words: [0, kBarWords) grid barrier,
+// [kPeelTotal] total assigned, [kPeelTotal + 1] last round.  A0 / A1:
+// assigned-set bitsets in sorted order (W words each), double-buffered across
+// rounds.
+//
+// Workgroup g owns row blocks g, g + G, g + 2G, ... (G workgroups, spread over
+// the whole sort order).  Round k:
+//   A  the wave owning a block checks, lane per row, the chunk holding each
+//      live row's cursor against the set assigned before round k; rows that
+//      clear it check the next kAheadChunks chunks in one batch.  Rows whose
+//      dominator words run out are front k; rows still clear with words left
+//      go to the workgroup's task queue in LDS as (row, cursor).
+//   B  the workgroup's 16 waves take the queued rows and scan each one's
+//      remaining words, 512 per round trip.  A clear row joins front k, a
+//      blocked row keeps its cursor at the first blocking word.
+// Rows, cursors and tasks never leave their workgroup; only the assigned set
+// and the total cross workgroups, at the one grid barrier per round.  Each
+// word of a row is found clear at most once over all rounds.
 struct PeelArgs {
     const uint64_t* B;
+    const int32_t* hi;       // (W,) last word block of each row block
+    const int32_t* perm;     // (n,) sorted position -> input index
     int n, W, Wc, stop_at;
-    int32_t* fronts;      // (n,) out
-    int32_t* cursor;      // (n,) scratch
+    int blocks_per_group;    // ceil(W / gridDim.x)
+    int32_t* fronts;         // (n,) out, input order
+    int32_t* cursor;         // (n,) scratch, sorted order
     uint64_t* A0;
     uint64_t* A1;
-    uint32_t* state;      // barrier + counters (see above)
+    uint32_t* state;
     int32_t* out_max_front;  // (1,) out
 };
 
-constexpr int kPeelThreads = 256;
-constexpr int kMaxWordsLds = 4096;  // n <= 262144
+constexpr int kPeelTotal = kBarWords;
+constexpr int kPeelStateWords = kBarWords + 16;
+constexpr int kPeelThreads = 1024;
+constexpr int kPeelWaves = kPeelThreads / 64;
+constexpr int kAheadChunks = 3;     // phase A: chunks checked after a cleared cursor chunk
+constexpr int kScanLoads = 8;       // phase B: words per lane in flight
+
+// LDS layout of nds_peel (dynamic): s_A (W u64), s_next (bpg u64),
+// s_task (64 bpg u64), s_hi (W int).
+__host__ __device__ inline size_t peel_lds_bytes(int W, int bpg)
+{
+    return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + static_cast<size_t>(bpg) * 64 * 8 +
+           static_cast<size_t>(W) * 4;
+}
+
+__device__ inline uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
+{
+    const ulonglong2 h = v[i >> 1];
+    return (i & 1) ? h.y : h.x;
+}
 
 __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
 {
-    __shared__ uint64_t s_A[kMaxWordsLds];
+    extern __shared__ uint64_t s_dyn[];
+    const int nblk = P.W;
+    const int bpg = P.blocks_per_group;
+    uint64_t* s_A = s_dyn;                                          // (W) assigned before this round
+    uint64_t* s_next = s_A + nblk;                                  // (bpg) next masks of own blocks
+    uint64_t* s_task = s_next + bpg;                                // (64 bpg) queued (cursor << 32 | row)
+    int* s_hi = reinterpret_cast<int*>(s_task + static_cast<size_t>(bpg) * 64);   // (W)
+    __shared__ int s_ntask;
     __shared__ int s_count;
-    const int nblk = (P.n + 63) / 64;
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int waves_total = gridDim.x * (kPeelThreads / 64);
-    const int gw = blockIdx.x * (kPeelThreads / 64) + wave;
+    const int G = static_cast<int>(gridDim.x);
+    const int g = static_cast<int>(blockIdx.x);
+    const int lane = lane_id();
+    const int lw = wave_uniform(static_cast<int>(threadIdx.x >> 6));
     uint32_t phase = 0;
 
-    // init owned rows
-    for (int rb = gw; rb < nblk; rb += waves_total) {
+    for (int w = threadIdx.x; w < nblk; w += kPeelThreads) s_hi[w] = P.hi[w];
+    for (int j = lw; j < bpg; j += kPeelWaves) {
+        const int rb = g + j * G;
+        if (rb >= nblk) break;
         const int p = rb * 64 + lane;
-        if (p < P.n) { P.fronts[p] = -1; P.cursor[p] = 0; }
+        if (p < P.n) { P.fronts[P.perm[p]] = -1; P.cursor[p] = 0; }
         if (lane == 0) { st_relaxed_u64(&P.A0[rb], 0ull); st_relaxed_u64(&P.A1[rb], 0ull); }
     }
     if (!grid_barrier(P.state, gridDim.x, phase)) return;
@@ -213,59 +358,140 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
     for (int k = 0;; ++k) {
         const uint64_t* Acur = (k & 1) ? P.A1 : P.A0;
         uint64_t* Anext = (k & 1) ? P.A0 : P.A1;
-        for (int w = threadIdx.x; w < P.W; w += kPeelThreads) s_A[w] = ld_relaxed_u64(&Acur[w]);
-        if (threadIdx.x == 0) s_count = 0;
+        for (int w = threadIdx.x; w < nblk; w += kPeelThreads) s_A[w] = ld_relaxed_u64(&Acur[w]);
+        if (threadIdx.x == 0) { s_count = 0; s_ntask = 0; }
         __syncthreads();
 
+        // ---- phase A: lane per row
         int my_new = 0;
-        for (int rb = gw; rb < nblk; rb += waves_total) {
+        for (int j = lw; j < bpg; j += kPeelWaves) {
+            const int rb = g + j * G;
+            if (rb >= nblk) break;
+            const uint64_t amask = s_A[rb];
+            const int end = s_hi[rb] + 1;             // words 0 .. end-1 can hold dominators
             const int p = rb * 64 + lane;
-            const bool live = (p < P.n) && (P.fronts[p] < 0);
-            // phase A: is the row still blocked at its cursor word?
-            int cur = live ? P.cursor[p] : P.W;
-            bool unblocked = false;
+            const bool live = (p < P.n) && !((amask >> lane) & 1ull);
+            const uint64_t* rowbase = P.B + (static_cast<size_t>(rb) * P.Wc * 64 + lane) * 8;
+            int cur = end;
+            bool ready = false, queued = false;
             if (live) {
-                const uint64_t word = P.B[bword_index(p, cur, P.Wc)];
-                unblocked = (word & ~s_A[cur]) == 0;
-            }
-            uint64_t todo = ballot(unblocked);
-            uint64_t ready = 0;
-            // phase B: scan unblocked rows forward, 64 words per step, wave-cooperative
-            while (todo) {
-                const int r = __builtin_ctzll(todo);
-                todo &= todo - 1;
-                const int pr = rb * 64 + r;
-                int c = __shfl(cur, r);
-                while (true) {
-                    const int w = c + lane;
-                    bool blk = false;
-                    if (w < P.W) blk = (P.B[bword_index(pr, w, P.Wc)] & ~s_A[w]) != 0;
-                    const uint64_t bm = ballot(blk);
-                    if (bm) { c = c + __builtin_ctzll(bm); break; }
-                    c += 64;
-                    if (c >= P.W) { c = P.W; break; }
+                cur = P.cursor[p];
+                int c = cur >> 3;
+                // stage 1: the chunk holding the cursor
+                ulonglong2 v[4];
+                const ulonglong2* src = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(c) * 512);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = src[i];
+                int stop = min(c * 8 + 8, end);
+                int blocked = stop;
+#pragma unroll
+                for (int i = 7; i >= 0; --i) {
+                    const int ww = c * 8 + i;
+                    if (ww >= cur && ww < stop && (chunk_word(v, i) & ~s_A[ww]) != 0) blocked = ww;
                 }
-                if (lane == r) cur = c;
-                if (c >= P.W) ready |= 1ull << r;
+                cur = blocked;
+                // stage 2: the next chunks, one batch
+                if (cur == stop && cur < end) {
+                    ulonglong2 u[kAheadChunks][4];
+#pragma unroll
+                    for (int a = 0; a < kAheadChunks; ++a) {
+                        const int ca = c + 1 + a;
+                        if (ca * 8 < end) {
+                            const ulonglong2* s2 = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(ca) * 512);
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) u[a][i] = s2[i];
+                        }
+                    }
+                    const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
+                    int blocked2 = stop2;
+#pragma unroll
+                    for (int a = kAheadChunks - 1; a >= 0; --a) {
+#pragma unroll
+                        for (int i = 7; i >= 0; --i) {
+                            const int ww = (c + 1 + a) * 8 + i;
+                            if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
+                        }
+                    }
+                    cur = blocked2;
+                    stop = stop2;
+                }
+                ready = (cur >= end);
+                queued = !ready && (cur == stop);
+                if (!ready) P.cursor[p] = cur;
             }
-            if (live) P.cursor[p] = cur;
-            if ((ready >> lane) & 1ull) P.fronts[p] = k;
+            const uint64_t ready_m = ballot(ready);
+            const uint64_t task_m = ballot(queued);
+            if (ready) P.fronts[P.perm[p]] = k;
+            if (task_m) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (queued) {
+                    const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                        static_cast<uint32_t>(task_m >> 32),
+                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
+                    s_task[base + below] = (static_cast<uint64_t>(cur) << 32) | static_cast<uint32_t>(p);
+                }
+            }
             if (lane == 0) {
-                st_relaxed_u64(&Anext[rb], s_A[rb] | ready);
-                my_new += __builtin_popcountll(ready);
+                s_next[j] = amask | ready_m;
+                my_new += __builtin_popcountll(ready_m);
+            }
+        }
+        __syncthreads();
+
+        // ---- phase B: the workgroup's waves scan the queued rows
+        const int ntask = s_ntask;
+        for (int t = lw; t < ntask; t += kPeelWaves) {
+            const uint64_t rec = s_task[t];
+            const int pr = wave_uniform(static_cast<int>(static_cast<uint32_t>(rec)));
+            int c = wave_uniform(static_cast<int>(rec >> 32));
+            const int rb = pr >> 6;
+            const int end = s_hi[rb] + 1;
+            bool clear = true;
+            while (c < end) {
+                uint64_t bl[kScanLoads];
+#pragma unroll
+                for (int i = 0; i < kScanLoads; ++i) {
+                    const int w = c + i * 64 + lane;
+                    bl[i] = (w < end) ? P.B[bword_index(pr, w, P.Wc)] : 0ull;
+                }
+                int first = -1;
+#pragma unroll
+                for (int i = kScanLoads - 1; i >= 0; --i) {
+                    const int w = c + i * 64 + lane;
+                    const uint64_t bm = ballot(w < end && (bl[i] & ~s_A[w]) != 0);
+                    if (bm) first = i * 64 + __builtin_ctzll(bm);
+                }
+                if (first >= 0) { c += first; clear = false; break; }
+                c += kScanLoads * 64;
+            }
+            if (lane == 0) {
+                if (clear) {
+                    P.fronts[P.perm[pr]] = k;
+                    atomicOr(reinterpret_cast<unsigned long long*>(&s_next[(rb - g) / G]),
+                             static_cast<unsigned long long>(1ull << (pr & 63)));
+                    ++my_new;
+                } else {
+                    P.cursor[pr] = c;
+                }
             }
         }
         if (lane == 0 && my_new) atomicAdd(&s_count, my_new);
         __syncthreads();
+        for (int j = threadIdx.x; j < bpg; j += kPeelThreads) {
+            const int rb = g + j * G;
+            if (rb < nblk) st_relaxed_u64(&Anext[rb], s_next[j]);
+        }
         if (threadIdx.x == 0 && s_count)
-            __hip_atomic_fetch_add(&P.state[3], static_cast<uint32_t>(s_count), __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(&P.state[kPeelTotal], static_cast<uint32_t>(s_count), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         if (!grid_barrier(P.state, gridDim.x, phase)) return;
-        const int total = static_cast<int>(ld_acquire_u32(&P.state[3]));
+        const int total = static_cast<int>(ld_relaxed_u32(&P.state[kPeelTotal]));
         if (total >= P.stop_at || total == total_prev || total >= P.n) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 P.out_max_front[0] = (total == total_prev) ? k - 1 : k;
-                P.state[4] = static_cast<uint32_t>(k);
+                P.state[kPeelTotal + 1] = static_cast<uint32_t>(k);
             }
             return;
         }
@@ -279,91 +505,132 @@ __global__ void zero_u32(uint32_t* p, int count)
     if (i < count) p[i] = 0u;
 }
 
+constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
+
+struct NdsLayout {
+    uint64_t* B;
+    uint64_t* keys_in;
+    uint64_t* keys_out;
+    int32_t* vals_in;
+    int32_t* perm;
+    void* sort_temp;
+    size_t sort_bytes;
+    float* fs;
+    float* cvs;
+    int32_t* hi;
+    uint8_t* kind;
+    int32_t* cursor;
+    uint64_t* A0;
+    uint64_t* A1;
+    uint32_t* state;
+};
+
+size_t carve_nds(void* ws, int n, NdsLayout& L)
+{
+    const int W = ceil_div(n, 64);
+    const int Wc = ceil_div(W, 8);
+    const int npad = W * 64;
+    Carve c(ws, ~size_t{0});
+    L.B = c.take<uint64_t>(static_cast<size_t>(W) * Wc * 8 * 64);
+    L.keys_in = c.take<uint64_t>(n);
+    L.keys_out = c.take<uint64_t>(n);
+    L.vals_in = c.take<int32_t>(n);
+    L.perm = c.take<int32_t>(n);
+    L.sort_bytes = sort_pairs_temp_bytes(n, kKeyBits);
+    L.sort_temp = c.take_bytes(L.sort_bytes);
+    L.fs = c.take<float>(static_cast<size_t>(kMaxM) * npad);
+    L.cvs = c.take<float>(npad);
+    L.hi = c.take<int32_t>(W);
+    L.kind = c.take<uint8_t>(W);
+    L.cursor = c.take<int32_t>(n);
+    L.A0 = c.take<uint64_t>(W);
+    L.A1 = c.take<uint64_t>(W);
+    L.state = c.take<uint32_t>(kPeelStateWords);
+    return c.used;
+}
+
+// Cooperative grid of the peel: at most one 1024-thread workgroup per CU and
+// about 4 row blocks per workgroup; 0 if the kernel cannot be resident.
+int peel_grid_size(int nblk)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    }
+    int grid = ceil_div(nblk, 4);
+    if (grid > cus) grid = cus;
+    if (grid < 1) grid = 1;
+    const int bpg = ceil_div(nblk, grid);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(nds_peel),
+                                                    kPeelThreads, peel_lds_bytes(nblk, bpg)) != hipSuccess ||
+        per_cu < 1)
+        return 0;
+    return grid;
+}
+
 }  // namespace
 
 size_t nds_workspace_bytes(int n)
 {
-    const int W = ceil_div(n, 64);
-    const int Wc = ceil_div(W, 8);
-    size_t b = 0;
-    b += align_up(static_cast<size_t>(W) * Wc * 8 * 64 * sizeof(uint64_t), 256);  // B
-    b += align_up(static_cast<size_t>(W), 256);                                   // flags
-    b += align_up(static_cast<size_t>(n) * sizeof(int32_t), 256);                 // cursor
-    b += 2 * align_up(static_cast<size_t>(W) * sizeof(uint64_t), 256);            // A0, A1
-    b += 256;                                                                      // state
-    return b;
+    NdsLayout L;
+    return carve_nds(nullptr, n, L);
 }
 
-int nds_max_n() { return kMaxWordsLds * 64; }
+int nds_max_n() { return kMaxWords * 64; }
 
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
             int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream)
 {
-    if (n <= 0 || m <= 0 || m > kMaxM || !f || !cv || !fronts || !max_front)
+    if (n <= 0 || m <= 0 || m > kMaxM || !f || !cv || !fronts || !max_front || ldf < n)
         return fail(kInvalidArgument, "nds: invalid arguments");
     if (n > nds_max_n()) return fail(kUnsupported, "nds: population larger than supported");
-    if (ws_bytes < nds_workspace_bytes(n)) return fail(kWorkspaceTooSmall, "nds: workspace too small");
+    if (!ws || ws_bytes < nds_workspace_bytes(n)) return fail(kWorkspaceTooSmall, "nds: workspace too small");
     const int W = ceil_div(n, 64);
     const int Wc = ceil_div(W, 8);
-    char* base = static_cast<char*>(ws);
-    uint64_t* B = reinterpret_cast<uint64_t*>(base);
-    base += align_up(static_cast<size_t>(W) * Wc * 8 * 64 * sizeof(uint64_t), 256);
-    uint8_t* flags = reinterpret_cast<uint8_t*>(base);
-    base += align_up(static_cast<size_t>(W), 256);
-    int32_t* cursor = reinterpret_cast<int32_t*>(base);
-    base += align_up(static_cast<size_t>(n) * sizeof(int32_t), 256);
-    uint64_t* A0 = reinterpret_cast<uint64_t*>(base);
-    base += align_up(static_cast<size_t>(W) * sizeof(uint64_t), 256);
-    uint64_t* A1 = reinterpret_cast<uint64_t*>(base);
-    base += align_up(static_cast<size_t>(W) * sizeof(uint64_t), 256);
-    uint32_t* state = reinterpret_cast<uint32_t*>(base);
+    const int npad = W * 64;
+    NdsLayout L;
+    carve_nds(ws, n, L);
 
-    hipLaunchKernelGGL(nds_block_flags, dim3(W), dim3(64), 0, stream, f, ldf, cv, n, m, flags);
+    stage_begin(kStNdsSort, stream);
+    const int tb = 256;
+    hipLaunchKernelGGL(nds_keys, dim3(ceil_div(n, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
+                       L.keys_in, L.vals_in);
     MOEA_HIP(hipGetLastError());
-    const int tri = Wc * (Wc + 1) / 2;
+    MOEA_CHECK(sort_pairs(L.sort_temp, L.sort_bytes, L.keys_in, L.keys_out, L.vals_in, L.perm, n,
+                          stream, kKeyBits));
+    hipLaunchKernelGGL(nds_gather, dim3(ceil_div(npad, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
+                       npad, L.perm, L.fs, L.cvs);
+    MOEA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi);
+    MOEA_HIP(hipGetLastError());
+    stage_end(kStNdsSort, stream);
+
     stage_begin(kStDominance, stream);
-#define MOEA_DOM_CASE(MM)                                                                   \
-    case MM:                                                                               \
-        hipLaunchKernelGGL(nds_dominance<MM>, dim3(tri), dim3(kDomThreads), 0, stream, f, ldf, \
-                           cv, n, W, Wc, flags, B);                                        \
-        break;
-    switch (m) {
-        MOEA_DOM_CASE(1) MOEA_DOM_CASE(2) MOEA_DOM_CASE(3) MOEA_DOM_CASE(4) MOEA_DOM_CASE(5)
-        MOEA_DOM_CASE(6) MOEA_DOM_CASE(7) MOEA_DOM_CASE(8) MOEA_DOM_CASE(9) MOEA_DOM_CASE(10)
-        default: return fail(kUnsupported, "nds: unsupported objective count");
-    }
-#undef MOEA_DOM_CASE
+    const long long units = 4ll * Wc * (Wc + 1);
+    const dim3 dgrid(static_cast<unsigned>((units + kDomWaves - 1) / kDomWaves));
+    const dim3 dblock(kDomThreads);
+    MOEA_LAUNCH_M(m, nds_dominance, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
+                  L.hi, L.B, units);
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, stream, state, 64);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(kPeelStateWords), 0, stream, L.state, kPeelStateWords);
     MOEA_HIP(hipGetLastError());
-    PeelArgs args{B, n, W, Wc, stop_at, fronts, cursor, A0, A1, state, max_front};
-    const int nblk = W;
-    int grid = peel_grid_size(nblk);
+    const int grid = peel_grid_size(W);
+    if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
+    const int bpg = ceil_div(W, grid);
+    const size_t lds = peel_lds_bytes(W, bpg);
+    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.cursor, L.A0, L.A1, L.state, max_front};
     void* kargs[] = {&args};
     stage_begin(kStPeel, stream);
     MOEA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(nds_peel), dim3(grid),
-                                        dim3(kPeelThreads), kargs, 0, stream));
+                                        dim3(kPeelThreads), kargs, static_cast<unsigned>(lds), stream));
+    MOEA_HIP(hipGetLastError());
     stage_end(kStPeel, stream);
     return kOk;
-}
-
-int peel_grid_size(int nblk)
-{
-    static int max_blocks = 0;
-    if (max_blocks == 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(nds_peel),
-                                                        kPeelThreads, 0) != hipSuccess)
-            per_cu = 1;
-        max_blocks = cus * (per_cu > 0 ? per_cu : 1);
-    }
-    const int want = ceil_div(nblk, kPeelThreads / 64);
-    return want < max_blocks ? (want > 0 ? want : 1) : max_blocks;
 }
 
 }  // namespace moea

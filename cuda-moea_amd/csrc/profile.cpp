@@ -17,7 +17,7 @@ namespace {
 
 const char* kStageNames[kNumStages] = {
     "reproduce", "evaluate", "nds_dominance", "nds_peel", "select_prep",
-    "association", "niching", "gather", "rvea_select", "rvea_adapt",
+    "association", "niching", "gather", "rvea_select", "rvea_adapt", "nds_sort",
 };
 
 struct Timer {

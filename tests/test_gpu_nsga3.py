@@ -65,6 +65,60 @@ def test_nds_constraints_and_nan(n, m, seed):
     np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
 
 
+@pytest.mark.parametrize("n,m,levels0,seed", [(1000, 3, 2, 11), (3000, 2, 3, 12), (777, 5, 1, 13),
+                                               (2100, 3, 40, 14), (640, 1, 3, 15)])
+def test_nds_f0_tie_groups(n, m, levels0, seed):
+    """f0 on very few levels: tie groups of equal f0 run across many 64-row
+    blocks, where a later row in sort order can dominate an earlier one."""
+    rng = np.random.default_rng(seed)
+    f = rng.random((m, n), dtype=np.float32)
+    f[0] = rng.integers(0, levels0, n).astype(np.float32)
+    if m > 1:
+        f[1, rng.random(n) < 0.2] = 0.5                    # ties in other objectives too
+    f[0, (f[0] == 0) & (rng.random(n) < 0.5)] = -0.0    # -0 and +0 compare equal
+    cv = np.zeros(n, np.float32)
+    for stop in (n, n // 3):
+        print(f"nds tie case n={n} m={m} levels0={levels0} seed={seed} stop={stop}", flush=True)
+        fr_o, mf_o = O.nds(f, cv, stop)
+        fr_g, mf_g = K.nds_rank(_f(f), _f(cv), stop)
+        assert mf_g == mf_o
+        np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
+
+
+def test_nds_duplicates_and_segments():
+    """Duplicate rows, feasible rows with NaN objectives, infeasible rows with
+    equal / NaN cv and +-inf objectives, interleaved across blocks."""
+    rng = np.random.default_rng(21)
+    n, m = 2500, 3
+    base = rng.integers(0, 5, size=(m, 60)).astype(np.float32) / np.float32(5)
+    f = base[:, rng.integers(0, 60, n)]
+    f = np.ascontiguousarray(f)
+    f[rng.integers(0, m, 40), rng.integers(0, n, 40)] = np.nan
+    f[1, rng.random(n) < 0.02] = -np.inf
+    f[2, rng.random(n) < 0.02] = np.inf
+    cv = np.where(rng.random(n) < 0.4, rng.integers(1, 4, n).astype(np.float32), 0.0).astype(np.float32)
+    cv[rng.random(n) < 0.02] = np.nan
+    cv[rng.random(n) < 0.05] = -0.0                     # feasible (== 0)
+    print("nds duplicates/segments case n=2500", flush=True)
+    for stop in (n, 700):
+        fr_o, mf_o = O.nds(f, cv, stop)
+        fr_g, mf_g = K.nds_rank(_f(f), _f(cv), stop)
+        assert mf_g == mf_o
+        np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
+
+
+def test_nds_larger_random():
+    rng = np.random.default_rng(31)
+    n, m = 20000, 3
+    f = rng.random((m, n), dtype=np.float32)
+    cv = np.zeros(n, np.float32)
+    print("nds random case n=20000", flush=True)
+    fr_o, mf_o = O.nds(f, cv, n // 2)
+    fr_g, mf_g = K.nds_rank(_f(f), _f(cv), n // 2)
+    assert mf_g == mf_o
+    np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
+
+
 def test_nds_selection_bench_shape():
     """Standalone selection bench input (2N x 3 rows s*u) at N = 4096."""
     rng = np.random.default_rng(123)
