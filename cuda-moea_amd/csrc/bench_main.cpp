@@ -2,12 +2,15 @@
 // (no Python, no torch), for profilers that want a plain native process.
 //
 //   moea_bench [--n N] [--m M] [--d D] [--gens K] [--warmup W] [--seed S] [--stages]
+//              [--copy-mib C]
 //
 // Workload: NSGA-III with quadratic-center objectives f_k(x) = sum_j (x_j - c_kj)^2,
 // x in [0, 1]^D, centers c drawn like torch.rand((M, D)) after manual_seed(seed)
 // (MT19937, 24-bit floats), Das-Dennis directions for N, SBX eta 30 p 1,
 // polynomial mutation eta 20 p 1, tournament mating -- the cuda_moea.NSGA3
 // defaults.  Prints one JSON line with gens/s over the K timed generations.
+// --copy-mib C first runs one plain copy kernel over C MiB (a dispatch of known
+// traffic, C MiB read and C MiB written, for calibrating HBM byte counters).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -48,6 +51,13 @@ T* dalloc(size_t count)
     return static_cast<T*>(p);
 }
 
+__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t count)
+{
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -55,6 +65,7 @@ int main(int argc, char** argv)
     int n = 32768, m = 3, d = 1024, gens = 50, warmup = 5;
     unsigned long long seed = 0;
     bool stages = false;
+    int copy_mib = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -68,6 +79,7 @@ int main(int argc, char** argv)
         else if (a == "--warmup") warmup = std::atoi(next());
         else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
         else if (a == "--stages") stages = true;
+        else if (a == "--copy-mib") copy_mib = std::atoi(next());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 1; }
     }
     if (n <= 0 || n % 2 || m <= 0 || d <= 0 || gens <= 0 || warmup < 0) {
@@ -77,6 +89,18 @@ int main(int argc, char** argv)
     CHECK_HIP(hipSetDevice(0));
     hipStream_t stream;
     CHECK_HIP(hipStreamCreate(&stream));
+    if (copy_mib > 0) {
+        const size_t count = static_cast<size_t>(copy_mib) * (1u << 20) / sizeof(float4);
+        float4* src = dalloc<float4>(count);
+        float4* dst = dalloc<float4>(count);
+        CHECK_HIP(hipMemsetAsync(src, 0, count * sizeof(float4), stream));
+        hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, stream, src, dst, count);
+        CHECK_HIP(hipGetLastError());
+        CHECK_HIP(hipStreamSynchronize(stream));
+        CHECK_HIP(hipFree(src));
+        CHECK_HIP(hipFree(dst));
+        std::printf("{\"copy_kernel_mib\": %d}\n", copy_mib);
+    }
 
     // directions (host Das-Dennis, as the Python API does)
     const int k = moea_das_dennis(n, m, 0, 0, nullptr, nullptr);

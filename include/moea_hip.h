@@ -4,8 +4,8 @@
  * One NSGA-III / RVEA generation on the GPU: constrained non-dominated sorting,
  * ideal point and FP64 hyperplane normalization, perpendicular-distance
  * association, niching, RVEA angle-penalized selection, tournament / random
- * mating, SBX + polynomial mutation and the built-in objectives, plus a batched
- * engine that advances many independent populations in one launch sequence.
+ * mating, SBX + polynomial mutation, the built-in objectives and a native run
+ * loop that chains them without host synchronization.
  *
  * Conventions (those of the reference, python/csrc/native/include/cuda_moea/
  * core/types.cuh:21-51):
