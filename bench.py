@@ -4,8 +4,11 @@
 A step is one generation (mating -> SBX -> PM -> objective evaluation ->
 NSGA-III selection) of the native run loop, inputs resident in HBM.  W warmup
 generations, then K timed generations between two synchronized HIP events on
-the run stream.  A second pass of K generations with per-stage HIP events
-gives the dominance kernel's average launch time for the roofline.  The CPU
+the run stream.  A second pass of K generations with per-stage HIP events on
+the same stream gives each stage's average time and the dominance kernel's
+average launch time for the roofline (VALU issue: unordered pairs x M lane
+instructions per launch against 256 CUs x 64 lanes x 2.4 GHz; the bitmask bytes
+against 8 TB/s are reported beside it).  The CPU
 baseline times one generation of the oracle (C restatement of the reference
 algorithm) on the host cores.
 """
@@ -22,13 +25,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "cuda-moea_amd"))
 
 N, M, D = 32768, 3, 1024
-VALU_PEAK_CMP = 256 * 64 * 2.4e9          # compare lanes / s (256 CUs x 64 lanes x 2.4 GHz)
+VALU_PEAK_CMP = 256 * 64 * 2.4e9          # VALU lane-ops / s (256 CUs x 64 lanes x 2.4 GHz)
 HBM_PEAK = 8.0e12
 
 
 def _traffic_from_profiles() -> float | None:
-    """HBM bytes per dominance launch from the committed rocprofv3 PMC summary."""
-    p = ROOT / "profiles" / "nds_dominance_n65536_m3_pmc.txt"
+    """HBM bytes per dominance launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size) from the committed rocprofv3 PMC summary."""
+    p = ROOT / "profiles" / "nds_dominance_n65536_m3_pmc.md"
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
@@ -102,9 +106,15 @@ def main() -> int:
     dom_ms, dom_n = stages["nds_dominance"]
     dom_avg_s = dom_ms / max(dom_n, 1) / 1e3
     nm = 2 * N
-    work = M * nm * (nm - 1)                      # element compares per launch
-    achieved = work / dom_avg_s / 1e9             # Gcmp/s
+    words = (nm + 63) // 64
+    # work of the sorted-order dominance kernel: every unordered pair once, M lane
+    # instructions each (M-1 compares into a lane mask, one bit shift-in)
+    pairs = nm * (nm - 1) // 2
+    lane_ops = pairs * M
+    achieved = lane_ops / dom_avg_s / 1e9                 # G lane-ops / s
     peak = VALU_PEAK_CMP / 1e9
+    # algorithmic bytes: the block-lower-triangular bitmask written, objectives read
+    alg_bytes = 64 * 8 * words * (words + 1) // 2 + nm * M * 4
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1]}
 
     cpu = None
@@ -128,9 +138,14 @@ def main() -> int:
                    "N": N, "M": M, "D": D, "K_directions": alg.k, "generations_timed": K,
                    "last_front_index": max_front},
         "roofline": {"bound": "valu", "kernel": "nds_dominance",
-                     "achieved": round(achieved, 1), "peak": round(peak, 1), "unit": "Gcmp/s",
+                     "achieved": round(achieved, 1), "peak": round(peak, 1), "unit": "Glane-ops/s",
                      "frac": round(achieved / peak, 4), "traffic": _traffic_from_profiles(),
-                     "work_per_launch": work, "avg_launch_ms": round(dom_avg_s * 1e3, 4)},
+                     "work_per_launch": lane_ops, "avg_launch_ms": round(dom_avg_s * 1e3, 4),
+                     "algorithmic_bytes": alg_bytes,
+                     "hbm_achieved_GBps": round(alg_bytes / dom_avg_s / 1e9, 1),
+                     "hbm_frac": round(alg_bytes / dom_avg_s / HBM_PEAK, 4),
+                     "reference_compares_per_launch": M * nm * (nm - 1),
+                     "frac_reference_work": round(M * nm * (nm - 1) / dom_avg_s / VALU_PEAK_CMP, 4)},
         "stage_ms_per_generation": stage_ms,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,121 @@
+// Batched engine: many independent NSGA-III populations of uneven N / M / D
+// advanced by one launch sequence per generation.
+//
+// Every per-instance buffer lives in one caller-provided workspace.  BInst is
+// the device-side record of one instance; a BMap sends each block of a batched
+// launch to (instance, block within the instance) through a prefix table of
+// per-instance block counts, so no block straddles two instances and each
+// instance computes exactly what a single-population launch would.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace moea {
+
+struct BInst {
+    // sizes
+    int n, m, d, k;
+    int nm, W, Wc, npad;          // NDS over the 2n merged rows
+    int max_gen;
+    int nchunks;                  // association chunks of 1024 directions
+    uint64_t seed;
+    float sbx_eta0, sbx_eta1, sbx_prob, sbx_copy, pm_eta0, pm_eta1, pm_prob;
+    // problem (caller-owned device arrays)
+    const float* centers;         // (m, d)
+    const float* bounds;          // (2d)
+    const float* dirs;            // (m, k)
+    // populations: three rotating slots
+    float* x[3];
+    float* f[3];
+    float* cv[3];
+    int32_t* fronts[2];           // parent fronts, ping-pong
+    int32_t* winners;             // (n) mating winners
+    float* ideal;                 // (m)
+    // selection scratch (2n merged rows)
+    float* fm;                    // (m, nm)
+    float* cvm;                   // (nm)
+    int32_t* fronts_mix;          // (nm)
+    int32_t* meta;                // (8) max_front, n_prior, n_last, ill, n_rem, chosen count
+    int32_t* gidx;                // (nm)
+    float* fn;                    // (m, nm)
+    float* a;                     // (kMaxM)
+    float* part_d2;               // (nchunks, nm)
+    int32_t* part_idx;            // (nchunks, nm)
+    float* dist;                  // (nm)
+    int32_t* ref;                 // (nm)
+    int32_t* rho;                 // (k)
+    uint8_t* flag;                // (nm)
+    int32_t* chosen;              // (n)
+    // NDS scratch (2n merged rows, or n for the initial parents)
+    uint64_t* B;                  // (W * Wc * 512) dominator bitmask, sorted order
+    float* fs;                    // (m, npad) sorted objectives
+    float* cvs;                   // (npad)
+    int32_t* perm;                // (nm) sorted position -> local row
+    int32_t* hi;                  // (W)
+    uint8_t* kind;                // (W)
+    int32_t* cursor;              // (nm)
+    uint64_t* A;                  // (W) assigned set
+    long long key_off;            // first of this instance's nm entries in the batch key arrays
+    long long key_off_n;          // first of this instance's n entries (NDS of the parents)
+};
+
+// Block map: block b of a batched launch belongs to instance i with
+// start[i] <= b < start[i + 1]; its local block index is b - start[i].
+struct BMap {
+    const int* start;             // (count + 1) prefix of per-instance block counts
+    int count;
+};
+
+// Instance of block b (wave-uniform; binary search over the prefix).
+__device__ inline int bmap_find(const BMap& mp, int b, int& local)
+{
+    int lo = 0, hi = mp.count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mp.start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    local = b - mp.start[lo];
+    return lo;
+}
+
+// A batched launch: its block map and total block count.
+struct BGrid {
+    BMap map;
+    int blocks;
+};
+
+// Batched stage launches (defined next to their single-population kernels).
+struct BatchLaunch {
+    BInst* insts;                 // device instance table
+    int count;
+    hipStream_t stream;
+};
+
+// Grids of one batched NDS call; mode 0 ranks the n parents of a slot, mode 1
+// the 2n merged rows.
+struct BNdsGrids {
+    BGrid rows256;                // nds keys: rows per instance, 256 per block
+    BGrid npad256;                // sorted gather: 64 W per instance, 256 per block
+    BGrid blocks64;               // block info: W one-wave blocks per instance
+    BGrid dom[kMaxM + 1];         // dominance units, 4 per block, of the instances with m objectives
+    int max_W;                    // largest W (peel LDS)
+    long long total_rows;         // sum of rows (sort length)
+};
+
+struct BNdsScratch {
+    uint64_t* keys_in;
+    uint64_t* keys_out;
+    int32_t* vals_in;
+    int32_t* vals_out;
+    void* sort_temp;
+    size_t sort_bytes;
+    int inst_bits;                // bits of the instance index above the 34-bit NDS key
+};
+
+}  // namespace moea

@@ -319,64 +319,63 @@ __device__ inline void st_relaxed_u64(uint64_t* p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier for cooperative launches, two-level to spread the arrival
-// atomics: block b arrives on the counter of group b % 8 (blocks are dealt to
-// the 8 XCDs round-robin, so a group is one XCD's blocks); the last arrival of
-// a group arrives on the top counter; the last top arrival advances the
-// generation word.  Waiting blocks poll the generation with relaxed loads and
-// take one acquire fence when it moves (an acquire per poll would invalidate
-// the XCD's L2 on every poll).  Every arrival is an acq_rel RMW, which chains
-// each block's writes before the barrier to every block's reads after it.
-// Words of `state` (each counter on its own 64-byte line), all zero at launch:
-constexpr int kBarGroups = 8;
-constexpr int kBarTop = kBarGroups * 16;      // top-level arrival count
-constexpr int kBarGen = kBarTop + 16;         // generation
+// Grid barrier for cooperative launches (at most kBarMaxBlocks blocks).  Each
+// block publishes its arrival with a release store to its own flag word;
+// every thread of block 0 polls a share of the flags with relaxed loads and
+// takes an acquire fence, then block 0 releases the generation word that the
+// other blocks poll (relaxed loads, one acquire fence when it moves).  There
+// is no read-modify-write on a shared counter, and no acquire per poll (that
+// would invalidate the XCD's L2 on every poll).
+// Words of `state`, all zero at launch:
+constexpr int kBarMaxBlocks = 1024;
+constexpr int kBarGen = kBarMaxBlocks;        // generation
 constexpr int kBarError = kBarGen + 16;       // set when a spin gives up
 constexpr int kBarWords = kBarError + 16;     // first word free for the caller
-// The spin is bounded: after kSpinLimit polls the block records an error and
-// returns false, so a broken launch ends instead of hanging the device.
+// The spin is bounded: after kSpinLimit polls a thread records an error and
+// the barrier returns false, so a broken launch ends instead of hanging.
 constexpr uint32_t kSpinLimit = 1u << 24;
 
 __device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t& phase)
 {
     __shared__ int s_ok;
+    const uint32_t target = phase + 1;
+    if (threadIdx.x == 0) s_ok = 1;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        const uint32_t target = phase + 1;
-        const uint32_t g = blockIdx.x % kBarGroups;
-        const uint32_t group_size = (nblocks - g + kBarGroups - 1) / kBarGroups;
-        const uint32_t groups = nblocks < kBarGroups ? nblocks : kBarGroups;
-        bool released = false;
-        const uint32_t arrived =
-            __hip_atomic_fetch_add(&state[g * 16], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == group_size - 1) {
-            __hip_atomic_store(&state[g * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t top =
-                __hip_atomic_fetch_add(&state[kBarTop], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (top == groups - 1) {
-                __hip_atomic_store(&state[kBarTop], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&state[kBarGen], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                released = true;
-            }
-        }
-        if (!released) {
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&state[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0) {
+        for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
             uint32_t spins = 0;
-            while (__hip_atomic_load(&state[kBarGen], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            while (__hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > kSpinLimit) {
                     __hip_atomic_store(&state[kBarError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
+                    s_ok = 0;
                     break;
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        if (__hip_atomic_load(&state[kBarError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
-        s_ok = ok;
-        phase = target;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&state[kBarGen], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&state[kBarGen], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(&state[kBarError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_ok = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    if (threadIdx.x == 0 &&
+        __hip_atomic_load(&state[kBarError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        s_ok = 0;
     __syncthreads();
+    phase = target;
     return s_ok != 0;
 }
 

@@ -5,6 +5,7 @@
 // row once and produces all m objectives.  Fixed summation order (shared with
 // the oracle): lane l accumulates j = l, l+64, ... ascending with one fma per
 // element, then a pairwise tree over lanes (offsets 32, 16, ..., 1).
+#include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
 
@@ -12,15 +13,12 @@ namespace moea {
 
 namespace {
 
+// One wave computes the m objectives of one row.
 template <int M>
-__global__ __launch_bounds__(256) void quadratic_kernel(const float* __restrict__ x, int n, int d,
-                                                        const float* __restrict__ c,
-                                                        float* __restrict__ f, int ldf,
-                                                        float* __restrict__ cv)
+__device__ inline void quadratic_row(const float* __restrict__ x, int d, const float* __restrict__ c,
+                                     float* __restrict__ f, int ldf, float* __restrict__ cv, int row,
+                                     int lane)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
     const float* xr = x + static_cast<size_t>(row) * d;
     float acc[M];
 #pragma unroll
@@ -47,6 +45,30 @@ __global__ __launch_bounds__(256) void quadratic_kernel(const float* __restrict_
         f[static_cast<size_t>(lane) * ldf + row] = out;
     }
     if (lane == 0 && cv) cv[row] = 0.0f;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void quadratic_kernel(const float* __restrict__ x, int n, int d,
+                                                        const float* __restrict__ c,
+                                                        float* __restrict__ f, int ldf,
+                                                        float* __restrict__ cv)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    quadratic_row<M>(x, d, c, f, ldf, cv, row, static_cast<int>(threadIdx.x & 63));
+}
+
+// Batched: the instances with M objectives, 4 rows per block (slot `slot`).
+template <int M>
+__global__ __launch_bounds__(256) void b_quadratic_kernel(const BInst* __restrict__ insts, BMap mp, int slot)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (row >= I.n) return;
+    quadratic_row<M>(I.x[slot], I.d, I.centers, I.f[slot], I.n, I.cv[slot], row,
+                     static_cast<int>(threadIdx.x & 63));
 }
 
 // MLP hidden layer H = tanh(X W1^T): (n, d) x (hidden, d)^T -> (n, hidden).
@@ -168,6 +190,15 @@ int eval_quadratic(const float* x, int n, int d, const float* centers, int m, fl
 #undef MOEA_Q
         default: return fail(kUnsupported, "eval_quadratic: unsupported objective count");
     }
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int batch_eval_quadratic(const BatchLaunch& L, int m, const BMap& rows4, int nblocks, int slot)
+{
+    if (nblocks <= 0) return kOk;
+    const dim3 grid(nblocks), block(256);
+    MOEA_LAUNCH_M(m, b_quadratic_kernel, grid, block, 0, L.stream, L.insts, rows4, slot);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }

@@ -35,6 +35,7 @@
 //   word (p, w) lives at (((p/64) * Wc + w/8) * 64 + p%64) * 8 + w%8.
 // A 64-row x 8-word chunk is 4 KiB contiguous: the dominance kernel writes it
 // as one 64-byte segment per lane, the peel reads a row's words 8 at a time.
+#include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
 
@@ -60,21 +61,24 @@ __device__ inline uint32_t order_key(float x)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+__device__ inline uint64_t nds_key_of(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
+                                      int s, int m)
+{
+    const float c = cv[s];
+    if (c == 0.0f) {
+        bool nan = false;
+        for (int k = 0; k < m; ++k) nan = nan || isnan(f[static_cast<size_t>(k) * ldf + s]);
+        return nan ? (1ull << 32) : order_key(f[s]);
+    }
+    return (2ull << 32) | (isnan(c) ? 0xFFFFFFFFu : order_key(c));
+}
+
 __global__ void nds_keys(const float* __restrict__ f, int ldf, const float* __restrict__ cv, int n,
                          int m, uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const float c = cv[s];
-    uint64_t key;
-    if (c == 0.0f) {
-        bool nan = false;
-        for (int k = 0; k < m; ++k) nan = nan || isnan(f[static_cast<size_t>(k) * ldf + s]);
-        key = nan ? (1ull << 32) : order_key(f[s]);
-    } else {
-        key = (2ull << 32) | (isnan(c) ? 0xFFFFFFFFu : order_key(c));
-    }
-    keys[s] = key;
+    keys[s] = nds_key_of(f, ldf, cv, s, m);
     vals[s] = s;
 }
 
@@ -95,20 +99,20 @@ __global__ void nds_gather(const float* __restrict__ f, int ldf, const float* __
     }
 }
 
-// One wave per 64-row block: kind[b] and hi[b] (see the header comment).
-__global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
-                               int32_t* __restrict__ hi)
+// One wave per 64-row block: kind[b] and hi[b] (see the header comment).  The
+// segment is bits 32-33 of the key (higher bits may carry a batch index).
+__device__ inline void block_info_one(const uint64_t* __restrict__ keys, int n, int b,
+                                      uint8_t* __restrict__ kind, int32_t* __restrict__ hi)
 {
-    const int b = blockIdx.x;
     const int t = b * 64 + static_cast<int>(threadIdx.x);
-    const bool other = (t < n) && (keys[t] >> 32) != 0;
+    const bool other = (t < n) && ((keys[t] >> 32) & 3u) != 0;
     const uint64_t any_other = ballot(other);
     if (threadIdx.x != 0) return;
     kind[b] = any_other ? kMixed : kPlain;
     const int last = min(n, b * 64 + 64) - 1;
     const uint64_t kl = keys[last];
     int end = last;
-    if ((kl >> 32) == 0) {
+    if (((kl >> 32) & 3u) == 0) {
         // first position after `last` whose key differs (keys ascend)
         int lo = last + 1, hi_ = n;
         while (lo < hi_) {
@@ -119,6 +123,12 @@ __global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t
         end = lo - 1;
     }
     hi[b] = end >> 6;
+}
+
+__global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
+                               int32_t* __restrict__ hi)
+{
+    block_info_one(keys, n, static_cast<int>(blockIdx.x), kind, hi);
 }
 
 // Tile in which every earlier row q (block qb, wave-uniform) has f0 strictly
@@ -218,14 +228,10 @@ __device__ inline void decode_unit(long long u, int& pb, int& qc)
 // belong to tied tiles, whose backward direction the wave of the later block
 // stores.
 template <int M>
-__global__ __launch_bounds__(kDomThreads) void nds_dominance(
-    const float* __restrict__ fs, const float* __restrict__ cvs, int npad, int n, int W, int Wc,
-    const uint8_t* __restrict__ kind, const int32_t* __restrict__ hi, uint64_t* __restrict__ B,
-    long long units)
+__device__ inline void dom_unit(const float* __restrict__ fs, const float* __restrict__ cvs, int npad,
+                                int n, int W, int Wc, const uint8_t* __restrict__ kind,
+                                const int32_t* __restrict__ hi, uint64_t* __restrict__ B, long long u)
 {
-    const int wave = wave_uniform(static_cast<int>(threadIdx.x >> 6));
-    const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
-    if (u >= units) return;
     int pb, qc;
     decode_unit(u, pb, qc);
     pb = wave_uniform(pb);
@@ -274,6 +280,18 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
     }
 }
 
+template <int M>
+__global__ __launch_bounds__(kDomThreads) void nds_dominance(
+    const float* __restrict__ fs, const float* __restrict__ cvs, int npad, int n, int W, int Wc,
+    const uint8_t* __restrict__ kind, const int32_t* __restrict__ hi, uint64_t* __restrict__ B,
+    long long units)
+{
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x >> 6));
+    const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
+    if (u >= units) return;
+    dom_unit<M>(fs, cvs, npad, n, W, Wc, kind, hi, B, u);
+}
+
 // Peel state (device), u32 words: [0, kBarWords) grid barrier,
 // [kPeelTotal] total assigned, [kPeelTotal + 1] last round.  A0 / A1:
 // assigned-set bitsets in sorted order (W words each), double-buffered across
@@ -287,7 +305,7 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
 //      dominator words run out are front k; rows still clear with words left
 //      go to the workgroup's task queue in LDS as (row, cursor).
 //   B  the workgroup's 16 waves take the queued rows and scan each one's
-//      remaining words, 512 per round trip.  A clear row joins front k, a
+//      remaining words (64 in the first round trip, then 512 per round trip).  A clear row joins front k, a
 //      blocked row keeps its cursor at the first blocking word.
 // Rows, cursors and tasks never leave their workgroup; only the assigned set
 // and the total cross workgroups, at the one grid barrier per round.  Each
@@ -325,6 +343,82 @@ __device__ inline uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
 {
     const ulonglong2 h = v[i >> 1];
     return (i & 1) ? h.y : h.x;
+}
+
+// Phase A for the lane's row p of block rb (live rows only): checks the
+// chunk holding the cursor, and the next kAheadChunks chunks if it is clear.
+// Updates cur; ready: no dominator word left; queued: clear so far, words left.
+__device__ inline void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
+                                  const uint64_t* s_A, int& cur, bool& ready, bool& queued)
+{
+    const uint64_t* rowbase = B + (static_cast<size_t>(rb) * Wc * 64 + lane) * 8;
+    const int c = cur >> 3;
+    ulonglong2 v[4];
+    const ulonglong2* src = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(c) * 512);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = src[i];
+    int stop = min(c * 8 + 8, end);
+    int blocked = stop;
+#pragma unroll
+    for (int i = 7; i >= 0; --i) {
+        const int ww = c * 8 + i;
+        if (ww >= cur && ww < stop && (chunk_word(v, i) & ~s_A[ww]) != 0) blocked = ww;
+    }
+    cur = blocked;
+    if (cur == stop && cur < end) {
+        ulonglong2 u[kAheadChunks][4];
+#pragma unroll
+        for (int a = 0; a < kAheadChunks; ++a) {
+            const int ca = c + 1 + a;
+            if (ca * 8 < end) {
+                const ulonglong2* s2 = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(ca) * 512);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[a][i] = s2[i];
+            }
+        }
+        const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
+        int blocked2 = stop2;
+#pragma unroll
+        for (int a = kAheadChunks - 1; a >= 0; --a) {
+#pragma unroll
+            for (int i = 7; i >= 0; --i) {
+                const int ww = (c + 1 + a) * 8 + i;
+                if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
+            }
+        }
+        cur = blocked2;
+        stop = stop2;
+    }
+    ready = (cur >= end);
+    queued = !ready && (cur == stop);
+}
+
+// Phase B: the wave scans row pr from word c (wave-uniform) towards end, 64
+// words in the first round trip, then 512.  Returns the first word holding an
+// unassigned dominator, or end when the row is clear.
+__device__ inline int peel_scan(const uint64_t* __restrict__ B, int Wc, int pr, int c, int end,
+                                const uint64_t* s_A, int lane)
+{
+    int batch = 1;
+    while (c < end) {
+        uint64_t bl[kScanLoads];
+#pragma unroll
+        for (int i = 0; i < kScanLoads; ++i) {
+            const int w = c + i * 64 + lane;
+            bl[i] = (i < batch && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
+        }
+        int first = -1;
+#pragma unroll
+        for (int i = kScanLoads - 1; i >= 0; --i) {
+            const int w = c + i * 64 + lane;
+            const uint64_t bm = ballot(i < batch && w < end && (bl[i] & ~s_A[w]) != 0);
+            if (bm) first = i * 64 + __builtin_ctzll(bm);
+        }
+        if (first >= 0) return c + first;
+        c += batch * 64;
+        batch = kScanLoads;
+    }
+    return end;
 }
 
 __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
@@ -371,52 +465,11 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
             const int end = s_hi[rb] + 1;             // words 0 .. end-1 can hold dominators
             const int p = rb * 64 + lane;
             const bool live = (p < P.n) && !((amask >> lane) & 1ull);
-            const uint64_t* rowbase = P.B + (static_cast<size_t>(rb) * P.Wc * 64 + lane) * 8;
             int cur = end;
             bool ready = false, queued = false;
             if (live) {
                 cur = P.cursor[p];
-                int c = cur >> 3;
-                // stage 1: the chunk holding the cursor
-                ulonglong2 v[4];
-                const ulonglong2* src = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(c) * 512);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = src[i];
-                int stop = min(c * 8 + 8, end);
-                int blocked = stop;
-#pragma unroll
-                for (int i = 7; i >= 0; --i) {
-                    const int ww = c * 8 + i;
-                    if (ww >= cur && ww < stop && (chunk_word(v, i) & ~s_A[ww]) != 0) blocked = ww;
-                }
-                cur = blocked;
-                // stage 2: the next chunks, one batch
-                if (cur == stop && cur < end) {
-                    ulonglong2 u[kAheadChunks][4];
-#pragma unroll
-                    for (int a = 0; a < kAheadChunks; ++a) {
-                        const int ca = c + 1 + a;
-                        if (ca * 8 < end) {
-                            const ulonglong2* s2 = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(ca) * 512);
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) u[a][i] = s2[i];
-                        }
-                    }
-                    const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
-                    int blocked2 = stop2;
-#pragma unroll
-                    for (int a = kAheadChunks - 1; a >= 0; --a) {
-#pragma unroll
-                        for (int i = 7; i >= 0; --i) {
-                            const int ww = (c + 1 + a) * 8 + i;
-                            if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
-                        }
-                    }
-                    cur = blocked2;
-                    stop = stop2;
-                }
-                ready = (cur >= end);
-                queued = !ready && (cur == stop);
+                peel_check(P.B, P.Wc, rb, lane, end, s_A, cur, ready, queued);
                 if (!ready) P.cursor[p] = cur;
             }
             const uint64_t ready_m = ballot(ready);
@@ -445,27 +498,11 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
         for (int t = lw; t < ntask; t += kPeelWaves) {
             const uint64_t rec = s_task[t];
             const int pr = wave_uniform(static_cast<int>(static_cast<uint32_t>(rec)));
-            int c = wave_uniform(static_cast<int>(rec >> 32));
+            const int c0 = wave_uniform(static_cast<int>(rec >> 32));
             const int rb = pr >> 6;
             const int end = s_hi[rb] + 1;
-            bool clear = true;
-            while (c < end) {
-                uint64_t bl[kScanLoads];
-#pragma unroll
-                for (int i = 0; i < kScanLoads; ++i) {
-                    const int w = c + i * 64 + lane;
-                    bl[i] = (w < end) ? P.B[bword_index(pr, w, P.Wc)] : 0ull;
-                }
-                int first = -1;
-#pragma unroll
-                for (int i = kScanLoads - 1; i >= 0; --i) {
-                    const int w = c + i * 64 + lane;
-                    const uint64_t bm = ballot(w < end && (bl[i] & ~s_A[w]) != 0);
-                    if (bm) first = i * 64 + __builtin_ctzll(bm);
-                }
-                if (first >= 0) { c += first; clear = false; break; }
-                c += kScanLoads * 64;
-            }
+            const int c = peel_scan(P.B, P.Wc, pr, c0, end, s_A, lane);
+            const bool clear = c >= end;
             if (lane == 0) {
                 if (clear) {
                     P.fronts[P.perm[pr]] = k;
@@ -496,6 +533,181 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
             return;
         }
         total_prev = total;
+    }
+}
+
+// ---------------------------------------------------------------- batched --
+// The NDS of every instance of a batch: mode 0 ranks the n parents of slot
+// `slot` into fronts[fslot], mode 1 the 2n merged rows into fronts_mix and
+// meta[0].  Keys carry the instance index above the 34 NDS bits, so one
+// stable sort orders every instance's rows exactly as its own sort would.
+struct BRows {
+    const float* f;
+    const float* cv;
+    int rows;
+    long long key_off;
+    int32_t* fronts;
+};
+
+__device__ inline BRows brows(const BInst& I, int mode, int slot, int fslot)
+{
+    BRows r;
+    if (mode == 0) {
+        r.f = I.f[slot]; r.cv = I.cv[slot]; r.rows = I.n; r.key_off = I.key_off_n; r.fronts = I.fronts[fslot];
+    } else {
+        r.f = I.fm; r.cv = I.cvm; r.rows = I.nm; r.key_off = I.key_off; r.fronts = I.fronts_mix;
+    }
+    return r;
+}
+
+__global__ void b_nds_keys(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
+                           uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const BRows R = brows(I, mode, slot, fslot);
+    const int s = local * 256 + static_cast<int>(threadIdx.x);
+    if (s >= R.rows) return;
+    keys[R.key_off + s] = nds_key_of(R.f, R.rows, R.cv, s, I.m) | (static_cast<uint64_t>(i) << kKeyBits);
+    vals[R.key_off + s] = s;
+}
+
+__global__ void b_nds_gather(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
+                             const int32_t* __restrict__ vals_sorted)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const BRows R = brows(I, mode, slot, fslot);
+    const int npad = ((R.rows + 63) / 64) * 64;
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t >= npad) return;
+    if (t < R.rows) {
+        const int s = vals_sorted[R.key_off + t];
+        I.perm[t] = s;
+        for (int k = 0; k < I.m; ++k)
+            I.fs[static_cast<size_t>(k) * npad + t] = R.f[static_cast<size_t>(k) * R.rows + s];
+        I.cvs[t] = R.cv[s];
+    } else {
+        for (int k = 0; k < I.m; ++k) I.fs[static_cast<size_t>(k) * npad + t] = 0.0f;
+        I.cvs[t] = 0.0f;
+    }
+}
+
+__global__ void b_nds_block_info(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
+                                 const uint64_t* __restrict__ keys_sorted)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const BRows R = brows(I, mode, slot, fslot);
+    block_info_one(keys_sorted + R.key_off, R.rows, local, I.kind, I.hi);
+}
+
+template <int M>
+__global__ __launch_bounds__(kDomThreads) void b_nds_dominance(const BInst* __restrict__ insts, BMap mp,
+                                                              int mode, int slot, int fslot)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const BRows R = brows(I, mode, slot, fslot);
+    const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
+    const long long units = 4ll * Wc * (Wc + 1);
+    const long long u = static_cast<long long>(local) * kDomWaves + wave_uniform(static_cast<int>(threadIdx.x >> 6));
+    if (u >= units) return;
+    dom_unit<M>(I.fs, I.cvs, W * 64, R.rows, W, Wc, I.kind, I.hi, I.B, u);
+}
+
+// Front peeling of one instance per 1024-thread workgroup: the two phases of
+// nds_peel with workgroup barriers only.  Tasks go through a global per-
+// instance list (the gidx scratch, free during NDS).
+__global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restrict__ insts, int mode,
+                                                          int slot, int fslot)
+{
+    extern __shared__ uint64_t s_dyn[];
+    __shared__ int s_ntask;
+    __shared__ int s_count;
+    const BInst& I = insts[blockIdx.x];
+    const BRows R = brows(I, mode, slot, fslot);
+    const int rows = R.rows;
+    const int W = (rows + 63) / 64, Wc = (W + 7) / 8;
+    uint64_t* s_A = s_dyn;            // (W) assigned before this round
+    uint64_t* s_next = s_dyn + W;     // (W)
+    int32_t* tasks = I.gidx;
+    const int lane = lane_id();
+    const int lw = wave_uniform(static_cast<int>(threadIdx.x >> 6));
+
+    for (int p = threadIdx.x; p < rows; p += kPeelThreads) {
+        R.fronts[I.perm[p]] = -1;
+        I.cursor[p] = 0;
+    }
+    for (int w = threadIdx.x; w < W; w += kPeelThreads) s_A[w] = 0ull;
+    int total = 0;
+    for (int k = 0;; ++k) {
+        __syncthreads();   // every thread has read the previous round's count
+        if (threadIdx.x == 0) { s_ntask = 0; s_count = 0; }
+        __syncthreads();
+        int my_new = 0;
+        for (int rb = lw; rb < W; rb += kPeelWaves) {
+            const uint64_t amask = s_A[rb];
+            const int end = I.hi[rb] + 1;
+            const int p = rb * 64 + lane;
+            const bool live = (p < rows) && !((amask >> lane) & 1ull);
+            int cur = end;
+            bool ready = false, queued = false;
+            if (live) {
+                cur = I.cursor[p];
+                peel_check(I.B, Wc, rb, lane, end, s_A, cur, ready, queued);
+                if (!ready) I.cursor[p] = cur;
+            }
+            const uint64_t ready_m = ballot(ready);
+            const uint64_t task_m = ballot(queued);
+            if (ready) R.fronts[I.perm[p]] = k;
+            if (task_m) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (queued) {
+                    const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                        static_cast<uint32_t>(task_m >> 32),
+                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
+                    tasks[base + below] = p;
+                }
+            }
+            if (lane == 0) {
+                s_next[rb] = amask | ready_m;
+                my_new += __builtin_popcountll(ready_m);
+            }
+        }
+        __syncthreads();
+        const int ntask = s_ntask;
+        for (int t = lw; t < ntask; t += kPeelWaves) {
+            const int pr = wave_uniform(tasks[t]);
+            const int c0 = wave_uniform(I.cursor[pr]);
+            const int rb = pr >> 6;
+            const int c = peel_scan(I.B, Wc, pr, c0, I.hi[rb] + 1, s_A, lane);
+            if (lane == 0) {
+                if (c >= I.hi[rb] + 1) {
+                    R.fronts[I.perm[pr]] = k;
+                    atomicOr(reinterpret_cast<unsigned long long*>(&s_next[rb]),
+                             static_cast<unsigned long long>(1ull << (pr & 63)));
+                    ++my_new;
+                } else {
+                    I.cursor[pr] = c;
+                }
+            }
+        }
+        if (lane == 0 && my_new) atomicAdd(&s_count, my_new);
+        __syncthreads();
+        for (int w = threadIdx.x; w < W; w += kPeelThreads) s_A[w] = s_next[w];
+        const int added = s_count;
+        total += added;
+        if (total >= I.n || added == 0 || total >= rows) {
+            if (mode == 1 && threadIdx.x == 0) I.meta[0] = (added == 0) ? k - 1 : k;
+            return;
+        }
     }
 }
 
@@ -561,6 +773,7 @@ int peel_grid_size(int nblk)
     }
     int grid = ceil_div(nblk, 4);
     if (grid > cus) grid = cus;
+    if (grid > kBarMaxBlocks) grid = kBarMaxBlocks;
     if (grid < 1) grid = 1;
     const int bpg = ceil_div(nblk, grid);
     int per_cu = 0;
@@ -617,7 +830,8 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(kPeelStateWords), 0, stream, L.state, kPeelStateWords);
+    hipLaunchKernelGGL(zero_u32, dim3(ceil_div(kPeelStateWords, 256)), dim3(256), 0, stream, L.state,
+                       kPeelStateWords);
     MOEA_HIP(hipGetLastError());
     const int grid = peel_grid_size(W);
     if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
@@ -630,6 +844,37 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
                                         dim3(kPeelThreads), kargs, static_cast<unsigned>(lds), stream));
     MOEA_HIP(hipGetLastError());
     stage_end(kStPeel, stream);
+    return kOk;
+}
+
+size_t batch_nds_sort_bytes(long long total_rows, int inst_bits)
+{
+    return sort_pairs_temp_bytes(static_cast<int>(total_rows), kKeyBits + inst_bits);
+}
+
+int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int slot, int fslot)
+{
+    hipStream_t st = L.stream;
+    hipLaunchKernelGGL(b_nds_keys, dim3(G.rows256.blocks), dim3(256), 0, st, L.insts, G.rows256.map, mode, slot,
+                       fslot, S.keys_in, S.vals_in);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out,
+                          static_cast<int>(G.total_rows), st, kKeyBits + S.inst_bits));
+    hipLaunchKernelGGL(b_nds_gather, dim3(G.npad256.blocks), dim3(256), 0, st, L.insts, G.npad256.map, mode,
+                       slot, fslot, S.vals_out);
+    MOEA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(b_nds_block_info, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map,
+                       mode, slot, fslot, S.keys_out);
+    MOEA_HIP(hipGetLastError());
+    for (int m = 1; m <= kMaxM; ++m) {
+        if (G.dom[m].blocks <= 0) continue;
+        const dim3 grid(G.dom[m].blocks), block(kDomThreads);
+        MOEA_LAUNCH_M(m, b_nds_dominance, grid, block, 0, st, L.insts, G.dom[m].map, mode, slot, fslot);
+        MOEA_HIP(hipGetLastError());
+    }
+    const size_t lds = 2 * static_cast<size_t>(G.max_W) * sizeof(uint64_t);
+    hipLaunchKernelGGL(b_nds_peel, dim3(L.count), dim3(kPeelThreads), lds, st, L.insts, mode, slot, fslot);
+    MOEA_HIP(hipGetLastError());
     return kOk;
 }
 

@@ -71,33 +71,57 @@ __global__ void merge_objectives(const float* __restrict__ pf, const float* __re
 
 // prior = {front < max_front}, last = {front == max_front}, each ascending
 // (nds_frontmask_kernel + copy_if, non_dominated_sort.cu:1580-1629);
-// gidx = [prior; last].  Single 1024-thread block.
-__global__ __launch_bounds__(kScanThreads) void split_fronts(const int32_t* __restrict__ fronts,
-                                                             int nm, int n, int32_t* __restrict__ gidx,
-                                                             int32_t* __restrict__ meta)
+// gidx = [prior; last].  Two passes over blocks of kScanThreads entries:
+// per-block counts, then each block adds the counts of the blocks before it,
+// scans its own flags and scatters.
+__global__ __launch_bounds__(kScanThreads) void front_counts(const int32_t* __restrict__ fronts, int nm,
+                                                             const int32_t* __restrict__ meta,
+                                                             int32_t* __restrict__ counts)
 {
     __shared__ int s_wave[64];
     const int mf = meta[kMaxFront];
-    const int chunk = (nm + kScanThreads - 1) / kScanThreads;
-    const int b = threadIdx.x * chunk, e = min(nm, b + chunk);
-    int cp = 0, cl = 0;
-    for (int i = b; i < e; ++i) {
-        const int fr = fronts[i];
-        cp += (fr >= 0 && fr < mf);
-        cl += (fr == mf);
-    }
+    const int i = blockIdx.x * kScanThreads + threadIdx.x;
+    const int fr = i < nm ? fronts[i] : -1;
     int tp, tl;
-    int op = block_exclusive_scan(cp, s_wave, tp);
-    int ol = block_exclusive_scan(cl, s_wave, tl);
-    for (int i = b; i < e; ++i) {
-        const int fr = fronts[i];
-        if (fr >= 0 && fr < mf) gidx[op++] = i;
-        else if (fr == mf) gidx[tp + ol++] = i;
-    }
+    block_exclusive_scan((fr >= 0 && fr < mf) ? 1 : 0, s_wave, tp);
+    block_exclusive_scan((fr == mf) ? 1 : 0, s_wave, tl);
     if (threadIdx.x == 0) {
-        meta[kNPrior] = tp;
-        meta[kNLast] = tl;
-        meta[kNRem] = n - tp;
+        counts[2 * blockIdx.x] = tp;
+        counts[2 * blockIdx.x + 1] = tl;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __restrict__ fronts, int nm,
+                                                              int n, const int32_t* __restrict__ counts,
+                                                              int nblk, int32_t* __restrict__ gidx,
+                                                              int32_t* __restrict__ meta)
+{
+    __shared__ int s_wave[64];
+    const int mf = meta[kMaxFront];
+    const int b = blockIdx.x;
+    int bp = 0, bl = 0, ap = 0, al = 0;   // before this block, all blocks
+    for (int j = threadIdx.x; j < nblk; j += kScanThreads) {
+        const int cp = counts[2 * j], cl = counts[2 * j + 1];
+        ap += cp;
+        al += cl;
+        if (j < b) { bp += cp; bl += cl; }
+    }
+    int t;
+    block_exclusive_scan(bp, s_wave, bp);
+    block_exclusive_scan(bl, s_wave, bl);
+    block_exclusive_scan(ap, s_wave, ap);
+    block_exclusive_scan(al, s_wave, al);
+    const int i = b * kScanThreads + threadIdx.x;
+    const int fr = i < nm ? fronts[i] : -1;
+    const int fp = (fr >= 0 && fr < mf) ? 1 : 0, fl = (fr == mf) ? 1 : 0;
+    const int op = block_exclusive_scan(fp, s_wave, t);
+    const int ol = block_exclusive_scan(fl, s_wave, t);
+    if (fp) gidx[bp + op] = i;
+    if (fl) gidx[ap + bl + ol] = i;
+    if (b == 0 && threadIdx.x == 0) {
+        meta[kNPrior] = ap;
+        meta[kNLast] = al;
+        meta[kNRem] = n - ap;
         meta[kIll] = -1;
     }
 }
@@ -198,27 +222,58 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
 }
 
 // Extreme points, FP64 hyperplane or fallback, sanitize (normalization.cu:453-540,
-// 572-583, 1593-1636).  One thread.
-__global__ void norm_solve(const float* __restrict__ fn, int nm, int m,
-                           const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
-                           const float* __restrict__ row_max, int32_t* __restrict__ meta,
-                           float* __restrict__ a_out)
+// 572-583, 1593-1636).  256 threads reduce the per-block candidates, then one
+// thread solves.  Extreme point j is the (value, index)-smallest candidate with
+// a finite ASF value (-1 if none) -- an order-independent reduction.
+constexpr int kSolveThreads = 256;
+__global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,
+                                                            const float* __restrict__ asf_val,
+                                                            const int32_t* __restrict__ asf_idx,
+                                                            const float* __restrict__ row_max,
+                                                            int32_t* __restrict__ meta,
+                                                            float* __restrict__ a_out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ float s_best[kSolveThreads / 64][kMaxM];
+    __shared__ int s_bidx[kSolveThreads / 64][kMaxM];
+    __shared__ float s_rmax[kSolveThreads / 64][kMaxM];
     if (meta[kNRem] <= 0) return;
     const int n_nds = meta[kNPrior] + meta[kNLast];
     const int nb = (n_nds + 255) / 256;
-    int ext[kMaxM];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int j = 0; j < m; ++j) {
-        float best = INFINITY;
+        float best = INFINITY, rm = -INFINITY;
         int bi = -1;
-        for (int b = 0; b < nb; ++b) {
+        for (int b = threadIdx.x; b < nb; b += kSolveThreads) {
             float v = asf_val[b * m + j];
             const int idx = asf_idx[b * m + j];
             v = (v != v) ? INFINITY : v;
-            if (v < best || (v == best && idx < bi)) { best = v; bi = idx; }
+            if (idx >= 0 && v < INFINITY && (bi < 0 || v < best || (v == best && idx < bi))) { best = v; bi = idx; }
+            rm = fmaxf(rm, row_max[b * m + j]);
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float ov = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (oi >= 0 && ov < INFINITY && (bi < 0 || ov < best || (ov == best && oi < bi))) { best = ov; bi = oi; }
+            rm = fmaxf(rm, __shfl_xor(rm, o));
+        }
+        if (lane == 0) { s_best[wave][j] = best; s_bidx[wave][j] = bi; s_rmax[wave][j] = rm; }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int ext[kMaxM];
+    float rmax[kMaxM];
+    for (int j = 0; j < m; ++j) {
+        float best = INFINITY, rm = -INFINITY;
+        int bi = -1;
+        for (int w = 0; w < kSolveThreads / 64; ++w) {
+            const float ov = s_best[w][j];
+            const int oi = s_bidx[w][j];
+            if (oi >= 0 && ov < INFINITY && (bi < 0 || ov < best || (ov == best && oi < bi))) { best = ov; bi = oi; }
+            rm = fmaxf(rm, s_rmax[w][j]);
         }
         ext[j] = bi;
+        rmax[j] = rm;
     }
     double E[kMaxM * kMaxM], a[kMaxM];
     for (int i = 0; i < m; ++i)
@@ -228,11 +283,8 @@ __global__ void norm_solve(const float* __restrict__ fn, int nm, int m,
                                : static_cast<double>(INFINITY);
     const int ill = hyperplane_fp64(E, m, a);
     if (ill) {
-        for (int k = 0; k < m; ++k) {
-            float mx = -INFINITY;
-            for (int b = 0; b < nb; ++b) mx = fmaxf(mx, row_max[b * m + k]);
-            a[k] = (mx > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(mx, 1e-6f)) : 1.0;
-        }
+        for (int k = 0; k < m; ++k)
+            a[k] = (rmax[k] > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(rmax[k], 1e-6f)) : 1.0;
     }
     for (int k = 0; k < m; ++k) {
         float af = static_cast<float>(a[k]);
@@ -329,19 +381,23 @@ __global__ void rho_hist(const int32_t* __restrict__ ref, const int32_t* __restr
 }
 
 // Phase-2 keys: ref << 32 | bits(dist), value = last-front local index
-// (build_lf_asrpts_asdist_key_kernel, niching.cu:68-92).  Slots past N_last
-// get the all-ones key; a stable sort keeps them after every real entry.
+// (build_lf_asrpts_asdist_key_kernel, niching.cu:68-92).  The high word is
+// the direction, with an unassociated row (ref -1, whose u32 sorts after every
+// direction) written as k and slots past N_last as k + 1 with an all-ones low
+// word: the same order in 32 + bitlen(k + 1) key bits.
+__device__ inline uint32_t ref_word(int r, int k) { return r < 0 ? static_cast<uint32_t>(k) : static_cast<uint32_t>(r); }
+
 __global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __restrict__ dist,
-                            const int32_t* __restrict__ meta, int nm, uint64_t* __restrict__ key,
+                            const int32_t* __restrict__ meta, int nm, int k, uint64_t* __restrict__ key,
                             int32_t* __restrict__ val)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm) return;
     const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
     const int n_prior = meta[kNPrior];
-    uint64_t kk = ~0ull;
+    uint64_t kk = (static_cast<uint64_t>(k + 1) << 32) | 0xFFFFFFFFull;
     if (t < n_last) {
-        const uint32_t r = static_cast<uint32_t>(ref[n_prior + t]);
+        const uint32_t r = ref_word(ref[n_prior + t], k);
         kk = (static_cast<uint64_t>(r) << 32) | __float_as_uint(dist[n_prior + t]);
     }
     key[t] = kk;
@@ -377,7 +433,7 @@ __global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* _
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm) return;
     const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
-    uint64_t kk = ~0ull;
+    uint64_t kk = (static_cast<uint64_t>(k + 1) << 32) | 0xFFFFFFFFull;
     if (t < n_last) {
         const uint32_t r = static_cast<uint32_t>(key1s[t] >> 32);
         const int local = t - segment_head(key1s, t);
@@ -390,6 +446,8 @@ __global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* _
 }
 
 // Phase-5 keys: (rho + rank) << 32 | draw_b (compute_niching_keys_kernel, niching.cu:157-178).
+// rho + rank < N_prior + N_last <= nm, so slots past N_last take the high word
+// nm: 32 + bitlen(nm) key bits.
 __global__ void niche_keys3(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
                             const int32_t* __restrict__ rho, int k, int nm,
                             const uint32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
@@ -398,7 +456,7 @@ __global__ void niche_keys3(const uint64_t* __restrict__ key2s, const int32_t* _
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm) return;
     const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
-    uint64_t kk = ~0ull;
+    uint64_t kk = (static_cast<uint64_t>(nm) << 32) | 0xFFFFFFFFull;
     if (t < n_last) {
         const int ri = static_cast<int>(static_cast<uint32_t>(key2s[t] >> 32));
         const int rank = t - segment_head(key2s, t);
@@ -430,21 +488,39 @@ __global__ void mark_selected(const int32_t* __restrict__ val3s, const int32_t* 
 }
 
 // Chosen indices in ascending merged order (thrust::sort of the selection,
-// nsga3_population_update.cu:269).  Single 1024-thread block.
-__global__ __launch_bounds__(kScanThreads) void compact_flags(const uint8_t* __restrict__ flag, int nm,
-                                                              int n, int32_t* __restrict__ chosen,
-                                                              int32_t* __restrict__ meta)
+// nsga3_population_update.cu:269): the same two passes as front_counts /
+// front_scatter over the survivor flags.
+__global__ __launch_bounds__(kScanThreads) void flag_counts(const uint8_t* __restrict__ flag, int nm,
+                                                            int32_t* __restrict__ counts)
 {
     __shared__ int s_wave[64];
-    const int chunk = (nm + kScanThreads - 1) / kScanThreads;
-    const int b = threadIdx.x * chunk, e = min(nm, b + chunk);
-    int c = 0;
-    for (int i = b; i < e; ++i) c += flag[i];
+    const int i = blockIdx.x * kScanThreads + threadIdx.x;
     int tot;
-    int o = block_exclusive_scan(c, s_wave, tot);
-    for (int i = b; i < e; ++i)
-        if (flag[i] && o < n) chosen[o++] = i;
-    if (threadIdx.x == 0) meta[5] = tot;
+    block_exclusive_scan(i < nm ? static_cast<int>(flag[i]) : 0, s_wave, tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void flag_scatter(const uint8_t* __restrict__ flag, int nm,
+                                                             int n, const int32_t* __restrict__ counts,
+                                                             int nblk, int32_t* __restrict__ chosen,
+                                                             int32_t* __restrict__ meta)
+{
+    __shared__ int s_wave[64];
+    const int b = blockIdx.x;
+    int before = 0, all = 0;
+    for (int j = threadIdx.x; j < nblk; j += kScanThreads) {
+        const int c = counts[j];
+        all += c;
+        if (j < b) before += c;
+    }
+    block_exclusive_scan(before, s_wave, before);
+    block_exclusive_scan(all, s_wave, all);
+    const int i = b * kScanThreads + threadIdx.x;
+    const int f = (i < nm) ? static_cast<int>(flag[i]) : 0;
+    int t;
+    const int o = before + block_exclusive_scan(f, s_wave, t);
+    if (f && o < n) chosen[o] = i;
+    if (b == 0 && threadIdx.x == 0) meta[5] = all;
 }
 
 // Survivor gather (pop_update_*_kernel + fit_and_fronts_update_kernel,
@@ -491,7 +567,7 @@ struct Layout {
     float* asf_val; int32_t* asf_idx; float* row_max; float* a;
     float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho;
     uint64_t* k0; uint64_t* k1; int32_t* v0; int32_t* v1; void* sort_tmp; size_t sort_bytes;
-    uint8_t* flag; int32_t* chosen; void* nds_ws; size_t nds_bytes;
+    uint8_t* flag; int32_t* chosen; int32_t* counts; void* nds_ws; size_t nds_bytes;
 };
 
 Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
@@ -523,6 +599,7 @@ Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
     L.sort_tmp = c.take_bytes(L.sort_bytes);
     L.flag = c.take<uint8_t>(nm);
     L.chosen = c.take<int32_t>(n);
+    L.counts = c.take<int32_t>(2 * static_cast<size_t>(ceil_div(nm, kScanThreads)));
     L.nds_bytes = nds_workspace_bytes(nm);
     L.nds_ws = c.take_bytes(L.nds_bytes);
     (void)ws; (void)cap;
@@ -573,13 +650,16 @@ int nsga3_select(const moea_nsga3_select_args& A)
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(nds_run(L.fm, nm, L.cvm, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
     stage_begin(kStSelectPrep, st);
-    hipLaunchKernelGGL(split_fronts, dim3(1), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.gidx, L.meta);
+    const int nsb = ceil_div(nm, kScanThreads);
+    hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);
+    hipLaunchKernelGGL(front_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.counts, nsb,
+                       L.gidx, L.meta);
     hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, st, L.fm, nm, n, n, m, A.ideal, 0);
     const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
     MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, A.ideal, L.meta, L.fn,
                   L.asf_val, L.asf_idx, L.row_max);
-    hipLaunchKernelGGL(norm_solve, dim3(1), dim3(64), 0, st, L.fn, nm, m, L.asf_val, L.asf_idx,
-                       L.row_max, L.meta, L.a);
+    hipLaunchKernelGGL(norm_solve, dim3(1), dim3(kSolveThreads), 0, st, L.fn, nm, m, L.asf_val,
+                       L.asf_idx, L.row_max, L.meta, L.a);
     stage_end(kStSelectPrep, st);
     stage_begin(kStAssociation, st);
     MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
@@ -590,21 +670,25 @@ int nsga3_select(const moea_nsga3_select_args& A)
     stage_begin(kStNiching, st);
     hipLaunchKernelGGL(rho_clear, dim3(ceil_div(k, 256)), dim3(256), 0, st, L.rho, k);
     hipLaunchKernelGGL(rho_hist, dim3(nb), dim3(256), 0, st, L.ref, L.meta, k, L.rho);
-    hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, L.k0, L.v0);
+    const int bits_ref = 32 + (32 - __builtin_clz(static_cast<unsigned>(k + 1)));
+    const int bits_rank = 32 + (32 - __builtin_clz(static_cast<unsigned>(nm)));
+    hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
     MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st));
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_ref));
     hipLaunchKernelGGL(niche_keys2, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
                        dr ? dr->niche_a : nullptr, A.seed, gen, L.k0);
     MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v1, L.v0, nm, st));
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v1, L.v0, nm, st, bits_ref));
     hipLaunchKernelGGL(niche_keys3, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
                        dr ? dr->niche_b : nullptr, A.seed, gen, L.k0);
     MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st));
+    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_rank));
     hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
     hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
                        L.flag);
-    hipLaunchKernelGGL(compact_flags, dim3(1), dim3(kScanThreads), 0, st, L.flag, nm, n, L.chosen, L.meta);
+    hipLaunchKernelGGL(flag_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, L.counts);
+    hipLaunchKernelGGL(flag_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, n, L.counts, nsb,
+                       L.chosen, L.meta);
     stage_end(kStNiching, st);
     stage_begin(kStGather, st);
     hipLaunchKernelGGL(gather_next, dim3(ceil_div(n, 4)), dim3(256), 0, st, L.chosen, n, d, m,

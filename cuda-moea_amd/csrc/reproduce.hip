@@ -6,6 +6,7 @@
 // reference materializes every draw array with separate cuRAND kernels; here
 // draws are either injected or generated in registers (Philox4x32-10), and the
 // crossover and mutation of a pair happen in one pass over its two rows.
+#include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
 
@@ -13,15 +14,11 @@ namespace moea {
 
 namespace {
 
-__global__ void mating_kernel(int n, int active_host, const int32_t* __restrict__ active_dev, int mating,
-                              const int32_t* __restrict__ rank,
-                              const float* __restrict__ cv, const int32_t* __restrict__ inj_a,
-                              const int32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
-                              int32_t* __restrict__ winners)
+// Mating winner of slot t (candidates from [0, active)).
+__device__ inline int mating_one(int t, int n, int active, int mating, const int32_t* __restrict__ rank,
+                                 const float* __restrict__ cv, const int32_t* __restrict__ inj_a,
+                                 const int32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int active = active_dev ? active_dev[0] : active_host;
     active = active < 1 ? 1 : (active > n ? n : active);
     int a, b;
     if (inj_a) {
@@ -33,15 +30,25 @@ __global__ void mating_kernel(int n, int active_host, const int32_t* __restrict_
         a = static_cast<int>(r.x % static_cast<uint32_t>(active));
         b = static_cast<int>(r.y % static_cast<uint32_t>(active));
     }
-    if (mating == MOEA_MATING_RANDOM) { winners[t] = a; return; }
+    if (mating == MOEA_MATING_RANDOM) return a;
     // tournament_winner_kernel (mating_selection.cu:11-35)
     const float ca = cv[a], cb = cv[b];
-    int w;
-    if (ca == 0.0f && cb != 0.0f) w = a;
-    else if (ca != 0.0f && cb == 0.0f) w = b;
-    else if (ca != 0.0f && cb != 0.0f) w = (ca <= cb) ? a : b;
-    else w = (rank[a] <= rank[b]) ? a : b;
-    winners[t] = w;
+    if (ca == 0.0f && cb != 0.0f) return a;
+    if (ca != 0.0f && cb == 0.0f) return b;
+    if (ca != 0.0f && cb != 0.0f) return (ca <= cb) ? a : b;
+    return (rank[a] <= rank[b]) ? a : b;
+}
+
+__global__ void mating_kernel(int n, int active_host, const int32_t* __restrict__ active_dev, int mating,
+                              const int32_t* __restrict__ rank,
+                              const float* __restrict__ cv, const int32_t* __restrict__ inj_a,
+                              const int32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
+                              int32_t* __restrict__ winners)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int active = active_dev ? active_dev[0] : active_host;
+    winners[t] = mating_one(t, n, active, mating, rank, cv, inj_a, inj_b, seed, gen);
 }
 
 // SBX of one (pair, variable) element.  In midpoint +/- delta both operands are
@@ -95,11 +102,10 @@ struct RepArgs {
     float* ox;
 };
 
-// One thread per (pair, variable); the odd tail row (n odd) copies its parent
-// (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
-__global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
+// Element e of the (pair, variable) space; the odd tail row (n odd) copies its
+// parent (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
+__device__ inline void sbx_pm_elem(const RepArgs& R, long long e)
 {
-    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long total_pair = static_cast<long long>(R.pairs) * R.d;
     const bool tail = (R.n & 1) != 0;
     if (e >= total_pair + (tail ? R.d : 0)) return;
@@ -155,22 +161,35 @@ __global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
     }
 }
 
+// One thread per (pair, variable).
+__global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
+{
+    sbx_pm_elem(R, static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
 // Population::initialize_random (core/population.cu:97-115):
 // lower + u * (upper - lower), contracted to one fma by nvcc.
-__global__ void init_kernel(float* __restrict__ x, long long count, int d,
-                            const float* __restrict__ bounds, uint64_t seed)
+__device__ inline void init_elem(float* __restrict__ x, long long e, int d, const float* __restrict__ bounds,
+                                 uint64_t seed)
 {
-    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= count) return;
     const int j = static_cast<int>(e % d);
     const U4 r = draw4(seed, static_cast<uint64_t>(e), 0u, kStInit);
     const float lo = bounds[2 * j], hi = bounds[2 * j + 1];
     x[e] = fmaf(u01(r.x), hi - lo, lo);
 }
 
-}  // namespace
+__global__ void init_kernel(float* __restrict__ x, long long count, int d,
+                            const float* __restrict__ bounds, uint64_t seed)
+{
+    const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < count) init_elem(x, e, d, bounds, seed);
+}
 
-float annealed(float begin, float end, int generation, int total)
+// ---------------------------------------------------------------- batched --
+// The same element work per instance of a batch (tournament mating, SBX and
+// polynomial mutation with the instance's annealed etas, in-kernel draws).
+
+__host__ __device__ inline float annealed_hd(float begin, float end, int generation, int total)
 {
     float progress = 1.0f;
     if (total > 0) {
@@ -179,6 +198,51 @@ float annealed(float begin, float end, int generation, int total)
         if (progress > 1.0f) progress = 1.0f;
     }
     return begin + (end - begin) * progress;
+}
+
+__global__ void b_init_kernel(const BInst* __restrict__ insts, BMap mp, int cur)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const long long e = static_cast<long long>(local) * blockDim.x + threadIdx.x;
+    if (e < static_cast<long long>(I.n) * I.d) init_elem(I.x[cur], e, I.d, I.bounds, I.seed);
+}
+
+__global__ void b_mating_kernel(const BInst* __restrict__ insts, BMap mp, int cur, int fcur, uint32_t gen)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (t >= I.n) return;
+    I.winners[t] = mating_one(t, I.n, I.n, MOEA_MATING_TOURNAMENT, I.fronts[fcur], I.cv[cur], nullptr,
+                              nullptr, I.seed, gen);
+}
+
+__global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__ insts, BMap mp, int cur,
+                                                       int off, int gen)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    RepArgs R{};
+    R.n = I.n; R.d = I.d; R.pairs = I.n / 2;
+    R.px = I.x[cur]; R.winners = I.winners; R.bounds = I.bounds;
+    R.eta_c = annealed_hd(I.sbx_eta0, I.sbx_eta1, gen, I.max_gen);
+    R.prob_c = I.sbx_prob; R.copy_c = I.sbx_copy;
+    R.mutate = 1;
+    R.eta_m = annealed_hd(I.pm_eta0, I.pm_eta1, gen, I.max_gen);
+    R.rate_m = I.pm_prob / static_cast<float>(I.d);
+    R.seed = I.seed; R.gen = static_cast<uint32_t>(gen); R.ox = I.x[off];
+    sbx_pm_elem(R, static_cast<long long>(local) * blockDim.x + threadIdx.x);
+}
+
+}  // namespace
+
+float annealed(float begin, float end, int generation, int total)
+{
+    return annealed_hd(begin, end, generation, total);
 }
 
 int reproduce(const moea_reproduce_args& A)
@@ -225,6 +289,25 @@ int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, 
     const long long count = static_cast<long long>(n) * d;
     hipLaunchKernelGGL(init_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, stream,
                        x, count, d, bounds, seed);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks, int cur)
+{
+    hipLaunchKernelGGL(b_init_kernel, dim3(nblocks), dim3(256), 0, L.stream, L.insts, mp, cur);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int batch_reproduce(const BatchLaunch& L, const BMap& rows256, int nrows_blocks, const BMap& sbx256,
+                    int nsbx_blocks, int cur, int off, int fcur, int generation)
+{
+    hipLaunchKernelGGL(b_mating_kernel, dim3(nrows_blocks), dim3(256), 0, L.stream, L.insts, rows256, cur,
+                       fcur, static_cast<uint32_t>(generation));
+    MOEA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(b_sbx_pm_kernel, dim3(nsbx_blocks), dim3(256), 0, L.stream, L.insts, sbx256, cur,
+                       off, generation);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
