@@ -61,12 +61,79 @@ def cpu_baseline(threads: int) -> dict:
                       f"{ms / 1000:.1f} s"}
 
 
+def batched_config(generations: int, count: int, plain_sample: int) -> dict:
+    """Config 3: `count` independent NSGA-III runs (N log-uniform in [64, 4096],
+    even; M in {3, 5, 8} with Das-Dennis H = 12 / 6 / 3; D log-uniform in
+    [10, 1000]; per-run seeded quadratic centers) advanced together by one
+    BatchNSGA3, against the plain one-run-at-a-time GPU path (cuda_moea.NSGA3)
+    timed on the first `plain_sample` runs.  Runs/s count whole runs of
+    `generations` generations; the initial evaluation is outside the timing,
+    as in Result.total_ms."""
+    import math
+    import numpy as np
+    import torch
+    import cuda_moea as cm
+    rng = np.random.default_rng(2024)
+    parts = {3: 12, 5: 6, 8: 3}
+    specs = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(64), math.log(4096)))))
+        n += n & 1
+        m = int(rng.choice([3, 5, 8]))
+        d = int(round(math.exp(rng.uniform(math.log(10), math.log(1000)))))
+        specs.append((n, m, d))
+
+    def make(i):
+        n, m, d = specs[i]
+        return cm.NSGA3(population_size=n, max_generations=generations,
+                        problem=cm.QuadraticCenters(d, m, seed=10_000 + i), seed=i,
+                        reference_directions=cm.DasDennisDirections(partitions=parts[m]),
+                        print_progress=False)
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    batch = cm.BatchNSGA3([make(i) for i in range(count)])
+    batch.initialize()
+    stream.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    batch.step(generations)
+    t1.record(stream)
+    t1.synchronize()
+    batched_s = t0.elapsed_time(t1) / 1e3
+    plain_s = 0.0
+    for i in range(plain_sample):
+        alg = make(i)
+        alg.initialize()
+        stream.synchronize()
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a0.record(stream)
+        alg._call(cm.algorithm.lib.moea_run_step, generations)
+        a1.record(stream)
+        a1.synchronize()
+        plain_s += a0.elapsed_time(a1) / 1e3
+    batched_rps = count / batched_s
+    plain_rps = plain_sample / plain_s
+    return {"workload": "config 3: NSGA-III, N log-uniform [64,4096], M in {3,5,8}, D log-uniform [10,1000], "
+                        "quadratic-center",
+            "instances": count, "generations": generations,
+            "batched_runs_per_s": round(batched_rps, 2), "batched_s": round(batched_s, 3),
+            "ms_per_batched_generation": round(batched_s / generations * 1e3, 4),
+            "plain_runs_per_s": round(plain_rps, 3), "plain_sample_runs": plain_sample,
+            "speedup_vs_plain": round(batched_rps / plain_rps, 2),
+            "total_individuals": sum(s[0] for s in specs)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-instances", type=int, default=256)
+    ap.add_argument("--batched-generations", type=int, default=300)
+    ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -117,6 +184,10 @@ def main() -> int:
     alg_bytes = 64 * 8 * words * (words + 1) // 2 + nm * M * 4
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1]}
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_config(args.batched_generations, args.batched_instances, args.plain_sample)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_threads)
@@ -147,6 +218,7 @@ def main() -> int:
                      "reference_compares_per_launch": M * nm * (nm - 1),
                      "frac_reference_work": round(M * nm * (nm - 1) / dom_avg_s / VALU_PEAK_CMP, 4)},
         "stage_ms_per_generation": stage_ms,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

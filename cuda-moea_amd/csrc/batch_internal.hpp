@@ -43,6 +43,9 @@ struct BInst {
     int32_t* meta;                // (8) max_front, n_prior, n_last, ill, n_rem, chosen count
     int32_t* gidx;                // (nm)
     float* fn;                    // (m, nm)
+    float* asf_val;               // (ceil(nm / 256), m) extreme-point candidates per block
+    int32_t* asf_idx;             // (ceil(nm / 256), m)
+    float* row_max;               // (ceil(nm / 256), m)
     float* a;                     // (kMaxM)
     float* part_d2;               // (nchunks, nm)
     int32_t* part_idx;            // (nchunks, nm)
@@ -108,6 +111,18 @@ struct BNdsGrids {
     long long total_rows;         // sum of rows (sort length)
 };
 
+// Grids of the batched selection stages.
+struct BSelGrids {
+    BGrid merged256;              // 2n entries per instance, 256 per block
+    BGrid rows4;                  // n rows per instance, one wave per row
+    BGrid objectives;             // one 1024-thread block per objective per instance
+    BGrid norm_prep[kMaxM + 1];   // ceil(2n / 256) blocks, instances with m objectives
+    BGrid assoc[kMaxM + 1];       // ceil(2n / 256) x association chunks, instances with m objectives
+    int ref_bits;                 // bits of a direction index (with the two sentinels)
+    int rank_bits;                // bits of rho + rank (with the sentinel)
+    long long total_merged;       // sum of 2n (sort length)
+};
+
 struct BNdsScratch {
     uint64_t* keys_in;
     uint64_t* keys_out;
@@ -117,5 +132,17 @@ struct BNdsScratch {
     size_t sort_bytes;
     int inst_bits;                // bits of the instance index above the 34-bit NDS key
 };
+
+// Batched stage launchers (each defined next to its single-population kernels).
+int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks, int cur);       // reproduce.hip
+int batch_reproduce(const BatchLaunch& L, const BMap& rows256, int nrows_blocks, const BMap& sbx256,
+                    int nsbx_blocks, int cur, int off, int fcur, int generation);            // reproduce.hip
+int batch_eval_quadratic(const BatchLaunch& L, int m, const BMap& rows4, int nblocks, int slot);  // evaluate.hip
+size_t batch_nds_sort_bytes(long long total_rows, int inst_bits);                             // nds.hip
+int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int slot,
+              int fslot);                                                                     // nds.hip
+int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode, int slot);                // nsga3.hip
+int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S, int cur,
+                 int off, int nxt, int fnext, int generation);                                // nsga3.hip
 
 }  // namespace moea

@@ -9,6 +9,7 @@
 //
 // Merged index space: s in [0, n) is parent s, s in [n, 2n) is offspring s-n.
 // Variables are never merged; the survivor gather reads them in place.
+#include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
 
@@ -127,11 +128,10 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
 }
 
 // Running ideal point: ideal = fminf(ideal, min over offspring) (ideal_point.cu:25-165).
-__global__ void ideal_update(const float* __restrict__ f, int ldf, int col0, int count, int m,
-                             float* __restrict__ ideal, int init)
+__device__ inline void ideal_body(const float* __restrict__ f, int ldf, int col0, int count, int k,
+                                  float* __restrict__ ideal, int init)
 {
     __shared__ float s_min[1024 / 64];
-    const int k = blockIdx.x;
     float z = INFINITY;
     for (int i = threadIdx.x; i < count; i += blockDim.x)
         z = fminf(z, f[static_cast<size_t>(k) * ldf + col0 + i]);
@@ -146,25 +146,28 @@ __global__ void ideal_update(const float* __restrict__ f, int ldf, int col0, int
     }
 }
 
+__global__ void ideal_update(const float* __restrict__ f, int ldf, int col0, int count, int m,
+                             float* __restrict__ ideal, int init)
+{
+    ideal_body(f, ldf, col0, count, static_cast<int>(blockIdx.x), ideal, init);
+}
+
 // Translate, clamp and reduce the extreme-point / row-max candidates per block
 // (translated_ndsfit_kernel, compute_asf_legacy_kernel, find_intercepts_legacy_kernel:
 // normalization.cu:370-444, 647-729, 802-856).
 template <int M>
-__global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, int nm,
-                                                 const int32_t* __restrict__ gidx,
-                                                 const float* __restrict__ ideal,
-                                                 const int32_t* __restrict__ meta,
-                                                 float* __restrict__ fn, float* __restrict__ asf_val,
-                                                 int32_t* __restrict__ asf_idx,
-                                                 float* __restrict__ row_max)
+__device__ inline void norm_prep_body(const float* __restrict__ fm, int nm, const int32_t* __restrict__ gidx,
+                                      const float* __restrict__ ideal, const int32_t* __restrict__ meta,
+                                      float* __restrict__ fn, float* __restrict__ asf_val,
+                                      int32_t* __restrict__ asf_idx, float* __restrict__ row_max, int bx)
 {
     __shared__ float s_val[4][M];
     __shared__ int s_idx[4][M];
     __shared__ float s_max[4][M];
     const int n_nds = meta[kNPrior] + meta[kNLast];
     if (meta[kNRem] <= 0) return;
-    if (blockIdx.x * 256 >= n_nds) return;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (bx * 256 >= n_nds) return;
+    const int i = bx * 256 + threadIdx.x;
     const bool valid = i < n_nds;
     float v[M];
     if (valid) {
@@ -215,10 +218,22 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
             if (ov < best || (ov == best && oi < bi)) { best = ov; bi = oi; }
             rm = fmaxf(rm, s_max[w][j]);
         }
-        asf_val[blockIdx.x * M + j] = best;
-        asf_idx[blockIdx.x * M + j] = bi;
-        row_max[blockIdx.x * M + j] = rm;
+        asf_val[bx * M + j] = best;
+        asf_idx[bx * M + j] = bi;
+        row_max[bx * M + j] = rm;
     }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, int nm,
+                                                 const int32_t* __restrict__ gidx,
+                                                 const float* __restrict__ ideal,
+                                                 const int32_t* __restrict__ meta,
+                                                 float* __restrict__ fn, float* __restrict__ asf_val,
+                                                 int32_t* __restrict__ asf_idx,
+                                                 float* __restrict__ row_max)
+{
+    norm_prep_body<M>(fm, nm, gidx, ideal, meta, fn, asf_val, asf_idx, row_max, static_cast<int>(blockIdx.x));
 }
 
 // Extreme points, FP64 hyperplane or fallback, sanitize (normalization.cu:453-540,
@@ -226,12 +241,10 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
 // thread solves.  Extreme point j is the (value, index)-smallest candidate with
 // a finite ASF value (-1 if none) -- an order-independent reduction.
 constexpr int kSolveThreads = 256;
-__global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,
-                                                            const float* __restrict__ asf_val,
-                                                            const int32_t* __restrict__ asf_idx,
-                                                            const float* __restrict__ row_max,
-                                                            int32_t* __restrict__ meta,
-                                                            float* __restrict__ a_out)
+__device__ inline void norm_solve_body(const float* __restrict__ fn, int nm, int m,
+                                       const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
+                                       const float* __restrict__ row_max, int32_t* __restrict__ meta,
+                                       float* __restrict__ a_out)
 {
     __shared__ float s_best[kSolveThreads / 64][kMaxM];
     __shared__ int s_bidx[kSolveThreads / 64][kMaxM];
@@ -294,6 +307,16 @@ __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restr
     meta[kIll] = ill;
 }
 
+__global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,
+                                                            const float* __restrict__ asf_val,
+                                                            const int32_t* __restrict__ asf_idx,
+                                                            const float* __restrict__ row_max,
+                                                            int32_t* __restrict__ meta,
+                                                            float* __restrict__ a_out)
+{
+    norm_solve_body(fn, nm, m, asf_val, asf_idx, row_max, meta, a_out);
+}
+
 // Perpendicular-distance association, one individual per thread against a
 // chunk of directions staged in LDS (association.cu:85-188): dot by an fmaf
 // chain, residual x - dot*w as one fma, d2 by an fmaf chain, first minimum
@@ -301,25 +324,23 @@ __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restr
 constexpr int kAssocChunk = 1024;
 
 template <int M>
-__global__ __launch_bounds__(256) void associate_partial(const float* __restrict__ fn, int nm,
-                                                         const float* __restrict__ a,
-                                                         const float* __restrict__ w, int k,
-                                                         const int32_t* __restrict__ meta,
-                                                         float* __restrict__ part_d2,
-                                                         int32_t* __restrict__ part_idx)
+__device__ inline void assoc_partial_body(const float* __restrict__ fn, int nm, const float* __restrict__ a,
+                                          const float* __restrict__ w, int k, const int32_t* __restrict__ meta,
+                                          float* __restrict__ part_d2, int32_t* __restrict__ part_idx, int bx,
+                                          int by)
 {
     __shared__ float s_w[M][kAssocChunk];
     if (meta[kNRem] <= 0) return;
     const int n_nds = meta[kNPrior] + meta[kNLast];
-    if (blockIdx.x * 256 >= n_nds) return;
-    const int j0 = blockIdx.y * kAssocChunk;
+    if (bx * 256 >= n_nds) return;
+    const int j0 = by * kAssocChunk;
     const int jn = min(kAssocChunk, k - j0);
     for (int e = threadIdx.x; e < M * kAssocChunk; e += 256) {
         const int c = e / kAssocChunk, j = e % kAssocChunk;
         s_w[c][j] = (j < jn) ? w[static_cast<size_t>(c) * k + j0 + j] : 0.0f;
     }
     __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = bx * 256 + threadIdx.x;
     if (i >= n_nds) return;
     float x[M];
 #pragma unroll
@@ -338,18 +359,29 @@ __global__ __launch_bounds__(256) void associate_partial(const float* __restrict
         }
         if (d2 < best) { best = d2; bj = j0 + j; }
     }
-    part_d2[static_cast<size_t>(blockIdx.y) * nm + i] = best;
-    part_idx[static_cast<size_t>(blockIdx.y) * nm + i] = bj;
+    part_d2[static_cast<size_t>(by) * nm + i] = best;
+    part_idx[static_cast<size_t>(by) * nm + i] = bj;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void associate_partial(const float* __restrict__ fn, int nm,
+                                                         const float* __restrict__ a,
+                                                         const float* __restrict__ w, int k,
+                                                         const int32_t* __restrict__ meta,
+                                                         float* __restrict__ part_d2,
+                                                         int32_t* __restrict__ part_idx)
+{
+    assoc_partial_body<M>(fn, nm, a, w, k, meta, part_d2, part_idx, static_cast<int>(blockIdx.x),
+                          static_cast<int>(blockIdx.y));
 }
 
 // Reduce the chunk partials in chunk order (association_phase2_kernel, :397-447).
-__global__ void associate_final(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
-                                int nchunks, int nm, const int32_t* __restrict__ meta,
-                                float* __restrict__ dist, int32_t* __restrict__ ref)
+__device__ inline void assoc_final_body(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
+                                        int nchunks, int nm, const int32_t* __restrict__ meta,
+                                        float* __restrict__ dist, int32_t* __restrict__ ref, int i)
 {
     if (meta[kNRem] <= 0) return;
     const int n_nds = meta[kNPrior] + meta[kNLast];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_nds) return;
     float best = INFINITY;
     int bj = -1;
@@ -360,6 +392,14 @@ __global__ void associate_final(const float* __restrict__ part_d2, const int32_t
     }
     dist[i] = (best != INFINITY) ? sqrtf(best) : INFINITY;
     ref[i] = bj;
+}
+
+__global__ void associate_final(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
+                                int nchunks, int nm, const int32_t* __restrict__ meta,
+                                float* __restrict__ dist, int32_t* __restrict__ ref)
+{
+    assoc_final_body(part_d2, part_idx, nchunks, nm, meta, dist, ref,
+                     static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // rho_prior histogram (prior_histogram_kernel, niching.cu:21-33).  Directions
@@ -387,12 +427,11 @@ __global__ void rho_hist(const int32_t* __restrict__ ref, const int32_t* __restr
 // word: the same order in 32 + bitlen(k + 1) key bits.
 __device__ inline uint32_t ref_word(int r, int k) { return r < 0 ? static_cast<uint32_t>(k) : static_cast<uint32_t>(r); }
 
-__global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __restrict__ dist,
-                            const int32_t* __restrict__ meta, int nm, int k, uint64_t* __restrict__ key,
-                            int32_t* __restrict__ val)
+// hi_base: bits above the key (a batch index); key arrays are the instance's view.
+__device__ inline void niche_keys1_body(const int32_t* __restrict__ ref, const float* __restrict__ dist,
+                                        const int32_t* __restrict__ meta, int k, uint64_t hi_base,
+                                        uint64_t* __restrict__ key, int32_t* __restrict__ val, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nm) return;
     const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
     const int n_prior = meta[kNPrior];
     uint64_t kk = (static_cast<uint64_t>(k + 1) << 32) | 0xFFFFFFFFull;
@@ -400,8 +439,16 @@ __global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __rest
         const uint32_t r = ref_word(ref[n_prior + t], k);
         kk = (static_cast<uint64_t>(r) << 32) | __float_as_uint(dist[n_prior + t]);
     }
-    key[t] = kk;
+    key[t] = hi_base | kk;
     val[t] = t;
+}
+
+__global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __restrict__ dist,
+                            const int32_t* __restrict__ meta, int nm, int k, uint64_t* __restrict__ key,
+                            int32_t* __restrict__ val)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nm) niche_keys1_body(ref, dist, meta, k, 0ull, key, val, t);
 }
 
 // First position of the run of equal high words containing t (keys sorted).
@@ -425,46 +472,61 @@ __device__ inline uint32_t niche_draw(const uint32_t* injected, int which, int t
 
 // Phase-4 rank keys: ref << 32 | (rho == 0 && local == 0 ? 0 : draw_a | 1)
 // (compute_rank_keys_kernel, niching.cu:114-139).
-__global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
-                            const int32_t* __restrict__ rho, int k, int nm,
-                            const uint32_t* __restrict__ inj_a, uint64_t seed, uint32_t gen,
-                            uint64_t* __restrict__ key2)
+// word_mask: the direction field of the high word (below any batch index).
+__device__ inline void niche_keys2_body(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
+                                        const int32_t* __restrict__ rho, int k, uint32_t word_mask,
+                                        uint64_t hi_base, const uint32_t* __restrict__ inj_a, uint64_t seed,
+                                        uint32_t gen, uint64_t* __restrict__ key2, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nm) return;
     const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
     uint64_t kk = (static_cast<uint64_t>(k + 1) << 32) | 0xFFFFFFFFull;
     if (t < n_last) {
-        const uint32_t r = static_cast<uint32_t>(key1s[t] >> 32);
+        const uint32_t r = static_cast<uint32_t>(key1s[t] >> 32) & word_mask;
         const int local = t - segment_head(key1s, t);
         const int ri = static_cast<int>(r);
         const int rh = (ri >= 0 && ri < k) ? rho[ri] : 0;
         const uint32_t low = (rh == 0 && local == 0) ? 0u : (niche_draw(inj_a, 0, t, seed, gen) | 1u);
         kk = (static_cast<uint64_t>(r) << 32) | low;
     }
-    key2[t] = kk;
+    key2[t] = hi_base | kk;
+}
+
+__global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
+                            const int32_t* __restrict__ rho, int k, int nm,
+                            const uint32_t* __restrict__ inj_a, uint64_t seed, uint32_t gen,
+                            uint64_t* __restrict__ key2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nm) niche_keys2_body(key1s, meta, rho, k, 0xFFFFFFFFu, 0ull, inj_a, seed, gen, key2, t);
 }
 
 // Phase-5 keys: (rho + rank) << 32 | draw_b (compute_niching_keys_kernel, niching.cu:157-178).
 // rho + rank < N_prior + N_last <= nm, so slots past N_last take the high word
 // nm: 32 + bitlen(nm) key bits.
+__device__ inline void niche_keys3_body(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
+                                        const int32_t* __restrict__ rho, int k, int nm, uint32_t word_mask,
+                                        uint64_t hi_base, const uint32_t* __restrict__ inj_b, uint64_t seed,
+                                        uint32_t gen, uint64_t* __restrict__ key3, int t)
+{
+    const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
+    uint64_t kk = (static_cast<uint64_t>(nm) << 32) | 0xFFFFFFFFull;
+    if (t < n_last) {
+        const int ri = static_cast<int>(static_cast<uint32_t>(key2s[t] >> 32) & word_mask);
+        const int rank = t - segment_head(key2s, t);
+        const int rh = (ri >= 0 && ri < k) ? rho[ri] : 0;
+        kk = (static_cast<uint64_t>(static_cast<uint32_t>(rh + rank)) << 32) |
+             niche_draw(inj_b, 1, t, seed, gen);
+    }
+    key3[t] = hi_base | kk;
+}
+
 __global__ void niche_keys3(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
                             const int32_t* __restrict__ rho, int k, int nm,
                             const uint32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen,
                             uint64_t* __restrict__ key3)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nm) return;
-    const int n_last = (meta[kNRem] > 0) ? meta[kNLast] : 0;
-    uint64_t kk = (static_cast<uint64_t>(nm) << 32) | 0xFFFFFFFFull;
-    if (t < n_last) {
-        const int ri = static_cast<int>(static_cast<uint32_t>(key2s[t] >> 32));
-        const int rank = t - segment_head(key2s, t);
-        const int rh = (ri >= 0 && ri < k) ? rho[ri] : 0;
-        kk = (static_cast<uint64_t>(static_cast<uint32_t>(rh + rank)) << 32) |
-             niche_draw(inj_b, 1, t, seed, gen);
-    }
-    key3[t] = kk;
+    if (t < nm) niche_keys3_body(key2s, meta, rho, k, nm, 0xFFFFFFFFu, 0ull, inj_b, seed, gen, key3, t);
 }
 
 // Survivor flags: all prior-front members plus the first N_rem niching picks
@@ -525,19 +587,13 @@ __global__ __launch_bounds__(kScanThreads) void flag_scatter(const uint8_t* __re
 
 // Survivor gather (pop_update_*_kernel + fit_and_fronts_update_kernel,
 // nsga3_population_update.cu:119-225); one wave per row.
-__global__ __launch_bounds__(256) void gather_next(const int32_t* __restrict__ chosen, int n, int d,
-                                                   int m, const float* __restrict__ px,
-                                                   const float* __restrict__ ox,
-                                                   const float* __restrict__ fm,
-                                                   const float* __restrict__ cvm,
-                                                   const int32_t* __restrict__ fronts_mix,
-                                                   float* __restrict__ nx, float* __restrict__ nf,
-                                                   float* __restrict__ ncv,
-                                                   int32_t* __restrict__ nfronts)
+__device__ inline void gather_row(const int32_t* __restrict__ chosen, int n, int d, int m,
+                                  const float* __restrict__ px, const float* __restrict__ ox,
+                                  const float* __restrict__ fm, const float* __restrict__ cvm,
+                                  const int32_t* __restrict__ fronts_mix, float* __restrict__ nx,
+                                  float* __restrict__ nf, float* __restrict__ ncv,
+                                  int32_t* __restrict__ nfronts, int row, int lane)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
     int s = chosen[row];
     if (s < 0 || s >= 2 * n) s = 0;  // unreachable: exactly n survivors are flagged
     const float* src = (s < n) ? px + static_cast<size_t>(s) * d : ox + static_cast<size_t>(s - n) * d;
@@ -555,6 +611,22 @@ __global__ __launch_bounds__(256) void gather_next(const int32_t* __restrict__ c
         ncv[row] = cvm[s];
         nfronts[row] = fronts_mix[s];
     }
+}
+
+__global__ __launch_bounds__(256) void gather_next(const int32_t* __restrict__ chosen, int n, int d,
+                                                   int m, const float* __restrict__ px,
+                                                   const float* __restrict__ ox,
+                                                   const float* __restrict__ fm,
+                                                   const float* __restrict__ cvm,
+                                                   const int32_t* __restrict__ fronts_mix,
+                                                   float* __restrict__ nx, float* __restrict__ nf,
+                                                   float* __restrict__ ncv,
+                                                   int32_t* __restrict__ nfronts)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    gather_row(chosen, n, d, m, px, ox, fm, cvm, fronts_mix, nx, nf, ncv, nfronts, row,
+               static_cast<int>(threadIdx.x & 63));
 }
 
 __global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restrict__ stats)
@@ -696,6 +768,262 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        A.next.cv, A.next_fronts);
     stage_end(kStGather, st);
     if (A.stats) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- batched --
+// The selection of every instance of a batch, each through the same bodies as
+// the single-population kernels.  Sort keys carry the instance index above
+// their own bits, so each global stable sort orders every instance's entries
+// exactly as that instance's own sort would.
+
+__global__ void b_merge(const BInst* __restrict__ insts, BMap mp, int cur, int off)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t >= I.nm) return;
+    const bool par = t < I.n;
+    const int s = par ? t : t - I.n;
+    const float* pf = par ? I.f[cur] : I.f[off];
+    for (int k = 0; k < I.m; ++k) I.fm[static_cast<size_t>(k) * I.nm + t] = pf[static_cast<size_t>(k) * I.n + s];
+    I.cvm[t] = par ? I.cv[cur][s] : I.cv[off][s];
+}
+
+// prior / last split of one instance per workgroup: a counting pass, then a
+// scatter pass, each over chunks of kScanThreads entries.
+__global__ __launch_bounds__(kScanThreads) void b_split(const BInst* __restrict__ insts)
+{
+    __shared__ int s_wave[64];
+    const BInst& I = insts[blockIdx.x];
+    const int mf = I.meta[kMaxFront];
+    int tp = 0, tl = 0;
+    for (int base = 0; base < I.nm; base += kScanThreads) {
+        const int t = base + static_cast<int>(threadIdx.x);
+        const int fr = t < I.nm ? I.fronts_mix[t] : -1;
+        int cp, cl;
+        block_exclusive_scan((fr >= 0 && fr < mf) ? 1 : 0, s_wave, cp);
+        block_exclusive_scan((fr == mf) ? 1 : 0, s_wave, cl);
+        tp += cp;
+        tl += cl;
+    }
+    int op = 0, ol = 0;
+    for (int base = 0; base < I.nm; base += kScanThreads) {
+        const int t = base + static_cast<int>(threadIdx.x);
+        const int fr = t < I.nm ? I.fronts_mix[t] : -1;
+        const int fp = (fr >= 0 && fr < mf) ? 1 : 0, fl = (fr == mf) ? 1 : 0;
+        int cp, cl;
+        const int ep = block_exclusive_scan(fp, s_wave, cp);
+        const int el = block_exclusive_scan(fl, s_wave, cl);
+        if (fp) I.gidx[op + ep] = t;
+        if (fl) I.gidx[tp + ol + el] = t;
+        op += cp;
+        ol += cl;
+    }
+    if (threadIdx.x == 0) {
+        I.meta[kNPrior] = tp;
+        I.meta[kNLast] = tl;
+        I.meta[kNRem] = I.n - tp;
+        I.meta[kIll] = -1;
+    }
+}
+
+// mode 0: ideal point of the n parents of slot `slot`; mode 1: update with the
+// offspring columns of the merged objectives.
+__global__ void b_ideal(const BInst* __restrict__ insts, BMap mp, int mode, int slot)
+{
+    int k;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), k);
+    const BInst& I = insts[i];
+    if (mode == 0) ideal_body(I.f[slot], I.n, 0, I.n, k, I.ideal, 1);
+    else ideal_body(I.fm, I.nm, I.n, I.n, k, I.ideal, 0);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void b_norm_prep(const BInst* __restrict__ insts, BMap mp)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    norm_prep_body<M>(I.fm, I.nm, I.gidx, I.ideal, I.meta, I.fn, I.asf_val, I.asf_idx, I.row_max, local);
+}
+
+__global__ __launch_bounds__(kSolveThreads) void b_norm_solve(const BInst* __restrict__ insts)
+{
+    const BInst& I = insts[blockIdx.x];
+    norm_solve_body(I.fn, I.nm, I.m, I.asf_val, I.asf_idx, I.row_max, I.meta, I.a);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void b_assoc_partial(const BInst* __restrict__ insts, BMap mp)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int nbx = (I.nm + 255) / 256;
+    assoc_partial_body<M>(I.fn, I.nm, I.a, I.dirs, I.k, I.meta, I.part_d2, I.part_idx, local % nbx, local / nbx);
+}
+
+__global__ void b_assoc_final(const BInst* __restrict__ insts, BMap mp)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t < I.nm) assoc_final_body(I.part_d2, I.part_idx, I.nchunks, I.nm, I.meta, I.dist, I.ref, t);
+}
+
+// rho_prior of one instance per workgroup (rho_clear + rho_hist).
+__global__ __launch_bounds__(1024) void b_rho(const BInst* __restrict__ insts)
+{
+    const BInst& I = insts[blockIdx.x];
+    for (int j = threadIdx.x; j < I.k; j += blockDim.x) I.rho[j] = 0;
+    __syncthreads();
+    if (I.meta[kNRem] <= 0) return;
+    const int n_prior = I.meta[kNPrior];
+    for (int t = threadIdx.x; t < n_prior; t += blockDim.x) {
+        const int r = I.ref[t];
+        if (r >= 0 && r < I.k) atomicAdd(&I.rho[r], 1);
+    }
+}
+
+__global__ void b_niche_keys1(const BInst* __restrict__ insts, BMap mp, int ref_bits, uint64_t* __restrict__ keys,
+                              int32_t* __restrict__ vals)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t >= I.nm) return;
+    niche_keys1_body(I.ref, I.dist, I.meta, I.k, static_cast<uint64_t>(i) << (32 + ref_bits), keys + I.key_off,
+                     vals + I.key_off, t);
+}
+
+__global__ void b_niche_keys2(const BInst* __restrict__ insts, BMap mp, int ref_bits,
+                              const uint64_t* __restrict__ key1s, uint64_t* __restrict__ key2, uint32_t gen)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t >= I.nm) return;
+    niche_keys2_body(key1s + I.key_off, I.meta, I.rho, I.k, (1u << ref_bits) - 1u,
+                     static_cast<uint64_t>(i) << (32 + ref_bits), nullptr, I.seed, gen, key2 + I.key_off, t);
+}
+
+__global__ void b_niche_keys3(const BInst* __restrict__ insts, BMap mp, int ref_bits, int rank_bits,
+                              const uint64_t* __restrict__ key2s, uint64_t* __restrict__ key3, uint32_t gen)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int t = local * 256 + static_cast<int>(threadIdx.x);
+    if (t >= I.nm) return;
+    niche_keys3_body(key2s + I.key_off, I.meta, I.rho, I.k, I.nm, (1u << ref_bits) - 1u,
+                     static_cast<uint64_t>(i) << (32 + rank_bits), nullptr, I.seed, gen, key3 + I.key_off, t);
+}
+
+// Survivor flags and their ascending compaction, one instance per workgroup.
+__global__ __launch_bounds__(kScanThreads) void b_survivors(const BInst* __restrict__ insts,
+                                                            const int32_t* __restrict__ val3s)
+{
+    __shared__ int s_wave[64];
+    const BInst& I = insts[blockIdx.x];
+    const int mf = I.meta[kMaxFront];
+    for (int t = threadIdx.x; t < I.nm; t += kScanThreads) {
+        const int fr = I.fronts_mix[t];
+        I.flag[t] = (fr >= 0 && fr < mf) ? 1 : 0;
+    }
+    __syncthreads();
+    const int n_rem = I.meta[kNRem];
+    if (n_rem > 0 && I.meta[kNLast] > 0) {
+        const int n_prior = I.meta[kNPrior];
+        for (int t = threadIdx.x; t < n_rem; t += kScanThreads) I.flag[I.gidx[n_prior + val3s[I.key_off + t]]] = 1;
+    }
+    __syncthreads();
+    int o = 0;
+    for (int base = 0; base < I.nm; base += kScanThreads) {
+        const int t = base + static_cast<int>(threadIdx.x);
+        const int f = t < I.nm ? static_cast<int>(I.flag[t]) : 0;
+        int c;
+        const int e = block_exclusive_scan(f, s_wave, c);
+        if (f && o + e < I.n) I.chosen[o + e] = t;
+        o += c;
+    }
+    if (threadIdx.x == 0) I.meta[5] = o;
+}
+
+__global__ __launch_bounds__(256) void b_gather(const BInst* __restrict__ insts, BMap mp, int cur, int off,
+                                                int nxt, int fnext)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (row >= I.n) return;
+    gather_row(I.chosen, I.n, I.d, I.m, I.x[cur], I.x[off], I.fm, I.cvm, I.fronts_mix, I.x[nxt], I.f[nxt],
+               I.cv[nxt], I.fronts[fnext], row, static_cast<int>(threadIdx.x & 63));
+}
+
+}  // namespace
+
+int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode, int slot)
+{
+    hipLaunchKernelGGL(b_ideal, dim3(G.objectives.blocks), dim3(1024), 0, L.stream, L.insts, G.objectives.map,
+                       mode, slot);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S, int cur,
+                 int off, int nxt, int fnext, int generation)
+{
+    hipStream_t st = L.stream;
+    const uint32_t gen = static_cast<uint32_t>(generation);
+    const int total = static_cast<int>(G.total_merged);
+    hipLaunchKernelGGL(b_merge, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map, cur, off);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(batch_nds(L, NG, S, 1, 0, 0));
+    hipLaunchKernelGGL(b_split, dim3(L.count), dim3(kScanThreads), 0, st, L.insts);
+    MOEA_CHECK(batch_ideal(L, G, 1, 0));
+    for (int m = 1; m <= kMaxM; ++m) {
+        if (G.norm_prep[m].blocks <= 0) continue;
+        const dim3 grid(G.norm_prep[m].blocks), block(256);
+        MOEA_LAUNCH_M(m, b_norm_prep, grid, block, 0, st, L.insts, G.norm_prep[m].map);
+    }
+    hipLaunchKernelGGL(b_norm_solve, dim3(L.count), dim3(kSolveThreads), 0, st, L.insts);
+    for (int m = 1; m <= kMaxM; ++m) {
+        if (G.assoc[m].blocks <= 0) continue;
+        const dim3 grid(G.assoc[m].blocks), block(256);
+        MOEA_LAUNCH_M(m, b_assoc_partial, grid, block, 0, st, L.insts, G.assoc[m].map);
+    }
+    hipLaunchKernelGGL(b_assoc_final, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map);
+    hipLaunchKernelGGL(b_rho, dim3(L.count), dim3(1024), 0, st, L.insts);
+    MOEA_HIP(hipGetLastError());
+    const int bits_ref = 32 + G.ref_bits + S.inst_bits;
+    const int bits_rank = 32 + G.rank_bits + S.inst_bits;
+    hipLaunchKernelGGL(b_niche_keys1, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,
+                       G.ref_bits, S.keys_in, S.vals_in);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out, total, st,
+                          bits_ref));
+    hipLaunchKernelGGL(b_niche_keys2, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,
+                       G.ref_bits, S.keys_out, S.keys_in, gen);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_out, S.vals_in, total, st,
+                          bits_ref));
+    hipLaunchKernelGGL(b_niche_keys3, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,
+                       G.ref_bits, G.rank_bits, S.keys_out, S.keys_in, gen);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out, total, st,
+                          bits_rank));
+    hipLaunchKernelGGL(b_survivors, dim3(L.count), dim3(kScanThreads), 0, st, L.insts, S.vals_out);
+    hipLaunchKernelGGL(b_gather, dim3(G.rows4.blocks), dim3(256), 0, st, L.insts, G.rows4.map, cur, off, nxt,
+                       fnext);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }

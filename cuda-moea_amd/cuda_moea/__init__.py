@@ -4,6 +4,7 @@ hand-written gfx950 HIP kernels in libmoea_hip.so."""
 import torch  # noqa: F401  (device memory and streams)
 
 from .algorithm import Algorithm, AlgorithmBuilder, NSGA3, Population, RVEA, Result
+from .batch import BatchNSGA3
 from . import problem as _problem
 from .problem import *  # noqa: F401,F403
 from ._specs import (
@@ -17,6 +18,7 @@ from ._specs import (
 __version__ = "0.2.0"
 
 __all__ = [
+    "BatchNSGA3",
     "Algorithm", "AlgorithmBuilder", "NSGA3", "RVEA", "Population", "Result",
     "CudaConfig", "SBX", "PolynomialMutation", "NoMutation",
     "RandomMating", "TournamentMating", "DasDennisDirections",

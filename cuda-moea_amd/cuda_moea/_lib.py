@@ -91,6 +91,15 @@ class RunArgs(ctypes.Structure):
     ]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int), ("max_generations", c_int),
+        ("seed", c_uint64), ("centers", c_void_p), ("bounds", c_void_p), ("directions", c_void_p),
+        ("sbx_eta", c_float * 2), ("sbx_prob", c_float), ("sbx_copy_prob", c_float),
+        ("pm_eta", c_float * 2), ("pm_prob", c_float),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/moea_hip.h declares
 SIGNATURES = {
     "moea_abi_version": (c_int, []),
@@ -118,6 +127,14 @@ SIGNATURES = {
     "moea_run_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "moea_run_init": (c_int, [POINTER(RunArgs)]),
     "moea_run_step": (c_int, [POINTER(RunArgs), c_int]),
+    "moea_batch_workspace_bytes": (c_size_t, [POINTER(BatchProblem), c_int]),
+    "moea_batch_create": (c_int, [POINTER(BatchProblem), c_int, c_void_p, c_size_t, c_void_p,
+                                  POINTER(c_void_p)]),
+    "moea_batch_init": (c_int, [c_void_p]),
+    "moea_batch_step": (c_int, [c_void_p, c_int]),
+    "moea_batch_generation": (c_int, [c_void_p]),
+    "moea_batch_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
+    "moea_batch_destroy": (None, [c_void_p]),
 }
 
 

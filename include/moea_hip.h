@@ -319,6 +319,45 @@ int moea_run_init(moea_run_args* args);
 int moea_run_step(moea_run_args* args, int generations);
 
 /* ------------------------------------------------------------------------ */
+/* Batched NSGA-III                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* One instance of a batch: an NSGA-III run with quadratic-center objectives,
+ * tournament mating, SBX and polynomial mutation (the moea_run_* operators).
+ * All instances of a batch run the same number of generations. */
+typedef struct moea_batch_problem {
+    int n, m, d, k;            /* population (even), objectives (<= 10), variables, directions */
+    int max_generations;
+    uint64_t seed;
+    const float* centers;      /* (m, d) device */
+    const float* bounds;       /* (2d) device */
+    const float* directions;   /* (m, k) device, unit length */
+    float sbx_eta[2], sbx_prob, sbx_copy_prob;
+    float pm_eta[2], pm_prob;
+} moea_batch_problem;
+
+typedef struct moea_batch moea_batch;   /* host handle */
+
+/* The reference has no batched engine: each EvolutionaryAlgorithm
+ * (algorithm/algorithm.cu:104-329) runs one population.  This advances many
+ * independent instances of uneven n / m / d with one launch sequence per
+ * generation; instance i computes exactly what moea_run_init / moea_run_step
+ * compute for it alone.  Every device buffer lives in the caller's workspace
+ * (size from moea_batch_workspace_bytes); the handle is host memory only. */
+size_t moea_batch_workspace_bytes(const moea_batch_problem* problems, int count);
+int moea_batch_create(const moea_batch_problem* problems, int count, void* workspace,
+                      size_t workspace_bytes, void* stream, moea_batch** out);
+/* Initial populations, evaluation, parent fronts and ideal points. */
+int moea_batch_init(moea_batch* batch);
+/* Up to `generations` generations of every instance; no host synchronization. */
+int moea_batch_step(moea_batch* batch, int generations);
+int moea_batch_generation(const moea_batch* batch);
+/* Device views of instance `index`'s current population (x (n, d), f (m, n),
+ * cv (n)) and its fronts (n); valid until the next moea_batch_step. */
+int moea_batch_view(const moea_batch* batch, int index, moea_population* out, int32_t** fronts);
+void moea_batch_destroy(moea_batch* batch);
+
+/* ------------------------------------------------------------------------ */
 /* Profiling                                                                 */
 /* ------------------------------------------------------------------------ */
 
