@@ -124,6 +124,38 @@ def batched_config(generations: int, count: int, plain_sample: int) -> dict:
             "total_individuals": sum(s[0] for s in specs)}
 
 
+def dtlz1_comparison(generations: int, warmup: int) -> dict:
+    """NSGA-III on DTLZ1, N = 32768, M = 3, D = 500, in the reference's own
+    benchmark setting (SBX eta 30, p 1, per-variable copy probability 0.5; PM
+    eta 20, rate 1/D; tournament mating; directions sized from N), timed like
+    Result.total_ms.  The reference publishes 5.1406 ms/gen for this shape on
+    an RTX PRO 6000 Blackwell (population_timing_summary.csv:9)."""
+    import torch
+    import cuda_moea as cm
+    from cuda_moea import _lib
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    alg = cm.NSGA3(population_size=32768, max_generations=warmup + generations,
+                   problem=cm.DTLZ1(dimension=500, objectives=3),
+                   crossover=cm.SBX(eta_initial=30.0, eta_final=30.0, probability=1.0,
+                                    variable_copy_probability=0.5),
+                   seed=0, print_progress=False)
+    alg.initialize()
+    if warmup:
+        alg._call(_lib.lib.moea_run_step, warmup)
+    stream.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    alg._call(_lib.lib.moea_run_step, generations)
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / generations
+    return {"workload": "NSGA-III DTLZ1 N=32768 M=3 D=500 (reference benchmark setting)",
+            "ms_per_gen": round(ms, 4), "gens_per_s": round(1000.0 / ms, 2), "generations_timed": generations,
+            "reference_ms_per_gen": 5.1406, "reference_hardware": "RTX PRO 6000 Blackwell",
+            "speedup_vs_reference": round(5.1406 / ms, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +163,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-dtlz1", action="store_true")
     ap.add_argument("--batched-instances", type=int, default=256)
     ap.add_argument("--batched-generations", type=int, default=300)
     ap.add_argument("--plain-sample", type=int, default=16)
@@ -188,6 +221,10 @@ def main() -> int:
     if not args.no_batched:
         batched = batched_config(args.batched_generations, args.batched_instances, args.plain_sample)
 
+    dtlz1 = None
+    if not args.no_dtlz1:
+        dtlz1 = dtlz1_comparison(50, 5)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_threads)
@@ -219,6 +256,7 @@ def main() -> int:
                      "frac_reference_work": round(M * nm * (nm - 1) / dom_avg_s / VALU_PEAK_CMP, 4)},
         "stage_ms_per_generation": stage_ms,
         "batched": batched,
+        "reference_comparison": dtlz1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

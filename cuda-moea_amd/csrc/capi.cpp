@@ -182,6 +182,12 @@ int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int 
     return eval_quadratic(x, n, d, centers, m, f, ldf, cv, static_cast<hipStream_t>(stream));
 }
 
+int moea_eval_dtlz(int problem, const float* x, int n, int d, int m, float* f, int ldf, float* cv,
+                   void* stream)
+{
+    return eval_dtlz(problem, x, n, d, m, f, ldf, cv, static_cast<hipStream_t>(stream));
+}
+
 int moea_eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2, int m,
                   float* f, int ldf, float* cv, float* hwork, void* stream)
 {

@@ -25,6 +25,10 @@ int evaluate(const moea_run_args& a, const moea_population& p, hipStream_t st, i
             return eval_quadratic(p.x, a.n, a.d, pr.centers, a.m, p.f, a.n, p.cv, st);
         case MOEA_PROBLEM_MLP:
             return eval_mlp(p.x, a.n, a.d, pr.w1, pr.hidden, pr.w2, a.m, p.f, a.n, p.cv, pr.hwork, st);
+        case MOEA_PROBLEM_DTLZ1:
+            return eval_dtlz(1, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
+        case MOEA_PROBLEM_DTLZ2:
+            return eval_dtlz(2, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
         case MOEA_PROBLEM_CALLBACK:
             if (!pr.callback) return fail(kInvalidArgument, "moea_run: callback problem without callback");
             return pr.callback(pr.user, p.x, a.n, a.d, a.m, p.f, p.cv, generation, a.max_generations, st);

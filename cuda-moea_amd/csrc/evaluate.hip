@@ -15,7 +15,7 @@ namespace {
 
 // One wave computes the m objectives of one row.
 template <int M>
-__device__ inline void quadratic_row(const float* __restrict__ x, int d, const float* __restrict__ c,
+__device__ __forceinline__ void quadratic_row(const float* __restrict__ x, int d, const float* __restrict__ c,
                                      float* __restrict__ f, int ldf, float* __restrict__ cv, int row,
                                      int lane)
 {
@@ -156,7 +156,77 @@ __global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__
     if (lane == 0 && cv) cv[row] = 0.0f;
 }
 
+// DTLZ1 / DTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) with the reference's
+// protections: dtlz1_func_kernel / dtlz2_func_kernel
+// (problem/dtlz/dtlz_kernels.cu:28-112, 118-190).  g sums over the last
+// k = d - m + 1 variables; DTLZ1 clamps x_j to [0, 1] in the products and
+// 1 - x to >= 0; DTLZ2 uses max(0, cos).  The angle arguments are formed in
+// double and rounded to float, as the reference's M_PI expressions are.  One
+// wave per row; lanes own the objectives.
+template <int M, int KIND>
+__device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf,
+                                 float* __restrict__ cv)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float* xr = x + static_cast<size_t>(row) * d;
+    float s = 0.0f;
+    for (int j = M - 1 + lane; j < d; j += 64) {
+        const float v = xr[j] - 0.5f;
+        if (KIND == 1) s = s + (v * v - cosf(static_cast<float>(20.0 * M_PI * static_cast<double>(v))));
+        else s = s + v * v;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s = s + __shfl_xor(s, o);
+    if (lane < M) {
+        const int k = d - (M - 1);
+        float out = (KIND == 1) ? 0.5f * (1.0f + 100.0f * (static_cast<float>(k) + s)) : 1.0f + s;
+        const int prod = M - 1 - lane;
+        for (int j = 0; j < prod; ++j) {
+            const float xj = xr[j];
+            if (KIND == 1) out *= fminf(fmaxf(xj, 0.0f), 1.0f);
+            else out *= fmaxf(0.0f, cosf(static_cast<float>(static_cast<double>(xj) * M_PI / 2.0)));
+        }
+        if (lane > 0) {
+            const float xi = xr[M - 1 - lane];
+            if (KIND == 1) out *= fmaxf(0.0f, 1.0f - xi);
+            else out *= sinf(static_cast<float>(static_cast<double>(xi) * M_PI / 2.0));
+        }
+        f[static_cast<size_t>(lane) * ldf + row] = out;
+    }
+    if (lane == 0 && cv) cv[row] = 0.0f;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void dtlz1_kernel(const float* __restrict__ x, int n, int d,
+                                                    float* __restrict__ f, int ldf, float* __restrict__ cv)
+{
+    dtlz_rows<M, 1>(x, n, d, f, ldf, cv);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void dtlz2_kernel(const float* __restrict__ x, int n, int d,
+                                                    float* __restrict__ f, int ldf, float* __restrict__ cv)
+{
+    dtlz_rows<M, 2>(x, n, d, f, ldf, cv);
+}
+
 }  // namespace
+
+int eval_dtlz(int kind, const float* x, int n, int d, int m, float* f, int ldf, float* cv, hipStream_t stream)
+{
+    if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || (kind != 1 && kind != 2))
+        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (DTLZ1/2 need 2 <= m <= d)");
+    const dim3 grid(ceil_div(n, 4)), block(256);
+    if (kind == 1) {
+        MOEA_LAUNCH_M(m, dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv);
+    } else {
+        MOEA_LAUNCH_M(m, dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv);
+    }
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
 
 int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2, int m,
              float* f, int ldf, float* cv, float* hwork, hipStream_t stream)

@@ -28,6 +28,7 @@ int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, 
 // evaluate.hip
 int eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f, int ldf,
                    float* cv, hipStream_t stream);
+int eval_dtlz(int kind, const float* x, int n, int d, int m, float* f, int ldf, float* cv, hipStream_t stream);
 int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const float* w2, int m,
              float* f, int ldf, float* cv, float* hwork, hipStream_t stream);
 

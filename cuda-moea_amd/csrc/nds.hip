@@ -49,19 +49,29 @@ constexpr int kKeyBits = 34;             // segment (2 bits) above a 32-bit orde
 constexpr uint8_t kPlain = 0;            // every row of the block is in segment 0
 constexpr uint8_t kMixed = 1;
 
-__host__ __device__ inline size_t bword_index(int p, int w, int Wc)
+__host__ __device__ __forceinline__ size_t bword_index(int p, int w, int Wc)
 {
     return ((static_cast<size_t>(p >> 6) * Wc + (w >> 3)) * 64 + (p & 63)) * 8 + (w & 7);
 }
 
+// Wave-uniform reads of data the kernel never writes, through the constant
+// address space so they compile to scalar loads even where alias analysis
+// cannot prove the pointer read-only (device functions, pointers from tables).
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, size_t i)
+{
+    using const_ptr = const __attribute__((address_space(4))) T*;
+    return ((const_ptr)p)[i];   // address-space conversion (a C-style cast performs it)
+}
+
 // Monotone map float -> u32 (numeric order for non-NaN values; -0 maps like +0).
-__device__ inline uint32_t order_key(float x)
+__device__ __forceinline__ uint32_t order_key(float x)
 {
     const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__device__ inline uint64_t nds_key_of(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
+__device__ __forceinline__ uint64_t nds_key_of(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
                                       int s, int m)
 {
     const float c = cv[s];
@@ -101,7 +111,7 @@ __global__ void nds_gather(const float* __restrict__ f, int ldf, const float* __
 
 // One wave per 64-row block: kind[b] and hi[b] (see the header comment).  The
 // segment is bits 32-33 of the key (higher bits may carry a batch index).
-__device__ inline void block_info_one(const uint64_t* __restrict__ keys, int n, int b,
+__device__ __forceinline__ void block_info_one(const uint64_t* __restrict__ keys, int n, int b,
                                       uint8_t* __restrict__ kind, int32_t* __restrict__ hi)
 {
     const int t = b * 64 + static_cast<int>(threadIdx.x);
@@ -135,7 +145,7 @@ __global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t
 // below every row p (lane): q dominates p iff q_k <= p_k for k = 1..M-1.
 // Returns the row word of lane p over the 64 rows q of the block.
 template <int M>
-__device__ inline uint64_t tile_fast(const float* __restrict__ fs, int npad, int q0,
+__device__ __forceinline__ uint64_t tile_fast(const float* __restrict__ fs, int npad, int q0,
                                      const float (&pv)[M])
 {
     constexpr int G = (M <= 5) ? 16 : 8;   // rows q per group of scalar loads
@@ -146,7 +156,7 @@ __device__ inline uint64_t tile_fast(const float* __restrict__ fs, int npad, int
 #pragma unroll
         for (int k = 1; k < M; ++k)
 #pragma unroll
-            for (int i = 0; i < G; ++i) qv[k - 1][i] = fs[static_cast<size_t>(k) * npad + q0 + g0 + i];
+            for (int i = 0; i < G; ++i) qv[k - 1][i] = ld_uniform(fs, static_cast<size_t>(k) * npad + q0 + g0 + i);
         static_for([&](auto IC) {
             constexpr int i = G - 1 - decltype(IC)::value;
             constexpr int j = g0 + i;                  // descending: bit j lands at position j
@@ -164,7 +174,7 @@ __device__ inline uint64_t tile_fast(const float* __restrict__ fs, int npad, int
 // of block qb (q < n, q != p); when `backward` is set, the words "p dominates
 // q" of the rows q are stored at word pb of their rows (tied tiles only).
 template <int M>
-__device__ inline uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
+__device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
                                         int npad, int n, int qb, int pb, const float (&pv)[M],
                                         float cp, bool backward, uint64_t* __restrict__ B, int Wc)
 {
@@ -180,12 +190,12 @@ __device__ inline uint64_t tile_general(const float* __restrict__ fs, const floa
         const int q = q0 + j;
         uint64_t qdp_m = 0, pdq_m = 0;
         if (q < n) {
-            const float cq = cvs[q];
+            const float cq = ld_uniform(cvs, q);
             const bool qf = (cq == 0.0f);
             bool le_qp = true, lt_qp = false, le_pq = true, lt_pq = false;
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                const float qk = fs[static_cast<size_t>(k) * npad + q];
+                const float qk = ld_uniform(fs, static_cast<size_t>(k) * npad + q);
                 le_qp = le_qp && (qk <= pv[k]);
                 lt_qp = lt_qp || (qk < pv[k]);
                 le_pq = le_pq && (pv[k] <= qk);
@@ -212,7 +222,7 @@ __device__ inline uint64_t tile_general(const float* __restrict__ fs, const floa
 
 // Work unit u -> (row block pb, chunk qc), qc <= pb / 8.  Units are grouped by
 // a = pb / 8: group a holds 8 row blocks x (a + 1) chunks.
-__device__ inline void decode_unit(long long u, int& pb, int& qc)
+__device__ __forceinline__ void decode_unit(long long u, int& pb, int& qc)
 {
     long long a = static_cast<long long>((sqrt(1.0 + static_cast<double>(u)) - 1.0) * 0.5);
     while (4 * a * (a + 1) > u) --a;
@@ -228,7 +238,7 @@ __device__ inline void decode_unit(long long u, int& pb, int& qc)
 // belong to tied tiles, whose backward direction the wave of the later block
 // stores.
 template <int M>
-__device__ inline void dom_unit(const float* __restrict__ fs, const float* __restrict__ cvs, int npad,
+__device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const float* __restrict__ cvs, int npad,
                                 int n, int W, int Wc, const uint8_t* __restrict__ kind,
                                 const int32_t* __restrict__ hi, uint64_t* __restrict__ B, long long u)
 {
@@ -243,7 +253,7 @@ __device__ inline void dom_unit(const float* __restrict__ fs, const float* __res
 #pragma unroll
     for (int k = 0; k < M; ++k) pv[k] = fs[static_cast<size_t>(k) * npad + p];
     const float cp = cvs[p];
-    const bool p_plain = kind[pb] == kPlain;
+    const bool p_plain = ld_uniform(kind, pb) == kPlain;
 
     uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0, w7 = 0;
 #pragma unroll 1
@@ -251,8 +261,8 @@ __device__ inline void dom_unit(const float* __restrict__ fs, const float* __res
         const int qb = qc * 8 + t;
         uint64_t word = 0;
         if (qb < pb) {
-            const bool tied = hi[qb] >= pb;
-            if (!tied && p_plain && kind[qb] == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
+            const bool tied = ld_uniform(hi, qb) >= pb;
+            if (!tied && p_plain && ld_uniform(kind, qb) == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
             else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc);
         } else if (qb == pb) {
             word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc);
@@ -333,13 +343,13 @@ constexpr int kScanLoads = 8;       // phase B: words per lane in flight
 
 // LDS layout of nds_peel (dynamic): s_A (W u64), s_next (bpg u64),
 // s_task (64 bpg u64), s_hi (W int).
-__host__ __device__ inline size_t peel_lds_bytes(int W, int bpg)
+__host__ __device__ __forceinline__ size_t peel_lds_bytes(int W, int bpg)
 {
     return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + static_cast<size_t>(bpg) * 64 * 8 +
            static_cast<size_t>(W) * 4;
 }
 
-__device__ inline uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
+__device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
 {
     const ulonglong2 h = v[i >> 1];
     return (i & 1) ? h.y : h.x;
@@ -348,7 +358,7 @@ __device__ inline uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
 // Phase A for the lane's row p of block rb (live rows only): checks the
 // chunk holding the cursor, and the next kAheadChunks chunks if it is clear.
 // Updates cur; ready: no dominator word left; queued: clear so far, words left.
-__device__ inline void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
+__device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
                                   const uint64_t* s_A, int& cur, bool& ready, bool& queued)
 {
     const uint64_t* rowbase = B + (static_cast<size_t>(rb) * Wc * 64 + lane) * 8;
@@ -396,7 +406,7 @@ __device__ inline void peel_check(const uint64_t* __restrict__ B, int Wc, int rb
 // Phase B: the wave scans row pr from word c (wave-uniform) towards end, 64
 // words in the first round trip, then 512.  Returns the first word holding an
 // unassigned dominator, or end when the row is clear.
-__device__ inline int peel_scan(const uint64_t* __restrict__ B, int Wc, int pr, int c, int end,
+__device__ __forceinline__ int peel_scan(const uint64_t* __restrict__ B, int Wc, int pr, int c, int end,
                                 const uint64_t* s_A, int lane)
 {
     int batch = 1;
@@ -549,7 +559,7 @@ struct BRows {
     int32_t* fronts;
 };
 
-__device__ inline BRows brows(const BInst& I, int mode, int slot, int fslot)
+__device__ __forceinline__ BRows brows(const BInst& I, int mode, int slot, int fslot)
 {
     BRows r;
     if (mode == 0) {

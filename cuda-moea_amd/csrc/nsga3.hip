@@ -23,7 +23,7 @@ enum Meta : int { kMaxFront = 0, kNPrior = 1, kNLast = 2, kIll = 3, kNRem = 4, k
 constexpr int kScanThreads = 1024;
 
 // Exclusive scan of one int per thread across a 1024-thread block.
-__device__ inline int block_exclusive_scan(int v, int* s_wave, int& total)
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int incl = v;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
 }
 
 // Running ideal point: ideal = fminf(ideal, min over offspring) (ideal_point.cu:25-165).
-__device__ inline void ideal_body(const float* __restrict__ f, int ldf, int col0, int count, int k,
+__device__ __forceinline__ void ideal_body(const float* __restrict__ f, int ldf, int col0, int count, int k,
                                   float* __restrict__ ideal, int init)
 {
     __shared__ float s_min[1024 / 64];
@@ -156,7 +156,7 @@ __global__ void ideal_update(const float* __restrict__ f, int ldf, int col0, int
 // (translated_ndsfit_kernel, compute_asf_legacy_kernel, find_intercepts_legacy_kernel:
 // normalization.cu:370-444, 647-729, 802-856).
 template <int M>
-__device__ inline void norm_prep_body(const float* __restrict__ fm, int nm, const int32_t* __restrict__ gidx,
+__device__ __forceinline__ void norm_prep_body(const float* __restrict__ fm, int nm, const int32_t* __restrict__ gidx,
                                       const float* __restrict__ ideal, const int32_t* __restrict__ meta,
                                       float* __restrict__ fn, float* __restrict__ asf_val,
                                       int32_t* __restrict__ asf_idx, float* __restrict__ row_max, int bx)
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
 // thread solves.  Extreme point j is the (value, index)-smallest candidate with
 // a finite ASF value (-1 if none) -- an order-independent reduction.
 constexpr int kSolveThreads = 256;
-__device__ inline void norm_solve_body(const float* __restrict__ fn, int nm, int m,
+__device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, int nm, int m,
                                        const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
                                        const float* __restrict__ row_max, int32_t* __restrict__ meta,
                                        float* __restrict__ a_out)
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restr
 constexpr int kAssocChunk = 1024;
 
 template <int M>
-__device__ inline void assoc_partial_body(const float* __restrict__ fn, int nm, const float* __restrict__ a,
+__device__ __forceinline__ void assoc_partial_body(const float* __restrict__ fn, int nm, const float* __restrict__ a,
                                           const float* __restrict__ w, int k, const int32_t* __restrict__ meta,
                                           float* __restrict__ part_d2, int32_t* __restrict__ part_idx, int bx,
                                           int by)
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void associate_partial(const float* __restrict
 }
 
 // Reduce the chunk partials in chunk order (association_phase2_kernel, :397-447).
-__device__ inline void assoc_final_body(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
+__device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
                                         int nchunks, int nm, const int32_t* __restrict__ meta,
                                         float* __restrict__ dist, int32_t* __restrict__ ref, int i)
 {
@@ -425,10 +425,10 @@ __global__ void rho_hist(const int32_t* __restrict__ ref, const int32_t* __restr
 // the direction, with an unassociated row (ref -1, whose u32 sorts after every
 // direction) written as k and slots past N_last as k + 1 with an all-ones low
 // word: the same order in 32 + bitlen(k + 1) key bits.
-__device__ inline uint32_t ref_word(int r, int k) { return r < 0 ? static_cast<uint32_t>(k) : static_cast<uint32_t>(r); }
+__device__ __forceinline__ uint32_t ref_word(int r, int k) { return r < 0 ? static_cast<uint32_t>(k) : static_cast<uint32_t>(r); }
 
 // hi_base: bits above the key (a batch index); key arrays are the instance's view.
-__device__ inline void niche_keys1_body(const int32_t* __restrict__ ref, const float* __restrict__ dist,
+__device__ __forceinline__ void niche_keys1_body(const int32_t* __restrict__ ref, const float* __restrict__ dist,
                                         const int32_t* __restrict__ meta, int k, uint64_t hi_base,
                                         uint64_t* __restrict__ key, int32_t* __restrict__ val, int t)
 {
@@ -452,7 +452,7 @@ __global__ void niche_keys1(const int32_t* __restrict__ ref, const float* __rest
 }
 
 // First position of the run of equal high words containing t (keys sorted).
-__device__ inline int segment_head(const uint64_t* __restrict__ key, int t)
+__device__ __forceinline__ int segment_head(const uint64_t* __restrict__ key, int t)
 {
     const uint32_t hi = static_cast<uint32_t>(key[t] >> 32);
     int lo = 0, up = t;
@@ -464,7 +464,7 @@ __device__ inline int segment_head(const uint64_t* __restrict__ key, int t)
     return lo;
 }
 
-__device__ inline uint32_t niche_draw(const uint32_t* injected, int which, int t, uint64_t seed,
+__device__ __forceinline__ uint32_t niche_draw(const uint32_t* injected, int which, int t, uint64_t seed,
                                       uint32_t gen)
 {
     return injected ? injected[t] : splitmix32(niche_base(gen, which) + static_cast<uint64_t>(t), seed);
@@ -473,7 +473,7 @@ __device__ inline uint32_t niche_draw(const uint32_t* injected, int which, int t
 // Phase-4 rank keys: ref << 32 | (rho == 0 && local == 0 ? 0 : draw_a | 1)
 // (compute_rank_keys_kernel, niching.cu:114-139).
 // word_mask: the direction field of the high word (below any batch index).
-__device__ inline void niche_keys2_body(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
+__device__ __forceinline__ void niche_keys2_body(const uint64_t* __restrict__ key1s, const int32_t* __restrict__ meta,
                                         const int32_t* __restrict__ rho, int k, uint32_t word_mask,
                                         uint64_t hi_base, const uint32_t* __restrict__ inj_a, uint64_t seed,
                                         uint32_t gen, uint64_t* __restrict__ key2, int t)
@@ -503,7 +503,7 @@ __global__ void niche_keys2(const uint64_t* __restrict__ key1s, const int32_t* _
 // Phase-5 keys: (rho + rank) << 32 | draw_b (compute_niching_keys_kernel, niching.cu:157-178).
 // rho + rank < N_prior + N_last <= nm, so slots past N_last take the high word
 // nm: 32 + bitlen(nm) key bits.
-__device__ inline void niche_keys3_body(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
+__device__ __forceinline__ void niche_keys3_body(const uint64_t* __restrict__ key2s, const int32_t* __restrict__ meta,
                                         const int32_t* __restrict__ rho, int k, int nm, uint32_t word_mask,
                                         uint64_t hi_base, const uint32_t* __restrict__ inj_b, uint64_t seed,
                                         uint32_t gen, uint64_t* __restrict__ key3, int t)
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kScanThreads) void flag_scatter(const uint8_t* __re
 
 // Survivor gather (pop_update_*_kernel + fit_and_fronts_update_kernel,
 // nsga3_population_update.cu:119-225); one wave per row.
-__device__ inline void gather_row(const int32_t* __restrict__ chosen, int n, int d, int m,
+__device__ __forceinline__ void gather_row(const int32_t* __restrict__ chosen, int n, int d, int m,
                                   const float* __restrict__ px, const float* __restrict__ ox,
                                   const float* __restrict__ fm, const float* __restrict__ cvm,
                                   const int32_t* __restrict__ fronts_mix, float* __restrict__ nx,

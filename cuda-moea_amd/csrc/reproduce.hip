@@ -15,7 +15,7 @@ namespace moea {
 namespace {
 
 // Mating winner of slot t (candidates from [0, active)).
-__device__ inline int mating_one(int t, int n, int active, int mating, const int32_t* __restrict__ rank,
+__device__ __forceinline__ int mating_one(int t, int n, int active, int mating, const int32_t* __restrict__ rank,
                                  const float* __restrict__ cv, const int32_t* __restrict__ inj_a,
                                  const int32_t* __restrict__ inj_b, uint64_t seed, uint32_t gen)
 {
@@ -54,7 +54,7 @@ __global__ void mating_kernel(int n, int active_host, const int32_t* __restrict_
 // SBX of one (pair, variable) element.  In midpoint +/- delta both operands are
 // products; the NVPTX combiner fuses the first (0.5f * (x1 + x2), an exact
 // product), so the sums evaluate unfused as written.
-__device__ inline void sbx_one(float x1, float x2, float ub, float up, int sign, float uc, float lo,
+__device__ __forceinline__ void sbx_one(float x1, float x2, float ub, float up, int sign, float uc, float lo,
                                float hi, float eta, float prob, float copy_prob, float& ca, float& cb)
 {
     if (up >= prob || uc < copy_prob) { ca = x1; cb = x2; return; }
@@ -68,7 +68,7 @@ __device__ inline void sbx_one(float x1, float x2, float ub, float up, int sign,
 }
 
 // Deb's polynomial mutation of one element; value + delta * range is fused.
-__device__ inline float pm_one(float value, float ug, float ud, float lo, float hi, float eta, float rate)
+__device__ __forceinline__ float pm_one(float value, float ug, float ud, float lo, float hi, float eta, float rate)
 {
     if (ug >= rate) return value;
     const float range = hi - lo;
@@ -104,7 +104,7 @@ struct RepArgs {
 
 // Element e of the (pair, variable) space; the odd tail row (n odd) copies its
 // parent (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
-__device__ inline void sbx_pm_elem(const RepArgs& R, long long e)
+__device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
 {
     const long long total_pair = static_cast<long long>(R.pairs) * R.d;
     const bool tail = (R.n & 1) != 0;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
 
 // Population::initialize_random (core/population.cu:97-115):
 // lower + u * (upper - lower), contracted to one fma by nvcc.
-__device__ inline void init_elem(float* __restrict__ x, long long e, int d, const float* __restrict__ bounds,
+__device__ __forceinline__ void init_elem(float* __restrict__ x, long long e, int d, const float* __restrict__ bounds,
                                  uint64_t seed)
 {
     const int j = static_cast<int>(e % d);
@@ -189,7 +189,7 @@ __global__ void init_kernel(float* __restrict__ x, long long count, int d,
 // The same element work per instance of a batch (tournament mating, SBX and
 // polynomial mutation with the instance's annealed etas, in-kernel draws).
 
-__host__ __device__ inline float annealed_hd(float begin, float end, int generation, int total)
+__host__ __device__ __forceinline__ float annealed_hd(float begin, float end, int generation, int total)
 {
     float progress = 1.0f;
     if (total > 0) {

@@ -16,7 +16,7 @@ MOEA_OK = 0
 MATING_TOURNAMENT, MATING_RANDOM = 0, 1
 MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
 ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
-PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK = 0, 1, 2
+PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK, PROBLEM_DTLZ1, PROBLEM_DTLZ2 = 0, 1, 2, 3, 4
 MAX_OBJECTIVES = 10
 
 
@@ -116,6 +116,7 @@ SIGNATURES = {
     "moea_init_population": (c_int, [c_void_p, c_int, c_int, c_void_p, c_uint64, c_void_p]),
     "moea_eval_quadratic": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                     c_void_p, c_void_p]),
+    "moea_eval_dtlz": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "moea_eval_mlp": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
                               c_int, c_void_p, c_void_p, c_void_p]),
     "moea_rvea_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),

@@ -208,6 +208,17 @@ def eval_quadratic(x: torch.Tensor, centers: torch.Tensor):
     return f, cv
 
 
+def eval_dtlz(kind: int, x: torch.Tensor, m: int):
+    """DTLZ1 (kind 1) or DTLZ2 (kind 2) objectives (m, n) and zero cv of x (n, d)."""
+    _dev(x, _F32, "x")
+    n, d = x.shape
+    f = torch.empty(m, n, dtype=_F32, device=x.device)
+    cv = torch.empty(n, dtype=_F32, device=x.device)
+    check(lib.moea_eval_dtlz(int(kind), x.data_ptr(), n, d, m, f.data_ptr(), n, cv.data_ptr(),
+                             _stream(x.device)))
+    return f, cv
+
+
 def eval_mlp(x: torch.Tensor, w1: torch.Tensor, w2: torch.Tensor):
     _dev(x, _F32, "x")
     n, d = x.shape

@@ -202,6 +202,13 @@ int moea_init_population(float* x, int n, int d, const float* bounds, uint64_t s
 int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f,
                         int ldf, float* cv, void* stream);
 
+/* DTLZ1 (problem 1) or DTLZ2 (problem 2), as the reference's built-in kernels
+ * dtlz1_func_kernel / dtlz2_func_kernel (problem/dtlz/dtlz_kernels.cu:28-190),
+ * with their clamping of x to [0, 1] (DTLZ1) and max(0, cos) (DTLZ2); cv = 0.
+ * Needs 2 <= m <= d. */
+int moea_eval_dtlz(int problem, const float* x, int n, int d, int m, float* f, int ldf, float* cv,
+                   void* stream);
+
 /* Seeded MLP objective: h = tanh(W1 x) (hidden units), y = W2 h (m outputs),
  * f_c = y_c^2 + 0.01 * ||x - 0.5||^2; cv = 0.  w1: (hidden, d), w2: (m, hidden).
  * hwork: (n, hidden) scratch. */
@@ -256,6 +263,8 @@ int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_a
 #define MOEA_PROBLEM_QUADRATIC 0
 #define MOEA_PROBLEM_MLP 1
 #define MOEA_PROBLEM_CALLBACK 2
+#define MOEA_PROBLEM_DTLZ1 3
+#define MOEA_PROBLEM_DTLZ2 4
 
 /* Problem evaluator callback: the C form of IProblemEvaluator::evaluate
  * (problem/problem_evaluator.cuh:10-31).  Must write f (m, n) and cv (n) for
