@@ -1,13 +1,20 @@
-// Batched NSGA-III engine: many independent populations of uneven N / M / D
-// advanced together, one launch sequence per generation for the whole batch.
+// Batched and served NSGA-III: many independent populations of uneven N / M /
+// D advanced together, one launch sequence per generation for all of them.
 //
 // The reference has no batching: each EvolutionaryAlgorithm (NS/algorithm/
-// algorithm.cu:104-329) owns its CUDA context and runs one population.  Here
-// every instance's buffers are carved out of one caller-provided workspace and
-// each stage is one launch over all instances (see batch_internal.hpp), so the
-// cost of a generation is one set of launches however many runs it carries.
-// Each instance computes exactly what moea_run_init / moea_run_step compute
-// for it alone (same kernels bodies, same draws keyed by the instance's seed).
+// algorithm.cu:104-329) owns its stream and runs one population.  Here each
+// instance's buffers live in caller-provided device memory, a device table of
+// BInst records describes them, and every stage is one launch over the
+// instances (batch_internal.hpp).  Instance i computes exactly what
+// moea_run_init / moea_run_step compute for it alone: the same kernel bodies
+// and draws keyed by its own seed and generation.
+//
+// Two front ends share the core:
+//   moea_batch_*  a fixed set of instances carved from one workspace, all
+//                 running the same number of generations;
+//   moea_serve_*  continuous batching: instances are admitted into slots while
+//                 others run, their buffers allocated from an arena (the HBM
+//                 budget) and freed on release; each runs its own generations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +31,7 @@ namespace moea {
 namespace {
 
 constexpr int kAssocChunkB = 1024;   // association chunk (directions per LDS stage)
+constexpr int kStateFree = 0, kStateRunning = 1, kStateAdmitted = 2, kStateFinished = 3;
 
 int bitlen(long long v)
 {
@@ -32,54 +40,8 @@ int bitlen(long long v)
     return b < 1 ? 1 : b;
 }
 
-}  // namespace
-
-struct BatchState {
-    std::vector<moea_batch_problem> probs;
-    std::vector<BInst> host;          // host copy of the instance table
-    BInst* dev_insts = nullptr;
-    int count = 0;
-    hipStream_t stream{};
-    // grids
-    BGrid init256{}, rows256{}, sbx256{}, rows4_m[kMaxM + 1]{};
-    BSelGrids sel{};
-    BNdsGrids nds0{}, nds1{};
-    BNdsScratch scratch{};
-    // host copies of the prefix tables, copied to the device at init
-    std::vector<int> tables;
-    int* dev_tables = nullptr;
-    // run state
-    int current = 0, fronts_current = 0, generation = 0, max_generations = 0;
-    bool initialized = false;
-};
-
-namespace {
-
-// Per-instance block counts -> prefix table appended to `tables`; returns the
-// grid with its table offset in the map pointer (fixed up after the copy).
-BGrid add_grid(std::vector<int>& tables, const std::vector<long long>& counts)
-{
-    BGrid g{};
-    const size_t off = tables.size();
-    long long acc = 0;
-    for (long long c : counts) {
-        tables.push_back(static_cast<int>(acc));
-        acc += c;
-    }
-    tables.push_back(static_cast<int>(acc));
-    g.map.start = reinterpret_cast<const int*>(off);   // offset until fixed up
-    g.map.count = static_cast<int>(counts.size());
-    g.blocks = static_cast<int>(acc);
-    return g;
-}
-
-void fix(BGrid& g, int* base)
-{
-    g.map.start = base + reinterpret_cast<size_t>(g.map.start);
-}
-
 struct InstSizes {
-    int n, m, d, k, nm, W, Wc, npad, nchunks, nb;
+    int n, m, d, k, nm, W, Wc, npad, nchunks, nb, hidden;
 };
 
 InstSizes sizes_of(const moea_batch_problem& p)
@@ -92,286 +54,581 @@ InstSizes sizes_of(const moea_batch_problem& p)
     z.npad = z.W * 64;
     z.nchunks = ceil_div(p.k, kAssocChunkB);
     z.nb = ceil_div(z.nm, 256);
+    z.hidden = (p.problem == MOEA_PROBLEM_MLP) ? p.hidden : 0;
     return z;
 }
 
-// Lays out everything in the workspace (base may be null to size it).
-size_t layout(BatchState& B, char* base)
+// Carves one instance's buffers (c.base may be null to size them).
+void carve_instance(Carve& c, const moea_batch_problem& p, BInst& I)
 {
-    Carve c(base, ~size_t{0});
-    const int count = B.count;
-    B.dev_insts = c.take<BInst>(count);
-    B.dev_tables = c.take<int>(B.tables.size());
-    long long total_merged = 0, total_n = 0;
-    for (int i = 0; i < count; ++i) {
-        const InstSizes z = sizes_of(B.probs[i]);
-        BInst& I = B.host[i];
-        for (int s = 0; s < 3; ++s) {
-            I.x[s] = c.take<float>(static_cast<size_t>(z.n) * z.d);
-            I.f[s] = c.take<float>(static_cast<size_t>(z.m) * z.n);
-            I.cv[s] = c.take<float>(z.n);
-        }
-        I.fronts[0] = c.take<int32_t>(z.n);
-        I.fronts[1] = c.take<int32_t>(z.n);
-        I.winners = c.take<int32_t>(z.n);
-        I.ideal = c.take<float>(kMaxM);
-        I.fm = c.take<float>(static_cast<size_t>(z.m) * z.nm);
-        I.cvm = c.take<float>(z.nm);
-        I.fronts_mix = c.take<int32_t>(z.nm);
-        I.meta = c.take<int32_t>(8);
-        I.gidx = c.take<int32_t>(z.nm);
-        I.fn = c.take<float>(static_cast<size_t>(z.m) * z.nm);
-        I.asf_val = c.take<float>(static_cast<size_t>(z.nb) * z.m);
-        I.asf_idx = c.take<int32_t>(static_cast<size_t>(z.nb) * z.m);
-        I.row_max = c.take<float>(static_cast<size_t>(z.nb) * z.m);
-        I.a = c.take<float>(kMaxM);
-        I.part_d2 = c.take<float>(static_cast<size_t>(z.nchunks) * z.nm);
-        I.part_idx = c.take<int32_t>(static_cast<size_t>(z.nchunks) * z.nm);
-        I.dist = c.take<float>(z.nm);
-        I.ref = c.take<int32_t>(z.nm);
-        I.rho = c.take<int32_t>(z.k);
-        I.flag = c.take<uint8_t>(z.nm);
-        I.chosen = c.take<int32_t>(z.n);
-        I.B = c.take<uint64_t>(static_cast<size_t>(z.W) * z.Wc * 512);
-        I.fs = c.take<float>(static_cast<size_t>(z.m) * z.npad);
-        I.cvs = c.take<float>(z.npad);
-        I.perm = c.take<int32_t>(z.nm);
-        I.hi = c.take<int32_t>(z.W);
-        I.kind = c.take<uint8_t>(z.W);
-        I.cursor = c.take<int32_t>(z.nm);
-        I.A = nullptr;
-        I.key_off = total_merged;
-        I.key_off_n = total_n;
-        total_merged += z.nm;
-        total_n += z.n;
+    const InstSizes z = sizes_of(p);
+    for (int s = 0; s < 3; ++s) {
+        I.x[s] = c.take<float>(static_cast<size_t>(z.n) * z.d);
+        I.f[s] = c.take<float>(static_cast<size_t>(z.m) * z.n);
+        I.cv[s] = c.take<float>(z.n);
     }
-    BNdsScratch& S = B.scratch;
-    S.keys_in = c.take<uint64_t>(total_merged);
-    S.keys_out = c.take<uint64_t>(total_merged);
-    S.vals_in = c.take<int32_t>(total_merged);
-    S.vals_out = c.take<int32_t>(total_merged);
-    // temporary storage for the widest of the key widths the batch sorts with
-    const int ib = S.inst_bits;
-    const int widths[] = {34 + ib, 32 + B.sel.ref_bits + ib, 32 + B.sel.rank_bits + ib, 64};
-    S.sort_bytes = 0;
-    for (int w : widths)
-        S.sort_bytes = std::max(S.sort_bytes, sort_pairs_temp_bytes(static_cast<int>(total_merged), w));
-    S.sort_temp = c.take_bytes(S.sort_bytes);
+    I.fronts[0] = c.take<int32_t>(z.n);
+    I.fronts[1] = c.take<int32_t>(z.n);
+    I.winners = c.take<int32_t>(z.n);
+    I.ideal = c.take<float>(kMaxM);
+    I.hwork = z.hidden ? c.take<float>(static_cast<size_t>(z.n) * z.hidden) : nullptr;
+    I.fm = c.take<float>(static_cast<size_t>(z.m) * z.nm);
+    I.cvm = c.take<float>(z.nm);
+    I.fronts_mix = c.take<int32_t>(z.nm);
+    I.meta = c.take<int32_t>(8);
+    I.gidx = c.take<int32_t>(z.nm);
+    I.fn = c.take<float>(static_cast<size_t>(z.m) * z.nm);
+    I.asf_val = c.take<float>(static_cast<size_t>(z.nb) * z.m);
+    I.asf_idx = c.take<int32_t>(static_cast<size_t>(z.nb) * z.m);
+    I.row_max = c.take<float>(static_cast<size_t>(z.nb) * z.m);
+    I.a = c.take<float>(kMaxM);
+    I.part_d2 = c.take<float>(static_cast<size_t>(z.nchunks) * z.nm);
+    I.part_idx = c.take<int32_t>(static_cast<size_t>(z.nchunks) * z.nm);
+    I.dist = c.take<float>(z.nm);
+    I.ref = c.take<int32_t>(z.nm);
+    I.rho = c.take<int32_t>(z.k);
+    I.flag = c.take<uint8_t>(z.nm);
+    I.chosen = c.take<int32_t>(z.n);
+    I.B = c.take<uint64_t>(static_cast<size_t>(z.W) * z.Wc * 512);
+    I.fs = c.take<float>(static_cast<size_t>(z.m) * z.npad);
+    I.cvs = c.take<float>(z.npad);
+    I.perm = c.take<int32_t>(z.nm);
+    I.hi = c.take<int32_t>(z.W);
+    I.kind = c.take<uint8_t>(z.W);
+    I.cursor = c.take<int32_t>(z.nm);
+    I.A = nullptr;
+}
+
+size_t instance_bytes(const moea_batch_problem& p)
+{
+    Carve c(nullptr, ~size_t{0});
+    BInst I{};
+    carve_instance(c, p, I);
     return c.used;
 }
 
-// Fills the host instance records (sizes, problem, parameters) and grids.
-int plan(BatchState& B)
+void fill_record(const moea_batch_problem& p, BInst& I)
 {
-    const int count = B.count;
-    B.host.assign(count, BInst{});
-    B.tables.clear();
-    std::vector<long long> init256, rows256, sbx256, merged256, objectives, rows4;
-    std::vector<std::vector<long long>> rows4_m(kMaxM + 1, std::vector<long long>(count, 0));
-    std::vector<std::vector<long long>> prep_m(kMaxM + 1, std::vector<long long>(count, 0));
-    std::vector<std::vector<long long>> assoc_m(kMaxM + 1, std::vector<long long>(count, 0));
-    std::vector<long long> n0_rows, n0_npad, n0_w, n1_rows, n1_npad, n1_w;
-    std::vector<std::vector<long long>> dom0(kMaxM + 1, std::vector<long long>(count, 0));
-    std::vector<std::vector<long long>> dom1(kMaxM + 1, std::vector<long long>(count, 0));
+    const InstSizes z = sizes_of(p);
+    I.state = kStateAdmitted;
+    I.cur = 0; I.fcur = 0; I.gen = 0;
+    I.n = z.n; I.m = z.m; I.d = z.d; I.k = z.k;
+    I.nm = z.nm; I.W = z.W; I.Wc = z.Wc; I.npad = z.npad;
+    I.max_gen = p.max_generations;
+    I.nchunks = z.nchunks;
+    I.seed = p.seed;
+    I.sbx_eta0 = p.sbx_eta[0]; I.sbx_eta1 = p.sbx_eta[1];
+    I.sbx_prob = p.sbx_prob; I.sbx_copy = p.sbx_copy_prob;
+    I.pm_eta0 = p.pm_eta[0]; I.pm_eta1 = p.pm_eta[1]; I.pm_prob = p.pm_prob;
+    I.problem = p.problem;
+    I.w1 = p.w1; I.w2 = p.w2; I.hidden = z.hidden;
+    I.centers = p.centers; I.bounds = p.bounds; I.dirs = p.directions;
+}
+
+int validate_problem(const moea_batch_problem& p)
+{
+    if (p.n <= 0 || (p.n & 1) || p.m <= 0 || p.m > kMaxM || p.d <= 0 || p.k <= 0 || p.max_generations <= 0)
+        return fail(kInvalidArgument, "moea_batch: invalid sizes (n must be even and positive, 1 <= m <= 10)");
+    if (2 * p.n > nds_max_n()) return fail(kUnsupported, "moea_batch: population too large");
+    if (!p.bounds || !p.directions) return fail(kInvalidArgument, "moea_batch: bounds and directions are required");
+    if (p.problem == MOEA_PROBLEM_QUADRATIC) {
+        if (!p.centers) return fail(kInvalidArgument, "moea_batch: quadratic problem without centers");
+    } else if (p.problem == MOEA_PROBLEM_MLP) {
+        if (!p.w1 || !p.w2 || p.hidden <= 0) return fail(kInvalidArgument, "moea_batch: MLP problem needs w1, w2, hidden");
+    } else {
+        return fail(kInvalidArgument, "moea_batch: problem must be MOEA_PROBLEM_QUADRATIC or MOEA_PROBLEM_MLP");
+    }
+    return kOk;
+}
+
+// Grids of one state (the instances a set of launches serves).
+struct Grids {
+    BGrid init256{}, rows256{}, sbx256{};
+    BEvalGrids eval{};
+    BSelGrids sel{};
+    BNdsGrids nds0{}, nds1{};
+};
+
+// Grid offsets are kept as table indices until the table is on the device.
+struct TableBuilder {
+    std::vector<int> t;
+    std::vector<BGrid*> pending;
+    BGrid add(const std::vector<long long>& counts)
+    {
+        BGrid g{};
+        const size_t off = t.size();
+        long long acc = 0;
+        for (long long c : counts) {
+            t.push_back(static_cast<int>(acc));
+            acc += c;
+        }
+        t.push_back(static_cast<int>(acc));
+        g.map.start = reinterpret_cast<const int*>(off);
+        g.map.count = static_cast<int>(counts.size());
+        g.blocks = static_cast<int>(acc);
+        return g;
+    }
+};
+
+void fix(BGrid& g, const int* base) { g.map.start = base + reinterpret_cast<size_t>(g.map.start); }
+
+void fix_grids(Grids& G, const int* base)
+{
+    fix(G.init256, base); fix(G.rows256, base); fix(G.sbx256, base);
+    for (int m = 1; m <= kMaxM; ++m) {
+        fix(G.eval.quad_rows4[m], base);
+        fix(G.eval.mlp_rows4[m], base);
+        fix(G.sel.norm_prep[m], base);
+        fix(G.sel.assoc[m], base);
+        fix(G.nds0.dom[m], base);
+        fix(G.nds1.dom[m], base);
+    }
+    fix(G.eval.mlp_tiles, base);
+    fix(G.sel.merged256, base); fix(G.sel.rows4, base); fix(G.sel.objectives, base);
+    fix(G.nds0.rows256, base); fix(G.nds0.npad256, base); fix(G.nds0.blocks64, base);
+    fix(G.nds1.rows256, base); fix(G.nds1.npad256, base); fix(G.nds1.blocks64, base);
+}
+
+// Builds the grids for the instances in `state` and assigns their sort-key
+// offsets (contiguous in slot order).  Returns the rows sorted (sum of 2n).
+long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grids& G)
+{
+    const int count = static_cast<int>(host.size());
+    std::vector<long long> init256(count, 0), rows256(count, 0), sbx256(count, 0), merged256(count, 0),
+        objectives(count, 0), rows4(count, 0), mlp_tiles(count, 0), n0_rows(count, 0), n0_npad(count, 0),
+        n0_w(count, 0), n1_rows(count, 0), n1_npad(count, 0), n1_w(count, 0);
+    std::vector<std::vector<long long>> q4(kMaxM + 1, std::vector<long long>(count, 0)),
+        h4(kMaxM + 1, std::vector<long long>(count, 0)), prep(kMaxM + 1, std::vector<long long>(count, 0)),
+        assoc(kMaxM + 1, std::vector<long long>(count, 0)), dom0(kMaxM + 1, std::vector<long long>(count, 0)),
+        dom1(kMaxM + 1, std::vector<long long>(count, 0));
     int max_k = 1, max_nm = 1, max_W0 = 1, max_W1 = 1;
     long long total_merged = 0, total_n = 0;
     for (int i = 0; i < count; ++i) {
-        const moea_batch_problem& p = B.probs[i];
-        const InstSizes z = sizes_of(p);
-        BInst& I = B.host[i];
-        I.n = z.n; I.m = z.m; I.d = z.d; I.k = z.k;
-        I.nm = z.nm; I.W = z.W; I.Wc = z.Wc; I.npad = z.npad;
-        I.max_gen = p.max_generations;
-        I.nchunks = z.nchunks;
-        I.seed = p.seed;
-        I.sbx_eta0 = p.sbx_eta[0]; I.sbx_eta1 = p.sbx_eta[1];
-        I.sbx_prob = p.sbx_prob; I.sbx_copy = p.sbx_copy_prob;
-        I.pm_eta0 = p.pm_eta[0]; I.pm_eta1 = p.pm_eta[1]; I.pm_prob = p.pm_prob;
-        I.centers = p.centers; I.bounds = p.bounds; I.dirs = p.directions;
-        init256.push_back(ceil_div(static_cast<long long>(z.n) * z.d, 256));
-        rows256.push_back(ceil_div(z.n, 256));
-        sbx256.push_back(ceil_div(static_cast<long long>(z.n / 2) * z.d + ((z.n & 1) ? z.d : 0), 256));
-        merged256.push_back(z.nb);
-        objectives.push_back(z.m);
-        rows4.push_back(ceil_div(z.n, 4));
-        rows4_m[z.m][i] = ceil_div(z.n, 4);
-        prep_m[z.m][i] = z.nb;
-        assoc_m[z.m][i] = static_cast<long long>(z.nb) * z.nchunks;
-        const int W0 = ceil_div(z.n, 64), Wc0 = ceil_div(W0, 8);
-        n0_rows.push_back(ceil_div(z.n, 256));
-        n0_npad.push_back(ceil_div(W0 * 64, 256));
-        n0_w.push_back(W0);
-        dom0[z.m][i] = ceil_div(4ll * Wc0 * (Wc0 + 1), 4);
-        n1_rows.push_back(ceil_div(z.nm, 256));
-        n1_npad.push_back(ceil_div(z.npad, 256));
-        n1_w.push_back(z.W);
-        dom1[z.m][i] = ceil_div(4ll * z.Wc * (z.Wc + 1), 4);
-        max_k = std::max(max_k, z.k);
-        max_nm = std::max(max_nm, z.nm);
+        BInst& I = host[i];
+        if (I.state != state) continue;
+        const int n = I.n, m = I.m, nm = I.nm;
+        const int W0 = ceil_div(n, 64), Wc0 = ceil_div(W0, 8);
+        const int nb = ceil_div(nm, 256);
+        init256[i] = ceil_div(static_cast<long long>(n) * I.d, 256);
+        rows256[i] = ceil_div(n, 256);
+        sbx256[i] = ceil_div(static_cast<long long>(n / 2) * I.d + ((n & 1) ? I.d : 0), 256);
+        merged256[i] = nb;
+        objectives[i] = m;
+        rows4[i] = ceil_div(n, 4);
+        if (I.problem == MOEA_PROBLEM_MLP) {
+            mlp_tiles[i] = static_cast<long long>(ceil_div(n, 64)) * ceil_div(I.hidden, 64);
+            h4[m][i] = ceil_div(n, 4);
+        } else {
+            q4[m][i] = ceil_div(n, 4);
+        }
+        prep[m][i] = nb;
+        assoc[m][i] = static_cast<long long>(nb) * I.nchunks;
+        n0_rows[i] = ceil_div(n, 256);
+        n0_npad[i] = ceil_div(W0 * 64, 256);
+        n0_w[i] = W0;
+        dom0[m][i] = ceil_div(4ll * Wc0 * (Wc0 + 1), 4);
+        n1_rows[i] = ceil_div(nm, 256);
+        n1_npad[i] = ceil_div(I.npad, 256);
+        n1_w[i] = I.W;
+        dom1[m][i] = ceil_div(4ll * I.Wc * (I.Wc + 1), 4);
+        max_k = std::max(max_k, I.k);
+        max_nm = std::max(max_nm, nm);
         max_W0 = std::max(max_W0, W0);
-        max_W1 = std::max(max_W1, z.W);
-        total_merged += z.nm;
-        total_n += z.n;
+        max_W1 = std::max(max_W1, I.W);
+        I.key_off = total_merged;
+        I.key_off_n = total_n;
+        total_merged += nm;
+        total_n += n;
     }
-    std::vector<int>& T = B.tables;
-    B.init256 = add_grid(T, init256);
-    B.rows256 = add_grid(T, rows256);
-    B.sbx256 = add_grid(T, sbx256);
-    for (int m = 1; m <= kMaxM; ++m) B.rows4_m[m] = add_grid(T, rows4_m[m]);
-    BSelGrids& G = B.sel;
-    G.merged256 = add_grid(T, merged256);
-    G.rows4 = add_grid(T, rows4);
-    G.objectives = add_grid(T, objectives);
+    G.init256 = T.add(init256);
+    G.rows256 = T.add(rows256);
+    G.sbx256 = T.add(sbx256);
     for (int m = 1; m <= kMaxM; ++m) {
-        G.norm_prep[m] = add_grid(T, prep_m[m]);
-        G.assoc[m] = add_grid(T, assoc_m[m]);
+        G.eval.quad_rows4[m] = T.add(q4[m]);
+        G.eval.mlp_rows4[m] = T.add(h4[m]);
     }
-    G.ref_bits = bitlen(max_k + 1);
-    G.rank_bits = bitlen(max_nm);
-    G.total_merged = total_merged;
-    B.nds0.rows256 = add_grid(T, n0_rows);
-    B.nds0.npad256 = add_grid(T, n0_npad);
-    B.nds0.blocks64 = add_grid(T, n0_w);
-    B.nds1.rows256 = add_grid(T, n1_rows);
-    B.nds1.npad256 = add_grid(T, n1_npad);
-    B.nds1.blocks64 = add_grid(T, n1_w);
+    G.eval.mlp_tiles = T.add(mlp_tiles);
+    G.sel.merged256 = T.add(merged256);
+    G.sel.rows4 = T.add(rows4);
+    G.sel.objectives = T.add(objectives);
     for (int m = 1; m <= kMaxM; ++m) {
-        B.nds0.dom[m] = add_grid(T, dom0[m]);
-        B.nds1.dom[m] = add_grid(T, dom1[m]);
+        G.sel.norm_prep[m] = T.add(prep[m]);
+        G.sel.assoc[m] = T.add(assoc[m]);
     }
-    B.nds0.max_W = max_W0;
-    B.nds1.max_W = max_W1;
-    B.nds0.total_rows = total_n;
-    B.nds1.total_rows = total_merged;
-    B.scratch.inst_bits = bitlen(count - 1);
-    // the widest sort key: instance bits above 32 + rank bits (niching) or 34 (NDS)
-    if (32 + std::max(G.ref_bits, G.rank_bits) + B.scratch.inst_bits > 64 || 34 + B.scratch.inst_bits > 64)
-        return fail(kUnsupported, "moea_batch: too many instances for the sort key width");
+    G.sel.ref_bits = bitlen(max_k + 1);
+    G.sel.rank_bits = bitlen(max_nm);
+    G.sel.total_merged = total_merged;
+    G.nds0.rows256 = T.add(n0_rows);
+    G.nds0.npad256 = T.add(n0_npad);
+    G.nds0.blocks64 = T.add(n0_w);
+    G.nds1.rows256 = T.add(n1_rows);
+    G.nds1.npad256 = T.add(n1_npad);
+    G.nds1.blocks64 = T.add(n1_w);
+    for (int m = 1; m <= kMaxM; ++m) {
+        G.nds0.dom[m] = T.add(dom0[m]);
+        G.nds1.dom[m] = T.add(dom1[m]);
+    }
+    G.nds0.max_W = max_W0;
+    G.nds1.max_W = max_W1;
+    G.nds0.total_rows = total_n;
+    G.nds1.total_rows = total_merged;
+    return total_merged;
+}
+
+// Prefix tables per state: (1 + 3 + 2 kMaxM + 1 + 3 + 2 kMaxM + 3 + 3 + 2 kMaxM) grids.
+constexpr int kGridsPerState = 14 + 6 * kMaxM;
+
+}  // namespace
+
+// Shared state of a batch or a server.
+struct BatchCore {
+    int slots = 0;
+    std::vector<BInst> host;                  // host mirror of the device table
+    std::vector<moea_batch_problem> probs;
+    BInst* dev_insts = nullptr;
+    int* dev_tables = nullptr;
+    size_t table_ints = 0;
+    BNdsScratch scratch{};
+    long long max_rows = 0;                   // capacity of the sort scratch (sum of 2n)
+    hipStream_t stream{};
+    Grids run{}, init{};
+    bool dirty = true;                        // membership changed since the last upload
+};
+
+namespace {
+
+size_t core_reserved(int slots, long long max_rows, Carve& c, BatchCore* C)
+{
+    BInst* insts = c.take<BInst>(slots);
+    const size_t table_ints = static_cast<size_t>(2) * kGridsPerState * (slots + 1);
+    int* tables = c.take<int>(table_ints);
+    uint64_t* k_in = c.take<uint64_t>(max_rows);
+    uint64_t* k_out = c.take<uint64_t>(max_rows);
+    int32_t* v_in = c.take<int32_t>(max_rows);
+    int32_t* v_out = c.take<int32_t>(max_rows);
+    const int ib = bitlen(slots - 1);
+    size_t sort_bytes = 0;
+    for (int w = 34 + ib; w <= 64; ++w)
+        sort_bytes = std::max(sort_bytes, sort_pairs_temp_bytes(static_cast<int>(max_rows), w));
+    void* tmp = c.take_bytes(sort_bytes);
+    if (C) {
+        C->dev_insts = insts;
+        C->dev_tables = tables;
+        C->table_ints = table_ints;
+        C->scratch = BNdsScratch{k_in, k_out, v_in, v_out, tmp, sort_bytes, ib};
+    }
+    return c.used;
+}
+
+// Rebuilds both grid sets and uploads the tables and the instance table
+// (synchronizing the stream first: the device table may be in use).
+int core_upload(BatchCore& C)
+{
+    TableBuilder T;
+    const long long run_rows = build_grids(C.host, kStateRunning, T, C.run);
+    const long long init_rows = build_grids(C.host, kStateAdmitted, T, C.init);
+    if (run_rows > C.max_rows || init_rows > C.max_rows)
+        return fail(kWorkspaceTooSmall, "moea_batch: more rows than the sort scratch holds");
+    if (T.t.size() > C.table_ints) return fail(kInvalidArgument, "moea_batch: table capacity exceeded");
+    MOEA_HIP(hipStreamSynchronize(C.stream));
+    MOEA_HIP(hipMemcpy(C.dev_tables, T.t.data(), T.t.size() * sizeof(int), hipMemcpyHostToDevice));
+    MOEA_HIP(hipMemcpy(C.dev_insts, C.host.data(), C.host.size() * sizeof(BInst), hipMemcpyHostToDevice));
+    fix_grids(C.run, C.dev_tables);
+    fix_grids(C.init, C.dev_tables);
+    C.dirty = false;
     return kOk;
 }
 
-void fix_all(BatchState& B)
+bool any_state(const BatchCore& C, int state)
 {
-    int* base = B.dev_tables;
-    fix(B.init256, base);
-    fix(B.rows256, base);
-    fix(B.sbx256, base);
-    for (int m = 1; m <= kMaxM; ++m) fix(B.rows4_m[m], base);
-    fix(B.sel.merged256, base);
-    fix(B.sel.rows4, base);
-    fix(B.sel.objectives, base);
-    for (int m = 1; m <= kMaxM; ++m) {
-        fix(B.sel.norm_prep[m], base);
-        fix(B.sel.assoc[m], base);
-        fix(B.nds0.dom[m], base);
-        fix(B.nds1.dom[m], base);
-    }
-    fix(B.nds0.rows256, base);
-    fix(B.nds0.npad256, base);
-    fix(B.nds0.blocks64, base);
-    fix(B.nds1.rows256, base);
-    fix(B.nds1.npad256, base);
-    fix(B.nds1.blocks64, base);
+    for (const BInst& I : C.host)
+        if (I.state == state) return true;
+    return false;
 }
 
-int validate(const moea_batch_problem* probs, int count)
+// Initializes the admitted instances (state 2 -> 1): initial population,
+// evaluation at generation -1, parent fronts and ideal point.
+int core_init_admitted(BatchCore& C)
+{
+    if (!any_state(C, kStateAdmitted)) return kOk;
+    if (C.dirty) MOEA_CHECK(core_upload(C));
+    BatchLaunch L{C.dev_insts, C.slots, C.stream};
+    MOEA_CHECK(batch_init_population(L, C.init.init256.map, C.init.init256.blocks));
+    MOEA_CHECK(batch_evaluate(L, C.init.eval, 0));
+    MOEA_CHECK(batch_nds(L, C.init.nds0, C.scratch, 0, kStateAdmitted));
+    MOEA_CHECK(batch_ideal(L, C.init.sel, 0));
+    MOEA_CHECK(batch_advance(L, kStateAdmitted, kStateRunning, 0));
+    for (BInst& I : C.host)
+        if (I.state == kStateAdmitted) I.state = kStateRunning;
+    C.dirty = true;
+    return kOk;
+}
+
+// One generation of every running instance; instances reaching their
+// generation count become finished.  Appends finished slots to `done`.
+int core_step(BatchCore& C, std::vector<int>* done)
+{
+    if (!any_state(C, kStateRunning)) return kOk;
+    if (C.dirty) MOEA_CHECK(core_upload(C));
+    BatchLaunch L{C.dev_insts, C.slots, C.stream};
+    stage_begin(kStReproduce, C.stream);
+    MOEA_CHECK(batch_reproduce(L, C.run.rows256.map, C.run.rows256.blocks, C.run.sbx256.map, C.run.sbx256.blocks));
+    stage_end(kStReproduce, C.stream);
+    stage_begin(kStEvaluate, C.stream);
+    MOEA_CHECK(batch_evaluate(L, C.run.eval, 1));
+    stage_end(kStEvaluate, C.stream);
+    MOEA_CHECK(batch_select(L, C.run.sel, C.run.nds1, C.scratch));
+    MOEA_CHECK(batch_advance(L, kStateRunning, kStateRunning, 1));
+    for (int i = 0; i < C.slots; ++i) {
+        BInst& I = C.host[i];
+        if (I.state != kStateRunning) continue;
+        I.cur = (I.cur + 2) % 3;
+        I.fcur = 1 - I.fcur;
+        I.gen += 1;
+        if (I.gen >= I.max_gen) {
+            I.state = kStateFinished;
+            C.dirty = true;
+            if (done) done->push_back(i);
+        }
+    }
+    return kOk;
+}
+
+int core_view(const BatchCore& C, int slot, moea_population* out, int32_t** fronts)
+{
+    if (slot < 0 || slot >= C.slots || !out) return fail(kInvalidArgument, "moea_batch: slot out of range");
+    const BInst& I = C.host[slot];
+    if (I.state == kStateFree) return fail(kInvalidArgument, "moea_batch: slot is free");
+    out->x = I.x[I.cur];
+    out->f = I.f[I.cur];
+    out->cv = I.cv[I.cur];
+    if (fronts) *fronts = I.fronts[I.fcur];
+    return kOk;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batch --
+struct BatchState {
+    BatchCore core;
+    int generation = 0, max_generations = 0;
+};
+
+namespace {
+
+int validate_batch(const moea_batch_problem* probs, int count)
 {
     if (!probs || count <= 0) return fail(kInvalidArgument, "moea_batch: no problems");
     for (int i = 0; i < count; ++i) {
-        const moea_batch_problem& p = probs[i];
-        if (p.n <= 0 || (p.n & 1) || p.m <= 0 || p.m > kMaxM || p.d <= 0 || p.k <= 0 || p.max_generations <= 0)
-            return fail(kInvalidArgument, "moea_batch: invalid sizes (n must be even and positive, 1 <= m <= 10)");
-        if (2 * p.n > nds_max_n()) return fail(kUnsupported, "moea_batch: population too large");
-        if (!p.centers || !p.bounds || !p.directions)
-            return fail(kInvalidArgument, "moea_batch: centers, bounds and directions are required");
-        if (p.max_generations != probs[0].max_generations)
+        MOEA_CHECK(validate_problem(probs[i]));
+        if (probs[i].max_generations != probs[0].max_generations)
             return fail(kInvalidArgument, "moea_batch: every instance of a batch runs the same number of generations");
     }
     return kOk;
+}
+
+size_t batch_layout(const moea_batch_problem* probs, int count, char* base, BatchCore* C)
+{
+    long long rows = 0;
+    for (int i = 0; i < count; ++i) rows += 2ll * probs[i].n;
+    Carve c(base, ~size_t{0});
+    core_reserved(count, rows, c, C);
+    for (int i = 0; i < count; ++i) {
+        BInst I{};
+        carve_instance(c, probs[i], I);
+        if (C) {
+            fill_record(probs[i], I);
+            C->host[i] = I;
+        }
+    }
+    return c.used;
 }
 
 }  // namespace
 
 size_t batch_workspace_bytes(const moea_batch_problem* probs, int count)
 {
-    if (validate(probs, count) != kOk) return 0;
-    BatchState B;
-    B.count = count;
-    B.probs.assign(probs, probs + count);
-    if (plan(B) != kOk) return 0;
-    return layout(B, nullptr);
+    if (validate_batch(probs, count) != kOk) return 0;
+    return batch_layout(probs, count, nullptr, nullptr);
 }
 
 int batch_create(const moea_batch_problem* probs, int count, void* ws, size_t ws_bytes, hipStream_t stream,
                  BatchState** out)
 {
-    MOEA_CHECK(validate(probs, count));
+    MOEA_CHECK(validate_batch(probs, count));
     if (!out) return fail(kInvalidArgument, "moea_batch_create: null output handle");
+    const size_t need = batch_layout(probs, count, nullptr, nullptr);
+    if (!ws || ws_bytes < need) return fail(kWorkspaceTooSmall, "moea_batch_create: workspace too small");
     BatchState* B = new (std::nothrow) BatchState();
     if (!B) return fail(kInvalidArgument, "moea_batch_create: out of host memory");
-    B->count = count;
-    B->probs.assign(probs, probs + count);
-    B->stream = stream;
+    BatchCore& C = B->core;
+    C.slots = count;
+    C.host.assign(count, BInst{});
+    C.probs.assign(probs, probs + count);
+    C.stream = stream;
+    for (int i = 0; i < count; ++i) C.max_rows += 2ll * probs[i].n;
+    batch_layout(probs, count, static_cast<char*>(ws), &C);
     B->max_generations = probs[0].max_generations;
-    int st = plan(*B);
-    if (st != kOk) { delete B; return st; }
-    const size_t need = layout(*B, nullptr);
-    if (!ws || ws_bytes < need) {
-        delete B;
-        return fail(kWorkspaceTooSmall, "moea_batch_create: workspace too small");
-    }
-    layout(*B, static_cast<char*>(ws));
-    // tables, then the instance records that point into the workspace
-    hipError_t e = hipMemcpyAsync(B->dev_tables, B->tables.data(), B->tables.size() * sizeof(int),
-                                  hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(B->dev_insts, B->host.data(), B->host.size() * sizeof(BInst), hipMemcpyHostToDevice,
-                           stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) {
-        delete B;
-        return hip_fail(e, "moea_batch_create: table upload");
-    }
-    fix_all(*B);
     *out = B;
     return kOk;
 }
 
 int batch_init(BatchState& B)
 {
-    BatchLaunch L{B.dev_insts, B.count, B.stream};
-    B.current = 0;
-    B.fronts_current = 0;
-    MOEA_CHECK(batch_init_population(L, B.init256.map, B.init256.blocks, B.current));
-    for (int m = 1; m <= kMaxM; ++m)
-        MOEA_CHECK(batch_eval_quadratic(L, m, B.rows4_m[m].map, B.rows4_m[m].blocks, B.current));
-    MOEA_CHECK(batch_nds(L, B.nds0, B.scratch, 0, B.current, B.fronts_current));
-    MOEA_CHECK(batch_ideal(L, B.sel, 0, B.current));
+    BatchCore& C = B.core;
+    for (int i = 0; i < C.slots; ++i) {
+        BInst& I = C.host[i];
+        I.state = kStateAdmitted;
+        I.cur = 0; I.fcur = 0; I.gen = 0;
+    }
+    C.dirty = true;
+    MOEA_CHECK(core_init_admitted(C));
     B.generation = 0;
-    B.initialized = true;
     return kOk;
 }
 
 int batch_step(BatchState& B, int generations)
 {
-    if (!B.initialized) return fail(kInvalidArgument, "moea_batch_step: call moea_batch_init first");
-    BatchLaunch L{B.dev_insts, B.count, B.stream};
+    if (!any_state(B.core, kStateRunning) && B.generation == 0)
+        return fail(kInvalidArgument, "moea_batch_step: call moea_batch_init first");
     for (int it = 0; it < generations && B.generation < B.max_generations; ++it) {
-        const int g = B.generation;
-        const int cur = B.current, off = (cur + 1) % 3, nxt = (cur + 2) % 3;
-        stage_begin(kStReproduce, B.stream);
-        MOEA_CHECK(batch_reproduce(L, B.rows256.map, B.rows256.blocks, B.sbx256.map, B.sbx256.blocks, cur, off,
-                                   B.fronts_current, g));
-        stage_end(kStReproduce, B.stream);
-        stage_begin(kStEvaluate, B.stream);
-        for (int m = 1; m <= kMaxM; ++m)
-            MOEA_CHECK(batch_eval_quadratic(L, m, B.rows4_m[m].map, B.rows4_m[m].blocks, off));
-        stage_end(kStEvaluate, B.stream);
-        MOEA_CHECK(batch_select(L, B.sel, B.nds1, B.scratch, cur, off, nxt, 1 - B.fronts_current, g));
-        B.fronts_current = 1 - B.fronts_current;
-        B.current = nxt;
-        B.generation = g + 1;
+        MOEA_CHECK(core_step(B.core, nullptr));
+        B.generation += 1;
     }
+    return kOk;
+}
+
+// ------------------------------------------------------------------ serve --
+struct ServeState {
+    BatchCore core;
+    size_t arena_off = 0, arena_bytes = 0;    // instance arena inside the workspace
+    char* base = nullptr;
+    std::vector<std::pair<size_t, size_t>> free_list;   // (offset, bytes), sorted by offset
+    std::vector<std::pair<size_t, size_t>> owned;       // per slot: (offset, bytes)
+};
+
+namespace {
+
+constexpr size_t kArenaAlign = 256;
+
+bool arena_alloc(ServeState& S, size_t bytes, size_t& off)
+{
+    bytes = align_up(bytes, kArenaAlign);
+    for (size_t i = 0; i < S.free_list.size(); ++i) {
+        auto& r = S.free_list[i];
+        if (r.second >= bytes) {
+            off = r.first;
+            r.first += bytes;
+            r.second -= bytes;
+            if (r.second == 0) S.free_list.erase(S.free_list.begin() + static_cast<long>(i));
+            return true;
+        }
+    }
+    return false;
+}
+
+void arena_free(ServeState& S, size_t off, size_t bytes)
+{
+    bytes = align_up(bytes, kArenaAlign);
+    auto it = std::lower_bound(S.free_list.begin(), S.free_list.end(), std::make_pair(off, size_t{0}));
+    it = S.free_list.insert(it, {off, bytes});
+    // coalesce with the next and the previous range
+    auto nx = it + 1;
+    if (nx != S.free_list.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        S.free_list.erase(nx);
+    }
+    if (it != S.free_list.begin()) {
+        auto pv = it - 1;
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            S.free_list.erase(it);
+        }
+    }
+}
+
+}  // namespace
+
+size_t serve_reserved_bytes(int max_instances, long long max_rows)
+{
+    if (max_instances <= 0 || max_rows <= 0) return 0;
+    Carve c(nullptr, ~size_t{0});
+    return align_up(core_reserved(max_instances, max_rows, c, nullptr), kArenaAlign);
+}
+
+int serve_create(const moea_serve_config& cfg, void* ws, size_t ws_bytes, hipStream_t stream, ServeState** out)
+{
+    if (!out) return fail(kInvalidArgument, "moea_serve_create: null output handle");
+    if (cfg.max_instances <= 0 || cfg.max_rows <= 0 || cfg.max_rows > (1ll << 30))
+        return fail(kInvalidArgument, "moea_serve_create: invalid configuration");
+    const size_t reserved = serve_reserved_bytes(cfg.max_instances, cfg.max_rows);
+    if (!ws || ws_bytes <= reserved) return fail(kWorkspaceTooSmall, "moea_serve_create: workspace too small");
+    ServeState* S = new (std::nothrow) ServeState();
+    if (!S) return fail(kInvalidArgument, "moea_serve_create: out of host memory");
+    BatchCore& C = S->core;
+    C.slots = cfg.max_instances;
+    C.host.assign(C.slots, BInst{});
+    C.probs.assign(C.slots, moea_batch_problem{});
+    C.stream = stream;
+    C.max_rows = cfg.max_rows;
+    Carve c(ws, ws_bytes);
+    core_reserved(C.slots, C.max_rows, c, &C);
+    S->base = static_cast<char*>(ws);
+    S->arena_off = reserved;
+    S->arena_bytes = ws_bytes - reserved;
+    S->free_list.push_back({reserved, S->arena_bytes});
+    S->owned.assign(C.slots, {0, 0});
+    *out = S;
+    return kOk;
+}
+
+int serve_admit(ServeState& S, const moea_batch_problem& p, int* slot_out)
+{
+    MOEA_CHECK(validate_problem(p));
+    BatchCore& C = S.core;
+    int slot = -1;
+    for (int i = 0; i < C.slots; ++i)
+        if (C.host[i].state == kStateFree) { slot = i; break; }
+    if (slot < 0) return fail(kWorkspaceTooSmall, "moea_serve_admit: no free slot");
+    long long rows_run = 0, rows_init = 0;
+    for (const BInst& I : C.host) {
+        if (I.state == kStateRunning || I.state == kStateAdmitted) rows_run += I.nm;
+        if (I.state == kStateAdmitted) rows_init += I.n;
+    }
+    if (rows_run + 2ll * p.n > C.max_rows) return fail(kWorkspaceTooSmall, "moea_serve_admit: row capacity full");
+    const size_t bytes = instance_bytes(p);
+    size_t off = 0;
+    if (!arena_alloc(S, bytes, off)) return fail(kWorkspaceTooSmall, "moea_serve_admit: HBM budget full");
+    Carve c(S.base + off, bytes);
+    BInst I{};
+    carve_instance(c, p, I);
+    fill_record(p, I);
+    C.host[slot] = I;
+    C.probs[slot] = p;
+    S.owned[slot] = {off, bytes};
+    C.dirty = true;
+    (void)rows_init;
+    *slot_out = slot;
+    return kOk;
+}
+
+int serve_step(ServeState& S, std::vector<int>& done)
+{
+    MOEA_CHECK(core_init_admitted(S.core));
+    return core_step(S.core, &done);
+}
+
+int serve_release(ServeState& S, int slot)
+{
+    BatchCore& C = S.core;
+    if (slot < 0 || slot >= C.slots || C.host[slot].state == kStateFree)
+        return fail(kInvalidArgument, "moea_serve_release: slot is not in use");
+    // the device may still use the buffers: order the release after queued work
+    MOEA_HIP(hipStreamSynchronize(C.stream));
+    arena_free(S, S.owned[slot].first, S.owned[slot].second);
+    S.owned[slot] = {0, 0};
+    C.host[slot] = BInst{};
+    C.dirty = true;
     return kOk;
 }
 
@@ -381,6 +638,10 @@ using namespace moea;
 
 struct moea_batch {
     BatchState* state;
+};
+
+struct moea_serve {
+    ServeState* state;
 };
 
 extern "C" {
@@ -426,15 +687,8 @@ int moea_batch_generation(const moea_batch* batch)
 
 int moea_batch_view(const moea_batch* batch, int index, moea_population* out, int32_t** fronts)
 {
-    if (!batch || !batch->state || !out) return fail(kInvalidArgument, "moea_batch_view: null argument");
-    const BatchState& B = *batch->state;
-    if (index < 0 || index >= B.count) return fail(kInvalidArgument, "moea_batch_view: index out of range");
-    const BInst& I = B.host[index];
-    out->x = I.x[B.current];
-    out->f = I.f[B.current];
-    out->cv = I.cv[B.current];
-    if (fronts) *fronts = I.fronts[B.fronts_current];
-    return kOk;
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_view: null batch");
+    return core_view(batch->state->core, index, out, fronts);
 }
 
 void moea_batch_destroy(moea_batch* batch)
@@ -442,6 +696,80 @@ void moea_batch_destroy(moea_batch* batch)
     if (!batch) return;
     delete batch->state;
     delete batch;
+}
+
+size_t moea_serve_reserved_bytes(const moea_serve_config* config)
+{
+    if (!config) return 0;
+    return serve_reserved_bytes(config->max_instances, config->max_rows);
+}
+
+size_t moea_serve_instance_bytes(const moea_batch_problem* problem)
+{
+    if (!problem || validate_problem(*problem) != kOk) return 0;
+    return align_up(instance_bytes(*problem), 256);
+}
+
+int moea_serve_create(const moea_serve_config* config, void* workspace, size_t workspace_bytes, void* stream,
+                      moea_serve** out)
+{
+    if (!config || !out) return fail(kInvalidArgument, "moea_serve_create: null argument");
+    *out = nullptr;
+    ServeState* st = nullptr;
+    MOEA_CHECK(serve_create(*config, workspace, workspace_bytes, static_cast<hipStream_t>(stream), &st));
+    moea_serve* s = new (std::nothrow) moea_serve{st};
+    if (!s) {
+        delete st;
+        return fail(kInvalidArgument, "moea_serve_create: out of host memory");
+    }
+    *out = s;
+    return kOk;
+}
+
+int moea_serve_admit(moea_serve* server, const moea_batch_problem* problem, int* slot)
+{
+    if (!server || !server->state || !problem || !slot) return fail(kInvalidArgument, "moea_serve_admit: null argument");
+    return serve_admit(*server->state, *problem, slot);
+}
+
+int moea_serve_step(moea_serve* server, int32_t* finished, int max_finished, int* n_finished)
+{
+    if (!server || !server->state || !n_finished) return fail(kInvalidArgument, "moea_serve_step: null argument");
+    std::vector<int> done;
+    MOEA_CHECK(serve_step(*server->state, done));
+    const int n = static_cast<int>(done.size());
+    *n_finished = n;
+    if (finished)
+        for (int i = 0; i < n && i < max_finished; ++i) finished[i] = done[i];
+    return kOk;
+}
+
+int moea_serve_view(const moea_serve* server, int slot, moea_population* out, int32_t** fronts)
+{
+    if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_view: null server");
+    return core_view(server->state->core, slot, out, fronts);
+}
+
+int moea_serve_generation(const moea_serve* server, int slot)
+{
+    if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_generation: null server");
+    const BatchCore& C = server->state->core;
+    if (slot < 0 || slot >= C.slots || C.host[slot].state == 0)
+        return fail(kInvalidArgument, "moea_serve_generation: slot is not in use");
+    return C.host[slot].gen;
+}
+
+int moea_serve_release(moea_serve* server, int slot)
+{
+    if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_release: null server");
+    return serve_release(*server->state, slot);
+}
+
+void moea_serve_destroy(moea_serve* server)
+{
+    if (!server) return;
+    delete server->state;
+    delete server;
 }
 
 }  // extern "C"

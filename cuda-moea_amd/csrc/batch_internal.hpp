@@ -18,6 +18,9 @@
 namespace moea {
 
 struct BInst {
+    // lifecycle: 0 free, 1 running, 2 admitted (initialization pending)
+    int state;
+    int cur, fcur, gen;           // parent slot of x/f/cv, parent fronts slot, generation
     // sizes
     int n, m, d, k;
     int nm, W, Wc, npad;          // NDS over the 2n merged rows
@@ -26,6 +29,11 @@ struct BInst {
     uint64_t seed;
     float sbx_eta0, sbx_eta1, sbx_prob, sbx_copy, pm_eta0, pm_eta1, pm_prob;
     // problem (caller-owned device arrays)
+    int problem;                  // MOEA_PROBLEM_QUADRATIC or MOEA_PROBLEM_MLP
+    const float* w1;              // mlp: (hidden, d)
+    const float* w2;              // mlp: (m, hidden)
+    int hidden;
+    float* hwork;                 // mlp: (n, hidden) scratch
     const float* centers;         // (m, d)
     const float* bounds;          // (2d)
     const float* dirs;            // (m, k)
@@ -133,16 +141,26 @@ struct BNdsScratch {
     int inst_bits;                // bits of the instance index above the 34-bit NDS key
 };
 
+// Evaluation grids: per objective count, the quadratic rows and the MLP
+// hidden-layer tiles and head rows of the instances of that kind.
+struct BEvalGrids {
+    BGrid quad_rows4[kMaxM + 1];
+    BGrid mlp_tiles;              // 64 x 64 tiles of (n, hidden) per MLP instance
+    BGrid mlp_rows4[kMaxM + 1];
+};
+
 // Batched stage launchers (each defined next to its single-population kernels).
-int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks, int cur);       // reproduce.hip
+// `state` selects the instances a launch serves (1 running, 2 initializing);
+// the block maps hold blocks for those instances only.  `which` picks the
+// population slot: 0 the parents (slot cur), 1 the offspring (slot cur + 1).
+int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks);                 // reproduce.hip
 int batch_reproduce(const BatchLaunch& L, const BMap& rows256, int nrows_blocks, const BMap& sbx256,
-                    int nsbx_blocks, int cur, int off, int fcur, int generation);            // reproduce.hip
-int batch_eval_quadratic(const BatchLaunch& L, int m, const BMap& rows4, int nblocks, int slot);  // evaluate.hip
+                    int nsbx_blocks);                                                         // reproduce.hip
+int batch_evaluate(const BatchLaunch& L, const BEvalGrids& G, int which);                     // evaluate.hip
 size_t batch_nds_sort_bytes(long long total_rows, int inst_bits);                             // nds.hip
-int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int slot,
-              int fslot);                                                                     // nds.hip
-int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode, int slot);                // nsga3.hip
-int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S, int cur,
-                 int off, int nxt, int fnext, int generation);                                // nsga3.hip
+int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int state);  // nds.hip
+int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode);                          // nsga3.hip
+int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S);  // nsga3.hip
+int batch_advance(const BatchLaunch& L, int from_state, int to_state, int rotate);              // nsga3.hip
 
 }  // namespace moea

@@ -58,15 +58,17 @@ __global__ __launch_bounds__(256) void quadratic_kernel(const float* __restrict_
     quadratic_row<M>(x, d, c, f, ldf, cv, row, static_cast<int>(threadIdx.x & 63));
 }
 
-// Batched: the instances with M objectives, 4 rows per block (slot `slot`).
+// Batched: the instances with M objectives, 4 rows per block; `which` 0 is
+// the parent slot, 1 the offspring slot.
 template <int M>
-__global__ __launch_bounds__(256) void b_quadratic_kernel(const BInst* __restrict__ insts, BMap mp, int slot)
+__global__ __launch_bounds__(256) void b_quadratic_kernel(const BInst* __restrict__ insts, BMap mp, int which)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
     const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
     if (row >= I.n) return;
+    const int slot = (I.cur + which) % 3;
     quadratic_row<M>(I.x[slot], I.d, I.centers, I.f[slot], I.n, I.cv[slot], row,
                      static_cast<int>(threadIdx.x & 63));
 }
@@ -76,13 +78,12 @@ __global__ __launch_bounds__(256) void b_quadratic_kernel(const BInst* __restric
 // each thread owns a 4 x 4 register sub-tile.
 constexpr int kGT = 64, kGK = 16;
 
-__global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict__ x, int n, int d,
-                                                         const float* __restrict__ w1, int hidden,
-                                                         float* __restrict__ h)
+__device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int n, int d,
+                                                const float* __restrict__ w1, int hidden,
+                                                float* __restrict__ h, int r0, int c0)
 {
     __shared__ float s_x[kGK][kGT + 1];
     __shared__ float s_w[kGK][kGT + 1];
-    const int r0 = blockIdx.x * kGT, c0 = blockIdx.y * kGT;
     const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
     float acc[4][4] = {};
     for (int k0 = 0; k0 < d; k0 += kGK) {
@@ -117,17 +118,20 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict
         }
 }
 
+__global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict__ x, int n, int d,
+                                                         const float* __restrict__ w1, int hidden,
+                                                         float* __restrict__ h)
+{
+    mlp_hidden_tile(x, n, d, w1, hidden, h, static_cast<int>(blockIdx.x) * kGT,
+                    static_cast<int>(blockIdx.y) * kGT);
+}
+
 // Output head: y = W2 h, f_c = y_c^2 + 0.01 * ||x - 0.5||^2; one wave per row.
 template <int M>
-__global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__ x, int n, int d,
-                                                       const float* __restrict__ h, int hidden,
-                                                       const float* __restrict__ w2,
-                                                       float* __restrict__ f, int ldf,
-                                                       float* __restrict__ cv)
+__device__ __forceinline__ void mlp_head_row(const float* __restrict__ x, int d, const float* __restrict__ h,
+                                             int hidden, const float* __restrict__ w2, float* __restrict__ f,
+                                             int ldf, float* __restrict__ cv, int row, int lane)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
     float y[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) y[k] = 0.0f;
@@ -154,6 +158,42 @@ __global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__
         f[static_cast<size_t>(lane) * ldf + row] = fmaf(yk, yk, 0.01f * s);
     }
     if (lane == 0 && cv) cv[row] = 0.0f;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__ x, int n, int d,
+                                                       const float* __restrict__ h, int hidden,
+                                                       const float* __restrict__ w2,
+                                                       float* __restrict__ f, int ldf,
+                                                       float* __restrict__ cv)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    mlp_head_row<M>(x, d, h, hidden, w2, f, ldf, cv, row, static_cast<int>(threadIdx.x & 63));
+}
+
+// Batched MLP: hidden-layer tiles, then the head rows of the instances with M objectives.
+__global__ __launch_bounds__(256) void b_mlp_hidden_kernel(const BInst* __restrict__ insts, BMap mp, int which)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int slot = (I.cur + which) % 3;
+    const int nr = (I.n + kGT - 1) / kGT;
+    mlp_hidden_tile(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, (local % nr) * kGT, (local / nr) * kGT);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void b_mlp_head_kernel(const BInst* __restrict__ insts, BMap mp, int which)
+{
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (row >= I.n) return;
+    const int slot = (I.cur + which) % 3;
+    mlp_head_row<M>(I.x[slot], I.d, I.hwork, I.hidden, I.w2, I.f[slot], I.n, I.cv[slot], row,
+                    static_cast<int>(threadIdx.x & 63));
 }
 
 // DTLZ1 / DTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) with the reference's
@@ -264,12 +304,25 @@ int eval_quadratic(const float* x, int n, int d, const float* centers, int m, fl
     return kOk;
 }
 
-int batch_eval_quadratic(const BatchLaunch& L, int m, const BMap& rows4, int nblocks, int slot)
+int batch_evaluate(const BatchLaunch& L, const BEvalGrids& G, int which)
 {
-    if (nblocks <= 0) return kOk;
-    const dim3 grid(nblocks), block(256);
-    MOEA_LAUNCH_M(m, b_quadratic_kernel, grid, block, 0, L.stream, L.insts, rows4, slot);
-    MOEA_HIP(hipGetLastError());
+    for (int m = 1; m <= kMaxM; ++m) {
+        if (G.quad_rows4[m].blocks <= 0) continue;
+        const dim3 grid(G.quad_rows4[m].blocks), block(256);
+        MOEA_LAUNCH_M(m, b_quadratic_kernel, grid, block, 0, L.stream, L.insts, G.quad_rows4[m].map, which);
+        MOEA_HIP(hipGetLastError());
+    }
+    if (G.mlp_tiles.blocks > 0) {
+        hipLaunchKernelGGL(b_mlp_hidden_kernel, dim3(G.mlp_tiles.blocks), dim3(256), 0, L.stream, L.insts,
+                           G.mlp_tiles.map, which);
+        MOEA_HIP(hipGetLastError());
+        for (int m = 1; m <= kMaxM; ++m) {
+            if (G.mlp_rows4[m].blocks <= 0) continue;
+            const dim3 grid(G.mlp_rows4[m].blocks), block(256);
+            MOEA_LAUNCH_M(m, b_mlp_head_kernel, grid, block, 0, L.stream, L.insts, G.mlp_rows4[m].map, which);
+            MOEA_HIP(hipGetLastError());
+        }
+    }
     return kOk;
 }
 

@@ -547,9 +547,8 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
 }
 
 // ---------------------------------------------------------------- batched --
-// The NDS of every instance of a batch: mode 0 ranks the n parents of slot
-// `slot` into fronts[fslot], mode 1 the 2n merged rows into fronts_mix and
-// meta[0].  Keys carry the instance index above the 34 NDS bits, so one
+// The NDS of every instance of a batch: mode 0 ranks the n parents (slot cur)
+// into fronts[fcur], mode 1 the 2n merged rows into fronts_mix and meta[0].  Keys carry the instance index above the 34 NDS bits, so one
 // stable sort orders every instance's rows exactly as its own sort would.
 struct BRows {
     const float* f;
@@ -559,37 +558,37 @@ struct BRows {
     int32_t* fronts;
 };
 
-__device__ __forceinline__ BRows brows(const BInst& I, int mode, int slot, int fslot)
+__device__ __forceinline__ BRows brows(const BInst& I, int mode)
 {
     BRows r;
     if (mode == 0) {
-        r.f = I.f[slot]; r.cv = I.cv[slot]; r.rows = I.n; r.key_off = I.key_off_n; r.fronts = I.fronts[fslot];
+        r.f = I.f[I.cur]; r.cv = I.cv[I.cur]; r.rows = I.n; r.key_off = I.key_off_n; r.fronts = I.fronts[I.fcur];
     } else {
         r.f = I.fm; r.cv = I.cvm; r.rows = I.nm; r.key_off = I.key_off; r.fronts = I.fronts_mix;
     }
     return r;
 }
 
-__global__ void b_nds_keys(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
-                           uint64_t* __restrict__ keys, int32_t* __restrict__ vals)
+__global__ void b_nds_keys(const BInst* __restrict__ insts, BMap mp, int mode, uint64_t* __restrict__ keys,
+                           int32_t* __restrict__ vals)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
-    const BRows R = brows(I, mode, slot, fslot);
+    const BRows R = brows(I, mode);
     const int s = local * 256 + static_cast<int>(threadIdx.x);
     if (s >= R.rows) return;
     keys[R.key_off + s] = nds_key_of(R.f, R.rows, R.cv, s, I.m) | (static_cast<uint64_t>(i) << kKeyBits);
     vals[R.key_off + s] = s;
 }
 
-__global__ void b_nds_gather(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
+__global__ void b_nds_gather(const BInst* __restrict__ insts, BMap mp, int mode,
                              const int32_t* __restrict__ vals_sorted)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
-    const BRows R = brows(I, mode, slot, fslot);
+    const BRows R = brows(I, mode);
     const int npad = ((R.rows + 63) / 64) * 64;
     const int t = local * 256 + static_cast<int>(threadIdx.x);
     if (t >= npad) return;
@@ -605,24 +604,24 @@ __global__ void b_nds_gather(const BInst* __restrict__ insts, BMap mp, int mode,
     }
 }
 
-__global__ void b_nds_block_info(const BInst* __restrict__ insts, BMap mp, int mode, int slot, int fslot,
+__global__ void b_nds_block_info(const BInst* __restrict__ insts, BMap mp, int mode,
                                  const uint64_t* __restrict__ keys_sorted)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
-    const BRows R = brows(I, mode, slot, fslot);
+    const BRows R = brows(I, mode);
     block_info_one(keys_sorted + R.key_off, R.rows, local, I.kind, I.hi);
 }
 
 template <int M>
 __global__ __launch_bounds__(kDomThreads) void b_nds_dominance(const BInst* __restrict__ insts, BMap mp,
-                                                              int mode, int slot, int fslot)
+                                                              int mode)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
-    const BRows R = brows(I, mode, slot, fslot);
+    const BRows R = brows(I, mode);
     const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
     const long long units = 4ll * Wc * (Wc + 1);
     const long long u = static_cast<long long>(local) * kDomWaves + wave_uniform(static_cast<int>(threadIdx.x >> 6));
@@ -633,14 +632,14 @@ __global__ __launch_bounds__(kDomThreads) void b_nds_dominance(const BInst* __re
 // Front peeling of one instance per 1024-thread workgroup: the two phases of
 // nds_peel with workgroup barriers only.  Tasks go through a global per-
 // instance list (the gidx scratch, free during NDS).
-__global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restrict__ insts, int mode,
-                                                          int slot, int fslot)
+__global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restrict__ insts, int mode, int state)
 {
     extern __shared__ uint64_t s_dyn[];
     __shared__ int s_ntask;
     __shared__ int s_count;
     const BInst& I = insts[blockIdx.x];
-    const BRows R = brows(I, mode, slot, fslot);
+    if (I.state != state) return;
+    const BRows R = brows(I, mode);
     const int rows = R.rows;
     const int W = (rows + 63) / 64, Wc = (W + 7) / 8;
     uint64_t* s_A = s_dyn;            // (W) assigned before this round
@@ -862,28 +861,29 @@ size_t batch_nds_sort_bytes(long long total_rows, int inst_bits)
     return sort_pairs_temp_bytes(static_cast<int>(total_rows), kKeyBits + inst_bits);
 }
 
-int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int slot, int fslot)
+int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int state)
 {
     hipStream_t st = L.stream;
-    hipLaunchKernelGGL(b_nds_keys, dim3(G.rows256.blocks), dim3(256), 0, st, L.insts, G.rows256.map, mode, slot,
-                       fslot, S.keys_in, S.vals_in);
+    if (G.rows256.blocks <= 0) return kOk;
+    hipLaunchKernelGGL(b_nds_keys, dim3(G.rows256.blocks), dim3(256), 0, st, L.insts, G.rows256.map, mode,
+                       S.keys_in, S.vals_in);
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out,
                           static_cast<int>(G.total_rows), st, kKeyBits + S.inst_bits));
     hipLaunchKernelGGL(b_nds_gather, dim3(G.npad256.blocks), dim3(256), 0, st, L.insts, G.npad256.map, mode,
-                       slot, fslot, S.vals_out);
+                       S.vals_out);
     MOEA_HIP(hipGetLastError());
     hipLaunchKernelGGL(b_nds_block_info, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map,
-                       mode, slot, fslot, S.keys_out);
+                       mode, S.keys_out);
     MOEA_HIP(hipGetLastError());
     for (int m = 1; m <= kMaxM; ++m) {
         if (G.dom[m].blocks <= 0) continue;
         const dim3 grid(G.dom[m].blocks), block(kDomThreads);
-        MOEA_LAUNCH_M(m, b_nds_dominance, grid, block, 0, st, L.insts, G.dom[m].map, mode, slot, fslot);
+        MOEA_LAUNCH_M(m, b_nds_dominance, grid, block, 0, st, L.insts, G.dom[m].map, mode);
         MOEA_HIP(hipGetLastError());
     }
     const size_t lds = 2 * static_cast<size_t>(G.max_W) * sizeof(uint64_t);
-    hipLaunchKernelGGL(b_nds_peel, dim3(L.count), dim3(kPeelThreads), lds, st, L.insts, mode, slot, fslot);
+    hipLaunchKernelGGL(b_nds_peel, dim3(L.count), dim3(kPeelThreads), lds, st, L.insts, mode, state);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }

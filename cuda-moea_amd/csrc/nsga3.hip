@@ -780,11 +780,12 @@ namespace {
 // their own bits, so each global stable sort orders every instance's entries
 // exactly as that instance's own sort would.
 
-__global__ void b_merge(const BInst* __restrict__ insts, BMap mp, int cur, int off)
+__global__ void b_merge(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
+    const int cur = I.cur, off = (I.cur + 1) % 3;
     const int t = local * 256 + static_cast<int>(threadIdx.x);
     if (t >= I.nm) return;
     const bool par = t < I.n;
@@ -800,6 +801,7 @@ __global__ __launch_bounds__(kScanThreads) void b_split(const BInst* __restrict_
 {
     __shared__ int s_wave[64];
     const BInst& I = insts[blockIdx.x];
+    if (I.state != 1) return;
     const int mf = I.meta[kMaxFront];
     int tp = 0, tl = 0;
     for (int base = 0; base < I.nm; base += kScanThreads) {
@@ -832,14 +834,14 @@ __global__ __launch_bounds__(kScanThreads) void b_split(const BInst* __restrict_
     }
 }
 
-// mode 0: ideal point of the n parents of slot `slot`; mode 1: update with the
-// offspring columns of the merged objectives.
-__global__ void b_ideal(const BInst* __restrict__ insts, BMap mp, int mode, int slot)
+// mode 0: ideal point of the n parents; mode 1: update with the offspring
+// columns of the merged objectives.
+__global__ void b_ideal(const BInst* __restrict__ insts, BMap mp, int mode)
 {
     int k;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), k);
     const BInst& I = insts[i];
-    if (mode == 0) ideal_body(I.f[slot], I.n, 0, I.n, k, I.ideal, 1);
+    if (mode == 0) ideal_body(I.f[I.cur], I.n, 0, I.n, k, I.ideal, 1);
     else ideal_body(I.fm, I.nm, I.n, I.n, k, I.ideal, 0);
 }
 
@@ -855,6 +857,7 @@ __global__ __launch_bounds__(256) void b_norm_prep(const BInst* __restrict__ ins
 __global__ __launch_bounds__(kSolveThreads) void b_norm_solve(const BInst* __restrict__ insts)
 {
     const BInst& I = insts[blockIdx.x];
+    if (I.state != 1) return;
     norm_solve_body(I.fn, I.nm, I.m, I.asf_val, I.asf_idx, I.row_max, I.meta, I.a);
 }
 
@@ -881,6 +884,7 @@ __global__ void b_assoc_final(const BInst* __restrict__ insts, BMap mp)
 __global__ __launch_bounds__(1024) void b_rho(const BInst* __restrict__ insts)
 {
     const BInst& I = insts[blockIdx.x];
+    if (I.state != 1) return;
     for (int j = threadIdx.x; j < I.k; j += blockDim.x) I.rho[j] = 0;
     __syncthreads();
     if (I.meta[kNRem] <= 0) return;
@@ -904,7 +908,7 @@ __global__ void b_niche_keys1(const BInst* __restrict__ insts, BMap mp, int ref_
 }
 
 __global__ void b_niche_keys2(const BInst* __restrict__ insts, BMap mp, int ref_bits,
-                              const uint64_t* __restrict__ key1s, uint64_t* __restrict__ key2, uint32_t gen)
+                              const uint64_t* __restrict__ key1s, uint64_t* __restrict__ key2)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
@@ -912,11 +916,12 @@ __global__ void b_niche_keys2(const BInst* __restrict__ insts, BMap mp, int ref_
     const int t = local * 256 + static_cast<int>(threadIdx.x);
     if (t >= I.nm) return;
     niche_keys2_body(key1s + I.key_off, I.meta, I.rho, I.k, (1u << ref_bits) - 1u,
-                     static_cast<uint64_t>(i) << (32 + ref_bits), nullptr, I.seed, gen, key2 + I.key_off, t);
+                     static_cast<uint64_t>(i) << (32 + ref_bits), nullptr, I.seed, static_cast<uint32_t>(I.gen),
+                     key2 + I.key_off, t);
 }
 
 __global__ void b_niche_keys3(const BInst* __restrict__ insts, BMap mp, int ref_bits, int rank_bits,
-                              const uint64_t* __restrict__ key2s, uint64_t* __restrict__ key3, uint32_t gen)
+                              const uint64_t* __restrict__ key2s, uint64_t* __restrict__ key3)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
@@ -924,7 +929,8 @@ __global__ void b_niche_keys3(const BInst* __restrict__ insts, BMap mp, int ref_
     const int t = local * 256 + static_cast<int>(threadIdx.x);
     if (t >= I.nm) return;
     niche_keys3_body(key2s + I.key_off, I.meta, I.rho, I.k, I.nm, (1u << ref_bits) - 1u,
-                     static_cast<uint64_t>(i) << (32 + rank_bits), nullptr, I.seed, gen, key3 + I.key_off, t);
+                     static_cast<uint64_t>(i) << (32 + rank_bits), nullptr, I.seed, static_cast<uint32_t>(I.gen),
+                     key3 + I.key_off, t);
 }
 
 // Survivor flags and their ascending compaction, one instance per workgroup.
@@ -933,6 +939,7 @@ __global__ __launch_bounds__(kScanThreads) void b_survivors(const BInst* __restr
 {
     __shared__ int s_wave[64];
     const BInst& I = insts[blockIdx.x];
+    if (I.state != 1) return;
     const int mf = I.meta[kMaxFront];
     for (int t = threadIdx.x; t < I.nm; t += kScanThreads) {
         const int fr = I.fronts_mix[t];
@@ -957,39 +964,63 @@ __global__ __launch_bounds__(kScanThreads) void b_survivors(const BInst* __restr
     if (threadIdx.x == 0) I.meta[5] = o;
 }
 
-__global__ __launch_bounds__(256) void b_gather(const BInst* __restrict__ insts, BMap mp, int cur, int off,
-                                                int nxt, int fnext)
+__global__ __launch_bounds__(256) void b_gather(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
+    const int cur = I.cur, off = (I.cur + 1) % 3, nxt = (I.cur + 2) % 3, fnext = 1 - I.fcur;
     const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
     if (row >= I.n) return;
     gather_row(I.chosen, I.n, I.d, I.m, I.x[cur], I.x[off], I.fm, I.cvm, I.fronts_mix, I.x[nxt], I.f[nxt],
                I.cv[nxt], I.fronts[fnext], row, static_cast<int>(threadIdx.x & 63));
 }
 
+// After a generation (rotate = 1) the parents move to slot cur + 2 and the
+// fronts ping-pong; instances in `from_state` then take `to_state`.
+__global__ void b_advance(BInst* __restrict__ insts, int count, int from_state, int to_state, int rotate)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    BInst& I = insts[i];
+    if (I.state != from_state) return;
+    if (rotate) {
+        I.cur = (I.cur + 2) % 3;
+        I.fcur = 1 - I.fcur;
+        I.gen = I.gen + 1;
+    }
+    I.state = to_state;
+}
+
 }  // namespace
 
-int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode, int slot)
+int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode)
 {
+    if (G.objectives.blocks <= 0) return kOk;
     hipLaunchKernelGGL(b_ideal, dim3(G.objectives.blocks), dim3(1024), 0, L.stream, L.insts, G.objectives.map,
-                       mode, slot);
+                       mode);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
 
-int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S, int cur,
-                 int off, int nxt, int fnext, int generation)
+int batch_advance(const BatchLaunch& L, int from_state, int to_state, int rotate)
+{
+    hipLaunchKernelGGL(b_advance, dim3(ceil_div(L.count, 256)), dim3(256), 0, L.stream, L.insts, L.count,
+                       from_state, to_state, rotate);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S)
 {
     hipStream_t st = L.stream;
-    const uint32_t gen = static_cast<uint32_t>(generation);
     const int total = static_cast<int>(G.total_merged);
-    hipLaunchKernelGGL(b_merge, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map, cur, off);
+    if (G.merged256.blocks <= 0) return kOk;
+    hipLaunchKernelGGL(b_merge, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map);
     MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(batch_nds(L, NG, S, 1, 0, 0));
+    MOEA_CHECK(batch_nds(L, NG, S, 1, 1));
     hipLaunchKernelGGL(b_split, dim3(L.count), dim3(kScanThreads), 0, st, L.insts);
-    MOEA_CHECK(batch_ideal(L, G, 1, 0));
+    MOEA_CHECK(batch_ideal(L, G, 1));
     for (int m = 1; m <= kMaxM; ++m) {
         if (G.norm_prep[m].blocks <= 0) continue;
         const dim3 grid(G.norm_prep[m].blocks), block(256);
@@ -1012,18 +1043,17 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out, total, st,
                           bits_ref));
     hipLaunchKernelGGL(b_niche_keys2, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,
-                       G.ref_bits, S.keys_out, S.keys_in, gen);
+                       G.ref_bits, S.keys_out, S.keys_in);
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_out, S.vals_in, total, st,
                           bits_ref));
     hipLaunchKernelGGL(b_niche_keys3, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,
-                       G.ref_bits, G.rank_bits, S.keys_out, S.keys_in, gen);
+                       G.ref_bits, G.rank_bits, S.keys_out, S.keys_in);
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out, total, st,
                           bits_rank));
     hipLaunchKernelGGL(b_survivors, dim3(L.count), dim3(kScanThreads), 0, st, L.insts, S.vals_out);
-    hipLaunchKernelGGL(b_gather, dim3(G.rows4.blocks), dim3(256), 0, st, L.insts, G.rows4.map, cur, off, nxt,
-                       fnext);
+    hipLaunchKernelGGL(b_gather, dim3(G.rows4.blocks), dim3(256), 0, st, L.insts, G.rows4.map);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }

@@ -200,32 +200,32 @@ __host__ __device__ __forceinline__ float annealed_hd(float begin, float end, in
     return begin + (end - begin) * progress;
 }
 
-__global__ void b_init_kernel(const BInst* __restrict__ insts, BMap mp, int cur)
+__global__ void b_init_kernel(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
     const long long e = static_cast<long long>(local) * blockDim.x + threadIdx.x;
-    if (e < static_cast<long long>(I.n) * I.d) init_elem(I.x[cur], e, I.d, I.bounds, I.seed);
+    if (e < static_cast<long long>(I.n) * I.d) init_elem(I.x[I.cur], e, I.d, I.bounds, I.seed);
 }
 
-__global__ void b_mating_kernel(const BInst* __restrict__ insts, BMap mp, int cur, int fcur, uint32_t gen)
+__global__ void b_mating_kernel(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
     const int t = local * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     if (t >= I.n) return;
-    I.winners[t] = mating_one(t, I.n, I.n, MOEA_MATING_TOURNAMENT, I.fronts[fcur], I.cv[cur], nullptr,
-                              nullptr, I.seed, gen);
+    I.winners[t] = mating_one(t, I.n, I.n, MOEA_MATING_TOURNAMENT, I.fronts[I.fcur], I.cv[I.cur], nullptr,
+                              nullptr, I.seed, static_cast<uint32_t>(I.gen));
 }
 
-__global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__ insts, BMap mp, int cur,
-                                                       int off, int gen)
+__global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
+    const int cur = I.cur, off = (I.cur + 1) % 3, gen = I.gen;
     RepArgs R{};
     R.n = I.n; R.d = I.d; R.pairs = I.n / 2;
     R.px = I.x[cur]; R.winners = I.winners; R.bounds = I.bounds;
@@ -293,21 +293,21 @@ int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, 
     return kOk;
 }
 
-int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks, int cur)
+int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks)
 {
-    hipLaunchKernelGGL(b_init_kernel, dim3(nblocks), dim3(256), 0, L.stream, L.insts, mp, cur);
+    if (nblocks <= 0) return kOk;
+    hipLaunchKernelGGL(b_init_kernel, dim3(nblocks), dim3(256), 0, L.stream, L.insts, mp);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
 
 int batch_reproduce(const BatchLaunch& L, const BMap& rows256, int nrows_blocks, const BMap& sbx256,
-                    int nsbx_blocks, int cur, int off, int fcur, int generation)
+                    int nsbx_blocks)
 {
-    hipLaunchKernelGGL(b_mating_kernel, dim3(nrows_blocks), dim3(256), 0, L.stream, L.insts, rows256, cur,
-                       fcur, static_cast<uint32_t>(generation));
+    if (nrows_blocks <= 0) return kOk;
+    hipLaunchKernelGGL(b_mating_kernel, dim3(nrows_blocks), dim3(256), 0, L.stream, L.insts, rows256);
     MOEA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(b_sbx_pm_kernel, dim3(nsbx_blocks), dim3(256), 0, L.stream, L.insts, sbx256, cur,
-                       off, generation);
+    hipLaunchKernelGGL(b_sbx_pm_kernel, dim3(nsbx_blocks), dim3(256), 0, L.stream, L.insts, sbx256);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }

@@ -94,10 +94,15 @@ class RunArgs(ctypes.Structure):
 class BatchProblem(ctypes.Structure):
     _fields_ = [
         ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int), ("max_generations", c_int),
-        ("seed", c_uint64), ("centers", c_void_p), ("bounds", c_void_p), ("directions", c_void_p),
+        ("seed", c_uint64), ("problem", c_int), ("centers", c_void_p), ("w1", c_void_p),
+        ("w2", c_void_p), ("hidden", c_int), ("bounds", c_void_p), ("directions", c_void_p),
         ("sbx_eta", c_float * 2), ("sbx_prob", c_float), ("sbx_copy_prob", c_float),
         ("pm_eta", c_float * 2), ("pm_prob", c_float),
     ]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("max_instances", c_int), ("max_rows", ctypes.c_longlong)]
 
 
 # name -> (restype, argtypes); every symbol include/moea_hip.h declares
@@ -136,6 +141,15 @@ SIGNATURES = {
     "moea_batch_generation": (c_int, [c_void_p]),
     "moea_batch_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
     "moea_batch_destroy": (None, [c_void_p]),
+    "moea_serve_reserved_bytes": (c_size_t, [POINTER(ServeConfig)]),
+    "moea_serve_instance_bytes": (c_size_t, [POINTER(BatchProblem)]),
+    "moea_serve_create": (c_int, [POINTER(ServeConfig), c_void_p, c_size_t, c_void_p, POINTER(c_void_p)]),
+    "moea_serve_admit": (c_int, [c_void_p, POINTER(BatchProblem), POINTER(c_int)]),
+    "moea_serve_step": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_int)]),
+    "moea_serve_generation": (c_int, [c_void_p, c_int]),
+    "moea_serve_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
+    "moea_serve_release": (c_int, [c_void_p, c_int]),
+    "moea_serve_destroy": (None, [c_void_p]),
 }
 
 

@@ -331,14 +331,20 @@ int moea_run_step(moea_run_args* args, int generations);
 /* Batched NSGA-III                                                          */
 /* ------------------------------------------------------------------------ */
 
-/* One instance of a batch: an NSGA-III run with quadratic-center objectives,
- * tournament mating, SBX and polynomial mutation (the moea_run_* operators).
- * All instances of a batch run the same number of generations. */
+/* One instance of a batch: an NSGA-III run with tournament mating, SBX and
+ * polynomial mutation (the moea_run_* operators) on a built-in objective:
+ * quadratic-center (problem MOEA_PROBLEM_QUADRATIC, centers) or the seeded MLP
+ * (MOEA_PROBLEM_MLP, w1 / w2 / hidden).  All instances of a moea_batch run the
+ * same number of generations; a moea_serve instance runs its own. */
 typedef struct moea_batch_problem {
     int n, m, d, k;            /* population (even), objectives (<= 10), variables, directions */
     int max_generations;
     uint64_t seed;
-    const float* centers;      /* (m, d) device */
+    int problem;               /* MOEA_PROBLEM_QUADRATIC or MOEA_PROBLEM_MLP */
+    const float* centers;      /* quadratic: (m, d) device */
+    const float* w1;           /* mlp: (hidden, d) device */
+    const float* w2;           /* mlp: (m, hidden) device */
+    int hidden;
     const float* bounds;       /* (2d) device */
     const float* directions;   /* (m, k) device, unit length */
     float sbx_eta[2], sbx_prob, sbx_copy_prob;
@@ -365,6 +371,48 @@ int moea_batch_generation(const moea_batch* batch);
  * cv (n)) and its fronts (n); valid until the next moea_batch_step. */
 int moea_batch_view(const moea_batch* batch, int index, moea_population* out, int32_t** fronts);
 void moea_batch_destroy(moea_batch* batch);
+
+/* ------------------------------------------------------------------------ */
+/* Served NSGA-III (continuous batching)                                     */
+/* ------------------------------------------------------------------------ */
+
+/* No counterpart in the reference, which runs one population per
+ * EvolutionaryAlgorithm.  A server keeps up to max_instances runs in slots and
+ * advances every running one by a generation per moea_serve_step, with the
+ * launch sequence of moea_batch_step; runs are admitted and released between
+ * steps.  Each run's buffers (moea_serve_instance_bytes) come from an arena:
+ * the part of the caller's workspace beyond moea_serve_reserved_bytes, so the
+ * workspace size is the HBM budget.  A run computes exactly what moea_run_init
+ * and moea_run_step compute for it alone, whatever else shares the steps. */
+typedef struct moea_serve_config {
+    int max_instances;         /* slots */
+    long long max_rows;        /* bound on the sum of 2n over live runs (sort scratch) */
+} moea_serve_config;
+
+typedef struct moea_serve moea_serve;   /* host handle */
+
+size_t moea_serve_reserved_bytes(const moea_serve_config* config);
+/* Arena bytes one run takes (0 for an invalid problem). */
+size_t moea_serve_instance_bytes(const moea_batch_problem* problem);
+int moea_serve_create(const moea_serve_config* config, void* workspace, size_t workspace_bytes,
+                      void* stream, moea_serve** out);
+/* Places a run in a free slot (MOEA_E_WORKSPACE when no slot, rows or arena
+ * space is free); its initialization runs at the start of the next step.  The
+ * problem's device arrays must outlive the run. */
+int moea_serve_admit(moea_serve* server, const moea_batch_problem* problem, int* slot);
+/* Initializes newly admitted runs, then advances every running run by one
+ * generation.  Slots of runs that reached max_generations in this step are
+ * written to `finished` (up to max_finished; n_finished gets the full count).
+ * Synchronizes the stream only when the set of runs changed. */
+int moea_serve_step(moea_serve* server, int32_t* finished, int max_finished, int* n_finished);
+/* Generations completed by the run in `slot`. */
+int moea_serve_generation(const moea_serve* server, int slot);
+/* Device views of the run in `slot` (as moea_batch_view); valid until the next
+ * step or the run's release. */
+int moea_serve_view(const moea_serve* server, int slot, moea_population* out, int32_t** fronts);
+/* Frees the slot and its arena space (synchronizes the stream first). */
+int moea_serve_release(moea_serve* server, int slot);
+void moea_serve_destroy(moea_serve* server);
 
 /* ------------------------------------------------------------------------ */
 /* Profiling                                                                 */

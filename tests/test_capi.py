@@ -73,3 +73,44 @@ def test_api_validation_errors():
         cm.QuadraticCenters(12, 3))
     alg = b.build()
     assert isinstance(alg, cm.Algorithm) and alg.population_size == 92 and not alg.initialized
+
+
+def test_serve_admission_host_logic():
+    """Slot, row and arena limits of moea_serve_admit (host bookkeeping only:
+    the workspace is a host buffer that no device call touches)."""
+    import cuda_moea._lib as L
+    lib = L.lib
+    cfg = L.ServeConfig(max_instances=3, max_rows=1000)
+    reserved = lib.moea_serve_reserved_bytes(ctypes.byref(cfg))
+    assert reserved > 0 and reserved % 256 == 0
+    assert lib.moea_serve_reserved_bytes(ctypes.byref(L.ServeConfig(0, 10))) == 0
+    dummy = (ctypes.c_float * 8)()
+    p = L.BatchProblem()
+    p.n, p.m, p.d, p.k, p.max_generations, p.seed = 100, 3, 20, 91, 5, 1
+    p.problem, p.centers = L.PROBLEM_QUADRATIC, ctypes.addressof(dummy)
+    p.bounds = p.directions = ctypes.addressof(dummy)
+    one = lib.moea_serve_instance_bytes(ctypes.byref(p))
+    assert one > 100 * 20 * 4 * 3 and one % 256 == 0
+    # arena of two and a half instances: two fit, the third is refused for space
+    buf = ctypes.create_string_buffer(reserved + 2 * one + one // 2 + 256)
+    srv = ctypes.c_void_p()
+    base = (ctypes.addressof(buf) + 255) // 256 * 256
+    size = ctypes.sizeof(buf) - (base - ctypes.addressof(buf))
+    assert lib.moea_serve_create(ctypes.byref(cfg), base, size, None, ctypes.byref(srv)) == 0
+    slot = ctypes.c_int(-1)
+    assert lib.moea_serve_admit(srv, ctypes.byref(p), ctypes.byref(slot)) == 0 and slot.value == 0
+    assert lib.moea_serve_admit(srv, ctypes.byref(p), ctypes.byref(slot)) == 0 and slot.value == 1
+    assert lib.moea_serve_admit(srv, ctypes.byref(p), ctypes.byref(slot)) == -3
+    assert b"budget" in lib.moea_last_error()
+    assert lib.moea_serve_generation(srv, 1) == 0
+    assert lib.moea_serve_generation(srv, 2) < 0
+    # row capacity: 2n of a new run beyond max_rows
+    q = L.BatchProblem.from_buffer_copy(p)
+    q.n = 500
+    assert lib.moea_serve_admit(srv, ctypes.byref(q), ctypes.byref(slot)) == -3
+    # invalid problems
+    q.n = 101
+    assert lib.moea_serve_admit(srv, ctypes.byref(q), ctypes.byref(slot)) == -1
+    q.n, q.problem = 100, L.PROBLEM_MLP
+    assert lib.moea_serve_admit(srv, ctypes.byref(q), ctypes.byref(slot)) == -1
+    lib.moea_serve_destroy(srv)

@@ -17,9 +17,20 @@ MIX = [(92, 3, 12, 12, 0), (64, 5, 30, 6, 1), (130, 8, 7, 3, 2), (256, 2, 100, 2
        (200, 3, 64, 0, 4), (1000, 4, 50, 0, 5), (2, 3, 5, 12, 6)]
 
 
+# seeded-MLP instances: (n, m, d, partitions, seed, hidden); hidden need not be a multiple of 64
+MIX_MLP = [(96, 3, 40, 12, 7, 256), (64, 5, 130, 6, 8, 100), (130, 2, 9, 20, 9, 64)]
+
+
 def _run(n, m, d, parts, seed, gens):
     return cm.NSGA3(population_size=n, max_generations=gens,
                     problem=cm.QuadraticCenters(d, m, seed=seed + 100), seed=seed,
+                    reference_directions=cm.DasDennisDirections(partitions=parts),
+                    print_progress=False)
+
+
+def _run_mlp(n, m, d, parts, seed, hidden, gens):
+    return cm.NSGA3(population_size=n, max_generations=gens,
+                    problem=cm.SeededMLP(d, m, hidden=hidden, seed=seed + 200), seed=seed,
                     reference_directions=cm.DasDennisDirections(partitions=parts),
                     print_progress=False)
 
@@ -39,6 +50,16 @@ def test_batch_runs_equal_single_runs(gens):
     for c, res in zip(MIX, results):
         print(f"  instance {c}", flush=True)
         _same(res, _run(*c, gens).run())
+
+
+def test_batch_mixed_objectives_equal_single_runs():
+    gens = 8
+    runs = [_run(*c, gens) for c in MIX[:4]] + [_run_mlp(*c, gens) for c in MIX_MLP]
+    results = cm.BatchNSGA3(runs).run()
+    singles = [_run(*c, gens) for c in MIX[:4]] + [_run_mlp(*c, gens) for c in MIX_MLP]
+    for i, (res, single) in enumerate(zip(results, singles)):
+        print(f"  instance {i}", flush=True)
+        _same(res, single.run())
 
 
 def test_batch_alone_equals_in_mix():
