@@ -124,6 +124,144 @@ def batched_config(generations: int, count: int, plain_sample: int) -> dict:
             "total_individuals": sum(s[0] for s in specs)}
 
 
+def serving_config(seconds: float, rate: float, plain_sample: int, budget_bytes: int) -> dict:
+    """Config 4: a Poisson stream of NSGA-III run requests at `rate`/s for
+    `seconds` s (N log-uniform in [64, 8192], even; M in {3, 5, 8, 10}; D
+    log-uniform in [10, 4096]; G uniform in [50, 500]; quadratic-center or
+    seeded-MLP D->256->M objectives 50/50; directions sized from N) served by
+    one cuda_moea.Server whose workspace is the HBM budget.  The loop admits
+    arrived requests, runs one batched generation of every live run, waits for
+    it, and retires the runs that finished; a request's latency runs from its
+    arrival time to the end of the step that finished it.  One-at-a-time
+    serving is the same arrivals through a FIFO queue with one plain run at a
+    time: each request's service time is its G times a per-generation time
+    drawn from `plain_sample` requests of the stream run alone end to end
+    (cuda_moea.NSGA3.run, allocation and initialization included)."""
+    import collections
+    import math
+    import numpy as np
+    import torch
+    import cuda_moea as cm
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    rng = np.random.default_rng(777)
+    arrivals = []
+    t = rng.exponential(1.0 / rate)
+    while t < seconds:
+        arrivals.append(t)
+        t += rng.exponential(1.0 / rate)
+    total = len(arrivals)
+    specs = []
+    for i in range(total):
+        n = int(round(math.exp(rng.uniform(math.log(64), math.log(8192)))))
+        n += n & 1
+        m = int(rng.choice([3, 5, 8, 10]))
+        d = int(round(math.exp(rng.uniform(math.log(10), math.log(4096)))))
+        g = int(rng.integers(50, 501))
+        specs.append((n, m, d, g, bool(rng.integers(0, 2))))
+    # request payloads are resident in HBM before the stream starts
+    problems = [cm.SeededMLP(d, m, hidden=256, seed=50_000 + i, device=dev) if mlp
+                else cm.QuadraticCenters(d, m, seed=50_000 + i)
+                for i, (n, m, d, g, mlp) in enumerate(specs)]
+
+    def make(i, problem=None):
+        n, m, d, g, _ = specs[i]
+        return cm.NSGA3(population_size=n, max_generations=g, problem=problem or problems[i], seed=i,
+                        print_progress=False)
+
+    # warm-up: every kernel variant once, outside the measurement
+    warm = cm.Server(1 << 30, max_instances=8, max_rows=2 * 8192 * 8)
+    for m in (3, 5, 8, 10):
+        for mlp in (False, True):
+            prob = cm.SeededMLP(64, m, hidden=256, seed=1, device=dev) if mlp else cm.QuadraticCenters(64, m)
+            warm.admit(cm.NSGA3(population_size=128, max_generations=2, problem=prob, seed=0,
+                                print_progress=False))
+    while warm.live:
+        for slot in warm.step():
+            warm.release(slot)
+    stream.synchronize()
+    del warm
+
+    from cuda_moea import _lib
+    srv = cm.Server(budget_bytes, max_instances=512, max_rows=2 * 8192 * 512)
+    _lib.profile_enable(True)
+    owner, queue, latency = {}, collections.deque(), []
+    in_flight, step_ms = [], []
+    nxt, steps = 0, 0
+    t0 = time.perf_counter()
+    report = 10.0
+    while len(latency) < total:
+        now = time.perf_counter() - t0
+        while nxt < total and arrivals[nxt] <= now:
+            queue.append(nxt)
+            nxt += 1
+        while queue:
+            slot = srv.admit(make(queue[0]))
+            if slot is None:
+                break
+            owner[slot] = queue.popleft()
+        if not owner:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        a = time.perf_counter()
+        done = srv.step()
+        stream.synchronize()
+        b = time.perf_counter()
+        step_ms.append((b - a) * 1e3)
+        in_flight.append(len(owner))
+        steps += 1
+        for slot in done:
+            latency.append(b - t0 - arrivals[owner.pop(slot)])
+            srv.release(slot)
+        if b - t0 > report:
+            print(f"serving: t={b - t0:.0f}s arrived={nxt}/{total} done={len(latency)} "
+                  f"in_flight={len(owner)} queued={len(queue)}", file=sys.stderr, flush=True)
+            report += 10.0
+    makespan = time.perf_counter() - t0
+    stages = _lib.profile_read()
+    _lib.profile_enable(False)
+    del srv
+    torch.cuda.empty_cache()
+
+    # one-at-a-time: per-generation service times of sampled requests run alone
+    pick = rng.choice(total, size=min(plain_sample, total), replace=False)
+    per_gen = []
+    for i in pick:
+        stream.synchronize()
+        a = time.perf_counter()
+        make(int(i)).run()
+        stream.synchronize()
+        per_gen.append((time.perf_counter() - a) / specs[int(i)][3])
+    draw = rng.choice(np.asarray(per_gen), size=total)
+    finish, plain_lat = 0.0, []
+    for i in range(total):
+        start = max(arrivals[i], finish)
+        finish = start + draw[i] * specs[i][3]
+        plain_lat.append(finish - arrivals[i])
+    lat = np.asarray(latency)
+    plat = np.asarray(plain_lat)
+    return {"workload": f"config 4: Poisson arrivals {rate:g}/s for {seconds:g} s, N log-uniform [64,8192], "
+                        "M in {3,5,8,10}, D log-uniform [10,4096], G uniform [50,500], quadratic-center / "
+                        "seeded MLP D->256->M 50/50, directions sized from N",
+            "requests": total, "hbm_budget_bytes": budget_bytes,
+            "runs_per_s": round(total / makespan, 2), "makespan_s": round(makespan, 2),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+            "steps": steps, "mean_step_ms": round(float(np.mean(step_ms)), 3),
+            "mean_runs_in_flight": round(float(np.mean(in_flight)), 1),
+            "stage_ms_per_step": {k: round(v[0] / steps, 3) for k, v in stages.items() if v[1]},
+            "one_at_a_time": {
+                "runs_per_s": round(total / finish, 3),
+                "p50_latency_s": round(float(np.percentile(plat, 50)), 2),
+                "p99_latency_s": round(float(np.percentile(plat, 99)), 2),
+                "mean_service_s": round(float(np.mean(draw * np.asarray([s[3] for s in specs]))), 4),
+                "sampled_runs": len(per_gen),
+                "method": "FIFO queue over the same arrivals; service = G x per-generation time of a sampled "
+                          "request run alone (NSGA3.run, wall clock)"},
+            "p50_speedup": round(float(np.percentile(plat, 50) / np.percentile(lat, 50)), 1),
+            "p99_speedup": round(float(np.percentile(plat, 99) / np.percentile(lat, 99)), 1)}
+
+
 def dtlz1_comparison(generations: int, warmup: int) -> dict:
     """NSGA-III on DTLZ1, N = 32768, M = 3, D = 500, in the reference's own
     benchmark setting (SBX eta 30, p 1, per-variable copy probability 0.5; PM
@@ -167,6 +305,11 @@ def main() -> int:
     ap.add_argument("--batched-instances", type=int, default=256)
     ap.add_argument("--batched-generations", type=int, default=300)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-plain-sample", type=int, default=24)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -221,6 +364,11 @@ def main() -> int:
     if not args.no_batched:
         batched = batched_config(args.batched_generations, args.batched_instances, args.plain_sample)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_config(args.serve_seconds, args.serve_rate, args.serve_plain_sample,
+                                 int(args.serve_budget_gb * 1e9))
+
     dtlz1 = None
     if not args.no_dtlz1:
         dtlz1 = dtlz1_comparison(50, 5)
@@ -256,6 +404,7 @@ def main() -> int:
                      "frac_reference_work": round(M * nm * (nm - 1) / dom_avg_s / VALU_PEAK_CMP, 4)},
         "stage_ms_per_generation": stage_ms,
         "batched": batched,
+        "serving": serving,
         "reference_comparison": dtlz1,
         "cpu_baseline": cpu,
     }

@@ -96,6 +96,7 @@ void carve_instance(Carve& c, const moea_batch_problem& p, BInst& I)
     I.hi = c.take<int32_t>(z.W);
     I.kind = c.take<uint8_t>(z.W);
     I.cursor = c.take<int32_t>(z.nm);
+    I.rend = c.take<int32_t>(z.nm);
     I.A = nullptr;
 }
 

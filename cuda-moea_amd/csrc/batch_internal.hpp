@@ -70,6 +70,7 @@ struct BInst {
     int32_t* hi;                  // (W)
     uint8_t* kind;                // (W)
     int32_t* cursor;              // (nm)
+    int32_t* rend;                // (nm) sorted row -> 1 + last word holding a dominator (0: none)
     uint64_t* A;                  // (W) assigned set
     long long key_off;            // first of this instance's nm entries in the batch key arrays
     long long key_off_n;          // first of this instance's n entries (NDS of the parents)

@@ -3,6 +3,7 @@
 //
 //   moea_bench [--n N] [--m M] [--d D] [--gens K] [--warmup W] [--seed S] [--stages]
 //              [--copy-mib C]
+//   moea_bench --mix R [--gens K] [--warmup W] [--seed S] [--stages]
 //
 // Workload: NSGA-III with quadratic-center objectives f_k(x) = sum_j (x_j - c_kj)^2,
 // x in [0, 1]^D, centers c drawn like torch.rand((M, D)) after manual_seed(seed)
@@ -11,8 +12,15 @@
 // defaults.  Prints one JSON line with gens/s over the K timed generations.
 // --copy-mib C first runs one plain copy kernel over C MiB (a dispatch of known
 // traffic, C MiB read and C MiB written, for calibrating HBM byte counters).
+//
+// --mix R serves R concurrent runs through moea_serve_* instead: N log-uniform
+// in [64, 8192] (even), M in {3, 5, 8, 10}, D log-uniform in [10, 4096],
+// quadratic-center or seeded-MLP (hidden 256, weights N(0, 1/fan_in) from a host
+// normal generator) objectives 50/50, directions sized from N, each run long
+// enough not to finish; W warmup steps, then K timed steps (ms per step).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -58,6 +66,121 @@ __global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__
         dst[i] = src[i];
 }
 
+float* upload(const std::vector<float>& h)
+{
+    float* d = dalloc<float>(h.size());
+    CHECK_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    return d;
+}
+
+std::string stage_summary(int gens)
+{
+    std::string out;
+    double sms[32];
+    long long cnt[32];
+    const int ns = moea_profile_read(sms, cnt, 32);
+    for (int i = 0; i < ns && i < 32; ++i) {
+        if (!cnt[i]) continue;
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", out.empty() ? "" : ", ", moea_profile_stage_name(i),
+                      sms[i] / gens);
+        out += buf;
+    }
+    return out;
+}
+
+int serve_mix(int runs, int gens, int warmup, unsigned long long seed, bool stages, hipStream_t stream)
+{
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<float> Nrm(0.0f, 1.0f);
+    const int ms_choice[4] = {3, 5, 8, 10};
+    std::vector<moea_batch_problem> probs(runs);
+    long long rows = 0, n_sum = 0, mlp = 0;
+    for (int i = 0; i < runs; ++i) {
+        int n = static_cast<int>(std::lround(std::exp(std::log(64.0) + U(rng) * (std::log(8192.0) - std::log(64.0)))));
+        n += n & 1;
+        const int m = ms_choice[rng() % 4];
+        const int d = static_cast<int>(std::lround(std::exp(std::log(10.0) + U(rng) * (std::log(4096.0) - std::log(10.0)))));
+        const bool is_mlp = (rng() & 1) != 0;
+        moea_batch_problem& p = probs[i];
+        std::memset(&p, 0, sizeof(p));
+        p.n = n; p.m = m; p.d = d;
+        const int k = moea_das_dennis(n, m, 0, 0, nullptr, nullptr);
+        if (k <= 0) { std::fprintf(stderr, "das_dennis failed: %s\n", moea_last_error()); return 3; }
+        std::vector<float> dirs(static_cast<size_t>(m) * k), raw(static_cast<size_t>(m) * k);
+        CHECK_MOEA(moea_das_dennis(n, m, 0, 0, dirs.data(), raw.data()) < 0 ? -1 : 0);
+        p.k = k;
+        p.max_generations = warmup + 2 * gens + 10;
+        p.seed = seed + i;
+        std::vector<float> bounds(2 * static_cast<size_t>(d));
+        for (int j = 0; j < d; ++j) { bounds[2 * j] = 0.0f; bounds[2 * j + 1] = 1.0f; }
+        p.bounds = upload(bounds);
+        p.directions = upload(dirs);
+        if (is_mlp) {
+            const int hidden = 256;
+            std::vector<float> w1(static_cast<size_t>(hidden) * d), w2(static_cast<size_t>(m) * hidden);
+            for (auto& w : w1) w = Nrm(rng) / std::sqrt(static_cast<float>(d));
+            for (auto& w : w2) w = Nrm(rng) / std::sqrt(static_cast<float>(hidden));
+            p.problem = MOEA_PROBLEM_MLP;
+            p.w1 = upload(w1);
+            p.w2 = upload(w2);
+            p.hidden = hidden;
+            ++mlp;
+        } else {
+            std::vector<float> c(static_cast<size_t>(m) * d);
+            for (auto& v : c) v = static_cast<float>(U(rng));
+            p.problem = MOEA_PROBLEM_QUADRATIC;
+            p.centers = upload(c);
+        }
+        p.sbx_eta[0] = p.sbx_eta[1] = 30.0f;
+        p.sbx_prob = 1.0f;
+        p.pm_eta[0] = p.pm_eta[1] = 20.0f;
+        p.pm_prob = 1.0f;
+        rows += 2ll * n;
+        n_sum += n;
+    }
+    moea_serve_config cfg{runs, rows};
+    size_t bytes = moea_serve_reserved_bytes(&cfg);
+    for (auto& p : probs) bytes += moea_serve_instance_bytes(&p);
+    void* ws = nullptr;
+    CHECK_HIP(hipMalloc(&ws, bytes));
+    moea_serve* srv = nullptr;
+    CHECK_MOEA(moea_serve_create(&cfg, ws, bytes, stream, &srv));
+    for (auto& p : probs) {
+        int slot = -1;
+        CHECK_MOEA(moea_serve_admit(srv, &p, &slot));
+    }
+    int nf = 0;
+    for (int i = 0; i < warmup + 1; ++i) CHECK_MOEA(moea_serve_step(srv, nullptr, 0, &nf));
+    CHECK_HIP(hipStreamSynchronize(stream));
+    hipEvent_t e0, e1;
+    CHECK_HIP(hipEventCreate(&e0));
+    CHECK_HIP(hipEventCreate(&e1));
+    CHECK_HIP(hipEventRecord(e0, stream));
+    for (int i = 0; i < gens; ++i) CHECK_MOEA(moea_serve_step(srv, nullptr, 0, &nf));
+    CHECK_HIP(hipEventRecord(e1, stream));
+    CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    std::string stage_json;
+    if (stages) {
+        CHECK_MOEA(moea_profile_enable(1));
+        for (int i = 0; i < gens; ++i) CHECK_MOEA(moea_serve_step(srv, nullptr, 0, &nf));
+        CHECK_HIP(hipStreamSynchronize(stream));
+        stage_json = stage_summary(gens);
+        CHECK_MOEA(moea_profile_enable(0));
+    }
+    std::printf("{\"workload\": \"served NSGA-III mix\", \"runs\": %d, \"mlp_runs\": %lld, \"individuals\": %lld, "
+                "\"steps\": %d, \"warmup\": %d, \"ms_per_step\": %.4f, \"workspace_bytes\": %zu%s%s%s}\n",
+                runs, mlp, n_sum, gens, warmup, ms / gens, bytes, stages ? ", \"stage_ms_per_step\": {" : "",
+                stage_json.c_str(), stages ? "}" : "");
+    std::fflush(stdout);
+    moea_serve_destroy(srv);
+    CHECK_HIP(hipFree(ws));
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -66,6 +189,7 @@ int main(int argc, char** argv)
     unsigned long long seed = 0;
     bool stages = false;
     int copy_mib = 0;
+    int mix = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -80,6 +204,7 @@ int main(int argc, char** argv)
         else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
         else if (a == "--stages") stages = true;
         else if (a == "--copy-mib") copy_mib = std::atoi(next());
+        else if (a == "--mix") mix = std::atoi(next());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 1; }
     }
     if (n <= 0 || n % 2 || m <= 0 || d <= 0 || gens <= 0 || warmup < 0) {
@@ -101,6 +226,8 @@ int main(int argc, char** argv)
         CHECK_HIP(hipFree(dst));
         std::printf("{\"copy_kernel_mib\": %d}\n", copy_mib);
     }
+
+    if (mix > 0) return serve_mix(mix, gens, warmup, seed, stages, stream);
 
     // directions (host Das-Dennis, as the Python API does)
     const int k = moea_das_dennis(n, m, 0, 0, nullptr, nullptr);

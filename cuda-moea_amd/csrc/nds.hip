@@ -339,6 +339,7 @@ constexpr int kPeelStateWords = kBarWords + 16;
 constexpr int kPeelThreads = 1024;
 constexpr int kPeelWaves = kPeelThreads / 64;
 constexpr int kAheadChunks = 3;     // phase A: chunks checked after a cleared cursor chunk
+constexpr int kPeelGroup = 2;       // batched peel: row blocks whose loads one wave issues together
 constexpr int kScanLoads = 8;       // phase B: words per lane in flight
 
 // LDS layout of nds_peel (dynamic): s_A (W u64), s_next (bpg u64),
@@ -358,47 +359,66 @@ __device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
 // Phase A for the lane's row p of block rb (live rows only): checks the
 // chunk holding the cursor, and the next kAheadChunks chunks if it is clear.
 // Updates cur; ready: no dominator word left; queued: clear so far, words left.
-__device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
-                                  const uint64_t* s_A, int& cur, bool& ready, bool& queued)
+__device__ __forceinline__ const uint64_t* peel_rowbase(const uint64_t* __restrict__ B, int Wc, int rb, int lane)
 {
-    const uint64_t* rowbase = B + (static_cast<size_t>(rb) * Wc * 64 + lane) * 8;
-    const int c = cur >> 3;
-    ulonglong2 v[4];
+    return B + (static_cast<size_t>(rb) * Wc * 64 + lane) * 8;
+}
+
+__device__ __forceinline__ void peel_load_chunk(const uint64_t* rowbase, int c, ulonglong2 (&v)[4])
+{
     const ulonglong2* src = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(c) * 512);
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = src[i];
-    int stop = min(c * 8 + 8, end);
+}
+
+// The cursor chunk c (loaded in v): first word at or after cur holding an
+// unassigned dominator, or stop = end of the chunk (capped at end).
+__device__ __forceinline__ int peel_first(const ulonglong2 (&v)[4], int c, int cur, int end, const uint64_t* s_A,
+                                          int& stop)
+{
+    stop = min(c * 8 + 8, end);
     int blocked = stop;
 #pragma unroll
     for (int i = 7; i >= 0; --i) {
         const int ww = c * 8 + i;
         if (ww >= cur && ww < stop && (chunk_word(v, i) & ~s_A[ww]) != 0) blocked = ww;
     }
-    cur = blocked;
-    if (cur == stop && cur < end) {
-        ulonglong2 u[kAheadChunks][4];
+    return blocked;
+}
+
+// The kAheadChunks chunks after chunk c, loaded in one batch.
+__device__ __forceinline__ int peel_ahead(const uint64_t* rowbase, int c, int end, const uint64_t* s_A, int& stop)
+{
+    ulonglong2 u[kAheadChunks][4];
 #pragma unroll
-        for (int a = 0; a < kAheadChunks; ++a) {
-            const int ca = c + 1 + a;
-            if (ca * 8 < end) {
-                const ulonglong2* s2 = reinterpret_cast<const ulonglong2*>(rowbase + static_cast<size_t>(ca) * 512);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) u[a][i] = s2[i];
-            }
-        }
-        const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
-        int blocked2 = stop2;
-#pragma unroll
-        for (int a = kAheadChunks - 1; a >= 0; --a) {
-#pragma unroll
-            for (int i = 7; i >= 0; --i) {
-                const int ww = (c + 1 + a) * 8 + i;
-                if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
-            }
-        }
-        cur = blocked2;
-        stop = stop2;
+    for (int a = 0; a < kAheadChunks; ++a) {
+        const int ca = c + 1 + a;
+        if (ca * 8 < end) peel_load_chunk(rowbase, ca, u[a]);
     }
+    const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
+    int blocked2 = stop2;
+#pragma unroll
+    for (int a = kAheadChunks - 1; a >= 0; --a) {
+#pragma unroll
+        for (int i = 7; i >= 0; --i) {
+            const int ww = (c + 1 + a) * 8 + i;
+            if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
+        }
+    }
+    stop = stop2;
+    return blocked2;
+}
+
+__device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
+                                  const uint64_t* s_A, int& cur, bool& ready, bool& queued)
+{
+    const uint64_t* rowbase = peel_rowbase(B, Wc, rb, lane);
+    const int c = cur >> 3;
+    ulonglong2 v[4];
+    peel_load_chunk(rowbase, c, v);
+    int stop;
+    cur = peel_first(v, c, cur, end, s_A, stop);
+    if (cur == stop && cur < end) cur = peel_ahead(rowbase, c, end, s_A, stop);
     ready = (cur >= end);
     queued = !ready && (cur == stop);
 }
@@ -629,6 +649,35 @@ __global__ __launch_bounds__(kDomThreads) void b_nds_dominance(const BInst* __re
     dom_unit<M>(I.fs, I.cvs, W * 64, R.rows, W, Wc, I.kind, I.hi, I.B, u);
 }
 
+// Row ends: for each sorted row, one past the last word of its dominator
+// mask that is non-zero (0 when nothing dominates it).  Bits beyond it are
+// all clear, so the peel checks and scans stop there; for many objectives
+// most rows are dominated by few rows or none.  One wave per row block, lane
+// = row, the block's chunks read in order.
+__global__ __launch_bounds__(64) void b_nds_rowend(const BInst* __restrict__ insts, BMap mp, int mode)
+{
+    int rb;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), rb);
+    const BInst& I = insts[i];
+    const BRows R = brows(I, mode);
+    const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
+    const int lane = static_cast<int>(threadIdx.x);
+    const int p = rb * 64 + lane;
+    const int end = I.hi[rb] + 1;
+    const uint64_t* rowbase = peel_rowbase(I.B, Wc, rb, lane);
+    int last = -1;
+    for (int c = 0; c * 8 < end; ++c) {
+        ulonglong2 v[4];
+        peel_load_chunk(rowbase, c, v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int w = c * 8 + j;
+            if (w < end && chunk_word(v, j) != 0ull) last = w;
+        }
+    }
+    if (p < R.rows) I.rend[p] = last + 1;
+}
+
 // Front peeling of one instance per 1024-thread workgroup: the two phases of
 // nds_peel with workgroup barriers only.  Tasks go through a global per-
 // instance list (the gidx scratch, free during NDS).
@@ -659,35 +708,62 @@ __global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restri
         if (threadIdx.x == 0) { s_ntask = 0; s_count = 0; }
         __syncthreads();
         int my_new = 0;
-        for (int rb = lw; rb < W; rb += kPeelWaves) {
-            const uint64_t amask = s_A[rb];
-            const int end = I.hi[rb] + 1;
-            const int p = rb * 64 + lane;
-            const bool live = (p < rows) && !((amask >> lane) & 1ull);
-            int cur = end;
-            bool ready = false, queued = false;
-            if (live) {
-                cur = I.cursor[p];
-                peel_check(I.B, Wc, rb, lane, end, s_A, cur, ready, queued);
-                if (!ready) I.cursor[p] = cur;
+        // kPeelGroup row blocks per wave step: their cursor and chunk loads
+        // are issued together (one round trip each for the whole group)
+        for (int rb0 = lw * kPeelGroup; rb0 < W; rb0 += kPeelWaves * kPeelGroup) {
+            uint64_t amask[kPeelGroup];
+            int endg[kPeelGroup], curg[kPeelGroup];
+            bool live[kPeelGroup];
+#pragma unroll
+            for (int g = 0; g < kPeelGroup; ++g) {
+                const int rb = rb0 + g;
+                const bool vb = rb < W;
+                amask[g] = vb ? s_A[rb] : ~0ull;
+                const int p = rb * 64 + lane;
+                live[g] = vb && (p < rows) && !((amask[g] >> lane) & 1ull);
+                endg[g] = live[g] ? I.rend[p] : 0;
+                curg[g] = live[g] ? I.cursor[p] : 0;
             }
-            const uint64_t ready_m = ballot(ready);
-            const uint64_t task_m = ballot(queued);
-            if (ready) R.fronts[I.perm[p]] = k;
-            if (task_m) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (queued) {
-                    const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                        static_cast<uint32_t>(task_m >> 32),
-                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
-                    tasks[base + below] = p;
+            ulonglong2 v[kPeelGroup][4];
+#pragma unroll
+            for (int g = 0; g < kPeelGroup; ++g)
+                if (live[g] && curg[g] < endg[g]) peel_load_chunk(peel_rowbase(I.B, Wc, rb0 + g, lane), curg[g] >> 3, v[g]);
+#pragma unroll
+            for (int g = 0; g < kPeelGroup; ++g) {
+                const int rb = rb0 + g;
+                if (rb >= W) break;
+                const int p = rb * 64 + lane;
+                bool ready = false, queued = false;
+                if (live[g] && curg[g] >= endg[g]) {
+                    ready = true;
+                } else if (live[g]) {
+                    const int c = curg[g] >> 3;
+                    int stop;
+                    int cur = peel_first(v[g], c, curg[g], endg[g], s_A, stop);
+                    if (cur == stop && cur < endg[g])
+                        cur = peel_ahead(peel_rowbase(I.B, Wc, rb, lane), c, endg[g], s_A, stop);
+                    ready = (cur >= endg[g]);
+                    queued = !ready && (cur == stop);
+                    if (!ready) I.cursor[p] = cur;
                 }
-            }
-            if (lane == 0) {
-                s_next[rb] = amask | ready_m;
-                my_new += __builtin_popcountll(ready_m);
+                const uint64_t ready_m = ballot(ready);
+                const uint64_t task_m = ballot(queued);
+                if (ready) R.fronts[I.perm[p]] = k;
+                if (task_m) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (queued) {
+                        const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(task_m >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
+                        tasks[base + below] = p;
+                    }
+                }
+                if (lane == 0) {
+                    s_next[rb] = amask[g] | ready_m;
+                    my_new += __builtin_popcountll(ready_m);
+                }
             }
         }
         __syncthreads();
@@ -696,9 +772,10 @@ __global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restri
             const int pr = wave_uniform(tasks[t]);
             const int c0 = wave_uniform(I.cursor[pr]);
             const int rb = pr >> 6;
-            const int c = peel_scan(I.B, Wc, pr, c0, I.hi[rb] + 1, s_A, lane);
+            const int pend = wave_uniform(I.rend[pr]);
+            const int c = peel_scan(I.B, Wc, pr, c0, pend, s_A, lane);
             if (lane == 0) {
-                if (c >= I.hi[rb] + 1) {
+                if (c >= pend) {
                     R.fronts[I.perm[pr]] = k;
                     atomicOr(reinterpret_cast<unsigned long long*>(&s_next[rb]),
                              static_cast<unsigned long long>(1ull << (pr & 63)));
@@ -865,6 +942,7 @@ int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, in
 {
     hipStream_t st = L.stream;
     if (G.rows256.blocks <= 0) return kOk;
+    stage_begin(kStNdsSort, st);
     hipLaunchKernelGGL(b_nds_keys, dim3(G.rows256.blocks), dim3(256), 0, st, L.insts, G.rows256.map, mode,
                        S.keys_in, S.vals_in);
     MOEA_HIP(hipGetLastError());
@@ -876,15 +954,22 @@ int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, in
     hipLaunchKernelGGL(b_nds_block_info, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map,
                        mode, S.keys_out);
     MOEA_HIP(hipGetLastError());
+    stage_end(kStNdsSort, st);
+    stage_begin(kStDominance, st);
     for (int m = 1; m <= kMaxM; ++m) {
         if (G.dom[m].blocks <= 0) continue;
         const dim3 grid(G.dom[m].blocks), block(kDomThreads);
         MOEA_LAUNCH_M(m, b_nds_dominance, grid, block, 0, st, L.insts, G.dom[m].map, mode);
         MOEA_HIP(hipGetLastError());
     }
+    stage_end(kStDominance, st);
+    stage_begin(kStPeel, st);
+    hipLaunchKernelGGL(b_nds_rowend, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map, mode);
+    MOEA_HIP(hipGetLastError());
     const size_t lds = 2 * static_cast<size_t>(G.max_W) * sizeof(uint64_t);
     hipLaunchKernelGGL(b_nds_peel, dim3(L.count), dim3(kPeelThreads), lds, st, L.insts, mode, state);
     MOEA_HIP(hipGetLastError());
+    stage_end(kStPeel, st);
     return kOk;
 }
 

@@ -241,6 +241,129 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
 // thread solves.  Extreme point j is the (value, index)-smallest candidate with
 // a finite ASF value (-1 if none) -- an order-independent reduction.
 constexpr int kSolveThreads = 256;
+// hyperplane_fp64 (common.hpp) for one wave: lane i < M holds row i of A and
+// of V in registers.  Every element update is the same expression as in the
+// scalar solve and every sum is accumulated in the same order (row 0 first)
+// through cross-lane reads, so the result is bit-identical; the rotation
+// parameters are computed redundantly by every lane from the uniform sums.
+// Returns 1 for the row-max fallback (the scalar solve's conditions); lane r
+// gets a_out[r] in `a_lane`.
+// Lane i's value of v, uniform over the wave (two scalar lane reads).
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <int M>
+__device__ __forceinline__ int hyperplane_fp64_wave(const double (&row)[M], double& a_lane)
+{
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    double A[M], V[M];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        A[j] = lane < M ? row[j] : 0.0;
+        V[j] = (lane == j) ? 1.0 : 0.0;
+        bad = bad || !isfinite(A[j]);
+    }
+    if (__ballot(bad)) return 1;
+    int converged = 0;
+    for (int sweep = 0; sweep < 60 && !converged; ++sweep) {
+        int rotated = 0;
+#pragma unroll
+        for (int p = 0; p < M - 1; ++p) {
+#pragma unroll
+            for (int q = p + 1; q < M; ++q) {
+                const double pp = A[p] * A[p], qq = A[q] * A[q], pq = A[p] * A[q];
+                double alpha = 0.0, beta = 0.0, gamma = 0.0;
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    alpha = alpha + readlane_f64(pp, i);
+                    beta = beta + readlane_f64(qq, i);
+                    gamma = gamma + readlane_f64(pq, i);
+                }
+                if (gamma == 0.0 || fabs(gamma) <= 1e-15 * sqrt(alpha * beta)) continue;
+                rotated = 1;
+                const double zeta = (beta - alpha) / (2.0 * gamma);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double sn = c * t;
+                const double ap = A[p], aq = A[q];
+                A[p] = c * ap - sn * aq;
+                A[q] = sn * ap + c * aq;
+                const double vp = V[p], vq = V[q];
+                V[p] = c * vp - sn * vq;
+                V[q] = sn * vp + c * vq;
+            }
+        }
+        if (!rotated) converged = 1;
+    }
+    if (!converged) return 1;
+    double sig[M];
+    double smax = 0.0, smin = static_cast<double>(INFINITY);
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const double sq = A[j] * A[j];
+        double ss = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) ss = ss + readlane_f64(sq, i);
+        sig[j] = sqrt(ss);
+        if (!isfinite(sig[j])) return 1;
+        if (sig[j] > smax) smax = sig[j];
+        if (sig[j] < smin) smin = sig[j];
+    }
+    if (smin < 1e-10) return 1;
+    if (smax / smin > 1e6) return 1;
+    double out = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (!(sig[j] > 1e-5)) continue;
+        const double ud = A[j] / sig[j];
+        double usum = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) usum = usum + readlane_f64(ud, i);
+        const double w = usum / sig[j];
+        out = out + V[j] * w;
+    }
+    a_lane = out;
+    return 0;
+}
+
+// Wave 0 of the solve block: extreme-point matrix rows, the solve, the
+// row-max fallback and the float intercepts (lane k writes a_out[k]).
+template <int M>
+__device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, int nm, int n_nds,
+                                                const int (&ext)[kMaxM], const float (&rmax)[kMaxM],
+                                                int32_t* __restrict__ meta, float* __restrict__ a_out)
+{
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    int my_ext = -1;
+#pragma unroll
+    for (int i = 0; i < M; ++i) if (lane == i) my_ext = ext[i];
+    double row[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        row[k] = (lane < M && my_ext >= 0 && my_ext < n_nds)
+                     ? static_cast<double>(fn[static_cast<size_t>(k) * nm + my_ext])
+                     : static_cast<double>(INFINITY);
+    double a = 0.0;
+    const int ill = hyperplane_fp64_wave<M>(row, a);
+    if (lane >= M) return;
+    if (ill) {
+        float rm = rmax[0];
+#pragma unroll
+        for (int k = 1; k < M; ++k) if (lane == k) rm = rmax[k];
+        a = (rm > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(rm, 1e-6f)) : 1.0;
+    }
+    float af = static_cast<float>(a);
+    if (!isfinite(af) || af <= 0.0f) af = 1.0f;
+    a_out[lane] = af;
+    if (lane == 0) meta[kIll] = ill;
+}
+
 __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, int nm, int m,
                                        const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
                                        const float* __restrict__ row_max, int32_t* __restrict__ meta,
@@ -273,38 +396,31 @@ __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, in
         if (lane == 0) { s_best[wave][j] = best; s_bidx[wave][j] = bi; s_rmax[wave][j] = rm; }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (wave != 0) return;
     int ext[kMaxM];
     float rmax[kMaxM];
-    for (int j = 0; j < m; ++j) {
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j) {
         float best = INFINITY, rm = -INFINITY;
         int bi = -1;
-        for (int w = 0; w < kSolveThreads / 64; ++w) {
-            const float ov = s_best[w][j];
-            const int oi = s_bidx[w][j];
-            if (oi >= 0 && ov < INFINITY && (bi < 0 || ov < best || (ov == best && oi < bi))) { best = ov; bi = oi; }
-            rm = fmaxf(rm, s_rmax[w][j]);
+        if (j < m) {
+            for (int w = 0; w < kSolveThreads / 64; ++w) {
+                const float ov = s_best[w][j];
+                const int oi = s_bidx[w][j];
+                if (oi >= 0 && ov < INFINITY && (bi < 0 || ov < best || (ov == best && oi < bi))) { best = ov; bi = oi; }
+                rm = fmaxf(rm, s_rmax[w][j]);
+            }
         }
         ext[j] = bi;
         rmax[j] = rm;
     }
-    double E[kMaxM * kMaxM], a[kMaxM];
-    for (int i = 0; i < m; ++i)
-        for (int k = 0; k < m; ++k)
-            E[i * m + k] = (ext[i] >= 0 && ext[i] < n_nds)
-                               ? static_cast<double>(fn[static_cast<size_t>(k) * nm + ext[i]])
-                               : static_cast<double>(INFINITY);
-    const int ill = hyperplane_fp64(E, m, a);
-    if (ill) {
-        for (int k = 0; k < m; ++k)
-            a[k] = (rmax[k] > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(rmax[k], 1e-6f)) : 1.0;
+    switch (m) {
+#define MOEA_NS(MM) case MM: norm_solve_tail<MM>(fn, nm, n_nds, ext, rmax, meta, a_out); break;
+        MOEA_NS(1) MOEA_NS(2) MOEA_NS(3) MOEA_NS(4) MOEA_NS(5) MOEA_NS(6) MOEA_NS(7) MOEA_NS(8) MOEA_NS(9)
+        MOEA_NS(10)
+#undef MOEA_NS
+        default: break;
     }
-    for (int k = 0; k < m; ++k) {
-        float af = static_cast<float>(a[k]);
-        if (!isfinite(af) || af <= 0.0f) af = 1.0f;
-        a_out[k] = af;
-    }
-    meta[kIll] = ill;
 }
 
 __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,
@@ -1016,9 +1132,12 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     hipStream_t st = L.stream;
     const int total = static_cast<int>(G.total_merged);
     if (G.merged256.blocks <= 0) return kOk;
+    stage_begin(kStSelectPrep, st);
     hipLaunchKernelGGL(b_merge, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map);
     MOEA_HIP(hipGetLastError());
+    stage_end(kStSelectPrep, st);
     MOEA_CHECK(batch_nds(L, NG, S, 1, 1));
+    stage_begin(kStSelectPrep, st);
     hipLaunchKernelGGL(b_split, dim3(L.count), dim3(kScanThreads), 0, st, L.insts);
     MOEA_CHECK(batch_ideal(L, G, 1));
     for (int m = 1; m <= kMaxM; ++m) {
@@ -1027,12 +1146,18 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
         MOEA_LAUNCH_M(m, b_norm_prep, grid, block, 0, st, L.insts, G.norm_prep[m].map);
     }
     hipLaunchKernelGGL(b_norm_solve, dim3(L.count), dim3(kSolveThreads), 0, st, L.insts);
+    MOEA_HIP(hipGetLastError());
+    stage_end(kStSelectPrep, st);
+    stage_begin(kStAssociation, st);
     for (int m = 1; m <= kMaxM; ++m) {
         if (G.assoc[m].blocks <= 0) continue;
         const dim3 grid(G.assoc[m].blocks), block(256);
         MOEA_LAUNCH_M(m, b_assoc_partial, grid, block, 0, st, L.insts, G.assoc[m].map);
     }
     hipLaunchKernelGGL(b_assoc_final, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map);
+    MOEA_HIP(hipGetLastError());
+    stage_end(kStAssociation, st);
+    stage_begin(kStNiching, st);
     hipLaunchKernelGGL(b_rho, dim3(L.count), dim3(1024), 0, st, L.insts);
     MOEA_HIP(hipGetLastError());
     const int bits_ref = 32 + G.ref_bits + S.inst_bits;
@@ -1053,8 +1178,12 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     MOEA_CHECK(sort_pairs(S.sort_temp, S.sort_bytes, S.keys_in, S.keys_out, S.vals_in, S.vals_out, total, st,
                           bits_rank));
     hipLaunchKernelGGL(b_survivors, dim3(L.count), dim3(kScanThreads), 0, st, L.insts, S.vals_out);
+    MOEA_HIP(hipGetLastError());
+    stage_end(kStNiching, st);
+    stage_begin(kStGather, st);
     hipLaunchKernelGGL(b_gather, dim3(G.rows4.blocks), dim3(256), 0, st, L.insts, G.rows4.map);
     MOEA_HIP(hipGetLastError());
+    stage_end(kStGather, st);
     return kOk;
 }
 

@@ -24,7 +24,7 @@ struct Timer {
     std::mutex mu;
     bool enabled = false;
     std::vector<hipEvent_t> pool;
-    size_t next = 0;
+    std::vector<hipEvent_t> free;             // events not held by an open or unread record
     struct Rec { int stage; hipEvent_t a, b; };
     std::vector<Rec> open;
     std::vector<Rec> done;
@@ -40,8 +40,10 @@ Timer& timer()
 
 hipEvent_t take(Timer& t)
 {
-    if (t.next >= t.pool.size()) return nullptr;
-    return t.pool[t.next++];
+    if (t.free.empty()) return nullptr;
+    hipEvent_t e = t.free.back();
+    t.free.pop_back();
+    return e;
 }
 
 void drain(Timer& t)
@@ -52,9 +54,10 @@ void drain(Timer& t)
             t.ms[r.stage] += ms;
             t.count[r.stage] += 1;
         }
+        t.free.push_back(r.a);
+        t.free.push_back(r.b);
     }
     t.done.clear();
-    t.next = 0;
 }
 
 }  // namespace
@@ -64,9 +67,9 @@ void stage_begin(int stage, hipStream_t stream)
     Timer& t = timer();
     if (!t.enabled) return;
     std::lock_guard<std::mutex> lk(t.mu);
-    if (t.next + 2 > t.pool.size()) drain(t);
+    if (t.free.size() < 2) drain(t);
+    if (t.free.size() < 2) return;
     hipEvent_t a = take(t), b = take(t);
-    if (!a || !b) return;
     hipEventRecord(a, stream);
     t.open.push_back({stage, a, b});
 }
@@ -99,13 +102,14 @@ int moea_profile_enable(int enable)
     if (enable && !t.enabled) {
         t.pool.resize(8192);
         for (auto& e : t.pool) MOEA_HIP(hipEventCreate(&e));
-        t.next = 0;
+        t.free = t.pool;
         for (int i = 0; i < kNumStages; ++i) { t.ms[i] = 0.0; t.count[i] = 0; }
         t.enabled = true;
     } else if (!enable && t.enabled) {
         drain(t);
         for (auto& e : t.pool) hipEventDestroy(e);
         t.pool.clear();
+        t.free.clear();
         t.open.clear();
         t.enabled = false;
     }

@@ -43,17 +43,18 @@ class QuadraticCenters:
 class SeededMLP:
     """f_c(x) = y_c^2 + 0.01 ||x - 0.5||^2 with y = W2 tanh(W1 x), W1 ~ N(0, 1/d)
     (hidden x d), W2 ~ N(0, 1/hidden) (objectives x hidden), seeded from
-    ``torch.Generator('cpu').manual_seed(seed)``."""
+    ``torch.Generator(device).manual_seed(seed)`` (CPU by default; a GPU
+    generator draws other weights of the same distribution, in place)."""
 
     def __init__(self, dimension: int, objectives: int = 5, hidden: int = 256,
-                 seed: int = 0) -> None:
+                 seed: int = 0, device: Any = "cpu") -> None:
         self.dimension = int(dimension)
         self.objectives = int(objectives)
         self.hidden = int(hidden)
-        g = torch.Generator(device="cpu").manual_seed(int(seed))
-        self.w1 = (torch.randn(self.hidden, self.dimension, generator=g)
+        g = torch.Generator(device=device).manual_seed(int(seed))
+        self.w1 = (torch.randn(self.hidden, self.dimension, generator=g, device=device)
                    / math.sqrt(self.dimension)).contiguous()
-        self.w2 = (torch.randn(self.objectives, self.hidden, generator=g)
+        self.w2 = (torch.randn(self.objectives, self.hidden, generator=g, device=device)
                    / math.sqrt(self.hidden)).contiguous()
         self.lower_bounds = [0.0] * self.dimension
         self.upper_bounds = [1.0] * self.dimension
