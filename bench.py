@@ -262,6 +262,53 @@ def serving_config(seconds: float, rate: float, plain_sample: int, budget_bytes:
             "p99_speedup": round(float(np.percentile(plat, 99) / np.percentile(lat, 99)), 1)}
 
 
+def rvea_config(generations: int, warmup: int) -> dict:
+    """Config 2: RVEA, N = 16384, M = 5, D = 131072, adaptive reference
+    directions (frequency 0.1), alpha = 2, seeded-MLP objective D->256->5
+    (weights drawn on the GPU), K timed generations after W warmup
+    generations of the 200-generation schedule (the APD penalty and the
+    annealed operator parameters follow generation / 200)."""
+    import torch
+    import cuda_moea as cm
+    from cuda_moea import _lib
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    alg = cm.RVEA(population_size=16384, max_generations=200,
+                  problem=cm.SeededMLP(131072, 5, hidden=256, seed=0, device=dev),
+                  environment_selector=cm.RVEAEnvironmentSelector(alpha=2.0),
+                  reference_directions=cm.AdaptiveRVEADirections(frequency=0.1),
+                  seed=0, print_progress=False)
+    alg.initialize()
+    if warmup:
+        alg._call(_lib.lib.moea_run_step, warmup)
+    stream.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    alg._call(_lib.lib.moea_run_step, generations)
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / generations
+    _lib.profile_enable(True)
+    alg._call(_lib.lib.moea_run_step, 2)
+    stream.synchronize()
+    stages = _lib.profile_read()
+    _lib.profile_enable(False)
+    flops = 2.0 * 16384 * 131072 * 256
+    ev = stages.get("evaluate", (0.0, 0))
+    ev_ms = ev[0] / max(ev[1], 1)
+    out = {"workload": "config 2: RVEA N=16384 M=5 D=131072, seeded MLP D->256->5, adaptive directions "
+                       "(fr 0.1), alpha 2", "K_directions": alg.k,
+           "ms_per_gen": round(ms, 3), "gens_per_s": round(1000.0 / ms, 3),
+           "generations_timed": generations, "warmup": warmup,
+           "projected_200_generations_s": round(200 * ms / 1000.0, 2),
+           "stage_ms_per_generation": {k: round(v[0] / max(v[1], 1), 3) for k, v in stages.items() if v[1]},
+           "mlp_flop_per_gen": flops,
+           "mlp_tflops": round(flops / (ev_ms / 1e3) / 1e12, 1) if ev_ms > 0 else None}
+    del alg
+    torch.cuda.empty_cache()
+    return out
+
+
 def dtlz1_comparison(generations: int, warmup: int) -> dict:
     """NSGA-III on DTLZ1, N = 32768, M = 3, D = 500, in the reference's own
     benchmark setting (SBX eta 30, p 1, per-variable copy probability 0.5; PM
@@ -306,6 +353,8 @@ def main() -> int:
     ap.add_argument("--batched-generations", type=int, default=300)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-rvea", action="store_true")
+    ap.add_argument("--rvea-generations", type=int, default=5)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-plain-sample", type=int, default=24)
@@ -364,6 +413,10 @@ def main() -> int:
     if not args.no_batched:
         batched = batched_config(args.batched_generations, args.batched_instances, args.plain_sample)
 
+    rvea = None
+    if not args.no_rvea:
+        rvea = rvea_config(args.rvea_generations, 2)
+
     serving = None
     if not args.no_serving:
         serving = serving_config(args.serve_seconds, args.serve_rate, args.serve_plain_sample,
@@ -405,6 +458,7 @@ def main() -> int:
         "stage_ms_per_generation": stage_ms,
         "batched": batched,
         "serving": serving,
+        "rvea": rvea,
         "reference_comparison": dtlz1,
         "cpu_baseline": cpu,
     }

@@ -217,7 +217,7 @@ long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grid
         objectives[i] = m;
         rows4[i] = ceil_div(n, 4);
         if (I.problem == MOEA_PROBLEM_MLP) {
-            mlp_tiles[i] = static_cast<long long>(ceil_div(n, 64)) * ceil_div(I.hidden, 64);
+            mlp_tiles[i] = mlp_tile_count(n, I.hidden);
             h4[m][i] = ceil_div(n, 4);
         } else {
             q4[m][i] = ceil_div(n, 4);

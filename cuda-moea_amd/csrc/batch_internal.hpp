@@ -146,7 +146,7 @@ struct BNdsScratch {
 // hidden-layer tiles and head rows of the instances of that kind.
 struct BEvalGrids {
     BGrid quad_rows4[kMaxM + 1];
-    BGrid mlp_tiles;              // 64 x 64 tiles of (n, hidden) per MLP instance
+    BGrid mlp_tiles;              // mlp_tile_count(n, hidden) tile blocks per MLP instance
     BGrid mlp_rows4[kMaxM + 1];
 };
 
@@ -158,6 +158,7 @@ int batch_init_population(const BatchLaunch& L, const BMap& mp, int nblocks);   
 int batch_reproduce(const BatchLaunch& L, const BMap& rows256, int nrows_blocks, const BMap& sbx256,
                     int nsbx_blocks);                                                         // reproduce.hip
 int batch_evaluate(const BatchLaunch& L, const BEvalGrids& G, int which);                     // evaluate.hip
+int mlp_tile_count(int n, int hidden);            // blocks of one MLP hidden-layer launch       // evaluate.hip
 size_t batch_nds_sort_bytes(long long total_rows, int inst_bits);                             // nds.hip
 int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int state);  // nds.hip
 int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode);                          // nsga3.hip

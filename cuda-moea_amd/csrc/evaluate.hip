@@ -73,57 +73,168 @@ __global__ __launch_bounds__(256) void b_quadratic_kernel(const BInst* __restric
                      static_cast<int>(threadIdx.x & 63));
 }
 
-// MLP hidden layer H = tanh(X W1^T): (n, d) x (hidden, d)^T -> (n, hidden).
-// 64 x 64 output tile per 256-thread block, 16-wide k slices staged in LDS,
-// each thread owns a 4 x 4 register sub-tile.
-constexpr int kGT = 64, kGK = 16;
+// MLP hidden layer H = tanh(X W1^T): (n, d) x (hidden, d)^T -> (n, hidden),
+// on the FP32 matrix cores (v_mfma_f32_16x16x4_f32).  A 256-thread block
+// owns a 64 x 128 tile of H; its four waves each own 32 x 64 (2 x 4 MFMA
+// tiles).  K advances in slices of 32 staged in LDS (double-buffered, next
+// slice prefetched into registers during the current one's MFMAs).  Both
+// operands are K-contiguous, so a lane reads four consecutive k of its row
+// with one 16-byte LDS read; lane group g = lane / 16 takes k = 4g + s in MFMA
+// step s (the MFMA sums over its four lane groups), which covers every k of
+// the slice once.  Accumulation is FP32 throughout; the summation order is
+// fixed by the tiling, so a row of H does not depend on anything else in the
+// launch (the batched launch reproduces the single-run values bit for bit).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kHM = 64, kHN = 128, kHK = 32, kHS = kHK + 4;   // tile rows, cols, k slice, LDS row stride
 
+// (row tile, col tile) of tile index t: the col tiles of one row tile sit
+// 8 indices apart, so a round-robin dispatch over the 8 XCDs puts them on one
+// XCD and its L2 serves the shared X rows.  Returns false past the last row tile.
+__device__ __forceinline__ bool mlp_tile_of(int t, int nrt, int nct, int& rt, int& ct)
+{
+    const int xcd = t & 7, q = t >> 3;
+    ct = q % nct;
+    rt = (q / nct) * 8 + xcd;
+    return rt < nrt;
+}
+
+}  // namespace
+
+int mlp_tile_count(int n, int hidden)
+{
+    const int nrt = (n + kHM - 1) / kHM, nct = (hidden + kHN - 1) / kHN;
+    return ((nrt + 7) / 8) * 8 * nct;
+}
+
+namespace {
+
+struct MlpSmem {
+    float a[2][kHM * kHS];
+    float b[2][kHN * kHS];
+};
+
+// Loads are issued unconditionally from clamped (in-bounds) addresses and the
+// out-of-range elements are zeroed when the registers are stored to LDS, so
+// no load sits behind a branch (a branch join would wait for it).
+// VEC: ld % 4 == 0 (one 16-byte load); otherwise four scalar loads.
+template <bool VEC>
+__device__ __forceinline__ float4 mlp_raw4(const float* __restrict__ base, int row, int nrows, int ld, int k)
+{
+    const float* p = base + static_cast<size_t>(min(row, nrows - 1)) * ld;
+    if (VEC) return *reinterpret_cast<const float4*>(p + min(k, ld - 4));
+    float4 v;
+    v.x = p[min(k, ld - 1)];
+    v.y = p[min(k + 1, ld - 1)];
+    v.z = p[min(k + 2, ld - 1)];
+    v.w = p[min(k + 3, ld - 1)];
+    return v;
+}
+
+__device__ __forceinline__ float4 mlp_mask4(float4 v, int row, int nrows, int ld, int k)
+{
+    const bool r = row < nrows;
+    v.x = (r && k < ld) ? v.x : 0.0f;
+    v.y = (r && k + 1 < ld) ? v.y : 0.0f;
+    v.z = (r && k + 2 < ld) ? v.z : 0.0f;
+    v.w = (r && k + 3 < ld) ? v.w : 0.0f;
+    return v;
+}
+
+template <bool VEC>
 __device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int n, int d,
                                                 const float* __restrict__ w1, int hidden,
-                                                float* __restrict__ h, int r0, int c0)
+                                                float* __restrict__ h, int r0, int c0, MlpSmem& sm)
 {
-    __shared__ float s_x[kGK][kGT + 1];
-    __shared__ float s_w[kGK][kGT + 1];
-    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-    float acc[4][4] = {};
-    for (int k0 = 0; k0 < d; k0 += kGK) {
-        for (int e = threadIdx.x; e < kGT * kGK; e += 256) {
-            const int rr = e / kGK, kk = e % kGK;
-            const int gr = r0 + rr, gk = k0 + kk;
-            s_x[kk][rr] = (gr < n && gk < d) ? x[static_cast<size_t>(gr) * d + gk] : 0.0f;
-            const int gc = c0 + rr;
-            s_w[kk][rr] = (gc < hidden && gk < d) ? w1[static_cast<size_t>(gc) * d + gk] : 0.0f;
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 4, i = lane & 15;
+    // loader: 8 float4 per 32-float row slice; A 64 rows (2 per thread), B 128 rows (4 per thread)
+    const int lrow = tid >> 3, lk = (tid & 7) * 4;
+    // two register sets: slices s + 1 and s + 2 are in flight while slice s is in LDS
+    float4 pa[2][2], pb[2][4];
+    auto load = [&](int set, int k0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) pa[set][j] = mlp_raw4<VEC>(x, r0 + lrow + 32 * j, n, d, k0 + lk);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pb[set][j] = mlp_raw4<VEC>(w1, c0 + lrow + 32 * j, hidden, d, k0 + lk);
+    };
+    auto store = [&](int set, int buf, int k0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            *reinterpret_cast<float4*>(&sm.a[buf][(lrow + 32 * j) * kHS + lk]) =
+                mlp_mask4(pa[set][j], r0 + lrow + 32 * j, n, d, k0 + lk);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            *reinterpret_cast<float4*>(&sm.b[buf][(lrow + 32 * j) * kHS + lk]) =
+                mlp_mask4(pb[set][j], c0 + lrow + 32 * j, hidden, d, k0 + lk);
+    };
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            float4 a[2], b[4];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                a[t] = *reinterpret_cast<const float4*>(&sm.a[buf][(wm * 32 + t * 16 + i) * kHS + half * 16 + 4 * g]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                b[u] = *reinterpret_cast<const float4*>(&sm.b[buf][(wn * 64 + u * 16 + i) * kHS + half * 16 + 4 * g]);
+#pragma unroll
+            for (int st = 0; st < 4; ++st)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][st], b[u][st], acc[t][u], 0, 0, 0);
         }
+    };
+    // Loads and stores are unconditional (a slice past d loads clamped
+    // addresses and stores zeros), so every path through the loop has the
+    // same loads in flight and the waits stay exact.
+    load(0, 0);
+    store(0, 0, 0);
+    load(1, kHK);
+    load(0, 2 * kHK);
+    __syncthreads();
+    // slice s sits in LDS buffer s % 2; slice s + 1 in register set (s + 1) % 2
+    for (int k0 = 0;; k0 += 2 * kHK) {
+        compute(0);                                   // slice k0
+        store(1, 1, k0 + kHK);                        // slice k0 + 32
+        load(1, k0 + 3 * kHK);
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kGK; ++kk) {
-            float a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = s_x[kk][tr * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = s_w[kk][tc * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
-        }
+        if (k0 + kHK >= d) break;
+        compute(1);                                   // slice k0 + 32
+        store(0, 0, k0 + 2 * kHK);                    // slice k0 + 64
+        load(0, k0 + 4 * kHK);
         __syncthreads();
+        if (k0 + 2 * kHK >= d) break;
     }
+    // D of a 16 x 16 MFMA tile: lane (g, i) holds rows 4g .. 4g+3 of column i
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gr = r0 + tr * 4 + i, gc = c0 + tc * 4 + j;
-            if (gr < n && gc < hidden) h[static_cast<size_t>(gr) * hidden + gc] = tanhf(acc[i][j]);
-        }
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + wm * 32 + t * 16 + 4 * g + r;
+                const int col = c0 + wn * 64 + u * 16 + i;
+                if (row < n && col < hidden) h[static_cast<size_t>(row) * hidden + col] = tanhf(acc[t][u][r]);
+            }
 }
 
 __global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict__ x, int n, int d,
                                                          const float* __restrict__ w1, int hidden,
                                                          float* __restrict__ h)
 {
-    mlp_hidden_tile(x, n, d, w1, hidden, h, static_cast<int>(blockIdx.x) * kGT,
-                    static_cast<int>(blockIdx.y) * kGT);
+    __shared__ __align__(16) MlpSmem sm;
+    int rt, ct;
+    if (!mlp_tile_of(static_cast<int>(blockIdx.x), (n + kHM - 1) / kHM, (hidden + kHN - 1) / kHN, rt, ct)) return;
+    if ((d & 3) == 0) mlp_hidden_tile<true>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm);
+    else mlp_hidden_tile<false>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm);
 }
 
 // Output head: y = W2 h, f_c = y_c^2 + 0.01 * ||x - 0.5||^2; one wave per row.
@@ -179,8 +290,11 @@ __global__ __launch_bounds__(256) void b_mlp_hidden_kernel(const BInst* __restri
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
     const int slot = (I.cur + which) % 3;
-    const int nr = (I.n + kGT - 1) / kGT;
-    mlp_hidden_tile(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, (local % nr) * kGT, (local / nr) * kGT);
+    __shared__ __align__(16) MlpSmem sm;
+    int rt, ct;
+    if (!mlp_tile_of(local, (I.n + kHM - 1) / kHM, (I.hidden + kHN - 1) / kHN, rt, ct)) return;
+    if ((I.d & 3) == 0) mlp_hidden_tile<true>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm);
+    else mlp_hidden_tile<false>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm);
 }
 
 template <int M>
@@ -274,8 +388,8 @@ int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const fl
     if (!x || !w1 || !w2 || !f || !hwork || n <= 0 || d <= 0 || hidden <= 0 || m <= 0 || m > kMaxM ||
         ldf < n)
         return fail(kInvalidArgument, "eval_mlp: invalid arguments");
-    hipLaunchKernelGGL(mlp_hidden_kernel, dim3(ceil_div(n, kGT), ceil_div(hidden, kGT)), dim3(256), 0,
-                       stream, x, n, d, w1, hidden, hwork);
+    hipLaunchKernelGGL(mlp_hidden_kernel, dim3(mlp_tile_count(n, hidden)), dim3(256), 0, stream, x, n, d, w1,
+                       hidden, hwork);
     MOEA_HIP(hipGetLastError());
     const dim3 grid(ceil_div(n, 4)), block(256);
     switch (m) {
