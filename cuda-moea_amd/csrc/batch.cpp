@@ -97,7 +97,8 @@ void carve_instance(Carve& c, const moea_batch_problem& p, BInst& I)
     I.kind = c.take<uint8_t>(z.W);
     I.cursor = c.take<int32_t>(z.nm);
     I.rend = c.take<int32_t>(z.nm);
-    I.A = nullptr;
+    I.A = c.take<uint64_t>(2 * static_cast<size_t>(z.W));
+    I.pstate = c.take<uint32_t>(kBatchPeelStateWords);
 }
 
 size_t instance_bytes(const moea_batch_problem& p)
@@ -153,7 +154,13 @@ struct Grids {
 // Grid offsets are kept as table indices until the table is on the device.
 struct TableBuilder {
     std::vector<int> t;
-    std::vector<BGrid*> pending;
+    // appends `v` and returns its offset as a placeholder pointer (fixed after upload)
+    const int* raw(const std::vector<int>& v)
+    {
+        const size_t off = t.size();
+        t.insert(t.end(), v.begin(), v.end());
+        return reinterpret_cast<const int*>(off);
+    }
     BGrid add(const std::vector<long long>& counts)
     {
         BGrid g{};
@@ -172,6 +179,7 @@ struct TableBuilder {
 };
 
 void fix(BGrid& g, const int* base) { g.map.start = base + reinterpret_cast<size_t>(g.map.start); }
+void fix(const int*& p, const int* base) { p = base + reinterpret_cast<size_t>(p); }
 
 void fix_grids(Grids& G, const int* base)
 {
@@ -188,11 +196,14 @@ void fix_grids(Grids& G, const int* base)
     fix(G.sel.merged256, base); fix(G.sel.rows4, base); fix(G.sel.objectives, base);
     fix(G.nds0.rows256, base); fix(G.nds0.npad256, base); fix(G.nds0.blocks64, base);
     fix(G.nds1.rows256, base); fix(G.nds1.npad256, base); fix(G.nds1.blocks64, base);
+    fix(G.nds0.peel_first, base); fix(G.nds0.peel_jobs, base);
+    fix(G.nds1.peel_first, base); fix(G.nds1.peel_jobs, base);
 }
 
 // Builds the grids for the instances in `state` and assigns their sort-key
-// offsets (contiguous in slot order).  Returns the rows sorted (sum of 2n).
-long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grids& G)
+// offsets (contiguous in slot order).  Returns the rows sorted (sum of 2n),
+// or a negative status when the peel cannot be planned.
+long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, TableBuilder& T, Grids& G)
 {
     const int count = static_cast<int>(host.size());
     std::vector<long long> init256(count, 0), rows256(count, 0), sbx256(count, 0), merged256(count, 0),
@@ -202,7 +213,8 @@ long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grid
         h4(kMaxM + 1, std::vector<long long>(count, 0)), prep(kMaxM + 1, std::vector<long long>(count, 0)),
         assoc(kMaxM + 1, std::vector<long long>(count, 0)), dom0(kMaxM + 1, std::vector<long long>(count, 0)),
         dom1(kMaxM + 1, std::vector<long long>(count, 0));
-    int max_k = 1, max_nm = 1, max_W0 = 1, max_W1 = 1;
+    int max_k = 1, max_nm = 1;
+    std::vector<int> peel_inst, peel_W0, peel_W1;
     long long total_merged = 0, total_n = 0;
     for (int i = 0; i < count; ++i) {
         BInst& I = host[i];
@@ -234,8 +246,9 @@ long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grid
         dom1[m][i] = ceil_div(4ll * I.Wc * (I.Wc + 1), 4);
         max_k = std::max(max_k, I.k);
         max_nm = std::max(max_nm, nm);
-        max_W0 = std::max(max_W0, W0);
-        max_W1 = std::max(max_W1, I.W);
+        peel_inst.push_back(i);
+        peel_W0.push_back(W0);
+        peel_W1.push_back(I.W);
         I.key_off = total_merged;
         I.key_off_n = total_n;
         total_merged += nm;
@@ -269,8 +282,17 @@ long long build_grids(std::vector<BInst>& host, int state, TableBuilder& T, Grid
         G.nds0.dom[m] = T.add(dom0[m]);
         G.nds1.dom[m] = T.add(dom1[m]);
     }
-    G.nds0.max_W = max_W0;
-    G.nds1.max_W = max_W1;
+    for (int mode = 0; mode < 2; ++mode) {
+        BNdsGrids& N = mode ? G.nds1 : G.nds0;
+        std::vector<int> first, jobs;
+        size_t lds = 0;
+        const int st = batch_peel_plan(peel_inst, mode ? peel_W1 : peel_W0, peel_capacity, first, jobs, lds);
+        if (st != kOk) return st;
+        N.peel_wgs = static_cast<int>(first.size()) - 1;
+        N.peel_lds = lds;
+        N.peel_first = T.raw(first);
+        N.peel_jobs = T.raw(jobs);
+    }
     G.nds0.total_rows = total_n;
     G.nds1.total_rows = total_merged;
     return total_merged;
@@ -294,6 +316,7 @@ struct BatchCore {
     hipStream_t stream{};
     Grids run{}, init{};
     bool dirty = true;                        // membership changed since the last upload
+    int peel_capacity = 0;                    // co-resident blocks of the batched peel
 };
 
 namespace {
@@ -301,7 +324,10 @@ namespace {
 size_t core_reserved(int slots, long long max_rows, Carve& c, BatchCore* C)
 {
     BInst* insts = c.take<BInst>(slots);
-    const size_t table_ints = static_cast<size_t>(2) * kGridsPerState * (slots + 1);
+    // grid prefixes of both states, plus the peel plans of both states and modes
+    // (at most kBarMaxBlocks + 1 block starts and 256 jobs per instance each)
+    const size_t table_ints = static_cast<size_t>(2) * kGridsPerState * (slots + 1) +
+                              4 * (static_cast<size_t>(kBarMaxBlocks) + 1 + 3 * 256 * static_cast<size_t>(slots));
     int* tables = c.take<int>(table_ints);
     uint64_t* k_in = c.take<uint64_t>(max_rows);
     uint64_t* k_out = c.take<uint64_t>(max_rows);
@@ -325,9 +351,13 @@ size_t core_reserved(int slots, long long max_rows, Carve& c, BatchCore* C)
 // (synchronizing the stream first: the device table may be in use).
 int core_upload(BatchCore& C)
 {
+    if (C.peel_capacity == 0) C.peel_capacity = batch_peel_capacity();
+    if (C.peel_capacity <= 0) return fail(kUnsupported, "moea_batch: the peel kernel cannot be resident");
     TableBuilder T;
-    const long long run_rows = build_grids(C.host, kStateRunning, T, C.run);
-    const long long init_rows = build_grids(C.host, kStateAdmitted, T, C.init);
+    const long long run_rows = build_grids(C.host, kStateRunning, C.peel_capacity, T, C.run);
+    if (run_rows < 0) return static_cast<int>(run_rows);
+    const long long init_rows = build_grids(C.host, kStateAdmitted, C.peel_capacity, T, C.init);
+    if (init_rows < 0) return static_cast<int>(init_rows);
     if (run_rows > C.max_rows || init_rows > C.max_rows)
         return fail(kWorkspaceTooSmall, "moea_batch: more rows than the sort scratch holds");
     if (T.t.size() > C.table_ints) return fail(kInvalidArgument, "moea_batch: table capacity exceeded");

@@ -12,10 +12,14 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "common.hpp"
 
 namespace moea {
+
+constexpr int kBatchPeelStateWords = 1024 + 64;   // per instance: peel group barrier flags and totals
+constexpr int kBatchPeelMaxLds = 72 * 1024;        // LDS bound of one batched peel block
 
 struct BInst {
     // lifecycle: 0 free, 1 running, 2 admitted (initialization pending)
@@ -71,7 +75,8 @@ struct BInst {
     uint8_t* kind;                // (W)
     int32_t* cursor;              // (nm)
     int32_t* rend;                // (nm) sorted row -> 1 + last word holding a dominator (0: none)
-    uint64_t* A;                  // (W) assigned set
+    uint64_t* A;                  // (2 W) assigned sets of the peel (this round, next round)
+    uint32_t* pstate;             // (kBatchPeelStateWords) peel barrier and totals
     long long key_off;            // first of this instance's nm entries in the batch key arrays
     long long key_off_n;          // first of this instance's n entries (NDS of the parents)
 };
@@ -116,8 +121,13 @@ struct BNdsGrids {
     BGrid npad256;                // sorted gather: 64 W per instance, 256 per block
     BGrid blocks64;               // block info: W one-wave blocks per instance
     BGrid dom[kMaxM + 1];         // dominance units, 4 per block, of the instances with m objectives
-    int max_W;                    // largest W (peel LDS)
     long long total_rows;         // sum of rows (sort length)
+    // peel: cooperative launch of peel_wgs blocks; block w runs
+    // peel_jobs[peel_first[w] .. peel_first[w + 1]) (instance, rank, group size)
+    const int* peel_first;
+    const int* peel_jobs;
+    int peel_wgs;
+    size_t peel_lds;
 };
 
 // Grids of the batched selection stages.
@@ -161,6 +171,9 @@ int batch_evaluate(const BatchLaunch& L, const BEvalGrids& G, int which);       
 int mlp_tile_count(int n, int hidden);            // blocks of one MLP hidden-layer launch       // evaluate.hip
 size_t batch_nds_sort_bytes(long long total_rows, int inst_bits);                             // nds.hip
 int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int state);  // nds.hip
+int batch_peel_capacity();                                                                    // nds.hip
+int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, std::vector<int>& first,
+                    std::vector<int>& jobs, size_t& lds);                                     // nds.hip
 int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode);                          // nsga3.hip
 int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S);  // nsga3.hip
 int batch_advance(const BatchLaunch& L, int from_state, int to_state, int rotate);              // nsga3.hip

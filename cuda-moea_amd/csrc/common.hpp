@@ -335,15 +335,17 @@ constexpr int kBarWords = kBarError + 16;     // first word free for the caller
 // the barrier returns false, so a broken launch ends instead of hanging.
 constexpr uint32_t kSpinLimit = 1u << 24;
 
-__device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t& phase)
+// Barrier of the `nblocks` co-resident blocks that share `state`; `rank` is
+// this block's index among them (0 .. nblocks-1).
+__device__ inline bool group_barrier(uint32_t* state, uint32_t rank, uint32_t nblocks, uint32_t& phase)
 {
     __shared__ int s_ok;
     const uint32_t target = phase + 1;
     if (threadIdx.x == 0) s_ok = 1;
     __syncthreads();
     if (threadIdx.x == 0)
-        __hip_atomic_store(&state[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (blockIdx.x == 0) {
+        __hip_atomic_store(&state[rank], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (rank == 0) {
         for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
             uint32_t spins = 0;
             while (__hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
@@ -377,6 +379,11 @@ __device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t&
     __syncthreads();
     phase = target;
     return s_ok != 0;
+}
+
+__device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t& phase)
+{
+    return group_barrier(state, blockIdx.x, nblocks, phase);
 }
 
 // ----------------------------------------------------------------------------

@@ -35,6 +35,9 @@
 //   word (p, w) lives at (((p/64) * Wc + w/8) * 64 + p%64) * 8 + w%8.
 // A 64-row x 8-word chunk is 4 KiB contiguous: the dominance kernel writes it
 // as one 64-byte segment per lane, the peel reads a row's words 8 at a time.
+#include <algorithm>
+#include <vector>
+
 #include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
@@ -332,14 +335,15 @@ struct PeelArgs {
     uint64_t* A1;
     uint32_t* state;
     int32_t* out_max_front;  // (1,) out
+    const int32_t* rend;     // (n,) optional: one past each row's last non-zero word (sorted order)
 };
 
 constexpr int kPeelTotal = kBarWords;
 constexpr int kPeelStateWords = kBarWords + 16;
+static_assert(kPeelStateWords <= kBatchPeelStateWords, "batched peel state too small");
 constexpr int kPeelThreads = 1024;
 constexpr int kPeelWaves = kPeelThreads / 64;
 constexpr int kAheadChunks = 3;     // phase A: chunks checked after a cleared cursor chunk
-constexpr int kPeelGroup = 2;       // batched peel: row blocks whose loads one wave issues together
 constexpr int kScanLoads = 8;       // phase B: words per lane in flight
 
 // LDS layout of nds_peel (dynamic): s_A (W u64), s_next (bpg u64),
@@ -451,9 +455,10 @@ __device__ __forceinline__ int peel_scan(const uint64_t* __restrict__ B, int Wc,
     return end;
 }
 
-__global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
+// One peel over the G blocks of a group; g is this block's rank.  Returns
+// false when a barrier gave up (the launch is broken; the block exits).
+__device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint64_t* s_dyn)
 {
-    extern __shared__ uint64_t s_dyn[];
     const int nblk = P.W;
     const int bpg = P.blocks_per_group;
     uint64_t* s_A = s_dyn;                                          // (W) assigned before this round
@@ -462,8 +467,6 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
     int* s_hi = reinterpret_cast<int*>(s_task + static_cast<size_t>(bpg) * 64);   // (W)
     __shared__ int s_ntask;
     __shared__ int s_count;
-    const int G = static_cast<int>(gridDim.x);
-    const int g = static_cast<int>(blockIdx.x);
     const int lane = lane_id();
     const int lw = wave_uniform(static_cast<int>(threadIdx.x >> 6));
     uint32_t phase = 0;
@@ -476,7 +479,7 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
         if (p < P.n) { P.fronts[P.perm[p]] = -1; P.cursor[p] = 0; }
         if (lane == 0) { st_relaxed_u64(&P.A0[rb], 0ull); st_relaxed_u64(&P.A1[rb], 0ull); }
     }
-    if (!grid_barrier(P.state, gridDim.x, phase)) return;
+    if (!group_barrier(P.state, g, G, phase)) return false;
 
     int total_prev = 0;
     for (int k = 0;; ++k) {
@@ -492,8 +495,9 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
             const int rb = g + j * G;
             if (rb >= nblk) break;
             const uint64_t amask = s_A[rb];
-            const int end = s_hi[rb] + 1;             // words 0 .. end-1 can hold dominators
             const int p = rb * 64 + lane;
+            // words 0 .. end-1 can hold dominators
+            const int end = (P.rend && p < P.n) ? P.rend[p] : s_hi[rb] + 1;
             const bool live = (p < P.n) && !((amask >> lane) & 1ull);
             int cur = end;
             bool ready = false, queued = false;
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
             const int pr = wave_uniform(static_cast<int>(static_cast<uint32_t>(rec)));
             const int c0 = wave_uniform(static_cast<int>(rec >> 32));
             const int rb = pr >> 6;
-            const int end = s_hi[rb] + 1;
+            const int end = P.rend ? wave_uniform(P.rend[pr]) : s_hi[rb] + 1;
             const int c = peel_scan(P.B, P.Wc, pr, c0, end, s_A, lane);
             const bool clear = c >= end;
             if (lane == 0) {
@@ -553,17 +557,23 @@ __global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
         if (threadIdx.x == 0 && s_count)
             __hip_atomic_fetch_add(&P.state[kPeelTotal], static_cast<uint32_t>(s_count), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (!grid_barrier(P.state, gridDim.x, phase)) return;
+        if (!group_barrier(P.state, g, G, phase)) return false;
         const int total = static_cast<int>(ld_relaxed_u32(&P.state[kPeelTotal]));
         if (total >= P.stop_at || total == total_prev || total >= P.n) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (g == 0 && threadIdx.x == 0) {
                 P.out_max_front[0] = (total == total_prev) ? k - 1 : k;
                 P.state[kPeelTotal + 1] = static_cast<uint32_t>(k);
             }
-            return;
+            return true;
         }
         total_prev = total;
     }
+}
+
+__global__ __launch_bounds__(kPeelThreads) void nds_peel(PeelArgs P)
+{
+    extern __shared__ uint64_t s_dyn[];
+    peel_body(P, static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), s_dyn);
 }
 
 // ---------------------------------------------------------------- batched --
@@ -678,123 +688,38 @@ __global__ __launch_bounds__(64) void b_nds_rowend(const BInst* __restrict__ ins
     if (p < R.rows) I.rend[p] = last + 1;
 }
 
-// Front peeling of one instance per 1024-thread workgroup: the two phases of
-// nds_peel with workgroup barriers only.  Tasks go through a global per-
-// instance list (the gidx scratch, free during NDS).
-__global__ __launch_bounds__(kPeelThreads) void b_nds_peel(const BInst* __restrict__ insts, int mode, int state)
+// Front peeling of every instance of a batch in one cooperative launch.  The
+// blocks form groups; each group peels its instances one after another
+// (peel_body, with the group barrier on the instance's own state words), so
+// a large instance gets many blocks and small ones share single blocks.
+// Jobs of block w: jobs[first[w] .. first[w + 1]) as (instance, rank, group size).
+__global__ __launch_bounds__(kPeelThreads) void b_nds_peel_groups(const BInst* __restrict__ insts, int mode,
+                                                                 const int* __restrict__ first,
+                                                                 const int* __restrict__ jobs)
 {
     extern __shared__ uint64_t s_dyn[];
-    __shared__ int s_ntask;
-    __shared__ int s_count;
+    const int w = static_cast<int>(blockIdx.x);
+    const int j1 = first[w + 1];
+    for (int j = first[w]; j < j1; ++j) {
+        const int i = jobs[3 * j], g = jobs[3 * j + 1], G = jobs[3 * j + 2];
+        const BInst& I = insts[i];
+        const BRows R = brows(I, mode);
+        const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
+        PeelArgs P{I.B, I.hi, I.perm, R.rows, W, Wc, mode == 1 ? I.n : R.rows, (W + G - 1) / G, R.fronts,
+                   I.cursor, I.A, I.A + W, I.pstate,
+                   mode == 1 ? I.meta : reinterpret_cast<int32_t*>(I.pstate + kPeelTotal + 2), I.rend};
+        __syncthreads();
+        if (!peel_body(P, g, G, s_dyn)) return;
+    }
+}
+
+// Clears the peel state words of the instances in `state` (barrier flags and
+// totals start at zero).
+__global__ void b_peel_reset(const BInst* __restrict__ insts, int state)
+{
     const BInst& I = insts[blockIdx.x];
     if (I.state != state) return;
-    const BRows R = brows(I, mode);
-    const int rows = R.rows;
-    const int W = (rows + 63) / 64, Wc = (W + 7) / 8;
-    uint64_t* s_A = s_dyn;            // (W) assigned before this round
-    uint64_t* s_next = s_dyn + W;     // (W)
-    int32_t* tasks = I.gidx;
-    const int lane = lane_id();
-    const int lw = wave_uniform(static_cast<int>(threadIdx.x >> 6));
-
-    for (int p = threadIdx.x; p < rows; p += kPeelThreads) {
-        R.fronts[I.perm[p]] = -1;
-        I.cursor[p] = 0;
-    }
-    for (int w = threadIdx.x; w < W; w += kPeelThreads) s_A[w] = 0ull;
-    int total = 0;
-    for (int k = 0;; ++k) {
-        __syncthreads();   // every thread has read the previous round's count
-        if (threadIdx.x == 0) { s_ntask = 0; s_count = 0; }
-        __syncthreads();
-        int my_new = 0;
-        // kPeelGroup row blocks per wave step: their cursor and chunk loads
-        // are issued together (one round trip each for the whole group)
-        for (int rb0 = lw * kPeelGroup; rb0 < W; rb0 += kPeelWaves * kPeelGroup) {
-            uint64_t amask[kPeelGroup];
-            int endg[kPeelGroup], curg[kPeelGroup];
-            bool live[kPeelGroup];
-#pragma unroll
-            for (int g = 0; g < kPeelGroup; ++g) {
-                const int rb = rb0 + g;
-                const bool vb = rb < W;
-                amask[g] = vb ? s_A[rb] : ~0ull;
-                const int p = rb * 64 + lane;
-                live[g] = vb && (p < rows) && !((amask[g] >> lane) & 1ull);
-                endg[g] = live[g] ? I.rend[p] : 0;
-                curg[g] = live[g] ? I.cursor[p] : 0;
-            }
-            ulonglong2 v[kPeelGroup][4];
-#pragma unroll
-            for (int g = 0; g < kPeelGroup; ++g)
-                if (live[g] && curg[g] < endg[g]) peel_load_chunk(peel_rowbase(I.B, Wc, rb0 + g, lane), curg[g] >> 3, v[g]);
-#pragma unroll
-            for (int g = 0; g < kPeelGroup; ++g) {
-                const int rb = rb0 + g;
-                if (rb >= W) break;
-                const int p = rb * 64 + lane;
-                bool ready = false, queued = false;
-                if (live[g] && curg[g] >= endg[g]) {
-                    ready = true;
-                } else if (live[g]) {
-                    const int c = curg[g] >> 3;
-                    int stop;
-                    int cur = peel_first(v[g], c, curg[g], endg[g], s_A, stop);
-                    if (cur == stop && cur < endg[g])
-                        cur = peel_ahead(peel_rowbase(I.B, Wc, rb, lane), c, endg[g], s_A, stop);
-                    ready = (cur >= endg[g]);
-                    queued = !ready && (cur == stop);
-                    if (!ready) I.cursor[p] = cur;
-                }
-                const uint64_t ready_m = ballot(ready);
-                const uint64_t task_m = ballot(queued);
-                if (ready) R.fronts[I.perm[p]] = k;
-                if (task_m) {
-                    int base = 0;
-                    if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
-                    base = __builtin_amdgcn_readfirstlane(base);
-                    if (queued) {
-                        const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                            static_cast<uint32_t>(task_m >> 32),
-                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
-                        tasks[base + below] = p;
-                    }
-                }
-                if (lane == 0) {
-                    s_next[rb] = amask[g] | ready_m;
-                    my_new += __builtin_popcountll(ready_m);
-                }
-            }
-        }
-        __syncthreads();
-        const int ntask = s_ntask;
-        for (int t = lw; t < ntask; t += kPeelWaves) {
-            const int pr = wave_uniform(tasks[t]);
-            const int c0 = wave_uniform(I.cursor[pr]);
-            const int rb = pr >> 6;
-            const int pend = wave_uniform(I.rend[pr]);
-            const int c = peel_scan(I.B, Wc, pr, c0, pend, s_A, lane);
-            if (lane == 0) {
-                if (c >= pend) {
-                    R.fronts[I.perm[pr]] = k;
-                    atomicOr(reinterpret_cast<unsigned long long*>(&s_next[rb]),
-                             static_cast<unsigned long long>(1ull << (pr & 63)));
-                    ++my_new;
-                } else {
-                    I.cursor[pr] = c;
-                }
-            }
-        }
-        if (lane == 0 && my_new) atomicAdd(&s_count, my_new);
-        __syncthreads();
-        for (int w = threadIdx.x; w < W; w += kPeelThreads) s_A[w] = s_next[w];
-        const int added = s_count;
-        total += added;
-        if (total >= I.n || added == 0 || total >= rows) {
-            if (mode == 1 && threadIdx.x == 0) I.meta[0] = (added == 0) ? k - 1 : k;
-            return;
-        }
-    }
+    for (int t = threadIdx.x; t < kPeelStateWords; t += blockDim.x) I.pstate[t] = 0u;
 }
 
 __global__ void zero_u32(uint32_t* p, int count)
@@ -933,6 +858,80 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     return kOk;
 }
 
+int batch_peel_capacity()
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(b_nds_peel_groups),
+                                                    kPeelThreads, kBatchPeelMaxLds) != hipSuccess)
+        return 0;
+    const int cap = cus * per_cu;
+    return cap < kBarMaxBlocks ? cap : kBarMaxBlocks;
+}
+
+// Groups for the batched peel.  An instance of W row blocks wants one block
+// per 16 row blocks (one row block per wave) and needs at least one per 32
+// (the per-block task queue is sized for 32).  When the wanted blocks exceed
+// the capacity, the largest instances still get groups of their own (scaled
+// down) and the rest join the least-loaded group large enough for them; a
+// group's blocks run its instances in the same order, so no two groups wait
+// on each other.
+int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, std::vector<int>& first,
+                    std::vector<int>& jobs, size_t& lds)
+{
+    const int count = static_cast<int>(inst.size());
+    first.assign(1, 0);
+    jobs.clear();
+    lds = 0;
+    if (count == 0) return 0;
+    if (capacity < 1) return fail(kUnsupported, "batched peel cannot be resident on this device");
+    std::vector<int> want(count), need(count), order(count);
+    long long demand = 0;
+    for (int i = 0; i < count; ++i) {
+        want[i] = std::max(1, std::min(ceil_div(W[i], 16), capacity));
+        need[i] = std::max(1, ceil_div(W[i], 32));
+        if (need[i] > capacity) return fail(kUnsupported, "batched peel: population too large for one device");
+        demand += want[i];
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return W[a] > W[b]; });
+    struct Group { int size; double load; std::vector<int> members; };
+    std::vector<Group> groups;
+    int used = 0;
+    for (int oi = 0; oi < count; ++oi) {
+        const int i = order[oi];
+        int size = want[i];
+        if (demand > capacity)
+            size = std::max(need[i], static_cast<int>(static_cast<double>(want[i]) * capacity / demand));
+        const double cost = static_cast<double>(W[i]) * W[i];
+        if (used + size <= capacity) {
+            groups.push_back({size, cost / size, {i}});
+            used += size;
+            continue;
+        }
+        int best = -1;
+        for (int gi = 0; gi < static_cast<int>(groups.size()); ++gi)
+            if (groups[gi].size >= need[i] && (best < 0 || groups[gi].load < groups[best].load)) best = gi;
+        if (best < 0) return fail(kUnsupported, "batched peel: no group can hold an instance");
+        groups[best].members.push_back(i);
+        groups[best].load += cost / groups[best].size;
+    }
+    for (const Group& gr : groups) {
+        for (int r = 0; r < gr.size; ++r) {
+            for (int i : gr.members) {
+                jobs.push_back(inst[i]);
+                jobs.push_back(r);
+                jobs.push_back(gr.size);
+            }
+            first.push_back(static_cast<int>(jobs.size() / 3));
+        }
+        for (int i : gr.members) lds = std::max(lds, peel_lds_bytes(W[i], ceil_div(W[i], gr.size)));
+    }
+    if (lds > static_cast<size_t>(kBatchPeelMaxLds)) return fail(kUnsupported, "batched peel: LDS bound exceeded");
+    return kOk;
+}
+
 size_t batch_nds_sort_bytes(long long total_rows, int inst_bits)
 {
     return sort_pairs_temp_bytes(static_cast<int>(total_rows), kKeyBits + inst_bits);
@@ -966,8 +965,15 @@ int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, in
     stage_begin(kStPeel, st);
     hipLaunchKernelGGL(b_nds_rowend, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map, mode);
     MOEA_HIP(hipGetLastError());
-    const size_t lds = 2 * static_cast<size_t>(G.max_W) * sizeof(uint64_t);
-    hipLaunchKernelGGL(b_nds_peel, dim3(L.count), dim3(kPeelThreads), lds, st, L.insts, mode, state);
+    hipLaunchKernelGGL(b_peel_reset, dim3(L.count), dim3(256), 0, st, L.insts, state);
+    MOEA_HIP(hipGetLastError());
+    int mode_arg = mode;
+    const BInst* insts_arg = L.insts;
+    const int* first_arg = G.peel_first;
+    const int* jobs_arg = G.peel_jobs;
+    void* kargs[] = {&insts_arg, &mode_arg, &first_arg, &jobs_arg};
+    MOEA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(b_nds_peel_groups), dim3(G.peel_wgs),
+                                        dim3(kPeelThreads), kargs, static_cast<unsigned>(G.peel_lds), st));
     MOEA_HIP(hipGetLastError());
     stage_end(kStPeel, st);
     return kOk;
