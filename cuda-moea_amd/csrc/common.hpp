@@ -336,8 +336,13 @@ constexpr int kBarWords = kBarError + 16;     // first word free for the caller
 constexpr uint32_t kSpinLimit = 1u << 24;
 
 // Barrier of the `nblocks` co-resident blocks that share `state`; `rank` is
-// this block's index among them (0 .. nblocks-1).
-__device__ inline bool group_barrier(uint32_t* state, uint32_t rank, uint32_t nblocks, uint32_t& phase)
+// this block's index among them (0 .. nblocks-1).  With snap_src >= 0, block
+// 0 copies state[snap_src] to state[snap_dst] after every block has arrived
+// and before any leaves: a value accumulated before the barrier, read by
+// every block afterwards from snap_dst, is then the same for all of them even
+// if fast blocks start adding to snap_src for the next round.
+__device__ inline bool group_barrier(uint32_t* state, uint32_t rank, uint32_t nblocks, uint32_t& phase,
+                                     int snap_src = -1, int snap_dst = -1)
 {
     __shared__ int s_ok;
     const uint32_t target = phase + 1;
@@ -359,8 +364,13 @@ __device__ inline bool group_barrier(uint32_t* state, uint32_t rank, uint32_t nb
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
+            if (snap_src >= 0)
+                __hip_atomic_store(&state[snap_dst],
+                                   __hip_atomic_load(&state[snap_src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&state[kBarGen], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
     } else if (threadIdx.x == 0) {
         uint32_t spins = 0;
         while (__hip_atomic_load(&state[kBarGen], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {

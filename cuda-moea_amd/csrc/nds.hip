@@ -305,8 +305,10 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
     dom_unit<M>(fs, cvs, npad, n, W, Wc, kind, hi, B, u);
 }
 
-// Peel state (device), u32 words: [0, kBarWords) grid barrier,
-// [kPeelTotal] total assigned, [kPeelTotal + 1] last round.  A0 / A1:
+// Peel state (device), u32 words: [0, kBarWords) group barrier,
+// [kPeelTotal] total assigned, [kPeelTotal + 1] last round, [kPeelTotal + 2]
+// spare output word, [kPeelTotal + 4 + (k & 1)] the total as of the barrier
+// ending round k (snapshot read by every block).  A0 / A1:
 // assigned-set bitsets in sorted order (W words each), double-buffered across
 // rounds.
 //
@@ -557,8 +559,13 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         if (threadIdx.x == 0 && s_count)
             __hip_atomic_fetch_add(&P.state[kPeelTotal], static_cast<uint32_t>(s_count), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (!group_barrier(P.state, g, G, phase)) return false;
-        const int total = static_cast<int>(ld_relaxed_u32(&P.state[kPeelTotal]));
+        // the running total is read from the snapshot block 0 takes inside
+        // the barrier: a block that reads late must not see counts that
+        // faster blocks already add for the next round, or the blocks could
+        // disagree about stopping and strand each other at the next barrier
+        const int snap = kPeelTotal + 4 + (k & 1);
+        if (!group_barrier(P.state, g, G, phase, kPeelTotal, snap)) return false;
+        const int total = static_cast<int>(ld_relaxed_u32(&P.state[snap]));
         if (total >= P.stop_at || total == total_prev || total >= P.n) {
             if (g == 0 && threadIdx.x == 0) {
                 P.out_max_front[0] = (total == total_prev) ? k - 1 : k;
