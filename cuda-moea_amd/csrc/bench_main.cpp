@@ -4,6 +4,7 @@
 //   moea_bench [--n N] [--m M] [--d D] [--gens K] [--warmup W] [--seed S] [--stages]
 //              [--copy-mib C]
 //   moea_bench --mix R [--gens K] [--warmup W] [--seed S] [--stages]
+//   moea_bench --mlp --n N --m M --d D [--hidden H] [--gens K] [--warmup W]
 //
 // Workload: NSGA-III with quadratic-center objectives f_k(x) = sum_j (x_j - c_kj)^2,
 // x in [0, 1]^D, centers c drawn like torch.rand((M, D)) after manual_seed(seed)
@@ -18,6 +19,9 @@
 // quadratic-center or seeded-MLP (hidden 256, weights N(0, 1/fan_in) from a host
 // normal generator) objectives 50/50, directions sized from N, each run long
 // enough not to finish; W warmup steps, then K timed steps (ms per step).
+//
+// --mlp times the seeded-MLP objective alone (moea_eval_mlp: the FP32 MFMA
+// hidden layer and the head) on an (N, D) population: K timed calls.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -181,6 +185,46 @@ int serve_mix(int runs, int gens, int warmup, unsigned long long seed, bool stag
     return 0;
 }
 
+int mlp_only(int n, int m, int d, int hidden, int gens, int warmup, unsigned long long seed, hipStream_t stream)
+{
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> Nrm(0.0f, 1.0f);
+    std::vector<float> w1(static_cast<size_t>(hidden) * d), w2(static_cast<size_t>(m) * hidden);
+    for (auto& w : w1) w = Nrm(rng) / std::sqrt(static_cast<float>(d));
+    for (auto& w : w2) w = Nrm(rng) / std::sqrt(static_cast<float>(hidden));
+    float* dw1 = upload(w1);
+    float* dw2 = upload(w2);
+    float* x = dalloc<float>(static_cast<size_t>(n) * d);
+    float* f = dalloc<float>(static_cast<size_t>(m) * n);
+    float* cv = dalloc<float>(n);
+    float* hw = dalloc<float>(static_cast<size_t>(n) * hidden);
+    std::vector<float> bounds(2 * static_cast<size_t>(d));
+    for (int j = 0; j < d; ++j) { bounds[2 * j] = 0.0f; bounds[2 * j + 1] = 1.0f; }
+    float* db = upload(bounds);
+    CHECK_MOEA(moea_init_population(x, n, d, db, seed, stream));
+    for (int i = 0; i < warmup; ++i) CHECK_MOEA(moea_eval_mlp(x, n, d, dw1, hidden, dw2, m, f, n, cv, hw, stream));
+    CHECK_HIP(hipStreamSynchronize(stream));
+    hipEvent_t e0, e1;
+    CHECK_HIP(hipEventCreate(&e0));
+    CHECK_HIP(hipEventCreate(&e1));
+    CHECK_HIP(hipEventRecord(e0, stream));
+    for (int i = 0; i < gens; ++i) CHECK_MOEA(moea_eval_mlp(x, n, d, dw1, hidden, dw2, m, f, n, cv, hw, stream));
+    CHECK_HIP(hipEventRecord(e1, stream));
+    CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const double flop = 2.0 * n * static_cast<double>(d) * hidden;
+    std::printf("{\"workload\": \"seeded MLP objective\", \"N\": %d, \"M\": %d, \"D\": %d, \"hidden\": %d, "
+                "\"calls\": %d, \"ms_per_call\": %.4f, \"hidden_layer_flop\": %.4g, \"tflops\": %.2f}\n",
+                n, m, d, hidden, gens, ms / gens, flop, flop / (ms / gens * 1e-3) / 1e12);
+    std::fflush(stdout);
+    CHECK_HIP(hipFree(x));
+    CHECK_HIP(hipFree(f));
+    CHECK_HIP(hipFree(cv));
+    CHECK_HIP(hipFree(hw));
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -190,6 +234,8 @@ int main(int argc, char** argv)
     bool stages = false;
     int copy_mib = 0;
     int mix = 0;
+    bool mlp = false;
+    int hidden = 256;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -205,6 +251,8 @@ int main(int argc, char** argv)
         else if (a == "--stages") stages = true;
         else if (a == "--copy-mib") copy_mib = std::atoi(next());
         else if (a == "--mix") mix = std::atoi(next());
+        else if (a == "--mlp") mlp = true;
+        else if (a == "--hidden") hidden = std::atoi(next());
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 1; }
     }
     if (n <= 0 || n % 2 || m <= 0 || d <= 0 || gens <= 0 || warmup < 0) {
@@ -228,6 +276,10 @@ int main(int argc, char** argv)
     }
 
     if (mix > 0) return serve_mix(mix, gens, warmup, seed, stages, stream);
+    if (mlp) {
+        if (hidden <= 0 || m > MOEA_MAX_OBJECTIVES) { std::fprintf(stderr, "invalid sizes\n"); return 1; }
+        return mlp_only(n, m, d, hidden, gens, warmup, seed, stream);
+    }
 
     // directions (host Das-Dennis, as the Python API does)
     const int k = moea_das_dennis(n, m, 0, 0, nullptr, nullptr);

@@ -140,10 +140,16 @@ __device__ __forceinline__ float4 mlp_mask4(float4 v, int row, int nrows, int ld
     return v;
 }
 
+// The blocks of column tile 0 also sum (x - 0.5)^2 over each row of their
+// row tile from the x slices they stage (each of the 8 loader lanes of a row
+// sums its 4 columns of every slice, then a fixed xor reduction over the 8)
+// and store it in rowsq[row]; the head reads it there instead of reading x
+// again.
 template <bool VEC>
 __device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int n, int d,
                                                 const float* __restrict__ w1, int hidden,
-                                                float* __restrict__ h, int r0, int c0, MlpSmem& sm)
+                                                float* __restrict__ h, int r0, int c0, MlpSmem& sm,
+                                                float* __restrict__ rowsq)
 {
     const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -158,11 +164,22 @@ __device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int
 #pragma unroll
         for (int j = 0; j < 4; ++j) pb[set][j] = mlp_raw4<VEC>(w1, c0 + lrow + 32 * j, hidden, d, k0 + lk);
     };
+    const bool sq = (c0 == 0);
+    float sqs[2] = {0.0f, 0.0f};
     auto store = [&](int set, int buf, int k0) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            *reinterpret_cast<float4*>(&sm.a[buf][(lrow + 32 * j) * kHS + lk]) =
-                mlp_mask4(pa[set][j], r0 + lrow + 32 * j, n, d, k0 + lk);
+        for (int j = 0; j < 2; ++j) {
+            const float4 v = mlp_mask4(pa[set][j], r0 + lrow + 32 * j, n, d, k0 + lk);
+            *reinterpret_cast<float4*>(&sm.a[buf][(lrow + 32 * j) * kHS + lk]) = v;
+            if (sq) {
+                const float t0 = v.x - 0.5f, t1 = v.y - 0.5f, t2 = v.z - 0.5f, t3 = v.w - 0.5f;
+                const int kk = k0 + lk;
+                sqs[j] = (kk < d) ? fmaf(t0, t0, sqs[j]) : sqs[j];
+                sqs[j] = (kk + 1 < d) ? fmaf(t1, t1, sqs[j]) : sqs[j];
+                sqs[j] = (kk + 2 < d) ? fmaf(t2, t2, sqs[j]) : sqs[j];
+                sqs[j] = (kk + 3 < d) ? fmaf(t3, t3, sqs[j]) : sqs[j];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             *reinterpret_cast<float4*>(&sm.b[buf][(lrow + 32 * j) * kHS + lk]) =
@@ -213,6 +230,17 @@ __device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int
         __syncthreads();
         if (k0 + 2 * kHK >= d) break;
     }
+    if (sq) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            float v = sqs[j];
+            v = v + __shfl_xor(v, 1);
+            v = v + __shfl_xor(v, 2);
+            v = v + __shfl_xor(v, 4);
+            const int row = r0 + lrow + 32 * j;
+            if ((tid & 7) == 0 && row < n) rowsq[row] = v;
+        }
+    }
     // D of a 16 x 16 MFMA tile: lane (g, i) holds rows 4g .. 4g+3 of column i
 #pragma unroll
     for (int t = 0; t < 2; ++t)
@@ -228,21 +256,24 @@ __device__ __forceinline__ void mlp_hidden_tile(const float* __restrict__ x, int
 
 __global__ __launch_bounds__(256) void mlp_hidden_kernel(const float* __restrict__ x, int n, int d,
                                                          const float* __restrict__ w1, int hidden,
-                                                         float* __restrict__ h)
+                                                         float* __restrict__ h, float* __restrict__ rowsq)
 {
     __shared__ __align__(16) MlpSmem sm;
     int rt, ct;
     if (!mlp_tile_of(static_cast<int>(blockIdx.x), (n + kHM - 1) / kHM, (hidden + kHN - 1) / kHN, rt, ct)) return;
-    if ((d & 3) == 0) mlp_hidden_tile<true>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm);
-    else mlp_hidden_tile<false>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm);
+    if ((d & 3) == 0) mlp_hidden_tile<true>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm, rowsq);
+    else mlp_hidden_tile<false>(x, n, d, w1, hidden, h, rt * kHM, ct * kHN, sm, rowsq);
 }
 
 // Output head: y = W2 h, f_c = y_c^2 + 0.01 * ||x - 0.5||^2; one wave per row.
+// ||x - 0.5||^2 comes from the hidden-layer launch, parked in f[0][row]
+// (read before any objective of the row is written).
 template <int M>
-__device__ __forceinline__ void mlp_head_row(const float* __restrict__ x, int d, const float* __restrict__ h,
-                                             int hidden, const float* __restrict__ w2, float* __restrict__ f,
-                                             int ldf, float* __restrict__ cv, int row, int lane)
+__device__ __forceinline__ void mlp_head_row(const float* __restrict__ h, int hidden, const float* __restrict__ w2,
+                                             float* __restrict__ f, int ldf, float* __restrict__ cv, int row,
+                                             int lane)
 {
+    const float s = f[row];
     float y[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) y[k] = 0.0f;
@@ -251,14 +282,8 @@ __device__ __forceinline__ void mlp_head_row(const float* __restrict__ x, int d,
 #pragma unroll
         for (int k = 0; k < M; ++k) y[k] = fmaf(w2[static_cast<size_t>(k) * hidden + j], hv, y[k]);
     }
-    float s = 0.0f;
-    for (int j = lane; j < d; j += 64) {
-        const float t = x[static_cast<size_t>(row) * d + j] - 0.5f;
-        s = fmaf(t, t, s);
-    }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
-        s = s + __shfl_xor(s, o);
 #pragma unroll
         for (int k = 0; k < M; ++k) y[k] = y[k] + __shfl_xor(y[k], o);
     }
@@ -272,15 +297,13 @@ __device__ __forceinline__ void mlp_head_row(const float* __restrict__ x, int d,
 }
 
 template <int M>
-__global__ __launch_bounds__(256) void mlp_head_kernel(const float* __restrict__ x, int n, int d,
-                                                       const float* __restrict__ h, int hidden,
-                                                       const float* __restrict__ w2,
-                                                       float* __restrict__ f, int ldf,
+__global__ __launch_bounds__(256) void mlp_head_kernel(int n, const float* __restrict__ h, int hidden,
+                                                       const float* __restrict__ w2, float* __restrict__ f, int ldf,
                                                        float* __restrict__ cv)
 {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
-    mlp_head_row<M>(x, d, h, hidden, w2, f, ldf, cv, row, static_cast<int>(threadIdx.x & 63));
+    mlp_head_row<M>(h, hidden, w2, f, ldf, cv, row, static_cast<int>(threadIdx.x & 63));
 }
 
 // Batched MLP: hidden-layer tiles, then the head rows of the instances with M objectives.
@@ -293,8 +316,10 @@ __global__ __launch_bounds__(256) void b_mlp_hidden_kernel(const BInst* __restri
     __shared__ __align__(16) MlpSmem sm;
     int rt, ct;
     if (!mlp_tile_of(local, (I.n + kHM - 1) / kHM, (I.hidden + kHN - 1) / kHN, rt, ct)) return;
-    if ((I.d & 3) == 0) mlp_hidden_tile<true>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm);
-    else mlp_hidden_tile<false>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm);
+    if ((I.d & 3) == 0)
+        mlp_hidden_tile<true>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm, I.f[slot]);
+    else
+        mlp_hidden_tile<false>(I.x[slot], I.n, I.d, I.w1, I.hidden, I.hwork, rt * kHM, ct * kHN, sm, I.f[slot]);
 }
 
 template <int M>
@@ -306,7 +331,7 @@ __global__ __launch_bounds__(256) void b_mlp_head_kernel(const BInst* __restrict
     const int row = local * 4 + static_cast<int>(threadIdx.x >> 6);
     if (row >= I.n) return;
     const int slot = (I.cur + which) % 3;
-    mlp_head_row<M>(I.x[slot], I.d, I.hwork, I.hidden, I.w2, I.f[slot], I.n, I.cv[slot], row,
+    mlp_head_row<M>(I.hwork, I.hidden, I.w2, I.f[slot], I.n, I.cv[slot], row,
                     static_cast<int>(threadIdx.x & 63));
 }
 
@@ -389,11 +414,11 @@ int eval_mlp(const float* x, int n, int d, const float* w1, int hidden, const fl
         ldf < n)
         return fail(kInvalidArgument, "eval_mlp: invalid arguments");
     hipLaunchKernelGGL(mlp_hidden_kernel, dim3(mlp_tile_count(n, hidden)), dim3(256), 0, stream, x, n, d, w1,
-                       hidden, hwork);
+                       hidden, hwork, f);
     MOEA_HIP(hipGetLastError());
     const dim3 grid(ceil_div(n, 4)), block(256);
     switch (m) {
-#define MOEA_H(MM) case MM: hipLaunchKernelGGL(mlp_head_kernel<MM>, grid, block, 0, stream, x, n, d, hwork, hidden, w2, f, ldf, cv); break;
+#define MOEA_H(MM) case MM: hipLaunchKernelGGL(mlp_head_kernel<MM>, grid, block, 0, stream, n, hwork, hidden, w2, f, ldf, cv); break;
         MOEA_H(1) MOEA_H(2) MOEA_H(3) MOEA_H(4) MOEA_H(5) MOEA_H(6) MOEA_H(7) MOEA_H(8) MOEA_H(9) MOEA_H(10)
 #undef MOEA_H
         default: return fail(kUnsupported, "eval_mlp: unsupported objective count");
