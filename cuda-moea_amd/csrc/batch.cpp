@@ -118,6 +118,7 @@ void fill_record(const moea_batch_problem& p, BInst& I)
     I.nm = z.nm; I.W = z.W; I.Wc = z.Wc; I.npad = z.npad;
     I.max_gen = p.max_generations;
     I.nchunks = z.nchunks;
+    I.ddiv = fast_div_make(static_cast<uint32_t>(z.d));
     I.seed = p.seed;
     I.sbx_eta0 = p.sbx_eta[0]; I.sbx_eta1 = p.sbx_eta[1];
     I.sbx_prob = p.sbx_prob; I.sbx_copy = p.sbx_copy_prob;

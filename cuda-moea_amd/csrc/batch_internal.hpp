@@ -30,6 +30,7 @@ struct BInst {
     int nm, W, Wc, npad;          // NDS over the 2n merged rows
     int max_gen;
     int nchunks;                  // association chunks of 1024 directions
+    FastDiv ddiv;                 // division by d (reproduction element indices)
     uint64_t seed;
     float sbx_eta0, sbx_eta1, sbx_prob, sbx_copy, pm_eta0, pm_eta1, pm_prob;
     // problem (caller-owned device arrays)

@@ -102,6 +102,23 @@ __host__ __device__ inline U4 draw4(uint64_t seed, uint64_t index, uint32_t gen,
 // 24-bit uniform in [0, 1); never 1.0.
 __host__ __device__ inline float u01(uint32_t u) { return static_cast<float>(u >> 8) * 5.9604644775390625e-8f; }
 
+// In-kernel draws of one (pair, variable) element e = pair * d + j.  One
+// Philox block of stream kStSbx carries the SBX draws and both children's
+// mutation gates: u_beta, u_prob, u_copy from the top 24 bits of words x, y, z
+// (u01), the gate of child a from the top 24 bits of w, the gate of child b
+// from the low bytes of x, y, z, and the SBX direction from bit 7 of w.  The
+// mutation deltas come from a second block (stream kStPm: child a word x,
+// child b word y), generated only when a gate opens; with the usual rate 1/D
+// that is a small fraction of the elements, so most elements cost one Philox
+// block instead of two.  The tail row of an odd population takes child a's
+// draws at index pairs * d + j.
+__host__ __device__ inline float gate_b(const U4& r)
+{
+    return static_cast<float>(((r.x & 0xFFu) << 16) | ((r.y & 0xFFu) << 8) | (r.z & 0xFFu)) *
+           5.9604644775390625e-8f;
+}
+__host__ __device__ inline int sbx_direction(const U4& r) { return static_cast<int>((r.w >> 7) & 1u); }
+
 // The reference's niching draw: SplitMix64 finalizer of (counter ^ seed), low
 // 32 bits (NS/core/cuda/cuda_random.cu:1606-1614).
 __host__ __device__ inline uint32_t splitmix32(uint64_t counter, uint64_t seed)
@@ -118,6 +135,38 @@ __host__ __device__ inline uint32_t splitmix32(uint64_t counter, uint64_t seed)
 __host__ __device__ inline uint64_t niche_base(uint32_t gen, int which)
 {
     return (static_cast<uint64_t>(gen) << 33) | (static_cast<uint64_t>(which & 1) << 32);
+}
+
+// ----------------------------------------------------------------------------
+// Division by an invariant divisor d >= 1 for dividends below 2^31: one
+// high multiply and a shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994): with l = ceil(log2 d) and
+// p = 31 + l, m = ceil(2^p / d) < 2^32 and floor(x / d) = (x * m) >> p.
+// ----------------------------------------------------------------------------
+struct FastDiv {
+    uint32_t d, mul, shr;
+};
+
+__host__ __device__ inline FastDiv fast_div_make(uint32_t d)
+{
+    FastDiv f{d, 0u, 0u};
+    if (d <= 1) return f;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    const uint32_t p = 31 + l;
+    f.mul = static_cast<uint32_t>(((1ull << p) + d - 1) / d);
+    f.shr = p - 32;
+    return f;
+}
+
+__host__ __device__ inline uint32_t fast_div(uint32_t x, const FastDiv& f)
+{
+    if (f.d <= 1) return x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(x, f.mul) >> f.shr;
+#else
+    return static_cast<uint32_t>((static_cast<uint64_t>(x) * f.mul) >> 32) >> f.shr;
+#endif
 }
 
 // ----------------------------------------------------------------------------

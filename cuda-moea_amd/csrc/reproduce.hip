@@ -89,6 +89,8 @@ __device__ __forceinline__ float pm_one(float value, float ug, float ud, float l
 
 struct RepArgs {
     int n, d, pairs;
+    FastDiv dd;          // division by d
+    int small;           // pairs * d + d < 2^31: element indices fit the fast division
     const float* px;
     const int32_t* winners;
     const float* bounds;
@@ -110,31 +112,45 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
     const bool tail = (R.n & 1) != 0;
     if (e >= total_pair + (tail ? R.d : 0)) return;
     if (e < total_pair) {
-        const int pair = static_cast<int>(e / R.d);
-        const int j = static_cast<int>(e - static_cast<long long>(pair) * R.d);
+        int pair, j;
+        if (R.small) {
+            const uint32_t e32 = static_cast<uint32_t>(e);
+            pair = static_cast<int>(fast_div(e32, R.dd));
+            j = static_cast<int>(e32 - static_cast<uint32_t>(pair) * static_cast<uint32_t>(R.d));
+        } else {
+            pair = static_cast<int>(e / R.d);
+            j = static_cast<int>(e - static_cast<long long>(pair) * R.d);
+        }
         const int w0 = R.winners[2 * pair], w1 = R.winners[2 * pair + 1];
         const float x1 = R.px[static_cast<size_t>(w0) * R.d + j];
         const float x2 = R.px[static_cast<size_t>(w1) * R.d + j];
         const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
         float ub, up, uc;
         int sg;
+        U4 rs{0u, 0u, 0u, 0u};
+        const bool draw_s = !R.sbx_u || (R.mutate && !R.pm_gate);
+        if (draw_s) rs = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
         if (R.sbx_u) {
             ub = R.sbx_u[e]; up = R.sbx_p[e]; sg = R.sbx_sign[e]; uc = R.sbx_copy[e];
         } else {
-            const U4 r = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
-            ub = u01(r.x); up = u01(r.y); sg = static_cast<int>(r.z >> 31); uc = u01(r.w);
+            ub = u01(rs.x); up = u01(rs.y); uc = u01(rs.z); sg = sbx_direction(rs);
         }
         float ca, cb;
         sbx_one(x1, x2, ub, up, sg, uc, lo, hi, R.eta_c, R.prob_c, R.copy_c, ca, cb);
         if (R.mutate) {
             const size_t ra = static_cast<size_t>(2 * pair) * R.d + j;
             const size_t rb = ra + R.d;
-            float ga, da, gb, db;
+            float ga, da = 0.0f, gb, db = 0.0f;
             if (R.pm_gate) {
                 ga = R.pm_gate[ra]; da = R.pm_delta[ra]; gb = R.pm_gate[rb]; db = R.pm_delta[rb];
             } else {
-                const U4 r = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm);
-                ga = u01(r.x); da = u01(r.y); gb = u01(r.z); db = u01(r.w);
+                ga = u01(rs.w);
+                gb = gate_b(rs);
+                if (ga < R.rate_m || gb < R.rate_m) {
+                    const U4 rp = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm);
+                    da = u01(rp.x);
+                    db = u01(rp.y);
+                }
             }
             ca = pm_one(ca, ga, da, lo, hi, R.eta_m, R.rate_m);
             cb = pm_one(cb, gb, db, lo, hi, R.eta_m, R.rate_m);
@@ -148,12 +164,13 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
         float v = R.px[static_cast<size_t>(R.winners[row]) * R.d + j];
         if (R.mutate) {
             const size_t ra = static_cast<size_t>(row) * R.d + j;
-            float g, dd;
+            float g, dd = 0.0f;
             if (R.pm_gate) {
                 g = R.pm_gate[ra]; dd = R.pm_delta[ra];
             } else {
-                const U4 r = draw4(R.seed, static_cast<uint64_t>(row >> 1) * R.d + j, R.gen, kStPm);
-                g = u01(r.x); dd = u01(r.y);
+                const U4 rs = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
+                g = u01(rs.w);
+                if (g < R.rate_m) dd = u01(draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm).x);
             }
             v = pm_one(v, g, dd, lo, hi, R.eta_m, R.rate_m);
         }
@@ -228,6 +245,8 @@ __global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__
     const int cur = I.cur, off = (I.cur + 1) % 3, gen = I.gen;
     RepArgs R{};
     R.n = I.n; R.d = I.d; R.pairs = I.n / 2;
+    R.dd = I.ddiv;
+    R.small = (static_cast<long long>(R.pairs) + 1) * R.d < (1ll << 31);
     R.px = I.x[cur]; R.winners = I.winners; R.bounds = I.bounds;
     R.eta_c = annealed_hd(I.sbx_eta0, I.sbx_eta1, gen, I.max_gen);
     R.prob_c = I.sbx_prob; R.copy_c = I.sbx_copy;
@@ -264,6 +283,8 @@ int reproduce(const moea_reproduce_args& A)
     MOEA_HIP(hipGetLastError());
     RepArgs R{};
     R.n = A.n; R.d = A.d; R.pairs = A.n / 2;
+    R.dd = fast_div_make(static_cast<uint32_t>(A.d));
+    R.small = (static_cast<long long>(R.pairs) + 1) * R.d < (1ll << 31);
     R.px = A.parents_x; R.winners = A.parent_indices; R.bounds = A.bounds;
     R.eta_c = A.sbx_eta; R.prob_c = A.sbx_prob; R.copy_c = A.sbx_copy_prob;
     R.mutate = (A.mutation == MOEA_MUTATION_POLYNOMIAL) ? 1 : 0;

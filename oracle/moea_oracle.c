@@ -915,6 +915,19 @@ ORC_API void orc_draw_mating(uint64_t seed, uint32_t gen, int count, int active,
     }
 }
 
+/* In-kernel draw layout of one (pair, variable) element e = pair * d + j (the
+ * device's sbx_pm_elem): one Philox block of stream ST_SBX gives u_beta,
+ * u_prob, u_copy (top 24 bits of words 0..2), the SBX direction (bit 7 of
+ * word 3), child a's mutation gate (top 24 bits of word 3) and child b's gate
+ * (the low bytes of words 0..2, most significant first).  Mutation deltas come
+ * from a block of stream ST_PM (child a word 0, child b word 1).  The device
+ * draws that block only when a gate opens; the values it uses are the same.
+ * The tail row of an odd population takes child a's draws at pairs * d + j. */
+static float gate_b(const uint32_t o[4])
+{
+    return (float)(((o[0] & 0xFFu) << 16) | ((o[1] & 0xFFu) << 8) | (o[2] & 0xFFu)) * 5.9604644775390625e-8f;
+}
+
 ORC_API void orc_draw_sbx(uint64_t seed, uint32_t gen, int pairs, int d, float* ub, float* up,
                           int32_t* sign, float* uc)
 {
@@ -926,8 +939,8 @@ ORC_API void orc_draw_sbx(uint64_t seed, uint32_t gen, int pairs, int d, float* 
             philox_at(seed, e, gen, ST_SBX, o);
             ub[e] = u01(o[0]);
             up[e] = u01(o[1]);
-            sign[e] = (int32_t)(o[2] >> 31);
-            uc[e] = u01(o[3]);
+            uc[e] = u01(o[2]);
+            sign[e] = (int32_t)((o[3] >> 7) & 1u);
         }
 }
 
@@ -936,11 +949,18 @@ ORC_API void orc_draw_pm(uint64_t seed, uint32_t gen, int n, int d, float* ug, f
 #pragma omp parallel for schedule(static)
     for (int r = 0; r < n; ++r)
         for (int j = 0; j < d; ++j) {
-            uint32_t o[4];
-            philox_at(seed, (uint64_t)(r >> 1) * d + j, gen, ST_PM, o);
-            const int w = (r & 1) * 2;
-            ug[(size_t)r * d + j] = u01(o[w]);
-            ud[(size_t)r * d + j] = u01(o[w + 1]);
+            uint32_t os[4], op[4];
+            const uint64_t e = (uint64_t)(r >> 1) * d + j;
+            philox_at(seed, e, gen, ST_SBX, os);
+            philox_at(seed, e, gen, ST_PM, op);
+            const size_t i = (size_t)r * d + j;
+            if ((r & 1) == 0) {
+                ug[i] = u01(os[3]);
+                ud[i] = u01(op[0]);
+            } else {
+                ug[i] = gate_b(os);
+                ud[i] = u01(op[1]);
+            }
         }
 }
 

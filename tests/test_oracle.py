@@ -44,6 +44,39 @@ def _ulp32(x):
     return float(np.spacing(np.abs(x)))
 
 
+def test_sbx_pm_draws_uniform_and_independent():
+    """The SBX draws and both children's mutation gates share one Philox block
+    per element: every value must still be uniform on [0, 1) and the values
+    of one element pairwise uncorrelated (a layout that reused bits would
+    correlate two of them), the SBX direction a fair coin, and the deltas of
+    rows 2p and 2p + 1 distinct streams."""
+    pairs, d = 2000, 257
+    ub, up, sg, uc = O.draw_sbx(12345, 7, pairs, d)
+    ug, ud = O.draw_pm(12345, 7, 2 * pairs, d)
+    ga = ug.reshape(pairs, 2, d)[:, 0].ravel()
+    gb = ug.reshape(pairs, 2, d)[:, 1].ravel()
+    da = ud.reshape(pairs, 2, d)[:, 0].ravel()
+    db = ud.reshape(pairs, 2, d)[:, 1].ravel()
+    vals = {"ub": ub, "up": up, "uc": uc, "ga": ga, "gb": gb, "da": da, "db": db}
+    n = ub.size
+    for name, v in vals.items():
+        assert v.min() >= 0.0 and v.max() < 1.0, name
+        assert abs(v.mean() - 0.5) < 4 * math.sqrt(1 / 12 / n), name
+        hist, _ = np.histogram(v, bins=16, range=(0.0, 1.0))
+        chi2 = ((hist - n / 16) ** 2 / (n / 16)).sum()
+        assert chi2 < 50, (name, chi2)        # 15 dof: p < 1e-5 beyond 50
+    names = list(vals)
+    for a in range(len(names)):
+        for b in range(a + 1, len(names)):
+            r = np.corrcoef(vals[names[a]], vals[names[b]])[0, 1]
+            assert abs(r) < 5 / math.sqrt(n), (names[a], names[b], r)
+    assert abs(sg.mean() - 0.5) < 4 * math.sqrt(0.25 / n)
+    # a small gate rate opens as often as it should
+    rate = 1.0 / 64
+    assert abs((gb < rate).mean() - rate) < 5 * math.sqrt(rate / n)
+    assert abs((ga < rate).mean() - rate) < 5 * math.sqrt(rate / n)
+
+
 def test_powf_close_to_fp64():
     rng = np.random.default_rng(0)
     # the SBX / PM call sites: base in [0, 2] or up to 2^24 with a root exponent,
