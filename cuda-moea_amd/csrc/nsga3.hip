@@ -241,13 +241,6 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
 // thread solves.  Extreme point j is the (value, index)-smallest candidate with
 // a finite ASF value (-1 if none) -- an order-independent reduction.
 constexpr int kSolveThreads = 256;
-// hyperplane_fp64 (common.hpp) for one wave: lane i < M holds row i of A and
-// of V in registers.  Every element update is the same expression as in the
-// scalar solve and every sum is accumulated in the same order (row 0 first)
-// through cross-lane reads, so the result is bit-identical; the rotation
-// parameters are computed redundantly by every lane from the uniform sums.
-// Returns 1 for the row-max fallback (the scalar solve's conditions); lane r
-// gets a_out[r] in `a_lane`.
 // Lane i's value of v, uniform over the wave (two scalar lane reads).
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
@@ -257,107 +250,159 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-template <int M>
-__device__ __forceinline__ int hyperplane_fp64_wave(const double (&row)[M], double& a_lane)
+// LDS of the one-wave hyperplane solve: A and V row-major (kMaxM x kMaxM),
+// three sum rows, and the column weights.
+struct SolveLds {
+    double a[kMaxM * kMaxM];
+    double v[kMaxM * kMaxM];
+    double sums[3][16];
+    double w[kMaxM];
+    int skip[kMaxM];
+};
+
+// hyperplane_fp64 (common.hpp) on one wave, bit-identical to it: the same
+// rotation sequence, every element update the same expression, every sum
+// accumulated in the same order.  Lane i < m updates row i of A and V (in
+// LDS); each rotation's three column sums are taken by lanes 0, 1, 2, each
+// summing its row of products in order, and broadcast by lane reads; the
+// rotation parameters are then computed redundantly by every lane.  The
+// singular values and weights are computed by lane j for column j.  The code
+// is a compact loop: a fully unrolled 10 x 10 sweep does not fit the
+// instruction cache once other kernels' code shares it, and then runs several
+// times slower.  Returns 1 for the row-max fallback (the scalar solve's
+// conditions); lane r < m gets a_out[r] in `a_lane`.
+__device__ __forceinline__ int hyperplane_fp64_wave(int m, SolveLds& L, double& a_lane)
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
-    double A[M], V[M];
+    const bool own = lane < m;
+    double* ra = L.a + lane * kMaxM;
+    double* rv = L.v + lane * kMaxM;
     bool bad = false;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        A[j] = lane < M ? row[j] : 0.0;
-        V[j] = (lane == j) ? 1.0 : 0.0;
-        bad = bad || !isfinite(A[j]);
-    }
+    if (own)
+        for (int j = 0; j < m; ++j) {
+            bad = bad || !isfinite(ra[j]);
+            rv[j] = (lane == j) ? 1.0 : 0.0;
+        }
     if (__ballot(bad)) return 1;
+    // Two equal rows (the same individual extreme on two axes, as a rule at
+    // M >= 8) make the result the fallback whatever the sweeps do: rotations
+    // act on rows alike, so the rows stay equal, the columns lie in an
+    // (m-1)-dimensional subspace, and m of them cannot be orthogonal unless one
+    // is (numerically) zero -- the sweeps then either never meet the 1e-15
+    // test (60 sweeps, fallback) or end with sigma_min / sigma_max far below
+    // 1e-6 (fallback).  Taking the fallback at once saves the 60 sweeps.
+    bool dup = false;
+    if (own)
+        for (int i = 0; i < lane; ++i) {
+            bool eq = true;
+            for (int j = 0; j < m; ++j) eq = eq && (L.a[i * kMaxM + j] == ra[j]);
+            dup = dup || eq;
+        }
+    if (__ballot(dup)) return 1;
     int converged = 0;
     for (int sweep = 0; sweep < 60 && !converged; ++sweep) {
         int rotated = 0;
-#pragma unroll
-        for (int p = 0; p < M - 1; ++p) {
-#pragma unroll
-            for (int q = p + 1; q < M; ++q) {
-                const double pp = A[p] * A[p], qq = A[q] * A[q], pq = A[p] * A[q];
-                double alpha = 0.0, beta = 0.0, gamma = 0.0;
-#pragma unroll
-                for (int i = 0; i < M; ++i) {
-                    alpha = alpha + readlane_f64(pp, i);
-                    beta = beta + readlane_f64(qq, i);
-                    gamma = gamma + readlane_f64(pq, i);
+        for (int p = 0; p < m - 1; ++p) {
+            for (int q = p + 1; q < m; ++q) {
+                const double ap = own ? ra[p] : 0.0, aq = own ? ra[q] : 0.0;
+                if (own) {
+                    L.sums[0][lane] = ap * ap;
+                    L.sums[1][lane] = aq * aq;
+                    L.sums[2][lane] = ap * aq;
                 }
+                __builtin_amdgcn_wave_barrier();
+                double acc = 0.0;
+                if (lane < 3) {
+                    double t[kMaxM];
+#pragma unroll
+                    for (int i = 0; i < kMaxM; ++i) t[i] = L.sums[lane][i];
+#pragma unroll
+                    for (int i = 0; i < kMaxM; ++i)
+                        if (i < m) acc = acc + t[i];
+                }
+                const double alpha = readlane_f64(acc, 0), beta = readlane_f64(acc, 1),
+                             gamma = readlane_f64(acc, 2);
+                __builtin_amdgcn_wave_barrier();
                 if (gamma == 0.0 || fabs(gamma) <= 1e-15 * sqrt(alpha * beta)) continue;
                 rotated = 1;
                 const double zeta = (beta - alpha) / (2.0 * gamma);
                 const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 const double c = 1.0 / sqrt(1.0 + t * t);
                 const double sn = c * t;
-                const double ap = A[p], aq = A[q];
-                A[p] = c * ap - sn * aq;
-                A[q] = sn * ap + c * aq;
-                const double vp = V[p], vq = V[q];
-                V[p] = c * vp - sn * vq;
-                V[q] = sn * vp + c * vq;
+                if (own) {
+                    ra[p] = c * ap - sn * aq;
+                    ra[q] = sn * ap + c * aq;
+                    const double vp = rv[p], vq = rv[q];
+                    rv[p] = c * vp - sn * vq;
+                    rv[q] = sn * vp + c * vq;
+                }
             }
         }
         if (!rotated) converged = 1;
     }
     if (!converged) return 1;
-    double sig[M];
-    double smax = 0.0, smin = static_cast<double>(INFINITY);
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const double sq = A[j] * A[j];
+    // lane j: singular value of column j, then its weight
+    double sig = 0.0;
+    if (own) {
         double ss = 0.0;
-#pragma unroll
-        for (int i = 0; i < M; ++i) ss = ss + readlane_f64(sq, i);
-        sig[j] = sqrt(ss);
-        if (!isfinite(sig[j])) return 1;
-        if (sig[j] > smax) smax = sig[j];
-        if (sig[j] < smin) smin = sig[j];
+        for (int i = 0; i < m; ++i) {
+            const double e = L.a[i * kMaxM + lane];
+            ss = ss + e * e;
+        }
+        sig = sqrt(ss);
+    }
+    const bool finite_bad = own && !isfinite(sig);
+    if (__ballot(finite_bad)) return 1;
+    double smax = 0.0, smin = static_cast<double>(INFINITY);
+    for (int j = 0; j < m; ++j) {
+        const double sj = readlane_f64(sig, j);
+        if (sj > smax) smax = sj;
+        if (sj < smin) smin = sj;
     }
     if (smin < 1e-10) return 1;
     if (smax / smin > 1e6) return 1;
-    double out = 0.0;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        if (!(sig[j] > 1e-5)) continue;
-        const double ud = A[j] / sig[j];
-        double usum = 0.0;
-#pragma unroll
-        for (int i = 0; i < M; ++i) usum = usum + readlane_f64(ud, i);
-        const double w = usum / sig[j];
-        out = out + V[j] * w;
+    if (own) {
+        const bool use = sig > 1e-5;
+        double w = 0.0;
+        if (use) {
+            double usum = 0.0;
+            for (int i = 0; i < m; ++i) usum = usum + L.a[i * kMaxM + lane] / sig;
+            w = usum / sig;
+        }
+        L.w[lane] = w;
+        L.skip[lane] = use ? 0 : 1;
     }
+    __builtin_amdgcn_wave_barrier();
+    double out = 0.0;
+    if (own)
+        for (int j = 0; j < m; ++j)
+            if (!L.skip[j]) out = out + rv[j] * L.w[j];
     a_lane = out;
     return 0;
 }
 
 // Wave 0 of the solve block: extreme-point matrix rows, the solve, the
 // row-max fallback and the float intercepts (lane k writes a_out[k]).
-template <int M>
-__device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, int nm, int n_nds,
+__device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, int nm, int m, int n_nds,
                                                 const int (&ext)[kMaxM], const float (&rmax)[kMaxM],
-                                                int32_t* __restrict__ meta, float* __restrict__ a_out)
+                                                SolveLds& L, int32_t* __restrict__ meta, float* __restrict__ a_out)
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     int my_ext = -1;
+    float my_rmax = 0.0f;
 #pragma unroll
-    for (int i = 0; i < M; ++i) if (lane == i) my_ext = ext[i];
-    double row[M];
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-        row[k] = (lane < M && my_ext >= 0 && my_ext < n_nds)
-                     ? static_cast<double>(fn[static_cast<size_t>(k) * nm + my_ext])
-                     : static_cast<double>(INFINITY);
+    for (int i = 0; i < kMaxM; ++i)
+        if (lane == i) { my_ext = ext[i]; my_rmax = rmax[i]; }
+    if (lane < m)
+        for (int k = 0; k < m; ++k)
+            L.a[lane * kMaxM + k] = (my_ext >= 0 && my_ext < n_nds)
+                                        ? static_cast<double>(fn[static_cast<size_t>(k) * nm + my_ext])
+                                        : static_cast<double>(INFINITY);
+    __builtin_amdgcn_wave_barrier();
     double a = 0.0;
-    const int ill = hyperplane_fp64_wave<M>(row, a);
-    if (lane >= M) return;
-    if (ill) {
-        float rm = rmax[0];
-#pragma unroll
-        for (int k = 1; k < M; ++k) if (lane == k) rm = rmax[k];
-        a = (rm > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(rm, 1e-6f)) : 1.0;
-    }
+    const int ill = hyperplane_fp64_wave(m, L, a);
+    if (lane >= m) return;
+    if (ill) a = (my_rmax > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(my_rmax, 1e-6f)) : 1.0;
     float af = static_cast<float>(a);
     if (!isfinite(af) || af <= 0.0f) af = 1.0f;
     a_out[lane] = af;
@@ -414,13 +459,8 @@ __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, in
         ext[j] = bi;
         rmax[j] = rm;
     }
-    switch (m) {
-#define MOEA_NS(MM) case MM: norm_solve_tail<MM>(fn, nm, n_nds, ext, rmax, meta, a_out); break;
-        MOEA_NS(1) MOEA_NS(2) MOEA_NS(3) MOEA_NS(4) MOEA_NS(5) MOEA_NS(6) MOEA_NS(7) MOEA_NS(8) MOEA_NS(9)
-        MOEA_NS(10)
-#undef MOEA_NS
-        default: break;
-    }
+    __shared__ SolveLds s_solve;
+    if (m >= 1 && m <= kMaxM) norm_solve_tail(fn, nm, m, n_nds, ext, rmax, s_solve, meta, a_out);
 }
 
 __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,

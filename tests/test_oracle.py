@@ -219,6 +219,21 @@ def test_hyperplane_matches_lapack(m, seed):
     np.testing.assert_allclose(a, np.linalg.solve(E, np.ones(m)), rtol=1e-12)
 
 
+def test_hyperplane_equal_rows_always_fall_back():
+    """Two equal rows of E (one individual extreme on two axes) always end in
+    the row-max fallback: the device takes it without sweeping, which is
+    exact only if the full solve never accepts such a matrix."""
+    rng = np.random.default_rng(11)
+    for m in range(2, 11):
+        for trial in range(40):
+            scale = 10.0 ** rng.uniform(-6, 6)
+            E = (np.abs(rng.normal(0, 0.3, (m, m))) + np.eye(m)) * scale
+            i, j = rng.choice(m, 2, replace=False)
+            E[j] = E[i]
+            ill, _ = O.hyperplane_fp64(E)
+            assert ill == 1, (m, trial)
+
+
 def test_hyperplane_truncated_pinv():
     # sigma_min in (1e-10, 1e-5] with kappa <= 1e6: that singular value is dropped
     U, _ = np.linalg.qr(np.random.default_rng(0).standard_normal((3, 3)))
