@@ -62,6 +62,39 @@ def cpu_baseline(threads: int) -> dict:
                       f"{ms / 1000:.1f} s"}
 
 
+def config0_cpu(threads: int) -> dict:
+    """Config 0 on the oracle (the C restatement; the reference has no CPU
+    path): NSGA-III N = 92, M = 3, H = 12 (K = 91), D = 12, quadratic-center
+    objectives with centers from torch.rand after manual_seed(0), SBX eta 30
+    p 1, PM eta 20 rate 1/D, tournament mating, 200 generations, seed 0;
+    timed single-threaded and on `threads` OpenMP threads."""
+    import platform
+    import numpy as np
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    g = torch.Generator().manual_seed(0)
+    centers = torch.rand(3, 12, generator=g).numpy()
+    bounds = np.tile(np.array([0.0, 1.0], np.float32), 12)
+    out = {}
+    for t in (1, threads):
+        O.set_threads(t)
+        _, _, _, _, ms = O.nsga3_run(92, 3, 12, 200, 0, bounds, centers, partitions=12)
+        out[f"ms_{t}_threads"] = round(ms, 2)
+    O.set_threads(threads)
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    out.update({"workload": "config 0: NSGA-III N=92 M=3 K=91 D=12, 200 generations, oracle on the host",
+                "nproc": os.cpu_count(), "cpu_model": cpu, "machine": platform.machine()})
+    return out
+
+
 def batched_config(generations: int, count: int, plain_sample: int) -> dict:
     """Config 3: `count` independent NSGA-III runs (N log-uniform in [64, 4096],
     even; M in {3, 5, 8} with Das-Dennis H = 12 / 6 / 3; D log-uniform in
@@ -263,6 +296,43 @@ def serving_config(seconds: float, rate: float, plain_sample: int, budget_bytes:
             "p99_speedup": round(float(np.percentile(plat, 99) / np.percentile(lat, 99)), 1)}
 
 
+def selection_bench(calls: int) -> dict:
+    """The standalone selection bench of config 1: one NSGA-III selection
+    (moea_nsga3_select: constrained NDS, split, normalization, association,
+    niching, gather) over 2N = 65536 merged rows s*u (u uniform on the
+    positive unit-sphere orthant, s in [1, 2]), all feasible, K = 32640,
+    in-kernel draws; `calls` selections between HIP events."""
+    import numpy as np
+    import torch
+    from cuda_moea import kernels as K
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    n, m = N, M
+    rng = np.random.default_rng(2024)
+    g = np.abs(rng.standard_normal((2 * n, m)))
+    u = g / np.linalg.norm(g, axis=1, keepdims=True)
+    fm = np.ascontiguousarray((rng.uniform(1.0, 2.0, (2 * n, 1)) * u).T.astype(np.float32))
+    f = torch.from_numpy(fm).to(dev)
+    cv = torch.zeros(2 * n, dtype=torch.float32, device=dev)
+    x = torch.zeros(2 * n, 2, dtype=torch.float32, device=dev)
+    w = torch.from_numpy(K.das_dennis(n, m)).to(dev)
+    ideal = f[:, :n].min(dim=1).values.contiguous()
+    par = K.Population(x[:n].contiguous(), f[:, :n].contiguous(), cv[:n].contiguous())
+    ofs = K.Population(x[n:].contiguous(), f[:, n:].contiguous(), cv[n:].contiguous())
+    K.nsga3_select(par, ofs, w, ideal.clone(), seed=1, generation=0)
+    stream.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for c in range(calls):
+        _, _, st = K.nsga3_select(par, ofs, w, ideal.clone(), seed=1, generation=c)
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / calls
+    return {"workload": "NSGA-III selection alone, 2N = 65536 rows s*u on the positive orthant, K = 32640",
+            "ms_per_selection": round(ms, 4), "calls": calls, "max_front": int(st[0].item()),
+            "n_prior": int(st[1].item()), "n_last": int(st[2].item())}
+
+
 def rvea_config(generations: int, warmup: int) -> dict:
     """Config 2: RVEA, N = 16384, M = 5, D = 131072, adaptive reference
     directions (frequency 0.1), alpha = 2, seeded-MLP objective D->256->5
@@ -414,6 +484,8 @@ def main() -> int:
     if not args.no_batched:
         batched = batched_config(args.batched_generations, args.batched_instances, args.plain_sample)
 
+    selection = selection_bench(20)
+
     rvea = None
     if not args.no_rvea:
         rvea = rvea_config(args.rvea_generations, 2)
@@ -428,8 +500,10 @@ def main() -> int:
         dtlz1 = dtlz1_comparison(50, 5)
 
     cpu = None
+    config0 = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_threads)
+        config0 = config0_cpu(args.cpu_threads)
 
     line = {
         "metric": "gens/sec NSGA-III N=32768 M=3; batched runs/sec + p99 latency vs plain",
@@ -458,10 +532,12 @@ def main() -> int:
                      "frac_reference_work": round(M * nm * (nm - 1) / dom_avg_s / VALU_PEAK_CMP, 4)},
         "stage_ms_per_generation": stage_ms,
         "batched": batched,
+        "selection_bench": selection,
         "serving": serving,
         "rvea": rvea,
         "reference_comparison": dtlz1,
         "cpu_baseline": cpu,
+        "config0_cpu": config0,
     }
     print(json.dumps(line))
     return 0

@@ -755,9 +755,19 @@ __device__ __forceinline__ void gather_row(const int32_t* __restrict__ chosen, i
     const float* src = (s < n) ? px + static_cast<size_t>(s) * d : ox + static_cast<size_t>(s - n) * d;
     float* dst = nx + static_cast<size_t>(row) * d;
     if ((d & 3) == 0) {
+        // four 16-byte loads in flight per lane before their stores
         const float4* s4 = reinterpret_cast<const float4*>(src);
         float4* d4 = reinterpret_cast<float4*>(dst);
-        for (int j = lane; j < (d >> 2); j += 64) d4[j] = s4[j];
+        const int d4n = d >> 2;
+        int j = lane;
+        for (; j + 192 < d4n; j += 256) {
+            const float4 a = s4[j], b = s4[j + 64], c = s4[j + 128], e = s4[j + 192];
+            d4[j] = a;
+            d4[j + 64] = b;
+            d4[j + 128] = c;
+            d4[j + 192] = e;
+        }
+        for (; j < d4n; j += 64) d4[j] = s4[j];
     } else {
         for (int j = lane; j < d; j += 64) dst[j] = src[j];
     }

@@ -1260,3 +1260,12 @@ ORC_API int orc_num_threads(void)
     return 1;
 #endif
 }
+
+ORC_API void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

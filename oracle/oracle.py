@@ -80,6 +80,7 @@ def lib():
                                         c_float, i32p, vp, vp, vp]),
             "orc_rvea_adapt": (None, [f32p, f32p, c_int, c_int, c_int, c_int, f32p]),
             "orc_num_threads": (c_int, []),
+            "orc_set_threads": (None, [c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -323,3 +324,4 @@ def num_threads() -> int:
 
 def set_threads(n: int) -> None:
     os.environ["OMP_NUM_THREADS"] = str(n)
+    lib().orc_set_threads(int(n))

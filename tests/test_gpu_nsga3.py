@@ -248,6 +248,35 @@ def test_nsga3_select(n, m, kind, seed):
     np.testing.assert_array_equal(ideal_g.cpu().numpy(), ideal_o)
 
 
+def test_nsga3_select_bench_full_size():
+    """The standalone selection bench at its full size: 2N = 65536 rows s*u
+    (u uniform on the positive unit-sphere orthant, s in [1, 2]), all
+    feasible, K = 32640 Das-Dennis directions; fronts, the chosen set and
+    its order against the oracle."""
+    n, m = 32768, 3
+    rng = np.random.default_rng(2024)
+    fm = sphere_objectives(rng, m, 2 * n)
+    cvm = np.zeros(2 * n, np.float32)
+    xm = np.zeros((2 * n, 2), np.float32)
+    xm[:, 0] = np.arange(2 * n, dtype=np.float32)
+    w = O.das_dennis(n, m)
+    assert w.shape == (3, 32640)
+    ideal0 = fm[:, :n].min(axis=1).astype(np.float32)
+    ua = rng.integers(0, 2**32, 2 * n, dtype=np.uint64).astype(np.uint32)
+    ub = rng.integers(0, 2**32, 2 * n, dtype=np.uint64).astype(np.uint32)
+    print("nsga3_select selection bench n=32768 m=3 K=32640", flush=True)
+    chosen, fr_mix, st, ideal_o = O.nsga3_select(fm, cvm, w, ideal0, ua, ub)
+    par = K.Population(_f(xm[:n]), _f(fm[:, :n]), _f(cvm[:n]))
+    ofs = K.Population(_f(xm[n:]), _f(fm[:, n:]), _f(cvm[n:]))
+    ideal_g = _f(ideal0)
+    nxt, nfr, stg = K.nsga3_select(par, ofs, _f(w), ideal_g, draws={"niche_a": ua, "niche_b": ub})
+    stg = stg.cpu().numpy()
+    assert stg[0] == st[0] and stg[1] == st[1] and stg[2] == st[2], (stg, st)
+    np.testing.assert_array_equal(nxt.x[:, 0].cpu().numpy().astype(np.int64), chosen)
+    np.testing.assert_array_equal(nfr.cpu().numpy(), fr_mix[chosen])
+    np.testing.assert_array_equal(nxt.f.cpu().numpy(), fm[:, chosen])
+
+
 def test_nsga3_select_generated_draws():
     n, m, seed, gen = 300, 3, 2887, 4
     rng = np.random.default_rng(11)
