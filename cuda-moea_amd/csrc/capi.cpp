@@ -73,7 +73,7 @@ using namespace moea;
 
 extern "C" {
 
-int moea_abi_version(void) { return 1; }
+int moea_abi_version(void) { return 2; }
 
 const char* moea_last_error(void) { return g_last_error.c_str(); }
 
@@ -153,6 +153,15 @@ int moea_nsga3_select(const moea_nsga3_select_args* args)
 {
     if (!args) return fail(kInvalidArgument, "moea_nsga3_select: null args");
     return nsga3_select(*args);
+}
+
+size_t moea_associate_workspace_bytes(int ld, int m, int k) { return associate_workspace_bytes(ld, m, k); }
+
+int moea_associate(const float* fn, int ld, int n, int m, const float* a, const float* directions, int k,
+                   float* dist, int32_t* ref, int method, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return associate(fn, ld, n, m, a, directions, k, dist, ref, method, workspace, workspace_bytes,
+                     static_cast<hipStream_t>(stream));
 }
 
 int moea_ideal_point(const float* f, int ldf, int n, int m, float* ideal, void* stream)

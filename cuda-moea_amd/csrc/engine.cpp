@@ -90,6 +90,7 @@ int run_init(moea_run_args& a)
         MOEA_HIP(hipGetLastError());
     }
     a.generation = 0;
+    a.directions_ready = 0;   // run_init's sort used the workspace
     return kOk;
 }
 
@@ -142,10 +143,12 @@ int run_step(moea_run_args& a, int generations)
             s.draws = nullptr;
             s.seed = a.seed;
             s.generation = g;
+            s.directions_ready = a.directions_ready;
             s.stats = a.stats;
             s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
             s.stream = a.stream;
             MOEA_CHECK(nsga3_select(s));
+            a.directions_ready = 1;
             a.fronts_current = 1 - a.fronts_current;
         } else {
             moea_rvea_select_args s{};

@@ -20,6 +20,9 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 size_t nsga3_workspace_bytes(int n, int m, int k);
 int nsga3_select(const moea_nsga3_select_args& a);
 int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream);
+size_t associate_workspace_bytes(int ld, int m, int k);
+int associate(const float* fn, int ld, int n, int m, const float* a, const float* w, int k, float* dist,
+              int32_t* ref, int method, void* ws, size_t ws_bytes, hipStream_t st);
 
 // reproduce.hip
 int reproduce(const moea_reproduce_args& a);

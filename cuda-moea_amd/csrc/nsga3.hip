@@ -550,6 +550,304 @@ __device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_
     ref[i] = bj;
 }
 
+// ------------------------------------------------------ pruned association --
+// Exact association with pruning, for many directions.  Directions are cut
+// into blocks of 64 consecutive indices; each block gets a cone (unit center c
+// and half-angle alpha covering every direction of the block).  For an
+// individual x the kernel's d2 for direction w (dot by an fmaf chain,
+// r = fmaf(-dot, w, x), d2 by an fmaf chain) satisfies
+//     d2 >= (1 - 1e-6) |x|^2 max(0, sin(theta) - 1e-6 |w|^2)^2,
+// theta the angle between x and w: the perpendicular residual of the exact
+// product is at least |x| sin(theta) (|w|^2 (2 - |w|^2) <= 1), the dot
+// product's error is at most gamma_M |x| |w| and every later rounding is
+// relative (gamma_10 < 6.1e-7; all terms of the sums are non-negative).
+// With cur the best d2 found so far, every w with sin(theta) > S, where
+//     S = sqrt(max(cur, 2^-80) / ((1 - 1e-6) |x|^2)) + 1e-6 max|w|^2,
+// has d2 > cur: it is neither a minimum nor a tie (the floor keeps the bound
+// where every float rounding of d2 is relative, not subnormal).  Every direction of a block
+// lies within alpha of its center, whose angle to x is theta_c, so if
+//     |cos(theta_c)| < cos(alpha + asin S) = cos(alpha) sqrt(1 - S^2) - sin(alpha) S
+// (alpha + asin S < pi/2) all its angles lie in (asin S, pi - asin S) and the
+// block is skipped; the others are evaluated with the kernel's own arithmetic.
+// S is rounded up, sqrt(1 - S^2) down, cos(alpha) down and sin(alpha) up, and
+// the compare keeps a margin of 1e-6 over the float rounding of cos(theta_c).
+// The result (smallest d2, then smallest index, non-finite d2 never chosen) is
+// therefore the brute-force result bit for bit.  One wave per individual: its
+// lanes compute cos(theta_c) for every block, evaluate the block with the
+// nearest center first, set the threshold from that block's minimum, then
+// evaluate every block that passes it.
+constexpr int kConeMaxBlocks = 1024;   // directions up to 65536
+constexpr int kConeMinK = 2048;        // below this the brute-force kernel is cheaper
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Cone of direction block b: center c (M doubles), cos alpha, sin alpha, and
+// the largest |w|^2 of the block, both angle terms rounded outwards.
+// Layout: (ceil(k / 64), M + 3) doubles.
+template <int M>
+__device__ __forceinline__ void assoc_cone_body(const float* __restrict__ w, int k, const int32_t* __restrict__ perm,
+                                                float* __restrict__ wp, double* __restrict__ cones, int b)
+{
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const bool valid = b * 64 + lane < k;
+    const int j = valid ? perm[b * 64 + lane] : 0;
+    double u[M];
+    double nn = 0.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        const float wc = valid ? w[static_cast<size_t>(c) * k + j] : 0.0f;
+        if (valid) wp[static_cast<size_t>(c) * k + b * 64 + lane] = wc;   // directions in block order
+        u[c] = static_cast<double>(wc);
+        nn += u[c] * u[c];
+    }
+    const bool unit = valid && nn > 0.0 && nn == nn;
+    const double inv = unit ? 1.0 / sqrt(nn) : 0.0;
+    double cen[M];
+    double cn = 0.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        u[c] *= inv;
+        cen[c] = wave_sum_f64(u[c]);
+        cn += cen[c] * cen[c];
+    }
+    cn = sqrt(cn);
+#pragma unroll
+    for (int c = 0; c < M; ++c) cen[c] = (cn > 0.0) ? cen[c] / cn : (c == 0 ? 1.0 : 0.0);
+    double cj = 1.0;                               // lanes past k do not widen the cone
+    if (valid) {
+        if (unit) {
+            cj = 0.0;
+#pragma unroll
+            for (int c = 0; c < M; ++c) cj += cen[c] * u[c];
+        } else {
+            cj = -1.0;                             // zero or non-finite direction: no pruning
+        }
+    }
+    double w2 = valid ? nn : 0.0;
+    if (!(w2 == w2)) w2 = INFINITY;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        cj = fmin(cj, __shfl_xor(cj, o));
+        w2 = fmax(w2, __shfl_xor(w2, o));
+    }
+    double ca = cj - 1e-12;
+    if (ca < -1.0) ca = -1.0;
+    const double sa = sqrt(fmax(0.0, 1.0 - ca * ca)) + 1e-12;
+    if (lane == 0) {
+        double* out = cones + static_cast<size_t>(b) * (M + 3);
+#pragma unroll
+        for (int c = 0; c < M; ++c) out[c] = cen[c];
+        out[M] = ca;
+        out[M + 1] = sa;
+        out[M + 2] = w2;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void assoc_cones(const float* __restrict__ w, int k, const int32_t* __restrict__ perm,
+                                                   float* __restrict__ wp, double* __restrict__ cones)
+{
+    const int b = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (b >= (k + 63) / 64) return;
+    assoc_cone_body<M>(w, k, perm, wp, cones, b);
+}
+
+// Spatial order of the directions for the cone blocks: a Morton key of the
+// first M - 1 coordinates of each unit direction (the last is implied on the
+// simplex), so 64 consecutive keys cover a compact patch.  Any order gives
+// the same association; this one gives tight cones.
+constexpr int kMortonBits = 60;
+
+__host__ __device__ __forceinline__ int morton_coords(int m) { return m > 1 ? m - 1 : 1; }
+
+template <int M>
+__global__ void assoc_dir_keys(const float* __restrict__ w, int k, uint64_t* __restrict__ keys,
+                               int32_t* __restrict__ vals)
+{
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= k) return;
+    constexpr int C = (M > 1) ? M - 1 : 1;
+    constexpr int bits = (kMortonBits / C) < 16 ? (kMortonBits / C) : 16;
+    double u[M];
+    double nn = 0.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        u[c] = static_cast<double>(w[static_cast<size_t>(c) * k + j]);
+        nn += u[c] * u[c];
+    }
+    const double inv = (nn > 0.0 && nn == nn) ? 1.0 / sqrt(nn) : 0.0;
+    uint32_t q[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        double v = u[c] * inv;
+        v = (v == v) ? fmin(1.0, fmax(0.0, v)) : 0.0;   // negative components fold to 0: order only
+        q[c] = static_cast<uint32_t>(v * static_cast<double>((1u << bits) - 1) + 0.5);
+    }
+    uint64_t key = 0;
+    for (int bit = bits - 1; bit >= 0; --bit)
+#pragma unroll
+        for (int c = 0; c < C; ++c) key = (key << 1) | ((q[c] >> bit) & 1u);
+    keys[j] = key;
+    vals[j] = j;
+}
+
+template <int M>
+__device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, int nm, const float* __restrict__ a,
+                                                  const float* __restrict__ wp, int k,
+                                                  const int32_t* __restrict__ perm,
+                                                  const double* __restrict__ cones,
+                                                  const int32_t* __restrict__ meta, float* __restrict__ dist,
+                                                  int32_t* __restrict__ ref, int i, uint16_t* __restrict__ list)
+{
+    constexpr int T = kConeMaxBlocks / 64;   // blocks per lane: lane, lane + 64, ...
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int nbk = (k + 63) / 64;
+    float x[M];
+#pragma unroll
+    for (int c = 0; c < M; ++c) x[c] = fn[static_cast<size_t>(c) * nm + i] * a[c];
+    double xx = 0.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) xx += static_cast<double>(x[c]) * static_cast<double>(x[c]);
+    const double xn = sqrt(xx);
+    const bool finite_x = xx < INFINITY;                    // false for an inf or nan coordinate
+    const double inv_xn = (xn > 0.0 && finite_x) ? 1.0 / xn : 0.0;
+    float cth[T];                                           // cos(theta_c) of blocks lane, lane + 64, ...
+    double best_cos = -3.0, w2max = 0.0;
+    int best_b = 0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int b = lane + 64 * t;
+        cth[t] = 0.0f;
+        if (b < nbk) {
+            const double* cb = cones + static_cast<size_t>(b) * (M + 3);
+            double xc = 0.0;
+#pragma unroll
+            for (int c = 0; c < M; ++c) xc += static_cast<double>(x[c]) * cb[c];
+            const double ct = xc * inv_xn;
+            cth[t] = static_cast<float>(ct);
+            w2max = fmax(w2max, cb[M + 2]);
+            if (ct > best_cos) { best_cos = ct; best_b = b; }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double oc = __shfl_xor(best_cos, o);
+        const int ob = __shfl_xor(best_b, o);
+        if (oc > best_cos || (oc == best_cos && ob < best_b)) { best_cos = oc; best_b = ob; }
+        w2max = fmax(w2max, __shfl_xor(w2max, o));
+    }
+    const int first = wave_uniform(best_b);
+    float bd = INFINITY;
+    int bj = -1;
+    // the kernel's arithmetic for one direction (original index j)
+    auto consider = [&](const float (&wj)[M], int j) {
+        float dot = 0.0f;
+#pragma unroll
+        for (int c = 0; c < M; ++c) dot = fmaf(x[c], wj[c], dot);
+        float d2 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+            const float r = fmaf(-dot, wj[c], x[c]);
+            d2 = fmaf(r, r, d2);
+        }
+        if (d2 < bd || (d2 == bd && d2 < INFINITY && j < bj)) { bd = d2; bj = j; }
+    };
+    {
+        const int q = first * 64 + lane;
+        if (q < k) {
+            float wj[M];
+#pragma unroll
+            for (int c = 0; c < M; ++c) wj[c] = wp[static_cast<size_t>(c) * k + q];
+            consider(wj, perm[q]);
+        }
+    }
+    float cur = bd;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cur = fminf(cur, __shfl_xor(cur, o));
+    // the skip threshold from the nearest block's minimum
+    bool prune = finite_x && xx > 0.0 && cur < INFINITY && w2max < INFINITY;
+    double S = 1.0, C = 0.0;
+    if (prune) {
+        // below 2^-80 the float d2 may carry subnormal (absolute) rounding
+        const double floor_cur = fmax(static_cast<double>(cur), 0x1p-80);
+        const double s0 = sqrt(floor_cur / ((1.0 - 1e-6) * xx)) * (1.0 + 1e-12);
+        S = s0 + 1e-6 * w2max * (1.0 + 1e-12);
+        prune = S < 1.0;
+        if (prune) C = sqrt(1.0 - S * S) * (1.0 - 1e-12);
+    }
+    // candidates: the blocks that fail the skip test, compacted in ascending
+    // order into the wave's list
+    int nc = 0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        if (64 * t >= nbk) break;
+        const int b = lane + 64 * t;
+        bool skip = false;
+        if (prune && b < nbk) {
+            const double* cb = cones + static_cast<size_t>(b) * (M + 3);
+            const double ca = cb[M], sa = cb[M + 1];
+            skip = ca >= 0.0 && static_cast<double>(fabsf(cth[t])) < ca * C - sa * S - 1e-6;
+        }
+        const bool cand = b < nbk && b != first && !skip;
+        const uint64_t mk = ballot(cand);
+        if (cand)
+            list[nc + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                          static_cast<uint32_t>(mk >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u)))] =
+                static_cast<uint16_t>(b);
+        nc += __builtin_popcountll(mk);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // evaluate them four at a time, their loads in flight together
+    constexpr int kBatch = 4;
+    for (int base = 0; base < nc; base += kBatch) {
+        float wv[kBatch][M];
+        int jv[kBatch];
+#pragma unroll
+        for (int v = 0; v < kBatch; ++v) {
+            const int q = (base + v < nc) ? static_cast<int>(list[base + v]) * 64 + lane : k;
+            const bool ok = q < k;
+            jv[v] = ok ? perm[q] : -1;
+#pragma unroll
+            for (int c = 0; c < M; ++c) wv[v][c] = ok ? wp[static_cast<size_t>(c) * k + q] : 0.0f;
+        }
+#pragma unroll
+        for (int v = 0; v < kBatch; ++v)
+            if (jv[v] >= 0) consider(wv[v], jv[v]);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float od = __shfl_xor(bd, o);
+        const int oj = __shfl_xor(bj, o);
+        if (od < bd || (od == bd && od < INFINITY && oj < bj)) { bd = od; bj = oj; }
+    }
+    if (lane == 0) {
+        dist[i] = (bd != INFINITY) ? sqrtf(bd) : INFINITY;
+        ref[i] = bj;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void associate_pruned(const float* __restrict__ fn, int nm,
+                                                        const float* __restrict__ a, const float* __restrict__ wp,
+                                                        int k, const int32_t* __restrict__ perm,
+                                                        const double* __restrict__ cones,
+                                                        const int32_t* __restrict__ meta, float* __restrict__ dist,
+                                                        int32_t* __restrict__ ref)
+{
+    __shared__ __align__(8) uint16_t s_list[4][kConeMaxBlocks];   // candidate blocks per wave
+    if (meta[kNRem] <= 0) return;
+    const int n_nds = meta[kNPrior] + meta[kNLast];
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int i = static_cast<int>(blockIdx.x) * 4 + wave;
+    if (i >= n_nds) return;
+    assoc_pruned_body<M>(fn, nm, a, wp, k, perm, cones, meta, dist, ref, i, s_list[wave]);
+}
+
 __global__ void associate_final(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
                                 int nchunks, int nm, const int32_t* __restrict__ meta,
                                 float* __restrict__ dist, int32_t* __restrict__ ref)
@@ -557,6 +855,46 @@ __global__ void associate_final(const float* __restrict__ part_d2, const int32_t
     assoc_final_body(part_d2, part_idx, nchunks, nm, meta, dist, ref,
                      static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
 }
+
+// Buffers of the cone preparation: direction keys and permutation, the sort
+// scratch and the cones.
+struct ConeBufs {
+    uint64_t* kin; uint64_t* kout; int32_t* vin; int32_t* perm; void* tmp; size_t tmp_bytes; double* cones;
+    float* wp;
+};
+
+__host__ __forceinline__ int morton_key_bits(int m)
+{
+    const int c = morton_coords(m);
+    const int bits = (kMortonBits / c) < 16 ? (kMortonBits / c) : 16;
+    return bits * c;
+}
+
+ConeBufs carve_cones(Carve& c, int m, int k)
+{
+    ConeBufs B{};
+    B.kin = c.take<uint64_t>(k);
+    B.kout = c.take<uint64_t>(k);
+    B.vin = c.take<int32_t>(k);
+    B.perm = c.take<int32_t>(k);
+    B.tmp_bytes = sort_pairs_temp_bytes(k, morton_key_bits(m));
+    B.tmp = c.take_bytes(B.tmp_bytes);
+    B.cones = c.take<double>(static_cast<size_t>(ceil_div(k, 64)) * (m + 3));
+    B.wp = c.take<float>(static_cast<size_t>(m) * k);
+    return B;
+}
+
+int cones_prepare(const float* w, int k, int m, const ConeBufs& B, hipStream_t st)
+{
+    const dim3 b256(256);
+    MOEA_LAUNCH_M(m, assoc_dir_keys, dim3(ceil_div(k, 256)), b256, 0, st, w, k, B.kin, B.vin);
+    MOEA_HIP(hipGetLastError());
+    MOEA_CHECK(sort_pairs(B.tmp, B.tmp_bytes, B.kin, B.kout, B.vin, B.perm, k, st, morton_key_bits(m)));
+    MOEA_LAUNCH_M(m, assoc_cones, dim3(ceil_div(ceil_div(k, 64), 4)), b256, 0, st, w, k, B.perm, B.wp, B.cones);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
 
 // rho_prior histogram (prior_histogram_kernel, niching.cu:21-33).  Directions
 // with no association (ref -1, from non-finite normalized objectives) count in
@@ -803,7 +1141,7 @@ __global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restric
 struct Layout {
     float* fm; float* cvm; int32_t* fronts; int32_t* meta; int32_t* gidx; float* fn;
     float* asf_val; int32_t* asf_idx; float* row_max; float* a;
-    float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho;
+    float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho; ConeBufs cones;
     uint64_t* k0; uint64_t* k1; int32_t* v0; int32_t* v1; void* sort_tmp; size_t sort_bytes;
     uint8_t* flag; int32_t* chosen; int32_t* counts; void* nds_ws; size_t nds_bytes;
 };
@@ -829,6 +1167,7 @@ Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
     L.dist = c.take<float>(nm);
     L.ref = c.take<int32_t>(nm);
     L.rho = c.take<int32_t>(k);
+    L.cones = carve_cones(c, m, k);
     L.k0 = c.take<uint64_t>(nm);
     L.k1 = c.take<uint64_t>(nm);
     L.v0 = c.take<int32_t>(nm);
@@ -851,6 +1190,74 @@ size_t nsga3_workspace_bytes(int n, int m, int k)
     Carve c(nullptr, 0);
     carve(nullptr, 0, n, m, k, c);
     return c.used;
+}
+
+namespace {
+
+__global__ void assoc_meta(int32_t* __restrict__ meta, int n)
+{
+    if (threadIdx.x == 0) {
+        meta[kMaxFront] = 0;
+        meta[kNPrior] = n;
+        meta[kNLast] = 0;
+        meta[kNRem] = 1;
+    }
+}
+
+struct AssocLayout {
+    int32_t* meta; ConeBufs cones; float* part_d2; int32_t* part_idx;
+};
+
+AssocLayout carve_assoc(Carve& c, int n, int m, int k)
+{
+    AssocLayout L{};
+    L.meta = c.take<int32_t>(kMetaLen);
+    L.cones = carve_cones(c, m, k);
+    L.part_d2 = c.take<float>(static_cast<size_t>(ceil_div(k, kAssocChunk)) * n);
+    L.part_idx = c.take<int32_t>(static_cast<size_t>(ceil_div(k, kAssocChunk)) * n);
+    return L;
+}
+
+}  // namespace
+
+// The brute-force partials are strided by the leading dimension: size by ld.
+size_t associate_workspace_bytes(int ld, int m, int k)
+{
+    if (ld <= 0 || m <= 0 || m > kMaxM || k <= 0) return 0;
+    Carve c(nullptr, 0);
+    carve_assoc(c, ld, m, k);
+    return c.used;
+}
+
+int associate(const float* fn, int ld, int n, int m, const float* a, const float* w, int k, float* dist,
+              int32_t* ref, int method, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    if (!fn || !a || !w || !dist || !ref || n <= 0 || m <= 0 || m > kMaxM || k <= 0 || ld < n ||
+        method < 0 || method > 2)
+        return fail(kInvalidArgument, "associate: invalid arguments");
+    const size_t need = associate_workspace_bytes(ld, m, k);
+    if (!ws || ws_bytes < need) return fail(kWorkspaceTooSmall, "associate: workspace too small");
+    Carve c(ws, ws_bytes);
+    const AssocLayout L = carve_assoc(c, ld, m, k);
+    const int nbk = ceil_div(k, 64);
+    const bool prunable = nbk <= kConeMaxBlocks;
+    if (method == 2 && !prunable) return fail(kUnsupported, "associate: too many directions for the pruned path");
+    const bool pruned = (method == 2) || (method == 0 && prunable && k >= kConeMinK);
+    hipLaunchKernelGGL(assoc_meta, dim3(1), dim3(64), 0, st, L.meta, n);
+    const dim3 b256(256);
+    if (pruned) {
+        MOEA_CHECK(cones_prepare(w, k, m, L.cones, st));
+        MOEA_LAUNCH_M(m, associate_pruned, dim3(ceil_div(n, 4)), b256, 0, st, fn, ld, a, L.cones.wp, k,
+                      L.cones.perm, L.cones.cones, L.meta, dist, ref);
+    } else {
+        const int nchunks = ceil_div(k, kAssocChunk);
+        MOEA_LAUNCH_M(m, associate_partial, dim3(ceil_div(n, 256), nchunks), b256, 0, st, fn, ld, a, w, k, L.meta,
+                      L.part_d2, L.part_idx);
+        hipLaunchKernelGGL(associate_final, dim3(ceil_div(n, 256)), b256, 0, st, L.part_d2, L.part_idx, nchunks, ld,
+                           L.meta, dist, ref);
+    }
+    MOEA_HIP(hipGetLastError());
+    return kOk;
 }
 
 int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream)
@@ -900,10 +1307,16 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        L.asf_idx, L.row_max, L.meta, L.a);
     stage_end(kStSelectPrep, st);
     stage_begin(kStAssociation, st);
-    MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
-                  L.meta, L.part_d2, L.part_idx);
-    hipLaunchKernelGGL(associate_final, dim3(nb), dim3(256), 0, st, L.part_d2, L.part_idx, nchunks,
-                       nm, L.meta, L.dist, L.ref);
+    if (k >= kConeMinK && ceil_div(k, 64) <= kConeMaxBlocks) {
+        if (!A.directions_ready) MOEA_CHECK(cones_prepare(A.directions, k, m, L.cones, st));
+        MOEA_LAUNCH_M(m, associate_pruned, dim3(ceil_div(nm, 4)), b256, 0, st, L.fn, nm, L.a, L.cones.wp, k,
+                      L.cones.perm, L.cones.cones, L.meta, L.dist, L.ref);
+    } else {
+        MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
+                      L.meta, L.part_d2, L.part_idx);
+        hipLaunchKernelGGL(associate_final, dim3(nb), dim3(256), 0, st, L.part_d2, L.part_idx, nchunks,
+                           nm, L.meta, L.dist, L.ref);
+    }
     stage_end(kStAssociation, st);
     stage_begin(kStNiching, st);
     hipLaunchKernelGGL(rho_clear, dim3(ceil_div(k, 256)), dim3(256), 0, st, L.rho, k);

@@ -36,6 +36,7 @@ class NSGA3SelectArgs(ctypes.Structure):
         ("parents", PopulationC), ("offspring", PopulationC), ("next", PopulationC),
         ("next_fronts", c_void_p), ("directions", c_void_p), ("ideal", c_void_p),
         ("draws", POINTER(Draws)), ("seed", c_uint64), ("generation", c_int),
+        ("directions_ready", c_int),
         ("stats", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
         ("stream", c_void_p),
     ]
@@ -87,6 +88,7 @@ class RunArgs(ctypes.Structure):
         ("sbx_copy_prob", c_float), ("pm_eta", c_float * 2), ("pm_prob", c_float),
         ("rvea_alpha", c_float), ("rvea_frequency", c_float), ("seed", c_uint64),
         ("initial_x_given", c_int), ("problem", ProblemDesc), ("generation", c_int),
+        ("directions_ready", c_int),
         ("workspace", c_void_p), ("workspace_bytes", c_size_t), ("stream", c_void_p),
     ]
 
@@ -116,6 +118,9 @@ SIGNATURES = {
     "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "moea_nsga3_select": (c_int, [POINTER(NSGA3SelectArgs)]),
     "moea_ideal_point": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "moea_associate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "moea_associate": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                               c_int, c_void_p, c_size_t, c_void_p]),
     "moea_reproduce": (c_int, [POINTER(ReproduceArgs)]),
     "moea_annealed": (c_float, [c_float, c_float, c_int, c_int]),
     "moea_init_population": (c_int, [c_void_p, c_int, c_int, c_void_p, c_uint64, c_void_p]),

@@ -129,6 +129,23 @@ def ideal_point(f: torch.Tensor) -> torch.Tensor:
     return z
 
 
+def associate(fn: torch.Tensor, a: torch.Tensor, directions: torch.Tensor, method: int = 0):
+    """Association of normalized objectives fn (m, n) scaled by intercepts a (m,)
+    with unit directions (m, k): (dist (n,), ref (n,)).  method 0 automatic,
+    1 every direction, 2 cone pruning (identical results)."""
+    _dev(fn, _F32, "fn")
+    m, n = fn.shape
+    _dev(a, _F32, "a", (m,))
+    _dev(directions, _F32, "directions")
+    k = directions.shape[1]
+    dist = torch.empty(n, dtype=_F32, device=fn.device)
+    ref = torch.empty(n, dtype=_I32, device=fn.device)
+    ws = workspace(lib.moea_associate_workspace_bytes(n, m, k), fn.device)
+    check(lib.moea_associate(fn.data_ptr(), n, n, m, a.data_ptr(), directions.data_ptr(), k, dist.data_ptr(),
+                             ref.data_ptr(), int(method), ws.data_ptr(), ws.numel(), _stream(fn.device)))
+    return dist, ref
+
+
 def nsga3_select(parents: Population, offspring: Population, directions: torch.Tensor,
                  ideal: torch.Tensor, draws: Optional[dict] = None, seed: int = 0,
                  generation: int = 0):

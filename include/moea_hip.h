@@ -130,6 +130,10 @@ typedef struct moea_nsga3_select_args {
     const moea_draws* draws;    /* NULL: all in-kernel */
     uint64_t seed;
     int generation;
+    int directions_ready;       /* nonzero: an earlier call on this workspace with these
+                                   same directions left their cone tables in it (the
+                                   workspace not used for anything else since); 0 builds
+                                   them */
     int32_t* stats;             /* optional (8,) device: max_front, n_prior, n_last,
                                    fallback flag, n_rem */
     void* workspace;
@@ -146,6 +150,21 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k);
  * hyperplane solve in FP64 and its row-max fallback, association, niching
  * and the survivor gather in ascending merged-index order. */
 int moea_nsga3_select(const moea_nsga3_select_args* args);
+
+/* Perpendicular-distance association of n normalized individuals: for
+ * individual i, x = fn[:, i] * a (a: (m,) intercepts) and ref[i] = the
+ * direction with the smallest d2 (dot by an fmaf chain, r = fmaf(-dot, w, x),
+ * d2 by an fmaf chain; smallest index on ties; non-finite d2 never chosen,
+ * ref -1 and dist +inf when none is finite), dist[i] = sqrtf(d2).  Replaces
+ * execute_association (selection/nsga3/kernels/association.cu:586-608).
+ * method 0: automatic; 1: every direction; 2: cone pruning (directions in
+ * blocks of 64 with a bounding cone each; blocks that cannot hold the minimum
+ * are skipped), same result as 1 bit for bit.  fn: (m, ld); workspace from
+ * moea_associate_workspace_bytes(ld, m, k). */
+size_t moea_associate_workspace_bytes(int ld, int m, int k);
+int moea_associate(const float* fn, int ld, int n, int m, const float* a, const float* directions, int k,
+                   float* dist, int32_t* ref, int method, void* workspace, size_t workspace_bytes,
+                   void* stream);
 
 /* Running ideal point over one population (ideal_point.cu:97-133); used by
  * NSGA3EnvironmentSelector::prepare on the initial parents. */
@@ -311,6 +330,8 @@ typedef struct moea_run_args {
     int initial_x_given;       /* 1: pops[current].x already holds the initial population */
     moea_problem_desc problem;
     int generation;
+    int directions_ready;      /* NSGA-III: cone tables of `directions` in the workspace;
+                                  cleared by moea_run_init, set after the first selection */
     void* workspace;
     size_t workspace_bytes;
     void* stream;

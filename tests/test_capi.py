@@ -38,13 +38,45 @@ def test_python_binding_covers_header():
 
 def test_host_side_calls():
     import cuda_moea._lib as L
-    assert L.lib.moea_abi_version() == 1
+    assert L.lib.moea_abi_version() == 2
     assert L.lib.moea_das_dennis(92, 3, 0, 0, None, None) == 91
     assert L.lib.moea_das_dennis(32768, 3, 0, 0, None, None) == 32640
     assert abs(L.lib.moea_annealed(30.0, 10.0, 50, 100) - 20.0) < 1e-6
     assert abs(L.lib.moea_rvea_penalty(5, 50, 100, 2.0) - 1.25) < 1e-6
     assert L.lib.moea_das_dennis(92, 11, 0, 0, None, None) < 0
     assert b"objective" in L.lib.moea_last_error()
+
+
+STRUCTS = {"Draws": "moea_draws", "PopulationC": "moea_population", "NSGA3SelectArgs": "moea_nsga3_select_args",
+           "ReproduceArgs": "moea_reproduce_args", "RVEASelectArgs": "moea_rvea_select_args",
+           "ProblemDesc": "moea_problem_desc", "RunArgs": "moea_run_args", "BatchProblem": "moea_batch_problem",
+           "ServeConfig": "moea_serve_config"}
+
+
+def test_ctypes_structs_match_header_layout(tmp_path):
+    """Every ctypes mirror has the header struct's size and field offsets
+    (gcc on the header itself)."""
+    import shutil
+    import pytest
+    import cuda_moea._lib as L
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "moea_hip.h"', "int main(void) {"]
+    expect = []
+    for py, c in STRUCTS.items():
+        cls = getattr(L, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        expect.append(ctypes.sizeof(cls))
+        for f in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {f[0]}));')
+            expect.append(getattr(cls, f[0]).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(HEADER.parent), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == expect
 
 
 def test_das_dennis_matches_oracle():
