@@ -149,6 +149,8 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k)
     return nsga3_workspace_bytes(n, m, k);
 }
 
+int moea_set_niching(int mode, int lds_cap) { return set_niching(mode, lds_cap); }
+
 int moea_nsga3_select(const moea_nsga3_select_args* args)
 {
     if (!args) return fail(kInvalidArgument, "moea_nsga3_select: null args");

@@ -19,6 +19,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 // nsga3.hip
 size_t nsga3_workspace_bytes(int n, int m, int k);
 int nsga3_select(const moea_nsga3_select_args& a);
+int set_niching(int mode, int lds_cap);
 int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream);
 size_t associate_workspace_bytes(int ld, int m, int k);
 int associate(const float* fn, int ld, int n, int m, const float* a, const float* w, int k, float* dist,

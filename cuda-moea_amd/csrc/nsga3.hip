@@ -1043,6 +1043,449 @@ __global__ void mark_selected(const int32_t* __restrict__ val3s, const int32_t* 
     flag[gidx[meta[kNPrior] + val3s[t]]] = 1;
 }
 
+// ------------------------------------------------------ one-block niching --
+// The three niching sorts and the survivor marks in one workgroup.  Only the
+// last front's n_last members are sorted (the device-wide path sorts all 2N
+// slots, the others as sentinels after them, so the members' order is the
+// same), in LDS when they fit, else in the workspace's key/value buffers.
+// Each sort is stable LSD radix with 8-bit digits.  Wave w owns the w-th of
+// 16 contiguous chunks: it counts its chunk's digits, one prefix over
+// (digit, wave) gives every wave its first position per digit, and the wave
+// then walks its chunk 64 members at a time, each member going to its digit's
+// next position plus its rank among the step's lanes with that digit (eight
+// ballots).  Four barriers a pass whatever n_last is; a pass whose digit is one
+// value for every member moves nothing and is skipped.  Keys and draws are
+// those of niche_keys1..3.
+constexpr int kNicheThreads = 1024;
+constexpr int kNicheWaves = kNicheThreads / 64;
+constexpr int kNicheLdsCap = 5888;   // members sorted in LDS
+
+struct NicheShared {
+    uint64_t key[2][kNicheLdsCap];
+    int32_t val[2][kNicheLdsCap];
+    int32_t row[kNicheWaves][256];   // per wave: digit counts, then next positions
+    int32_t wsum[kNicheWaves];
+    int32_t skip;
+};
+
+struct NicheLdsBufs {
+    NicheShared* S;
+    __device__ __forceinline__ uint64_t* key(int b) const { return b ? S->key[1] : S->key[0]; }
+    __device__ __forceinline__ int32_t* val(int b) const { return b ? S->val[1] : S->val[0]; }
+};
+struct NicheGlobalBufs {
+    uint64_t* k0;
+    uint64_t* k1;
+    int32_t* v0;
+    int32_t* v1;
+    __device__ __forceinline__ uint64_t* key(int b) const { return b ? k1 : k0; }
+    __device__ __forceinline__ int32_t* val(int b) const { return b ? v1 : v0; }
+};
+
+// Lanes of the wave whose digit equals this lane's (valid lanes only).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t dg, bool valid)
+{
+    uint64_t peer = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = ((dg >> b) & 1u) != 0;
+        const uint64_t m = ballot(bit);
+        peer &= bit ? m : ~m;
+    }
+    return peer;
+}
+
+// Stable sort of the n pairs in buffer src by key bits [0, nbits); returns the
+// buffer that holds the result.  pass: a counter unique to each pass of the
+// kernel (tags the skip flag).
+template <class Bufs>
+__device__ int niche_sort(const Bufs& B, int src, int n, int nbits, NicheShared& S, int& pass)
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63, w = tid >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int chunk = (n + kNicheWaves - 1) / kNicheWaves;
+    const int lo = min(n, w * chunk), hi = min(n, lo + chunk);
+    int32_t* row = S.row[w];
+    for (int shift = 0; shift < nbits; shift += 8, ++pass) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) row[lane * 4 + j] = 0;
+        for (int i0 = lo; i0 < hi; i0 += 64) {          // the chunk's digit counts
+            const int i = i0 + lane;
+            const bool valid = i < hi;
+            const uint32_t dg = valid ? static_cast<uint32_t>(B.key(src)[i] >> shift) & 255u : 0u;
+            const uint64_t peer = digit_peers(dg, valid);
+            if (valid && (peer & below) == 0) row[dg] += __popcll(peer);
+        }
+        __syncthreads();
+        // thread t: digit d = t / 4 and waves 4q .. 4q + 3 (q = t % 4); wave w holds digits 16w .. 16w + 15
+        const int d = tid >> 2, q = tid & 3;
+        int c[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[j] = S.row[4 * q + j][d];
+        const int sum = c[0] + c[1] + c[2] + c[3];
+        int incl = sum;                                 // over the 4 threads of the digit
+        {
+            const int u1 = __shfl_up(incl, 1, 4);
+            if (q >= 1) incl += u1;
+            const int u2 = __shfl_up(incl, 2, 4);
+            if (q >= 2) incl += u2;
+        }
+        const int before_q = incl - sum;
+        const int total = __shfl(incl, 3, 4);           // the digit's count
+        int dscan = total;                              // over the wave's 16 digits (lanes 4j .. 4j + 3)
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1) {
+            const int u = __shfl_up(dscan, o);
+            if (lane >= o) dscan += u;
+        }
+        if (lane == 63) S.wsum[w] = dscan;
+        if (total == n && q == 0) S.skip = pass;        // every member has digit d
+        __syncthreads();
+        int base = dscan - total;                       // digits of earlier waves
+        for (int v = 0; v < w; ++v) base += S.wsum[v];
+        int at = base + before_q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            S.row[4 * q + j][d] = at;
+            at += c[j];
+        }
+        __syncthreads();
+        if (S.skip == pass) continue;
+        const int dst = src ^ 1;
+        for (int i0 = lo; i0 < hi; i0 += 64) {
+            const int i = i0 + lane;
+            const bool valid = i < hi;
+            uint64_t kv = 0;
+            int32_t vv = 0;
+            uint32_t dg = 0;
+            if (valid) {
+                kv = B.key(src)[i];
+                vv = B.val(src)[i];
+                dg = static_cast<uint32_t>(kv >> shift) & 255u;
+            }
+            const uint64_t peer = digit_peers(dg, valid);
+            const int rank = __popcll(peer & below);
+            if (valid) {
+                const int pos = row[dg] + rank;
+                B.key(dst)[pos] = kv;
+                B.val(dst)[pos] = vv;
+            }
+            if (valid && rank == 0) row[dg] += __popcll(peer);
+        }
+        __syncthreads();
+        src = dst;
+    }
+    return src;
+}
+
+// First position of the run of equal high words containing t (keys sorted).
+__device__ __forceinline__ int run_head(const uint64_t* key, int t)
+{
+    const uint32_t hi = static_cast<uint32_t>(key[t] >> 32);
+    int lo = 0, up = t;
+    while (lo < up) {
+        const int mid = (lo + up) >> 1;
+        if (static_cast<uint32_t>(key[mid] >> 32) < hi) lo = mid + 1;
+        else up = mid;
+    }
+    return lo;
+}
+
+struct NicheArgs {
+    const int32_t* ref;
+    const float* dist;
+    const int32_t* rho;
+    int k, bits_ref, bits_rank;
+    const uint32_t* inj_a;
+    const uint32_t* inj_b;
+    uint64_t seed;
+    uint32_t gen;
+    const int32_t* gidx;
+    uint8_t* flag;
+    int bucket;   // 0: always the radix sorts
+};
+
+template <class Bufs>
+__device__ void niche_body(const Bufs& B, const NicheArgs& A, NicheShared& S, int n_prior, int n_last, int n_rem)
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const int k = A.k;
+    int pass = 0;
+    // phase 2: (direction, distance), value = last-front index
+    for (int t = tid; t < n_last; t += kNicheThreads) {
+        const uint32_t r = ref_word(A.ref[n_prior + t], k);
+        B.key(0)[t] = (static_cast<uint64_t>(r) << 32) | __float_as_uint(A.dist[n_prior + t]);
+        B.val(0)[t] = t;
+    }
+    __syncthreads();
+    int src = niche_sort(B, 0, n_last, A.bits_ref, S, pass);
+    // phase 4: (direction, 0 for the nearest member of an empty niche, else draw_a | 1)
+    for (int t = tid; t < n_last; t += kNicheThreads) {
+        const uint32_t r = static_cast<uint32_t>(B.key(src)[t] >> 32);
+        const int local = t - run_head(B.key(src), t);
+        const int ri = static_cast<int>(r);
+        const int rh = (ri >= 0 && ri < k) ? A.rho[ri] : 0;
+        const uint32_t low = (rh == 0 && local == 0) ? 0u : (niche_draw(A.inj_a, 0, t, A.seed, A.gen) | 1u);
+        B.key(src ^ 1)[t] = (static_cast<uint64_t>(r) << 32) | low;
+        B.val(src ^ 1)[t] = B.val(src)[t];
+    }
+    __syncthreads();
+    src = niche_sort(B, src ^ 1, n_last, A.bits_ref, S, pass);
+    // phase 5: (rho + rank in the niche, draw_b)
+    for (int t = tid; t < n_last; t += kNicheThreads) {
+        const int ri = static_cast<int>(static_cast<uint32_t>(B.key(src)[t] >> 32));
+        const int rank = t - run_head(B.key(src), t);
+        const int rh = (ri >= 0 && ri < k) ? A.rho[ri] : 0;
+        B.key(src ^ 1)[t] = (static_cast<uint64_t>(static_cast<uint32_t>(rh + rank)) << 32) |
+                            niche_draw(A.inj_b, 1, t, A.seed, A.gen);
+        B.val(src ^ 1)[t] = B.val(src)[t];
+    }
+    __syncthreads();
+    src = niche_sort(B, src ^ 1, n_last, A.bits_rank, S, pass);
+    const int take = n_rem < n_last ? n_rem : n_last;
+    for (int t = tid; t < take; t += kNicheThreads) A.flag[A.gidx[n_prior + B.val(src)[t]]] = 1;
+}
+
+// Direction buckets: the usual case.  Sort 1 and sort 2 keep every niche's
+// members together in direction order, so a member's position is its niche's
+// start plus its rank inside the niche; niches are small, so the ranks are
+// counted directly: by (distance bits, last-front index) for sort 1, by
+// (phase-4 low word, sort-1 position) for sort 2.  Sort 3 only decides which
+// n_rem members come first: its keys (rho + rank, draw_b, sort-2 position) are
+// distinct, so an 8-round radix select of the n_rem-th smallest finds the
+// chosen set exactly.  A niche of more than kBucketMaxRun members (counting
+// would be quadratic) or more than 65535 members in all (16-bit positions)
+// takes the radix sorts instead.
+constexpr int kNicheLdsBytes = 162816;
+constexpr int kBucketMaxRun = 128;
+constexpr int kBucketSlotBytes = 18;   // t, r, p1, lev, p2: 2 B each; a0, db: 4 B each
+
+struct BucketSlots {
+    uint16_t* t;     // last-front index
+    uint16_t* r;     // direction word (k: unassociated)
+    uint32_t* a0;    // distance bits, then the phase-4 low word
+    uint16_t* p1;    // sort-1 position
+    uint16_t* lev;   // rho + rank in the niche
+    uint32_t* db;    // draw_b
+    uint16_t* p2;    // sort-2 position
+};
+
+// LDS: the seven arrays back to back, `cap` members each.
+__device__ __forceinline__ BucketSlots bucket_slots(unsigned char* base, int cap)
+{
+    BucketSlots v;
+    v.a0 = reinterpret_cast<uint32_t*>(base);
+    v.db = v.a0 + cap;
+    v.t = reinterpret_cast<uint16_t*>(v.db + cap);
+    v.r = v.t + cap;
+    v.p1 = v.r + cap;
+    v.lev = v.p1 + cap;
+    v.p2 = v.lev + cap;
+    return v;
+}
+
+// Workspace (members past the LDS capacity): a0 and db in k0, the 16-bit
+// arrays in k1 and v0, each of 2n slots (8, 8 and 4 bytes a slot).
+__device__ __forceinline__ BucketSlots bucket_slots_global(uint64_t* k0, uint64_t* k1, int32_t* v0, int nm)
+{
+    BucketSlots v;
+    v.a0 = reinterpret_cast<uint32_t*>(k0);
+    v.db = v.a0 + nm;
+    v.t = reinterpret_cast<uint16_t*>(k1);
+    v.r = v.t + nm;
+    v.p1 = v.r + nm;
+    v.lev = v.p1 + nm;
+    v.p2 = reinterpret_cast<uint16_t*>(v0);
+    return v;
+}
+
+__device__ __forceinline__ int half_get(const uint32_t* table, int r)
+{
+    const uint32_t w = table[r >> 1];
+    return static_cast<int>((r & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+// Slots and ranks, then the selection; table holds each niche's start (16-bit).
+__device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* table, int32_t* misc,
+                                              const BucketSlots& V, int n_prior, int n_last, int n_rem)
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const int k = A.k;
+    // place each member in its niche (order inside a niche does not matter:
+    // the ranks below break ties explicitly); afterwards table holds the ends
+    for (int t = tid; t < n_last; t += kNicheThreads) {
+        const int r = static_cast<int>(ref_word(A.ref[n_prior + t], k));
+        const uint32_t old = atomicAdd(&table[r >> 1], (r & 1) ? 0x10000u : 1u);
+        const int slot = static_cast<int>((r & 1) ? (old >> 16) : (old & 0xFFFFu));
+        V.t[slot] = static_cast<uint16_t>(t);
+        V.r[slot] = static_cast<uint16_t>(r);
+        V.a0[slot] = __float_as_uint(A.dist[n_prior + t]);
+    }
+    __syncthreads();
+    // sort 1: rank by (distance bits, t) inside the niche
+    for (int s = tid; s < n_last; s += kNicheThreads) {
+        const int r = V.r[s];
+        const int st = r ? half_get(table, r - 1) : 0, en = half_get(table, r);
+        const uint32_t d = V.a0[s];
+        const int t = V.t[s];
+        int rank = 0;
+        for (int u = st; u < en; ++u) {
+            const uint32_t du = V.a0[u];
+            rank += (du < d || (du == d && static_cast<int>(V.t[u]) < t)) ? 1 : 0;
+        }
+        V.p1[s] = static_cast<uint16_t>(st + rank);
+    }
+    __syncthreads();
+    // phase 4 low word
+    for (int s = tid; s < n_last; s += kNicheThreads) {
+        const int r = V.r[s];
+        const int st = r ? half_get(table, r - 1) : 0;
+        const int p1 = V.p1[s];
+        const int rh = (r < k) ? A.rho[r] : 0;
+        V.a0[s] = (rh == 0 && p1 == st) ? 0u : (niche_draw(A.inj_a, 0, p1, A.seed, A.gen) | 1u);
+    }
+    __syncthreads();
+    // sort 2: rank by (low, p1) inside the niche; phase-5 level and draw
+    for (int s = tid; s < n_last; s += kNicheThreads) {
+        const int r = V.r[s];
+        const int st = r ? half_get(table, r - 1) : 0, en = half_get(table, r);
+        const uint32_t lo = V.a0[s];
+        const int p1 = V.p1[s];
+        int rank = 0;
+        for (int u = st; u < en; ++u) {
+            const uint32_t lu = V.a0[u];
+            rank += (lu < lo || (lu == lo && static_cast<int>(V.p1[u]) < p1)) ? 1 : 0;
+        }
+        const int rh = (r < k) ? A.rho[r] : 0;
+        const int p2 = st + rank;
+        V.lev[s] = static_cast<uint16_t>(rh + rank);
+        V.db[s] = niche_draw(A.inj_b, 1, p2, A.seed, A.gen);
+        V.p2[s] = static_cast<uint16_t>(p2);
+    }
+    __syncthreads();
+    // sort 3 as a selection: the n_rem-th smallest of the distinct keys
+    // (lev << 48 | draw_b << 16 | p2), 8 bits a round from the top
+    const int take = n_rem < n_last ? n_rem : n_last;
+    int32_t* hist = misc;              // 256
+    int32_t* pick = misc + 256;        // digit, count below
+    uint64_t prefix = 0, mask = 0;
+    int want = take;                   // rank of the target among the candidates (1-based)
+    for (int sh = 56; sh >= 0; sh -= 8) {
+        if (tid < 256) hist[tid] = 0;
+        __syncthreads();
+        for (int s = tid; s < n_last; s += kNicheThreads) {
+            const uint64_t key = (static_cast<uint64_t>(V.lev[s]) << 48) | (static_cast<uint64_t>(V.db[s]) << 16) |
+                                 static_cast<uint64_t>(V.p2[s]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> sh) & 255u], 1);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            int c[4];
+            int sum = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                c[j] = hist[tid * 4 + j];
+                sum += c[j];
+            }
+            int incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o);
+                if (tid >= o) incl += u;
+            }
+            int before = incl - sum;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (before < want && want <= before + c[j]) {
+                    pick[0] = tid * 4 + j;
+                    pick[1] = before;
+                }
+                before += c[j];
+            }
+        }
+        __syncthreads();
+        prefix |= static_cast<uint64_t>(pick[0]) << sh;
+        mask |= 255ull << sh;
+        want -= pick[1];
+        __syncthreads();
+    }
+    for (int s = tid; s < n_last; s += kNicheThreads) {
+        const uint64_t key = (static_cast<uint64_t>(V.lev[s]) << 48) | (static_cast<uint64_t>(V.db[s]) << 16) |
+                             static_cast<uint64_t>(V.p2[s]);
+        if (key <= prefix) A.flag[A.gidx[n_prior + V.t[s]]] = 1;
+    }
+}
+
+// rho must hold the prior-front counts and flag the prior-front marks.
+__global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, const int32_t* __restrict__ meta, int nm,
+                                                             uint64_t* gk0, uint64_t* gk1, int32_t* gv0,
+                                                             int32_t* gv1, int lds_cap)
+{
+    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
+    __shared__ int s_wave[64];
+    const int n_rem = meta[kNRem];
+    const int n_last = meta[kNLast];
+    if (n_rem <= 0 || n_last <= 0) return;
+    const int n_prior = meta[kNPrior];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int k = A.k;
+    // bucket table: k + 1 niches, 16-bit counts, two a word; then misc, then slots
+    const int words = ((k + 2) / 2 + 3) & ~3;
+    const int misc_ints = 512;
+    const int fixed = 4 * words + 4 * misc_ints;
+    const int cap_b = (fixed < kNicheLdsBytes) ? ((kNicheLdsBytes - fixed) / kBucketSlotBytes) & ~15 : 0;
+    if (A.bucket && cap_b >= 64 && n_last <= 65535 && n_prior <= 65535 - kBucketMaxRun) {
+        uint32_t* table = reinterpret_cast<uint32_t*>(s_raw);
+        int32_t* misc = reinterpret_cast<int32_t*>(s_raw + 4 * words);
+        for (int j = tid; j < words; j += kNicheThreads) table[j] = 0;
+        __syncthreads();
+        for (int t = tid; t < n_last; t += kNicheThreads) {
+            const int r = static_cast<int>(ref_word(A.ref[n_prior + t], k));
+            atomicAdd(&table[r >> 1], (r & 1) ? 0x10000u : 1u);
+        }
+        __syncthreads();
+        // niche starts in place: thread tid takes an even-aligned run of niches
+        const int per = 2 * ((k + 1 + 2 * kNicheThreads - 1) / (2 * kNicheThreads));
+        const int b0 = min(k + 1, tid * per), b1 = min(k + 1, b0 + per);
+        int sum = 0, run_max = 0;
+        for (int r = b0; r < b1; ++r) {
+            const int c = half_get(table, r);
+            sum += c;
+            run_max = max(run_max, c);
+        }
+        int total;
+        int at = block_exclusive_scan(sum, s_wave, total);
+        for (int r = b0; r < b1; r += 2) {
+            const int c0 = half_get(table, r);
+            const int c1 = (r + 1 < b1) ? half_get(table, r + 1) : 0;
+            const uint32_t lo = static_cast<uint32_t>(at), hi = static_cast<uint32_t>(at + c0);
+            table[r >> 1] = lo | (hi << 16);
+            at += c0 + c1;
+        }
+        // largest niche
+        int mx = run_max;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+        if ((tid & 63) == 0) misc[tid >> 6] = mx;
+        __syncthreads();
+        int big = 0;
+#pragma unroll
+        for (int j = 0; j < kNicheWaves; ++j) big = max(big, misc[j]);
+        __syncthreads();
+        if (big <= kBucketMaxRun) {
+            if (n_last <= cap_b && n_last <= lds_cap) bucket_finish(A, table, misc, bucket_slots(s_raw + fixed, cap_b), n_prior, n_last, n_rem);
+            else bucket_finish(A, table, misc, bucket_slots_global(gk0, gk1, gv0, nm), n_prior, n_last, n_rem);
+            return;
+        }
+    }
+    NicheShared& S = *reinterpret_cast<NicheShared*>(s_raw);
+    if (tid == 0) S.skip = -1;
+    __syncthreads();
+    if (n_last <= lds_cap) niche_body(NicheLdsBufs{&S}, A, S, n_prior, n_last, n_rem);
+    else niche_body(NicheGlobalBufs{gk0, gk1, gv0, gv1}, A, S, n_prior, n_last, n_rem);
+}
+
 // Chosen indices in ascending merged order (thrust::sort of the selection,
 // nsga3_population_update.cu:269): the same two passes as front_counts /
 // front_scatter over the survivor flags.
@@ -1268,6 +1711,29 @@ int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t
     return kOk;
 }
 
+namespace {
+// Niching path: 0 automatic (one block), 1 one block, 2 device-wide sorts,
+// 3 one block with its radix sorts only.
+int g_niching_mode = 0;
+int g_niching_lds_cap = kNicheLdsCap;
+
+bool niching_one_block(int nm)
+{
+    (void)nm;
+    return g_niching_mode != 2;
+}
+int niching_buckets() { return g_niching_mode == 3 ? 0 : 1; }
+int niching_lds_cap() { return g_niching_lds_cap; }
+}  // namespace
+
+int set_niching(int mode, int lds_cap)
+{
+    if (mode < 0 || mode > 3 || lds_cap < 0) return fail(kInvalidArgument, "set_niching: invalid mode or capacity");
+    g_niching_mode = mode;
+    g_niching_lds_cap = lds_cap < kNicheLdsCap ? lds_cap : kNicheLdsCap;
+    return kOk;
+}
+
 int nsga3_select(const moea_nsga3_select_args& A)
 {
     const int n = A.n, m = A.m, d = A.d, k = A.k;
@@ -1323,20 +1789,28 @@ int nsga3_select(const moea_nsga3_select_args& A)
     hipLaunchKernelGGL(rho_hist, dim3(nb), dim3(256), 0, st, L.ref, L.meta, k, L.rho);
     const int bits_ref = 32 + (32 - __builtin_clz(static_cast<unsigned>(k + 1)));
     const int bits_rank = 32 + (32 - __builtin_clz(static_cast<unsigned>(nm)));
-    hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
-    MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_ref));
-    hipLaunchKernelGGL(niche_keys2, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
-                       dr ? dr->niche_a : nullptr, A.seed, gen, L.k0);
-    MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v1, L.v0, nm, st, bits_ref));
-    hipLaunchKernelGGL(niche_keys3, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
-                       dr ? dr->niche_b : nullptr, A.seed, gen, L.k0);
-    MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_rank));
-    hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
-    hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
-                       L.flag);
+    if (niching_one_block(nm)) {
+        hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
+        NicheArgs na{L.ref, L.dist, L.rho, k, bits_ref, bits_rank, dr ? dr->niche_a : nullptr,
+                     dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets()};
+        hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, L.k0, L.k1, L.v0,
+                           L.v1, niching_lds_cap());
+    } else {
+        hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
+        MOEA_HIP(hipGetLastError());
+        MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_ref));
+        hipLaunchKernelGGL(niche_keys2, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
+                           dr ? dr->niche_a : nullptr, A.seed, gen, L.k0);
+        MOEA_HIP(hipGetLastError());
+        MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v1, L.v0, nm, st, bits_ref));
+        hipLaunchKernelGGL(niche_keys3, dim3(nb), dim3(256), 0, st, L.k1, L.meta, L.rho, k, nm,
+                           dr ? dr->niche_b : nullptr, A.seed, gen, L.k0);
+        MOEA_HIP(hipGetLastError());
+        MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_rank));
+        hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
+        hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
+                           L.flag);
+    }
     hipLaunchKernelGGL(flag_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, L.counts);
     hipLaunchKernelGGL(flag_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, n, L.counts, nsb,
                        L.chosen, L.meta);

@@ -116,6 +116,7 @@ SIGNATURES = {
                               c_void_p, c_size_t, c_void_p]),
     "moea_das_dennis": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "moea_set_niching": (c_int, [c_int, c_int]),
     "moea_nsga3_select": (c_int, [POINTER(NSGA3SelectArgs)]),
     "moea_ideal_point": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_associate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),

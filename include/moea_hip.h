@@ -151,6 +151,15 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k);
  * and the survivor gather in ascending merged-index order. */
 int moea_nsga3_select(const moea_nsga3_select_args* args);
 
+/* Niching path of moea_nsga3_select, process-wide (set it while no selection
+ * runs): mode 0 automatic, 1 one workgroup over only the last front's
+ * members, grouped by direction (a niche of more than 128 members falls back
+ * to radix sorts), 2 three device-wide sorts of all 2n slots, 3 the one
+ * workgroup's radix sorts only.  The one-workgroup paths keep the members in
+ * LDS up to lds_cap of them (and their own limit), else in the workspace.
+ * Every path selects the same survivors. */
+int moea_set_niching(int mode, int lds_cap);
+
 /* Perpendicular-distance association of n normalized individuals: for
  * individual i, x = fn[:, i] * a (a: (m,) intercepts) and ref[i] = the
  * direction with the smallest d2 (dot by an fmaf chain, r = fmaf(-dot, w, x),
