@@ -231,8 +231,11 @@ __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, i
                                                  const int32_t* __restrict__ meta,
                                                  float* __restrict__ fn, float* __restrict__ asf_val,
                                                  int32_t* __restrict__ asf_idx,
-                                                 float* __restrict__ row_max)
+                                                 float* __restrict__ row_max, int32_t* __restrict__ rho, int k)
 {
+    // the niche counts start at zero (the association kernels add the prior front)
+    for (int j = static_cast<int>(blockIdx.x * 256 + threadIdx.x); j < k; j += static_cast<int>(gridDim.x * 256))
+        rho[j] = 0;
     norm_prep_body<M>(fm, nm, gidx, ideal, meta, fn, asf_val, asf_idx, row_max, static_cast<int>(blockIdx.x));
 }
 
@@ -532,9 +535,12 @@ __global__ __launch_bounds__(256) void associate_partial(const float* __restrict
 }
 
 // Reduce the chunk partials in chunk order (association_phase2_kernel, :397-447).
+// rho (optional): prior-front individuals add one to their direction's count
+// (prior_histogram_kernel, niching.cu:21-33).
 __device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
                                         int nchunks, int nm, const int32_t* __restrict__ meta,
-                                        float* __restrict__ dist, int32_t* __restrict__ ref, int i)
+                                        float* __restrict__ dist, int32_t* __restrict__ ref, int i,
+                                        int32_t* __restrict__ rho = nullptr)
 {
     if (meta[kNRem] <= 0) return;
     const int n_nds = meta[kNPrior] + meta[kNLast];
@@ -548,6 +554,7 @@ __device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_
     }
     dist[i] = (best != INFINITY) ? sqrtf(best) : INFINITY;
     ref[i] = bj;
+    if (rho && bj >= 0 && i < meta[kNPrior]) atomicAdd(&rho[bj], 1);
 }
 
 // ------------------------------------------------------ pruned association --
@@ -702,7 +709,8 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
                                                   const int32_t* __restrict__ perm,
                                                   const double* __restrict__ cones,
                                                   const int32_t* __restrict__ meta, float* __restrict__ dist,
-                                                  int32_t* __restrict__ ref, int i, uint16_t* __restrict__ list)
+                                                  int32_t* __restrict__ ref, int i, uint16_t* __restrict__ list,
+                                                  int32_t* __restrict__ rho)
 {
     constexpr int T = kConeMaxBlocks / 64;   // blocks per lane: lane, lane + 64, ...
     const int lane = static_cast<int>(threadIdx.x & 63);
@@ -828,6 +836,7 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
     if (lane == 0) {
         dist[i] = (bd != INFINITY) ? sqrtf(bd) : INFINITY;
         ref[i] = bj;
+        if (rho && bj >= 0 && i < meta[kNPrior]) atomicAdd(&rho[bj], 1);
     }
 }
 
@@ -837,7 +846,7 @@ __global__ __launch_bounds__(256) void associate_pruned(const float* __restrict_
                                                         int k, const int32_t* __restrict__ perm,
                                                         const double* __restrict__ cones,
                                                         const int32_t* __restrict__ meta, float* __restrict__ dist,
-                                                        int32_t* __restrict__ ref)
+                                                        int32_t* __restrict__ ref, int32_t* __restrict__ rho)
 {
     __shared__ __align__(8) uint16_t s_list[4][kConeMaxBlocks];   // candidate blocks per wave
     if (meta[kNRem] <= 0) return;
@@ -845,15 +854,15 @@ __global__ __launch_bounds__(256) void associate_pruned(const float* __restrict_
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int i = static_cast<int>(blockIdx.x) * 4 + wave;
     if (i >= n_nds) return;
-    assoc_pruned_body<M>(fn, nm, a, wp, k, perm, cones, meta, dist, ref, i, s_list[wave]);
+    assoc_pruned_body<M>(fn, nm, a, wp, k, perm, cones, meta, dist, ref, i, s_list[wave], rho);
 }
 
 __global__ void associate_final(const float* __restrict__ part_d2, const int32_t* __restrict__ part_idx,
                                 int nchunks, int nm, const int32_t* __restrict__ meta,
-                                float* __restrict__ dist, int32_t* __restrict__ ref)
+                                float* __restrict__ dist, int32_t* __restrict__ ref, int32_t* __restrict__ rho)
 {
     assoc_final_body(part_d2, part_idx, nchunks, nm, meta, dist, ref,
-                     static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+                     static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), rho);
 }
 
 // Buffers of the cone preparation: direction keys and permutation, the sort
@@ -896,23 +905,11 @@ int cones_prepare(const float* w, int k, int m, const ConeBufs& B, hipStream_t s
 }
 
 
-// rho_prior histogram (prior_histogram_kernel, niching.cu:21-33).  Directions
-// with no association (ref -1, from non-finite normalized objectives) count in
-// no niche; the reference would index rho[-1] there.
-__global__ void rho_clear(int32_t* __restrict__ rho, int k)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < k) rho[i] = 0;
-}
-__global__ void rho_hist(const int32_t* __restrict__ ref, const int32_t* __restrict__ meta, int k,
-                         int32_t* __restrict__ rho)
-{
-    if (meta[kNRem] <= 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= meta[kNPrior]) return;
-    const int r = ref[i];
-    if (r >= 0 && r < k) atomicAdd(&rho[r], 1);
-}
+// rho_prior (prior_histogram_kernel, niching.cu:21-33) is cleared by norm_prep
+// and filled by the association kernels as they write the prior front's
+// directions.  Individuals with no association (ref -1, from non-finite
+// normalized objectives) count in no niche; the reference would index rho[-1]
+// there.
 
 // Phase-2 keys: ref << 32 | bits(dist), value = last-front local index
 // (build_lf_asrpts_asdist_key_kernel, niching.cu:68-92).  The high word is
@@ -1365,13 +1362,42 @@ __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* tabl
     }
     __syncthreads();
     // sort 3 as a selection: the n_rem-th smallest of the distinct keys
-    // (lev << 48 | draw_b << 16 | p2), 8 bits a round from the top
+    // (lev << 48 | draw_b << 16 | p2), 8 bits a round from the first byte in
+    // which the keys differ, until one candidate is left
     const int take = n_rem < n_last ? n_rem : n_last;
     int32_t* hist = misc;              // 256
-    int32_t* pick = misc + 256;        // digit, count below
-    uint64_t prefix = 0, mask = 0;
+    int32_t* pick = misc + 256;        // digit, count below, its count
+    uint64_t* ext = reinterpret_cast<uint64_t*>(misc + 264);   // per wave min, max
+    uint64_t kmin = ~0ull, kmax = 0;
+    for (int s = tid; s < n_last; s += kNicheThreads) {
+        const uint64_t key = (static_cast<uint64_t>(V.lev[s]) << 48) | (static_cast<uint64_t>(V.db[s]) << 16) |
+                             static_cast<uint64_t>(V.p2[s]);
+        kmin = key < kmin ? key : kmin;
+        kmax = key > kmax ? key : kmax;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t a = __shfl_xor(kmin, o), b = __shfl_xor(kmax, o);
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    if ((tid & 63) == 0) {
+        ext[2 * (tid >> 6)] = kmin;
+        ext[2 * (tid >> 6) + 1] = kmax;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kNicheWaves; ++j) {
+        const uint64_t a = ext[2 * j], b = ext[2 * j + 1];
+        kmin = a < kmin ? a : kmin;
+        kmax = b > kmax ? b : kmax;
+    }
+    const uint64_t diff = kmin ^ kmax;
+    const int top = diff ? (63 - __builtin_clzll(diff)) / 8 * 8 : -8;   // highest byte that differs
+    uint64_t mask = (top >= 56) ? 0ull : (~0ull << (top + 8));
+    uint64_t prefix = kmin & mask;
     int want = take;                   // rank of the target among the candidates (1-based)
-    for (int sh = 56; sh >= 0; sh -= 8) {
+    for (int sh = top; sh >= 0; sh -= 8) {
         if (tid < 256) hist[tid] = 0;
         __syncthreads();
         for (int s = tid; s < n_last; s += kNicheThreads) {
@@ -1400,6 +1426,7 @@ __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* tabl
                 if (before < want && want <= before + c[j]) {
                     pick[0] = tid * 4 + j;
                     pick[1] = before;
+                    pick[2] = c[j];
                 }
                 before += c[j];
             }
@@ -1408,30 +1435,34 @@ __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* tabl
         prefix |= static_cast<uint64_t>(pick[0]) << sh;
         mask |= 255ull << sh;
         want -= pick[1];
+        const int left = pick[2];
         __syncthreads();
+        if (left == 1) break;          // the target is the one candidate left
     }
+    // chosen: every key whose masked value does not exceed the target's (with
+    // one candidate left, or the full mask, exactly the `take` smallest)
     for (int s = tid; s < n_last; s += kNicheThreads) {
         const uint64_t key = (static_cast<uint64_t>(V.lev[s]) << 48) | (static_cast<uint64_t>(V.db[s]) << 16) |
                              static_cast<uint64_t>(V.p2[s]);
-        if (key <= prefix) A.flag[A.gidx[n_prior + V.t[s]]] = 1;
+        if ((key & mask) <= prefix) A.flag[A.gidx[n_prior + V.t[s]]] = 1;
     }
 }
 
-// rho must hold the prior-front counts and flag the prior-front marks.
-__global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, const int32_t* __restrict__ meta, int nm,
-                                                             uint64_t* gk0, uint64_t* gk1, int32_t* gv0,
-                                                             int32_t* gv1, int lds_cap)
+// The last front's niching (rho must hold the prior-front counts).
+__device__ __forceinline__ void niche_last(const NicheArgs& A, const int32_t* __restrict__ meta, int nm,
+                                           uint64_t* gk0, uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap,
+                                           unsigned char* s_raw, int* s_wave)
 {
-    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
-    __shared__ int s_wave[64];
     const int n_rem = meta[kNRem];
     const int n_last = meta[kNLast];
-    if (n_rem <= 0 || n_last <= 0) return;
     const int n_prior = meta[kNPrior];
     const int tid = static_cast<int>(threadIdx.x);
     const int k = A.k;
-    // bucket table: k + 1 niches, 16-bit counts, two a word; then misc, then slots
-    const int words = ((k + 2) / 2 + 3) & ~3;
+    // bucket table: k + 1 niches, 16-bit counts, two a word, wpt consecutive
+    // words a thread (a multiple of 4); then misc, then slots
+    const int nwords = (k + 2) / 2;                     // niches 0 .. k
+    const int wpt = ((nwords + kNicheThreads - 1) / kNicheThreads + 3) & ~3;
+    const int words = wpt * kNicheThreads;
     const int misc_ints = 512;
     const int fixed = 4 * words + 4 * misc_ints;
     const int cap_b = (fixed < kNicheLdsBytes) ? ((kNicheLdsBytes - fixed) / kBucketSlotBytes) & ~15 : 0;
@@ -1445,33 +1476,39 @@ __global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, const 
             atomicAdd(&table[r >> 1], (r & 1) ? 0x10000u : 1u);
         }
         __syncthreads();
-        // niche starts in place: thread tid takes an even-aligned run of niches
-        const int per = 2 * ((k + 1 + 2 * kNicheThreads - 1) / (2 * kNicheThreads));
-        const int b0 = min(k + 1, tid * per), b1 = min(k + 1, b0 + per);
+        // niche starts in place: thread t owns words [t wpt, (t + 1) wpt)
+        const int lane = tid & 63, w = tid >> 6;
+        uint4* mine = reinterpret_cast<uint4*>(table + tid * wpt);
         int sum = 0, run_max = 0;
-        for (int r = b0; r < b1; ++r) {
-            const int c = half_get(table, r);
-            sum += c;
-            run_max = max(run_max, c);
-        }
-        int total;
-        int at = block_exclusive_scan(sum, s_wave, total);
-        for (int r = b0; r < b1; r += 2) {
-            const int c0 = half_get(table, r);
-            const int c1 = (r + 1 < b1) ? half_get(table, r + 1) : 0;
-            const uint32_t lo = static_cast<uint32_t>(at), hi = static_cast<uint32_t>(at + c0);
-            table[r >> 1] = lo | (hi << 16);
-            at += c0 + c1;
-        }
-        // largest niche
-        int mx = run_max;
+        for (int j = 0; j < wpt / 4; ++j) {
+            const uint4 v = mine[j];
+            const uint32_t e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
-        if ((tid & 63) == 0) misc[tid >> 6] = mx;
-        __syncthreads();
+            for (int i = 0; i < 4; ++i) {
+                const int c0 = static_cast<int>(e[i] & 0xFFFFu), c1 = static_cast<int>(e[i] >> 16);
+                sum += c0 + c1;
+                run_max = max(run_max, max(c0, c1));
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) run_max = max(run_max, __shfl_xor(run_max, o));
+        if (lane == 0) misc[w] = run_max;
+        int total;
+        int at = block_exclusive_scan(sum, s_wave, total);   // its barriers also publish misc
         int big = 0;
 #pragma unroll
         for (int j = 0; j < kNicheWaves; ++j) big = max(big, misc[j]);
+        for (int j = 0; j < wpt / 4; ++j) {
+            const uint4 v = mine[j];
+            uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int c0 = static_cast<int>(e[i] & 0xFFFFu), c1 = static_cast<int>(e[i] >> 16);
+                e[i] = static_cast<uint32_t>(at) | (static_cast<uint32_t>(at + c0) << 16);
+                at += c0 + c1;
+            }
+            mine[j] = make_uint4(e[0], e[1], e[2], e[3]);
+        }
         __syncthreads();
         if (big <= kBucketMaxRun) {
             if (n_last <= cap_b && n_last <= lds_cap) bucket_finish(A, table, misc, bucket_slots(s_raw + fixed, cap_b), n_prior, n_last, n_rem);
@@ -1484,6 +1521,100 @@ __global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, const 
     __syncthreads();
     if (n_last <= lds_cap) niche_body(NicheLdsBufs{&S}, A, S, n_prior, n_last, n_rem);
     else niche_body(NicheGlobalBufs{gk0, gk1, gv0, gv1}, A, S, n_prior, n_last, n_rem);
+}
+
+// Survivor flags (the prior fronts, then the niching picks), the niching, and
+// the survivors in ascending merged order (gather_selected_kernel,
+// niching.cu:192-206, prior copy :303-312; thrust::sort of the selection,
+// nsga3_population_update.cu:269), in one workgroup.
+__global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_t* __restrict__ meta, int nm, int n,
+                                                             const int32_t* __restrict__ fronts,
+                                                             int32_t* __restrict__ chosen, uint64_t* gk0,
+                                                             uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap)
+{
+    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
+    __shared__ int s_wave[64];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63, w = tid >> 6;
+    const int mf = meta[kMaxFront];
+    // prior-front flags, four individuals a thread (nm = 2n is a multiple of 4)
+    const int nq = nm / 4;
+    uint32_t* fw = reinterpret_cast<uint32_t*>(A.flag);
+    constexpr int kRows = 16;          // loads in flight per thread
+    for (int q0 = tid; q0 < nq; q0 += kRows * kNicheThreads) {
+        int4 fr[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = q0 + u * kNicheThreads;
+            fr[u] = (q < nq) ? reinterpret_cast<const int4*>(fronts)[q] : make_int4(-1, -1, -1, -1);
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = q0 + u * kNicheThreads;
+            const int4 f = fr[u];
+            if (q < nq)
+                fw[q] = ((f.x >= 0 && f.x < mf) ? 1u : 0u) | ((f.y >= 0 && f.y < mf) ? 0x100u : 0u) |
+                        ((f.z >= 0 && f.z < mf) ? 0x10000u : 0u) | ((f.w >= 0 && f.w < mf) ? 0x1000000u : 0u);
+        }
+    }
+    __syncthreads();
+    if (meta[kNRem] > 0 && meta[kNLast] > 0) niche_last(A, meta, nm, gk0, gk1, gv0, gv1, lds_cap, s_raw, s_wave);
+    __syncthreads();
+    // survivors in ascending order: wave w takes a run of flag words in rows of
+    // 64 consecutive words, one word (four flags) a lane
+    const int per_w = ((nq + kNicheWaves - 1) / kNicheWaves + 63) / 64 * 64;
+    const int q0 = min(nq, w * per_w), q1 = min(nq, q0 + per_w);
+    int cnt = 0;
+    for (int qr = q0; qr < q1; qr += kRows * 64) {
+        uint32_t v[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = qr + u * 64 + lane;
+            v[u] = (q < q1) ? fw[q] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) cnt += __popcll(ballot(((v[u] >> (8 * j)) & 1u) != 0));
+    }
+    if (lane == 0) s_wave[w] = cnt;
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int j = 0; j < kNicheWaves; ++j) {
+        if (j < w) base += s_wave[j];
+        total += s_wave[j];
+    }
+    for (int qb = q0; qb < q1; qb += kRows * 64) {
+        uint32_t vv[kRows];
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = qb + u * 64 + lane;
+            vv[u] = (q < q1) ? (fw[q] & 0x01010101u) : 0u;
+        }
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+        for (int u = 0; u < kRows; ++u) {
+            const int q = qb + u * 64 + lane;
+            const uint32_t v = vv[u];
+            int o = base, row = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t b = ballot(((v >> (8 * j)) & 1u) != 0);
+                o += __popcll(b & below);
+                row += __popcll(b);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if ((v >> (8 * j)) & 1u) {
+                    if (o < n) chosen[o] = 4 * q + j;
+                    ++o;
+                }
+            }
+            base += row;
+        }
+    }
+    if (tid == 0) meta[5] = total;
 }
 
 // Chosen indices in ascending merged order (thrust::sort of the selection,
@@ -1691,13 +1822,13 @@ int associate(const float* fn, int ld, int n, int m, const float* a, const float
     if (pruned) {
         MOEA_CHECK(cones_prepare(w, k, m, L.cones, st));
         MOEA_LAUNCH_M(m, associate_pruned, dim3(ceil_div(n, 4)), b256, 0, st, fn, ld, a, L.cones.wp, k,
-                      L.cones.perm, L.cones.cones, L.meta, dist, ref);
+                      L.cones.perm, L.cones.cones, L.meta, dist, ref, nullptr);
     } else {
         const int nchunks = ceil_div(k, kAssocChunk);
         MOEA_LAUNCH_M(m, associate_partial, dim3(ceil_div(n, 256), nchunks), b256, 0, st, fn, ld, a, w, k, L.meta,
                       L.part_d2, L.part_idx);
         hipLaunchKernelGGL(associate_final, dim3(ceil_div(n, 256)), b256, 0, st, L.part_d2, L.part_idx, nchunks, ld,
-                           L.meta, dist, ref);
+                           L.meta, dist, ref, nullptr);
     }
     MOEA_HIP(hipGetLastError());
     return kOk;
@@ -1768,7 +1899,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
     hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, st, L.fm, nm, n, n, m, A.ideal, 0);
     const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
     MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, A.ideal, L.meta, L.fn,
-                  L.asf_val, L.asf_idx, L.row_max);
+                  L.asf_val, L.asf_idx, L.row_max, L.rho, k);
     hipLaunchKernelGGL(norm_solve, dim3(1), dim3(kSolveThreads), 0, st, L.fn, nm, m, L.asf_val,
                        L.asf_idx, L.row_max, L.meta, L.a);
     stage_end(kStSelectPrep, st);
@@ -1776,25 +1907,22 @@ int nsga3_select(const moea_nsga3_select_args& A)
     if (k >= kConeMinK && ceil_div(k, 64) <= kConeMaxBlocks) {
         if (!A.directions_ready) MOEA_CHECK(cones_prepare(A.directions, k, m, L.cones, st));
         MOEA_LAUNCH_M(m, associate_pruned, dim3(ceil_div(nm, 4)), b256, 0, st, L.fn, nm, L.a, L.cones.wp, k,
-                      L.cones.perm, L.cones.cones, L.meta, L.dist, L.ref);
+                      L.cones.perm, L.cones.cones, L.meta, L.dist, L.ref, L.rho);
     } else {
         MOEA_LAUNCH_M(m, associate_partial, g_assoc, b256, 0, st, L.fn, nm, L.a, A.directions, k,
                       L.meta, L.part_d2, L.part_idx);
         hipLaunchKernelGGL(associate_final, dim3(nb), dim3(256), 0, st, L.part_d2, L.part_idx, nchunks,
-                           nm, L.meta, L.dist, L.ref);
+                           nm, L.meta, L.dist, L.ref, L.rho);
     }
     stage_end(kStAssociation, st);
     stage_begin(kStNiching, st);
-    hipLaunchKernelGGL(rho_clear, dim3(ceil_div(k, 256)), dim3(256), 0, st, L.rho, k);
-    hipLaunchKernelGGL(rho_hist, dim3(nb), dim3(256), 0, st, L.ref, L.meta, k, L.rho);
     const int bits_ref = 32 + (32 - __builtin_clz(static_cast<unsigned>(k + 1)));
     const int bits_rank = 32 + (32 - __builtin_clz(static_cast<unsigned>(nm)));
     if (niching_one_block(nm)) {
-        hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
         NicheArgs na{L.ref, L.dist, L.rho, k, bits_ref, bits_rank, dr ? dr->niche_a : nullptr,
                      dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets()};
-        hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, L.k0, L.k1, L.v0,
-                           L.v1, niching_lds_cap());
+        hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, n, L.fronts, L.chosen,
+                           L.k0, L.k1, L.v0, L.v1, niching_lds_cap());
     } else {
         hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
         MOEA_HIP(hipGetLastError());
@@ -1810,10 +1938,10 @@ int nsga3_select(const moea_nsga3_select_args& A)
         hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
         hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
                            L.flag);
+        hipLaunchKernelGGL(flag_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, L.counts);
+        hipLaunchKernelGGL(flag_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, n, L.counts, nsb,
+                           L.chosen, L.meta);
     }
-    hipLaunchKernelGGL(flag_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, L.counts);
-    hipLaunchKernelGGL(flag_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, n, L.counts, nsb,
-                       L.chosen, L.meta);
     stage_end(kStNiching, st);
     stage_begin(kStGather, st);
     hipLaunchKernelGGL(gather_next, dim3(ceil_div(n, 4)), dim3(256), 0, st, L.chosen, n, d, m,
