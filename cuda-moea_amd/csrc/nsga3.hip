@@ -1256,7 +1256,8 @@ __device__ void niche_body(const Bufs& B, const NicheArgs& A, NicheShared& S, in
 // takes the radix sorts instead.
 constexpr int kNicheLdsBytes = 162816;
 constexpr int kBucketMaxRun = 128;
-constexpr int kBucketSlotBytes = 18;   // t, r, p1, lev, p2: 2 B each; a0, db: 4 B each
+constexpr int kBucketSlotBytes = 10;   // LDS: t, r / p2, p1 / lev: 2 B each; a0 / db: 4 B
+constexpr int kBucketRegs = 10;        // members a thread holds through the sort-2 step (LDS case)
 
 struct BucketSlots {
     uint16_t* t;     // last-front index
@@ -1268,17 +1269,19 @@ struct BucketSlots {
     uint16_t* p2;    // sort-2 position
 };
 
-// LDS: the seven arrays back to back, `cap` members each.
+// LDS: four arrays of `cap` members; the sort-2 step replaces a0 by db, p1 by
+// lev and r by p2 (its results wait in registers until every member's inputs
+// are read).
 __device__ __forceinline__ BucketSlots bucket_slots(unsigned char* base, int cap)
 {
     BucketSlots v;
     v.a0 = reinterpret_cast<uint32_t*>(base);
-    v.db = v.a0 + cap;
-    v.t = reinterpret_cast<uint16_t*>(v.db + cap);
+    v.db = v.a0;
+    v.t = reinterpret_cast<uint16_t*>(v.a0 + cap);
     v.r = v.t + cap;
+    v.p2 = v.r;
     v.p1 = v.r + cap;
-    v.lev = v.p1 + cap;
-    v.p2 = v.lev + cap;
+    v.lev = v.p1;
     return v;
 }
 
@@ -1304,6 +1307,8 @@ __device__ __forceinline__ int half_get(const uint32_t* table, int r)
 }
 
 // Slots and ranks, then the selection; table holds each niche's start (16-bit).
+// kShared: V aliases the sort-2 outputs onto its inputs (n_last <= 1024 kBucketRegs).
+template <bool kShared>
 __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* table, int32_t* misc,
                                               const BucketSlots& V, int n_prior, int n_last, int n_rem)
 {
@@ -1344,7 +1349,7 @@ __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* tabl
     }
     __syncthreads();
     // sort 2: rank by (low, p1) inside the niche; phase-5 level and draw
-    for (int s = tid; s < n_last; s += kNicheThreads) {
+    auto sort2 = [&](int s, int& lev, uint32_t& db, int& p2) {
         const int r = V.r[s];
         const int st = r ? half_get(table, r - 1) : 0, en = half_get(table, r);
         const uint32_t lo = V.a0[s];
@@ -1355,10 +1360,37 @@ __device__ __forceinline__ void bucket_finish(const NicheArgs& A, uint32_t* tabl
             rank += (lu < lo || (lu == lo && static_cast<int>(V.p1[u]) < p1)) ? 1 : 0;
         }
         const int rh = (r < k) ? A.rho[r] : 0;
-        const int p2 = st + rank;
-        V.lev[s] = static_cast<uint16_t>(rh + rank);
-        V.db[s] = niche_draw(A.inj_b, 1, p2, A.seed, A.gen);
-        V.p2[s] = static_cast<uint16_t>(p2);
+        p2 = st + rank;
+        lev = rh + rank;
+        db = niche_draw(A.inj_b, 1, p2, A.seed, A.gen);
+    };
+    if (kShared) {
+        int lev[kBucketRegs], p2[kBucketRegs];
+        uint32_t db[kBucketRegs];
+#pragma unroll
+        for (int j = 0; j < kBucketRegs; ++j) {
+            const int s = tid + j * kNicheThreads;
+            if (s < n_last) sort2(s, lev[j], db[j], p2[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBucketRegs; ++j) {
+            const int s = tid + j * kNicheThreads;
+            if (s < n_last) {
+                V.lev[s] = static_cast<uint16_t>(lev[j]);
+                V.db[s] = db[j];
+                V.p2[s] = static_cast<uint16_t>(p2[j]);
+            }
+        }
+    } else {
+        for (int s = tid; s < n_last; s += kNicheThreads) {
+            int lev, p2;
+            uint32_t db;
+            sort2(s, lev, db, p2);
+            V.lev[s] = static_cast<uint16_t>(lev);
+            V.db[s] = db;
+            V.p2[s] = static_cast<uint16_t>(p2);
+        }
     }
     __syncthreads();
     // sort 3 as a selection: the n_rem-th smallest of the distinct keys
@@ -1465,8 +1497,9 @@ __device__ __forceinline__ void niche_last(const NicheArgs& A, const int32_t* __
     const int words = wpt * kNicheThreads;
     const int misc_ints = 512;
     const int fixed = 4 * words + 4 * misc_ints;
-    const int cap_b = (fixed < kNicheLdsBytes) ? ((kNicheLdsBytes - fixed) / kBucketSlotBytes) & ~15 : 0;
-    if (A.bucket && cap_b >= 64 && n_last <= 65535 && n_prior <= 65535 - kBucketMaxRun) {
+    int cap_b = (fixed < kNicheLdsBytes) ? ((kNicheLdsBytes - fixed) / kBucketSlotBytes) & ~15 : 0;
+    cap_b = min(cap_b, kBucketRegs * kNicheThreads);
+    if (A.bucket && fixed <= kNicheLdsBytes && n_last <= 65535 && n_prior <= 65535 - kBucketMaxRun) {
         uint32_t* table = reinterpret_cast<uint32_t*>(s_raw);
         int32_t* misc = reinterpret_cast<int32_t*>(s_raw + 4 * words);
         for (int j = tid; j < words; j += kNicheThreads) table[j] = 0;
@@ -1511,15 +1544,17 @@ __device__ __forceinline__ void niche_last(const NicheArgs& A, const int32_t* __
         }
         __syncthreads();
         if (big <= kBucketMaxRun) {
-            if (n_last <= cap_b && n_last <= lds_cap) bucket_finish(A, table, misc, bucket_slots(s_raw + fixed, cap_b), n_prior, n_last, n_rem);
-            else bucket_finish(A, table, misc, bucket_slots_global(gk0, gk1, gv0, nm), n_prior, n_last, n_rem);
+            if (n_last <= cap_b && n_last <= lds_cap)
+                bucket_finish<true>(A, table, misc, bucket_slots(s_raw + fixed, cap_b), n_prior, n_last, n_rem);
+            else
+                bucket_finish<false>(A, table, misc, bucket_slots_global(gk0, gk1, gv0, nm), n_prior, n_last, n_rem);
             return;
         }
     }
     NicheShared& S = *reinterpret_cast<NicheShared*>(s_raw);
     if (tid == 0) S.skip = -1;
     __syncthreads();
-    if (n_last <= lds_cap) niche_body(NicheLdsBufs{&S}, A, S, n_prior, n_last, n_rem);
+    if (n_last <= lds_cap && n_last <= kNicheLdsCap) niche_body(NicheLdsBufs{&S}, A, S, n_prior, n_last, n_rem);
     else niche_body(NicheGlobalBufs{gk0, gk1, gv0, gv1}, A, S, n_prior, n_last, n_rem);
 }
 
@@ -1846,7 +1881,7 @@ namespace {
 // Niching path: 0 automatic (one block), 1 one block, 2 device-wide sorts,
 // 3 one block with its radix sorts only.
 int g_niching_mode = 0;
-int g_niching_lds_cap = kNicheLdsCap;
+int g_niching_lds_cap = 1 << 30;   // each path also stops at its own LDS capacity
 
 bool niching_one_block(int nm)
 {
@@ -1861,7 +1896,7 @@ int set_niching(int mode, int lds_cap)
 {
     if (mode < 0 || mode > 3 || lds_cap < 0) return fail(kInvalidArgument, "set_niching: invalid mode or capacity");
     g_niching_mode = mode;
-    g_niching_lds_cap = lds_cap < kNicheLdsCap ? lds_cap : kNicheLdsCap;
+    g_niching_lds_cap = lds_cap;
     return kOk;
 }
 

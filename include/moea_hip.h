@@ -156,8 +156,10 @@ int moea_nsga3_select(const moea_nsga3_select_args* args);
  * members, grouped by direction (a niche of more than 128 members falls back
  * to radix sorts), 2 three device-wide sorts of all 2n slots, 3 the one
  * workgroup's radix sorts only.  The one-workgroup paths keep the members in
- * LDS up to lds_cap of them (and their own limit), else in the workspace.
- * Every path selects the same survivors. */
+ * LDS when there are at most lds_cap of them and they fit (about 9500 for the
+ * grouping at K = 32640, 5888 for the radix sorts), else in the workspace;
+ * mode 0 with a large lds_cap is the default.  Every path selects the same
+ * survivors. */
 int moea_set_niching(int mode, int lds_cap);
 
 /* Perpendicular-distance association of n normalized individuals: for

@@ -16,13 +16,13 @@ from cuda_moea import kernels as K  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
-PATHS = [(1, 5888), (1, 0), (3, 5888), (3, 0), (2, 0)]     # (mode, LDS capacity)
+PATHS = [(1, 1 << 30), (1, 0), (3, 1 << 30), (3, 0), (2, 0)]     # (mode, LDS capacity)
 
 
 @pytest.fixture(autouse=True)
 def _restore_niching():
     yield
-    assert _lib.lib.moea_set_niching(0, 5888) == 0
+    assert _lib.lib.moea_set_niching(0, 1 << 30) == 0
 
 
 def _f(a):
@@ -47,7 +47,8 @@ def _objectives(rng, kind, m, nm):
 @pytest.mark.parametrize("n,m,kind,seed", [(92, 3, "ties", 1), (300, 5, "sphere", 2), (1024, 3, "sphere", 3),
                                            (4000, 3, "plane", 4), (8192, 3, "plane", 5),
                                            (2048, 8, "ties", 6), (3000, 10, "sphere", 7),
-                                           (700, 3, "clump", 8), (5000, 4, "clump", 9)])
+                                           (700, 3, "clump", 8), (5000, 4, "clump", 9),
+                                           (9000, 3, "sphere", 10), (4500, 3, "plane", 11)])
 def test_niching_paths_equal_oracle(n, m, kind, seed):
     rng = np.random.default_rng(seed)
     fm = _objectives(rng, kind, m, 2 * n)
