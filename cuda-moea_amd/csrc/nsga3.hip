@@ -1562,13 +1562,11 @@ __device__ __forceinline__ void niche_last(const NicheArgs& A, const int32_t* __
 // the survivors in ascending merged order (gather_selected_kernel,
 // niching.cu:192-206, prior copy :303-312; thrust::sort of the selection,
 // nsga3_population_update.cu:269), in one workgroup.
-__global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_t* __restrict__ meta, int nm, int n,
-                                                             const int32_t* __restrict__ fronts,
-                                                             int32_t* __restrict__ chosen, uint64_t* gk0,
-                                                             uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap)
+__device__ __forceinline__ void niche_select_body(const NicheArgs& A, int32_t* __restrict__ meta, int nm, int n,
+                                                  const int32_t* __restrict__ fronts, int32_t* __restrict__ chosen,
+                                                  uint64_t* gk0, uint64_t* gk1, int32_t* gv0, int32_t* gv1,
+                                                  int lds_cap, unsigned char* s_raw, int* s_wave)
 {
-    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
-    __shared__ int s_wave[64];
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = tid & 63, w = tid >> 6;
     const int mf = meta[kMaxFront];
@@ -1650,6 +1648,16 @@ __global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_
         }
     }
     if (tid == 0) meta[5] = total;
+}
+
+__global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_t* __restrict__ meta, int nm, int n,
+                                                             const int32_t* __restrict__ fronts,
+                                                             int32_t* __restrict__ chosen, uint64_t* gk0,
+                                                             uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap)
+{
+    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
+    __shared__ int s_wave[64];
+    niche_select_body(A, meta, nm, n, fronts, chosen, gk0, gk1, gv0, gv1, lds_cap, s_raw, s_wave);
 }
 
 // Chosen indices in ascending merged order (thrust::sort of the selection,
@@ -2149,6 +2157,25 @@ __global__ void b_niche_keys3(const BInst* __restrict__ insts, BMap mp, int ref_
                      key3 + I.key_off, t);
 }
 
+// The niching, survivor flags and their ascending compaction of each running
+// instance, one workgroup each (niche_select_body); the instance's slots of the
+// batched sort buffers are its workspace.
+__global__ __launch_bounds__(kNicheThreads) void b_niche(const BInst* __restrict__ insts, uint64_t* keys_in,
+                                                         uint64_t* keys_out, int32_t* vals_in, int32_t* vals_out,
+                                                         int bucket, int lds_cap)
+{
+    __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
+    __shared__ int s_wave[64];
+    const BInst& I = insts[blockIdx.x];
+    if (I.state != 1) return;
+    const int bits_ref = 32 + (32 - __clz(I.k + 1));
+    const int bits_rank = 32 + (32 - __clz(I.nm));
+    const NicheArgs A{I.ref, I.dist, I.rho, I.k, bits_ref, bits_rank, nullptr, nullptr, I.seed,
+                      static_cast<uint32_t>(I.gen), I.gidx, I.flag, bucket};
+    niche_select_body(A, I.meta, I.nm, I.n, I.fronts_mix, I.chosen, keys_in + I.key_off, keys_out + I.key_off,
+                      vals_in + I.key_off, vals_out + I.key_off, lds_cap, s_raw, s_wave);
+}
+
 // Survivor flags and their ascending compaction, one instance per workgroup.
 __global__ __launch_bounds__(kScanThreads) void b_survivors(const BInst* __restrict__ insts,
                                                             const int32_t* __restrict__ val3s)
@@ -2260,6 +2287,17 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     stage_begin(kStNiching, st);
     hipLaunchKernelGGL(b_rho, dim3(L.count), dim3(1024), 0, st, L.insts);
     MOEA_HIP(hipGetLastError());
+    if (niching_one_block(total)) {
+        hipLaunchKernelGGL(b_niche, dim3(L.count), dim3(kNicheThreads), 0, st, L.insts, S.keys_in, S.keys_out,
+                           S.vals_in, S.vals_out, niching_buckets(), niching_lds_cap());
+        MOEA_HIP(hipGetLastError());
+        stage_end(kStNiching, st);
+        stage_begin(kStGather, st);
+        hipLaunchKernelGGL(b_gather, dim3(G.rows4.blocks), dim3(256), 0, st, L.insts, G.rows4.map);
+        MOEA_HIP(hipGetLastError());
+        stage_end(kStGather, st);
+        return kOk;
+    }
     const int bits_ref = 32 + G.ref_bits + S.inst_bits;
     const int bits_rank = 32 + G.rank_bits + S.inst_bits;
     hipLaunchKernelGGL(b_niche_keys1, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map,

@@ -1,7 +1,7 @@
 """Niching paths of moea_nsga3_select select the same survivors as the
 oracle, bit for bit: one workgroup grouping the last front by direction (in
 LDS or, past its capacity, in the workspace), the same workgroup's radix
-sorts (LDS or workspace; also its fallback for a niche of more than 128
+sorts (LDS or workspace; also its fallback for a niche of more than 1024
 members), and three device-wide sorts of all 2N slots."""
 import numpy as np
 import pytest
