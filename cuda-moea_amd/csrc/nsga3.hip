@@ -55,19 +55,38 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& tot
 
 // f (m, 2n) and cv (2n) of the merged population (nsga3_population_update.cu:7-115
 // copies variables too; here only the O(M n) objective columns move).
-__global__ void merge_objectives(const float* __restrict__ pf, const float* __restrict__ of,
-                                 const float* __restrict__ pcv, const float* __restrict__ ocv,
-                                 int n, int m, float* __restrict__ fm, float* __restrict__ cvm)
+// omin (optional, 256-thread blocks): per block and objective, the fminf of
+// its offspring rows, for the running ideal point (norm_prep finishes it).
+__global__ __launch_bounds__(256) void merge_objectives(const float* __restrict__ pf, const float* __restrict__ of,
+                                                        const float* __restrict__ pcv, const float* __restrict__ ocv,
+                                                        int n, int m, float* __restrict__ fm, float* __restrict__ cvm,
+                                                        float* __restrict__ omin)
 {
+    __shared__ float s_min[4][kMaxM];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nm = 2 * n;
-    if (i >= nm) return;
+    const bool valid = i < nm;
     const bool par = i < n;
     const int s = par ? i : i - n;
-    for (int k = 0; k < m; ++k)
-        fm[static_cast<size_t>(k) * nm + i] = par ? pf[static_cast<size_t>(k) * n + s]
-                                                  : of[static_cast<size_t>(k) * n + s];
-    cvm[i] = par ? pcv[s] : ocv[s];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < m; ++k) {
+        float v = INFINITY;
+        if (valid) {
+            v = par ? pf[static_cast<size_t>(k) * n + s] : of[static_cast<size_t>(k) * n + s];
+            fm[static_cast<size_t>(k) * nm + i] = v;
+        }
+        float z = (valid && !par) ? v : INFINITY;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) z = fminf(z, __shfl_xor(z, o));
+        if (lane == 0) s_min[wave][k] = z;
+    }
+    if (valid) cvm[i] = par ? pcv[s] : ocv[s];
+    if (!omin) return;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < m) {
+        const int k = threadIdx.x;
+        omin[blockIdx.x * m + k] = fminf(fminf(s_min[0][k], s_min[1][k]), fminf(s_min[2][k], s_min[3][k]));
+    }
 }
 
 // prior = {front < max_front}, last = {front == max_front}, each ascending
@@ -227,16 +246,31 @@ __device__ __forceinline__ void norm_prep_body(const float* __restrict__ fm, int
 template <int M>
 __global__ __launch_bounds__(256) void norm_prep(const float* __restrict__ fm, int nm,
                                                  const int32_t* __restrict__ gidx,
-                                                 const float* __restrict__ ideal,
                                                  const int32_t* __restrict__ meta,
                                                  float* __restrict__ fn, float* __restrict__ asf_val,
                                                  int32_t* __restrict__ asf_idx,
-                                                 float* __restrict__ row_max, int32_t* __restrict__ rho, int k)
+                                                 float* __restrict__ row_max, int32_t* __restrict__ rho, int k,
+                                                 const float* __restrict__ omin, int nbo, float* ideal_io)
 {
     // the niche counts start at zero (the association kernels add the prior front)
     for (int j = static_cast<int>(blockIdx.x * 256 + threadIdx.x); j < k; j += static_cast<int>(gridDim.x * 256))
         rho[j] = 0;
-    norm_prep_body<M>(fm, nm, gidx, ideal, meta, fn, asf_val, asf_idx, row_max, static_cast<int>(blockIdx.x));
+    // running ideal: fminf(ideal, min over the offspring) from merge_objectives'
+    // per-block minima, reduced in one fixed order by every block; block 0
+    // stores it (a block reading the stored value gets the same result, since
+    // fminf(fminf(a, r), r) = fminf(a, r))
+    __shared__ float s_ideal[M];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = wave; j < M; j += 4) {
+        float z = INFINITY;
+        for (int b = lane; b < nbo; b += 64) z = fminf(z, omin[b * M + j]);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) z = fminf(z, __shfl_xor(z, o));
+        if (lane == 0) s_ideal[j] = fminf(ideal_io[j], z);
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < M) ideal_io[threadIdx.x] = s_ideal[threadIdx.x];
+    norm_prep_body<M>(fm, nm, gidx, s_ideal, meta, fn, asf_val, asf_idx, row_max, static_cast<int>(blockIdx.x));
 }
 
 // Extreme points, FP64 hyperplane or fallback, sanitize (normalization.cu:453-540,
@@ -1650,14 +1684,18 @@ __device__ __forceinline__ void niche_select_body(const NicheArgs& A, int32_t* _
     if (tid == 0) meta[5] = total;
 }
 
+// stats (optional): the selection's meta words, as write_stats.
 __global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_t* __restrict__ meta, int nm, int n,
                                                              const int32_t* __restrict__ fronts,
                                                              int32_t* __restrict__ chosen, uint64_t* gk0,
-                                                             uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap)
+                                                             uint64_t* gk1, int32_t* gv0, int32_t* gv1, int lds_cap,
+                                                             int32_t* __restrict__ stats)
 {
     __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
     __shared__ int s_wave[64];
     niche_select_body(A, meta, nm, n, fronts, chosen, gk0, gk1, gv0, gv1, lds_cap, s_raw, s_wave);
+    if (stats && threadIdx.x == 0)
+        for (int j = 0; j < kMetaLen; ++j) stats[j] = meta[j];
 }
 
 // Chosen indices in ascending merged order (thrust::sort of the selection,
@@ -1757,7 +1795,7 @@ __global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restric
 
 struct Layout {
     float* fm; float* cvm; int32_t* fronts; int32_t* meta; int32_t* gidx; float* fn;
-    float* asf_val; int32_t* asf_idx; float* row_max; float* a;
+    float* asf_val; int32_t* asf_idx; float* row_max; float* a; float* omin;
     float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho; ConeBufs cones;
     uint64_t* k0; uint64_t* k1; int32_t* v0; int32_t* v1; void* sort_tmp; size_t sort_bytes;
     uint8_t* flag; int32_t* chosen; int32_t* counts; void* nds_ws; size_t nds_bytes;
@@ -1779,6 +1817,7 @@ Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
     L.asf_idx = c.take<int32_t>(static_cast<size_t>(nb) * m);
     L.row_max = c.take<float>(static_cast<size_t>(nb) * m);
     L.a = c.take<float>(kMaxM);
+    L.omin = c.take<float>(static_cast<size_t>(nb) * m);
     L.part_d2 = c.take<float>(static_cast<size_t>(nchunks) * nm);
     L.part_idx = c.take<int32_t>(static_cast<size_t>(nchunks) * nm);
     L.dist = c.take<float>(nm);
@@ -1931,7 +1970,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
     const moea_draws* dr = A.draws;
 
     hipLaunchKernelGGL(merge_objectives, dim3(nb), dim3(256), 0, st, A.parents.f, A.offspring.f,
-                       A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm);
+                       A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm, L.omin);
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(nds_run(L.fm, nm, L.cvm, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
     stage_begin(kStSelectPrep, st);
@@ -1939,10 +1978,9 @@ int nsga3_select(const moea_nsga3_select_args& A)
     hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);
     hipLaunchKernelGGL(front_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.counts, nsb,
                        L.gidx, L.meta);
-    hipLaunchKernelGGL(ideal_update, dim3(m), dim3(1024), 0, st, L.fm, nm, n, n, m, A.ideal, 0);
     const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
-    MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, A.ideal, L.meta, L.fn,
-                  L.asf_val, L.asf_idx, L.row_max, L.rho, k);
+    MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, L.meta, L.fn, L.asf_val, L.asf_idx,
+                  L.row_max, L.rho, k, L.omin, nb, A.ideal);
     hipLaunchKernelGGL(norm_solve, dim3(1), dim3(kSolveThreads), 0, st, L.fn, nm, m, L.asf_val,
                        L.asf_idx, L.row_max, L.meta, L.a);
     stage_end(kStSelectPrep, st);
@@ -1965,7 +2003,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
         NicheArgs na{L.ref, L.dist, L.rho, k, bits_ref, bits_rank, dr ? dr->niche_a : nullptr,
                      dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets()};
         hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, n, L.fronts, L.chosen,
-                           L.k0, L.k1, L.v0, L.v1, niching_lds_cap());
+                           L.k0, L.k1, L.v0, L.v1, niching_lds_cap(), A.stats);
     } else {
         hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
         MOEA_HIP(hipGetLastError());
@@ -1991,7 +2029,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        A.parents.x, A.offspring.x, L.fm, L.cvm, L.fronts, A.next.x, A.next.f,
                        A.next.cv, A.next_fronts);
     stage_end(kStGather, st);
-    if (A.stats) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
+    if (A.stats && !niching_one_block(nm)) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
