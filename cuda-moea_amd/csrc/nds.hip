@@ -138,9 +138,13 @@ __device__ __forceinline__ void block_info_one(const uint64_t* __restrict__ keys
     hi[b] = end >> 6;
 }
 
+// zero (optional, `words` of them): cleared on the way, for a later launch.
 __global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
-                               int32_t* __restrict__ hi)
+                               int32_t* __restrict__ hi, uint32_t* __restrict__ zero, int words)
 {
+    for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < words;
+         j += static_cast<int>(gridDim.x * blockDim.x))
+        zero[j] = 0u;
     block_info_one(keys, n, static_cast<int>(blockIdx.x), kind, hi);
 }
 
@@ -729,11 +733,6 @@ __global__ void b_peel_reset(const BInst* __restrict__ insts, int state)
     for (int t = threadIdx.x; t < kPeelStateWords; t += blockDim.x) I.pstate[t] = 0u;
 }
 
-__global__ void zero_u32(uint32_t* p, int count)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = 0u;
-}
 
 constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
 
@@ -835,7 +834,9 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     hipLaunchKernelGGL(nds_gather, dim3(ceil_div(npad, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
                        npad, L.perm, L.fs, L.cvs);
     MOEA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi);
+    // also clears the peel's barrier state (not read before the peel)
+    hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi, L.state,
+                       kPeelStateWords);
     MOEA_HIP(hipGetLastError());
     stage_end(kStNdsSort, stream);
 
@@ -848,9 +849,6 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
-    hipLaunchKernelGGL(zero_u32, dim3(ceil_div(kPeelStateWords, 256)), dim3(256), 0, stream, L.state,
-                       kPeelStateWords);
-    MOEA_HIP(hipGetLastError());
     const int grid = peel_grid_size(W);
     if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
     const int bpg = ceil_div(W, grid);
