@@ -273,6 +273,7 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
     G.sel.ref_bits = bitlen(max_k + 1);
     G.sel.rank_bits = bitlen(max_nm);
     G.sel.total_merged = total_merged;
+    G.sel.max_nm = static_cast<int>(max_nm);
     G.nds0.rows256 = T.add(n0_rows);
     G.nds0.npad256 = T.add(n0_npad);
     G.nds0.blocks64 = T.add(n0_w);

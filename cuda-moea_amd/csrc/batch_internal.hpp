@@ -141,6 +141,7 @@ struct BSelGrids {
     int ref_bits;                 // bits of a direction index (with the two sentinels)
     int rank_bits;                // bits of rho + rank (with the sentinel)
     long long total_merged;       // sum of 2n (sort length)
+    int max_nm;                   // largest 2n of the state's instances
 };
 
 struct BNdsScratch {

@@ -1936,6 +1936,8 @@ bool niching_one_block(int nm)
     return g_niching_mode != 2;
 }
 int niching_buckets() { return g_niching_mode == 3 ? 0 : 1; }
+int niching_mode() { return g_niching_mode; }
+constexpr int kBatchNicheMaxNm = 8192;
 int niching_lds_cap() { return g_niching_lds_cap; }
 }  // namespace
 
@@ -2325,7 +2327,10 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     stage_begin(kStNiching, st);
     hipLaunchKernelGGL(b_rho, dim3(L.count), dim3(1024), 0, st, L.insts);
     MOEA_HIP(hipGetLastError());
-    if (niching_one_block(total)) {
+    // one workgroup per instance while every instance is small (2n <= 8192);
+    // larger runs (many objectives put most of 2n in the last front) sort
+    // faster device-wide
+    if (niching_one_block(total) && (niching_mode() == 1 || G.max_nm <= kBatchNicheMaxNm)) {
         hipLaunchKernelGGL(b_niche, dim3(L.count), dim3(kNicheThreads), 0, st, L.insts, S.keys_in, S.keys_out,
                            S.vals_in, S.vals_out, niching_buckets(), niching_lds_cap());
         MOEA_HIP(hipGetLastError());

@@ -158,8 +158,10 @@ int moea_nsga3_select(const moea_nsga3_select_args* args);
  * workgroup's radix sorts only.  The one-workgroup paths keep the members in
  * LDS when there are at most lds_cap of them and they fit (about 9500 for the
  * grouping at K = 32640, 5888 for the radix sorts), else in the workspace;
- * mode 0 with a large lds_cap is the default.  Every path selects the same
- * survivors. */
+ * mode 0 with a large lds_cap is the default.  The batched and served
+ * engines take one workgroup per instance in mode 1, and in mode 0 while
+ * every instance has 2n <= 8192 (device-wide sorts otherwise).  Every path
+ * selects the same survivors. */
 int moea_set_niching(int mode, int lds_cap);
 
 /* Perpendicular-distance association of n normalized individuals: for

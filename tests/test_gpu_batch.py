@@ -62,6 +62,24 @@ def test_batch_mixed_objectives_equal_single_runs():
         _same(res, single.run())
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_niching_paths_equal_single_runs(mode):
+    """The batch's niching paths (one workgroup per instance, chosen while every
+    2n <= 8192; device-wide sorts otherwise or in mode 2) against single runs."""
+    from cuda_moea import _lib
+    gens = 6
+    cases = MIX[:3] + [(4200, 3, 20, 0, 11)]        # 2n = 8400: device-wide in mode 0
+    print(f"batch niching mode {mode}", flush=True)
+    try:
+        assert _lib.lib.moea_set_niching(mode, 1 << 30) == 0
+        results = cm.BatchNSGA3([_run(*c, gens) for c in cases]).run()
+    finally:
+        assert _lib.lib.moea_set_niching(0, 1 << 30) == 0
+    for c, res in zip(cases, results):
+        print(f"  instance {c}", flush=True)
+        _same(res, _run(*c, gens).run())
+
+
 def test_batch_alone_equals_in_mix():
     gens = 10
     alone = cm.BatchNSGA3([_run(*MIX[5], gens)]).run()[0]
