@@ -99,6 +99,11 @@ void carve_instance(Carve& c, const moea_batch_problem& p, BInst& I)
     I.rend = c.take<int32_t>(z.nm);
     I.A = c.take<uint64_t>(2 * static_cast<size_t>(z.W));
     I.pstate = c.take<uint32_t>(kBatchPeelStateWords);
+    I.wp = nullptr; I.cperm = nullptr; I.cones = nullptr; I.cone_buf = nullptr;
+    if (assoc_prunable(z.k, z.m)) {
+        I.cone_buf = c.take_bytes(cone_bytes(z.m, z.k));
+        if (I.cone_buf) cone_views(I.cone_buf, z.m, z.k, &I.wp, &I.cperm, &I.cones);
+    }
 }
 
 size_t instance_bytes(const moea_batch_problem& p)
@@ -190,6 +195,7 @@ void fix_grids(Grids& G, const int* base)
         fix(G.eval.mlp_rows4[m], base);
         fix(G.sel.norm_prep[m], base);
         fix(G.sel.assoc[m], base);
+        fix(G.sel.assoc_pruned[m], base);
         fix(G.nds0.dom[m], base);
         fix(G.nds1.dom[m], base);
     }
@@ -213,7 +219,7 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
     std::vector<std::vector<long long>> q4(kMaxM + 1, std::vector<long long>(count, 0)),
         h4(kMaxM + 1, std::vector<long long>(count, 0)), prep(kMaxM + 1, std::vector<long long>(count, 0)),
         assoc(kMaxM + 1, std::vector<long long>(count, 0)), dom0(kMaxM + 1, std::vector<long long>(count, 0)),
-        dom1(kMaxM + 1, std::vector<long long>(count, 0));
+        dom1(kMaxM + 1, std::vector<long long>(count, 0)), apr(kMaxM + 1, std::vector<long long>(count, 0));
     int max_k = 1, max_nm = 1;
     std::vector<int> peel_inst, peel_W0, peel_W1;
     long long total_merged = 0, total_n = 0;
@@ -236,7 +242,8 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
             q4[m][i] = ceil_div(n, 4);
         }
         prep[m][i] = nb;
-        assoc[m][i] = static_cast<long long>(nb) * I.nchunks;
+        if (I.cones) apr[m][i] = ceil_div(nm, 4);
+        else assoc[m][i] = static_cast<long long>(nb) * I.nchunks;
         n0_rows[i] = ceil_div(n, 256);
         n0_npad[i] = ceil_div(W0 * 64, 256);
         n0_w[i] = W0;
@@ -269,6 +276,7 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
     for (int m = 1; m <= kMaxM; ++m) {
         G.sel.norm_prep[m] = T.add(prep[m]);
         G.sel.assoc[m] = T.add(assoc[m]);
+        G.sel.assoc_pruned[m] = T.add(apr[m]);
     }
     G.sel.ref_bits = bitlen(max_k + 1);
     G.sel.rank_bits = bitlen(max_nm);
@@ -300,8 +308,8 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
     return total_merged;
 }
 
-// Prefix tables per state: (1 + 3 + 2 kMaxM + 1 + 3 + 2 kMaxM + 3 + 3 + 2 kMaxM) grids.
-constexpr int kGridsPerState = 14 + 6 * kMaxM;
+// Prefix tables per state: (1 + 3 + 2 kMaxM + 1 + 3 + 3 kMaxM + 3 + 3 + 2 kMaxM) grids.
+constexpr int kGridsPerState = 14 + 7 * kMaxM;
 
 }  // namespace
 
@@ -386,6 +394,8 @@ int core_init_admitted(BatchCore& C)
     if (!any_state(C, kStateAdmitted)) return kOk;
     if (C.dirty) MOEA_CHECK(core_upload(C));
     BatchLaunch L{C.dev_insts, C.slots, C.stream};
+    for (const BInst& I : C.host)
+        if (I.state == kStateAdmitted && I.cone_buf) MOEA_CHECK(cones_build(I.dirs, I.k, I.m, I.cone_buf, C.stream));
     MOEA_CHECK(batch_init_population(L, C.init.init256.map, C.init.init256.blocks));
     MOEA_CHECK(batch_evaluate(L, C.init.eval, 0));
     MOEA_CHECK(batch_nds(L, C.init.nds0, C.scratch, 0, kStateAdmitted));

@@ -67,6 +67,11 @@ struct BInst {
     int32_t* rho;                 // (k)
     uint8_t* flag;                // (nm)
     int32_t* chosen;              // (n)
+    // pruned association (K >= 2048): cone tables built at admission, else null
+    float* wp;                    // (m, k) directions in block order
+    int32_t* cperm;               // (k) block position -> direction index
+    double* cones;                // (ceil(k / 64), m + 3)
+    void* cone_buf;               // the tables and their build scratch
     // NDS scratch (2n merged rows, or n for the initial parents)
     uint64_t* B;                  // (W * Wc * 512) dominator bitmask, sorted order
     float* fs;                    // (m, npad) sorted objectives
@@ -138,6 +143,7 @@ struct BSelGrids {
     BGrid objectives;             // one 1024-thread block per objective per instance
     BGrid norm_prep[kMaxM + 1];   // ceil(2n / 256) blocks, instances with m objectives
     BGrid assoc[kMaxM + 1];       // ceil(2n / 256) x association chunks, instances with m objectives
+    BGrid assoc_pruned[kMaxM + 1];// ceil(2n / 4), instances with m objectives and cone tables
     int ref_bits;                 // bits of a direction index (with the two sentinels)
     int rank_bits;                // bits of rho + rank (with the sentinel)
     long long total_merged;       // sum of 2n (sort length)

@@ -20,6 +20,11 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 size_t nsga3_workspace_bytes(int n, int m, int k);
 int nsga3_select(const moea_nsga3_select_args& a);
 int set_niching(int mode, int lds_cap);
+// Cone tables of the pruned association, built once per set of directions.
+bool assoc_prunable(int k, int m);
+size_t cone_bytes(int m, int k);
+void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, double** cones);
+int cones_build(const float* w, int k, int m, void* buf, hipStream_t st);
 int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream);
 size_t associate_workspace_bytes(int ld, int m, int k);
 int associate(const float* fn, int ld, int n, int m, const float* a, const float* w, int k, float* dist,

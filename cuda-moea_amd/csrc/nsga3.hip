@@ -619,6 +619,7 @@ __device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_
 // evaluate every block that passes it.
 constexpr int kConeMaxBlocks = 1024;   // directions up to 65536
 constexpr int kConeMinK = 2048;        // below this the brute-force kernel is cheaper
+constexpr int kConeAutoMaxM = 3;
 
 __device__ __forceinline__ double wave_sum_f64(double v)
 {
@@ -1841,6 +1842,37 @@ Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
 
 }  // namespace
 
+// Automatic pruning: at most 3 objectives (the cones of higher-dimensional
+// direction sets leave too many candidate blocks to beat the tiled kernel).
+bool assoc_prunable(int k, int m)
+{
+    return m <= kConeAutoMaxM && k >= kConeMinK && ceil_div(k, 64) <= kConeMaxBlocks;
+}
+
+size_t cone_bytes(int m, int k)
+{
+    Carve c(nullptr, 0);
+    carve_cones(c, m, k);
+    return c.used;
+}
+
+void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, double** cones)
+{
+    Carve c(buf, ~size_t{0});
+    const ConeBufs B = carve_cones(c, m, k);
+    *wp = B.wp;
+    *perm = B.perm;
+    *cones = B.cones;
+}
+
+int cones_build(const float* w, int k, int m, void* buf, hipStream_t st)
+{
+    if (!w || !buf || m <= 0 || m > kMaxM || !assoc_prunable(k, m))
+        return fail(kInvalidArgument, "cones_build: invalid");
+    Carve c(buf, ~size_t{0});
+    return cones_prepare(w, k, m, carve_cones(c, m, k), st);
+}
+
 size_t nsga3_workspace_bytes(int n, int m, int k)
 {
     Carve c(nullptr, 0);
@@ -1898,7 +1930,7 @@ int associate(const float* fn, int ld, int n, int m, const float* a, const float
     const int nbk = ceil_div(k, 64);
     const bool prunable = nbk <= kConeMaxBlocks;
     if (method == 2 && !prunable) return fail(kUnsupported, "associate: too many directions for the pruned path");
-    const bool pruned = (method == 2) || (method == 0 && prunable && k >= kConeMinK);
+    const bool pruned = (method == 2) || (method == 0 && assoc_prunable(k, m));
     hipLaunchKernelGGL(assoc_meta, dim3(1), dim3(64), 0, st, L.meta, n);
     const dim3 b256(256);
     if (pruned) {
@@ -1987,7 +2019,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        L.asf_idx, L.row_max, L.meta, L.a);
     stage_end(kStSelectPrep, st);
     stage_begin(kStAssociation, st);
-    if (k >= kConeMinK && ceil_div(k, 64) <= kConeMaxBlocks) {
+    if (assoc_prunable(k, m)) {
         if (!A.directions_ready) MOEA_CHECK(cones_prepare(A.directions, k, m, L.cones, st));
         MOEA_LAUNCH_M(m, associate_pruned, dim3(ceil_div(nm, 4)), b256, 0, st, L.fn, nm, L.a, L.cones.wp, k,
                       L.cones.perm, L.cones.cones, L.meta, L.dist, L.ref, L.rho);
@@ -2135,11 +2167,29 @@ __global__ __launch_bounds__(256) void b_assoc_partial(const BInst* __restrict__
     assoc_partial_body<M>(I.fn, I.nm, I.a, I.dirs, I.k, I.meta, I.part_d2, I.part_idx, local % nbx, local / nbx);
 }
 
+// One wave per individual of the instances with cone tables.
+template <int M>
+__global__ __launch_bounds__(256) void b_assoc_pruned(const BInst* __restrict__ insts, BMap mp)
+{
+    __shared__ __align__(8) uint16_t s_list[4][kConeMaxBlocks];
+    int local;
+    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
+    const BInst& I = insts[i];
+    if (I.meta[kNRem] <= 0) return;
+    const int n_nds = I.meta[kNPrior] + I.meta[kNLast];
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int p = local * 4 + wave;
+    if (p >= n_nds) return;
+    assoc_pruned_body<M>(I.fn, I.nm, I.a, I.wp, I.k, I.cperm, I.cones, I.meta, I.dist, I.ref, p, s_list[wave],
+                         nullptr);
+}
+
 __global__ void b_assoc_final(const BInst* __restrict__ insts, BMap mp)
 {
     int local;
     const int i = bmap_find(mp, static_cast<int>(blockIdx.x), local);
     const BInst& I = insts[i];
+    if (I.cones) return;                       // associated by b_assoc_pruned
     const int t = local * 256 + static_cast<int>(threadIdx.x);
     if (t < I.nm) assoc_final_body(I.part_d2, I.part_idx, I.nchunks, I.nm, I.meta, I.dist, I.ref, t);
 }
@@ -2320,6 +2370,11 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
         if (G.assoc[m].blocks <= 0) continue;
         const dim3 grid(G.assoc[m].blocks), block(256);
         MOEA_LAUNCH_M(m, b_assoc_partial, grid, block, 0, st, L.insts, G.assoc[m].map);
+    }
+    for (int m = 1; m <= kMaxM; ++m) {
+        if (G.assoc_pruned[m].blocks <= 0) continue;
+        const dim3 grid(G.assoc_pruned[m].blocks), block(256);
+        MOEA_LAUNCH_M(m, b_assoc_pruned, grid, block, 0, st, L.insts, G.assoc_pruned[m].map);
     }
     hipLaunchKernelGGL(b_assoc_final, dim3(G.merged256.blocks), dim3(256), 0, st, L.insts, G.merged256.map);
     MOEA_HIP(hipGetLastError());

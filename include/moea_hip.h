@@ -170,9 +170,10 @@ int moea_set_niching(int mode, int lds_cap);
  * d2 by an fmaf chain; smallest index on ties; non-finite d2 never chosen,
  * ref -1 and dist +inf when none is finite), dist[i] = sqrtf(d2).  Replaces
  * execute_association (selection/nsga3/kernels/association.cu:586-608).
- * method 0: automatic; 1: every direction; 2: cone pruning (directions in
- * blocks of 64 with a bounding cone each; blocks that cannot hold the minimum
- * are skipped), same result as 1 bit for bit.  fn: (m, ld); workspace from
+ * method 0: automatic (pruning for M <= 3 and K >= 2048); 1: every
+ * direction; 2: cone pruning (directions in blocks of 64 with a bounding cone
+ * each; blocks that cannot hold the minimum are skipped), same result as 1 bit
+ * for bit.  fn: (m, ld); workspace from
  * moea_associate_workspace_bytes(ld, m, k). */
 size_t moea_associate_workspace_bytes(int ld, int m, int k);
 int moea_associate(const float* fn, int ld, int n, int m, const float* a, const float* directions, int k,

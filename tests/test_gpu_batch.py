@@ -65,10 +65,12 @@ def test_batch_mixed_objectives_equal_single_runs():
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_batch_niching_paths_equal_single_runs(mode):
     """The batch's niching paths (one workgroup per instance, chosen while every
-    2n <= 8192; device-wide sorts otherwise or in mode 2) against single runs."""
+    2n <= 8192; device-wide sorts otherwise or in mode 2), and its pruned
+    association for instances with K >= 2048, against single runs."""
     from cuda_moea import _lib
     gens = 6
-    cases = MIX[:3] + [(4200, 3, 20, 0, 11)]        # 2n = 8400: device-wide in mode 0
+    # 2n = 8400: device-wide niching in mode 0; K >= 2048: pruned association
+    cases = MIX[:3] + [(4200, 3, 20, 0, 11), (2600, 5, 15, 0, 12)]
     print(f"batch niching mode {mode}", flush=True)
     try:
         assert _lib.lib.moea_set_niching(mode, 1 << 30) == 0
