@@ -23,7 +23,26 @@ __device__ __forceinline__ void quadratic_row(const float* __restrict__ x, int d
     float acc[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) acc[k] = 0.0f;
-    for (int j = lane; j < d; j += 64) {
+    // each lane sums its elements j = lane, lane + 64, ... in that order; four
+    // of them loaded at a time
+    int j = lane;
+    for (; j + 192 < d; j += 256) {
+        float v[4], cc[M][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = xr[j + 64 * u];
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cc[k][u] = c[static_cast<size_t>(k) * d + j + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const float t = v[u] - cc[k][u];
+                acc[k] = fmaf(t, t, acc[k]);
+            }
+    }
+    for (; j < d; j += 64) {
         const float v = xr[j];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
