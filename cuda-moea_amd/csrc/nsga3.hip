@@ -458,24 +458,39 @@ __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, in
     const int n_nds = meta[kNPrior] + meta[kNLast];
     const int nb = (n_nds + 255) / 256;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = 0; j < m; ++j) {
-        float best = INFINITY, rm = -INFINITY;
-        int bi = -1;
-        for (int b = threadIdx.x; b < nb; b += kSolveThreads) {
+    // every objective's candidates in one pass over the blocks (each objective
+    // still reduced in ascending block order per thread)
+    float best[kMaxM], rm[kMaxM];
+    int bi[kMaxM];
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j) { best[j] = INFINITY; rm[j] = -INFINITY; bi[j] = -1; }
+    for (int b = threadIdx.x; b < nb; b += kSolveThreads) {
+#pragma unroll
+        for (int j = 0; j < kMaxM; ++j) {
+            if (j >= m) break;
             float v = asf_val[b * m + j];
             const int idx = asf_idx[b * m + j];
             v = (v != v) ? INFINITY : v;
-            if (idx >= 0 && v < INFINITY && (bi < 0 || v < best || (v == best && idx < bi))) { best = v; bi = idx; }
-            rm = fmaxf(rm, row_max[b * m + j]);
+            if (idx >= 0 && v < INFINITY && (bi[j] < 0 || v < best[j] || (v == best[j] && idx < bi[j]))) {
+                best[j] = v;
+                bi[j] = idx;
+            }
+            rm[j] = fmaxf(rm[j], row_max[b * m + j]);
         }
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j) {
+        if (j >= m) break;
+        float bj = best[j], rj = rm[j];
+        int ij = bi[j];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
-            const float ov = __shfl_xor(best, o);
-            const int oi = __shfl_xor(bi, o);
-            if (oi >= 0 && ov < INFINITY && (bi < 0 || ov < best || (ov == best && oi < bi))) { best = ov; bi = oi; }
-            rm = fmaxf(rm, __shfl_xor(rm, o));
+            const float ov = __shfl_xor(bj, o);
+            const int oi = __shfl_xor(ij, o);
+            if (oi >= 0 && ov < INFINITY && (ij < 0 || ov < bj || (ov == bj && oi < ij))) { bj = ov; ij = oi; }
+            rj = fmaxf(rj, __shfl_xor(rj, o));
         }
-        if (lane == 0) { s_best[wave][j] = best; s_bidx[wave][j] = bi; s_rmax[wave][j] = rm; }
+        if (lane == 0) { s_best[wave][j] = bj; s_bidx[wave][j] = ij; s_rmax[wave][j] = rj; }
     }
     __syncthreads();
     if (wave != 0) return;
