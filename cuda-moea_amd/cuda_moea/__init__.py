@@ -6,6 +6,7 @@ import torch  # noqa: F401  (device memory and streams)
 from .algorithm import Algorithm, AlgorithmBuilder, NSGA3, Population, RVEA, Result
 from .batch import BatchNSGA3
 from .serve import Server
+from .record import SnapshotRecorder, load_snapshot
 from . import problem as _problem
 from .problem import *  # noqa: F401,F403
 from ._specs import (
@@ -19,7 +20,7 @@ from ._specs import (
 __version__ = "0.2.0"
 
 __all__ = [
-    "BatchNSGA3", "Server",
+    "BatchNSGA3", "Server", "SnapshotRecorder", "load_snapshot",
     "Algorithm", "AlgorithmBuilder", "NSGA3", "RVEA", "Population", "Result",
     "CudaConfig", "SBX", "PolynomialMutation", "NoMutation",
     "RandomMating", "TournamentMating", "DasDennisDirections",

@@ -26,6 +26,7 @@ from ._specs import (
 )
 from . import kernels as K
 from .problem import DTLZ2, DTLZProblem, QuadraticCenters, SeededMLP
+from .record import SnapshotRecorder
 
 
 @dataclass(frozen=True)
@@ -66,8 +67,13 @@ class Algorithm:
             cuda.seed = seed
         if enable_warmup is not None:
             cuda.enable_warmup = enable_warmup
-        if save_data is not None:
-            raise NotImplementedError("snapshot recording (save_data) is not supported by this engine")
+        if int(save_interval) <= 0:
+            # algorithm_factory.cu:84-87 checks the interval whether or not saving is on
+            raise ValueError("saveData generation interval must be positive")
+        # python/cuda_moea/algorithm.py:72-74: any non-None save_data enables saving
+        self._recorder = (SnapshotRecorder(str(save_data or "cuda_moea_data"), int(save_interval))
+                          if save_data is not None else None)
+        self._front0: np.ndarray | None = None
         self.kind = kind
         self.cuda = cuda
         self.population_size = int(population_size)
@@ -212,6 +218,16 @@ class Algorithm:
         self._call(lib.moea_run_init)
         self._initialized = True
         self._total_ms = 0.0
+        self._front0 = None
+        if self._recorder is not None:
+            # algorithm.cu:208-214: metadata, bounds and initial directions, then generation 0
+            self._recorder.initialize(
+                algorithm_name="NSGA-III" if self.kind == "NSGA3" else "RVEA",
+                problem_name=getattr(p, "name", type(p).__name__), population_size=n, dimension=d,
+                objective_count=m, constraint_count=int(getattr(p, "constraints", 0)),
+                max_generations=self.max_generations, random_seed=int(self.cuda.seed),
+                lower_bounds=lb, upper_bounds=ub, initial_directions=self._directions0.cpu().numpy())
+            self._snapshot(0)
 
     def _run_seed(self) -> int:
         s = int(self.cuda.seed) & (2**64 - 1)
@@ -310,7 +326,24 @@ class Algorithm:
         if self.finished:
             return
         self._call(lib.moea_run_step, 1)
+        self._snapshot(self.generation)
         self._progress()
+
+    def _snapshot(self, generation: int) -> None:
+        # algorithm.cu:70-85: the current population, the current directions and
+        # the selector's auxiliary indices (NSGA-III: the population's front
+        # ranks; RVEA: the front-zero indices of the last result(), none before)
+        rec = self._recorder
+        if rec is None or not rec.should_save(generation, self.max_generations):
+            return
+        torch.cuda.current_stream(self.device).synchronize()
+        pop = self._current()
+        if self.kind == "NSGA3":
+            aux, active = self._fronts[self._args.fronts_current].cpu().numpy(), self.n
+        else:
+            aux, active = self._front0, int(self._active.item())
+        rec.save(generation, pop.x.cpu().numpy(), pop.f.cpu().numpy(), pop.cv.cpu().numpy(),
+                 self._directions.cpu().numpy(), aux, active)
 
     def _progress(self) -> None:
         g = self.generation
@@ -329,7 +362,11 @@ class Algorithm:
                 todo = self.progress_interval - self.generation % self.progress_interval
             else:
                 todo = self.max_generations - self.generation
+            if self._recorder is not None:
+                # stop at every generation the recorder saves (algorithm.cu:271-273)
+                todo = min(todo, self._recorder.next_save(self.generation, self.max_generations) - self.generation)
             self._call(lib.moea_run_step, int(todo))
+            self._snapshot(self.generation)
             self._progress()
         stop.record(stream)
         stop.synchronize()
@@ -351,6 +388,8 @@ class Algorithm:
             active = int(self._active.item())
             fr, _ = K.nds_rank(pop.f[:, :active].contiguous(), pop.cv[:active].contiguous())
             aux = torch.nonzero(fr == 0).flatten().to(torch.int32)
+            self._front0 = aux.cpu().numpy()
+        self._snapshot(self.generation)          # algorithm.cu:336-339, after finalize
         x, f, cv = pop.x, pop.f.t(), pop.cv
         if copy:
             x, f, cv, aux = x.clone(), f.contiguous(), cv.clone(), aux.clone()
@@ -368,6 +407,8 @@ class Algorithm:
         self._args.initial_x_given = 0
         self._call(lib.moea_run_init)
         self._total_ms = 0.0
+        self._front0 = None
+        self._snapshot(0)                        # algorithm.cu:430
 
     def synchronize(self) -> None:
         if self._initialized:
