@@ -94,6 +94,19 @@ int run_init(moea_run_args& a)
     return kOk;
 }
 
+namespace {
+
+// AdaptiveRVEADirections adapts after generation g when g + 1 is a multiple of
+// round(frequency * max_generations), at least 1 (reference_directions.cu:132-140).
+bool adapt_now(const moea_run_args& a, int g)
+{
+    int interval = static_cast<int>(std::lround(static_cast<double>(a.rvea_frequency) * a.max_generations));
+    if (interval < 1) interval = 1;
+    return (g + 1) % interval == 0;
+}
+
+}  // namespace
+
 int run_step(moea_run_args& a, int generations)
 {
     MOEA_CHECK(check(a));
@@ -127,8 +140,31 @@ int run_step(moea_run_args& a, int generations)
         r.stream = a.stream;
         if (r.mating == MOEA_MATING_TOURNAMENT && !r.parents_rank)
             return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
+        const moea_strategy_hooks* H = a.hooks;
         stage_begin(kStReproduce, st);
-        MOEA_CHECK(reproduce(r));
+        if (!H || (!H->mate && !H->crossover && !H->mutate)) {
+            MOEA_CHECK(reproduce(r));
+        } else {
+            // ReproductionPipeline::generate (reproduction_pipeline.cu:37-56)
+            // with the caller's strategies in place of the built-in ones
+            if (H->mate) {
+                MOEA_CHECK(H->mate(H->user, &par, a.n, a.d, a.m, r.parents_rank, r.active_device,
+                                   a.parent_indices, g, a.max_generations, a.stream));
+                r.mating = MOEA_MATING_GIVEN;
+            }
+            if (H->crossover) {
+                if (!H->mate) MOEA_CHECK(mate(r));
+                MOEA_CHECK(H->crossover(H->user, &par, a.n, a.d, a.m, a.parent_indices, ofs.x, g,
+                                        a.max_generations, a.stream));
+                if (!H->mutate && a.mutation == MOEA_MUTATION_POLYNOMIAL)
+                    MOEA_CHECK(mutate_rows(ofs.x, ofs.x, a.n, a.d, a.bounds, r.pm_eta, r.pm_prob, a.seed, g, st));
+            } else {
+                if (H->mutate) r.mutation = MOEA_MUTATION_NONE;
+                MOEA_CHECK(reproduce(r));
+            }
+            if (H->mutate)
+                MOEA_CHECK(H->mutate(H->user, &ofs, a.n, a.d, a.m, g, a.max_generations, a.stream));
+        }
         stage_end(kStReproduce, st);
         stage_begin(kStEvaluate, st);
         MOEA_CHECK(evaluate(a, ofs, st, g));
@@ -150,6 +186,13 @@ int run_step(moea_run_args& a, int generations)
             MOEA_CHECK(nsga3_select(s));
             a.directions_ready = 1;
             a.fronts_current = 1 - a.fronts_current;
+            if (H && H->update_directions) {
+                const int rc = H->update_directions(H->user, &nx, a.n, a.d, a.m, nullptr,
+                                                    const_cast<float*>(a.directions), a.k, g, a.max_generations,
+                                                    a.stream);
+                if (rc < 0) return rc;
+                if (rc > 0) a.directions_ready = 0;     // cone tables follow the new directions
+            }
         } else {
             moea_rvea_select_args s{};
             s.n = a.n; s.m = a.m; s.d = a.d; s.k = a.k;
@@ -170,10 +213,14 @@ int run_step(moea_run_args& a, int generations)
             stage_begin(kStRveaSelect, st);
             MOEA_CHECK(rvea_select(s));
             stage_end(kStRveaSelect, st);
-            // AdaptiveRVEADirections::update (reference_directions.cu:132-162)
-            int interval = static_cast<int>(std::lround(static_cast<double>(a.rvea_frequency) * a.max_generations));
-            if (interval < 1) interval = 1;
-            if ((g + 1) % interval == 0) {
+            if (H && H->update_directions) {
+                const int rc = H->update_directions(H->user, &nx, a.n, a.d, a.m, a.active_device,
+                                                    const_cast<float*>(a.directions), a.k, g, a.max_generations,
+                                                    a.stream);
+                if (rc < 0) return rc;
+                if (rc > 0) MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
+            } else if (adapt_now(a, g)) {
+                // AdaptiveRVEADirections::update (reference_directions.cu:132-162)
                 stage_begin(kStRveaAdapt, st);
                 MOEA_CHECK(rvea_adapt(a.directions0, nx.f, a.n, a.active_device, a.m, a.k,
                                       const_cast<float*>(a.directions), a.adapt_scratch, st));

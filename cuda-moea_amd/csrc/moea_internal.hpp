@@ -32,6 +32,9 @@ int associate(const float* fn, int ld, int n, int m, const float* a, const float
 
 // reproduce.hip
 int reproduce(const moea_reproduce_args& a);
+int mate(const moea_reproduce_args& a);          // the mating kernel alone
+int mutate_rows(const float* x_in, float* x_out, int n, int d, const float* bounds, float pm_eta, float pm_prob,
+                uint64_t seed, int generation, hipStream_t stream);
 int init_population(float* x, int n, int d, const float* bounds, uint64_t seed, hipStream_t stream);
 
 // evaluate.hip

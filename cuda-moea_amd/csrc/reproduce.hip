@@ -98,6 +98,7 @@ struct RepArgs {
     const float* bounds;
     float eta_c, prob_c, copy_c;
     int mutate;
+    int winner_mode;     // 0: winners as computed, 1: caller's winners clamped into [0, n), 2: row itself
     float eta_m, rate_m;
     const float* sbx_u; const float* sbx_p; const int32_t* sbx_sign; const float* sbx_copy;
     const float* pm_gate; const float* pm_delta;
@@ -105,6 +106,14 @@ struct RepArgs {
     uint32_t gen;
     float* ox;
 };
+
+// Parent row crossed into offspring row `row`.
+__device__ __forceinline__ int winner_of(const RepArgs& R, int row)
+{
+    if (R.winner_mode == 2) return row;
+    const int w = R.winners[row];
+    return R.winner_mode == 1 ? min(max(w, 0), R.n - 1) : w;
+}
 
 // Element e of the (pair, variable) space; the odd tail row (n odd) copies its
 // parent (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
@@ -123,7 +132,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
             pair = static_cast<int>(e / R.d);
             j = static_cast<int>(e - static_cast<long long>(pair) * R.d);
         }
-        const int w0 = R.winners[2 * pair], w1 = R.winners[2 * pair + 1];
+        const int w0 = winner_of(R, 2 * pair), w1 = winner_of(R, 2 * pair + 1);
         const float x1 = R.px[static_cast<size_t>(w0) * R.d + j];
         const float x2 = R.px[static_cast<size_t>(w1) * R.d + j];
         const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
@@ -163,7 +172,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
         const int j = static_cast<int>(e - total_pair);
         const int row = R.n - 1;
         const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
-        float v = R.px[static_cast<size_t>(R.winners[row]) * R.d + j];
+        float v = R.px[static_cast<size_t>(winner_of(R, row)) * R.d + j];
         if (R.mutate) {
             const size_t ra = static_cast<size_t>(row) * R.d + j;
             float g, dd = 0.0f;
@@ -266,6 +275,44 @@ float annealed(float begin, float end, int generation, int total)
     return annealed_hd(begin, end, generation, total);
 }
 
+int mate(const moea_reproduce_args& A)
+{
+    if (A.mating != MOEA_MATING_TOURNAMENT && A.mating != MOEA_MATING_RANDOM)
+        return fail(kInvalidArgument, "mate: unknown mating kind");
+    if (A.mating == MOEA_MATING_TOURNAMENT && (!A.parents_rank || !A.parents_cv))
+        return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
+    const moea_draws* dr = A.draws;
+    hipLaunchKernelGGL(mating_kernel, dim3(ceil_div(A.n, 256)), dim3(256), 0, static_cast<hipStream_t>(A.stream),
+                       A.n, A.active, A.active_device, A.mating, A.parents_rank, A.parents_cv,
+                       dr ? dr->mate_a : nullptr, dr ? dr->mate_b : nullptr, A.seed,
+                       static_cast<uint32_t>(A.generation), A.parent_indices);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
+int mutate_rows(const float* x_in, float* x_out, int n, int d, const float* bounds, float pm_eta, float pm_prob,
+                uint64_t seed, int generation, hipStream_t st)
+{
+    // The element pass of reproduce with every row its own parent and crossover
+    // switched off (probability 0 copies), so each row gets the mutation it
+    // would get after SBX in the same generation.  x_in may equal x_out: every
+    // element is read and written by one thread.
+    if (n <= 0 || d <= 0 || !x_in || !x_out || !bounds)
+        return fail(kInvalidArgument, "mutate_rows: invalid arguments");
+    RepArgs R{};
+    R.n = n; R.d = d; R.pairs = n / 2;
+    R.dd = fast_div_make(static_cast<uint32_t>(d));
+    R.small = (static_cast<long long>(R.pairs) + 1) * d < (1ll << 31);
+    R.px = x_in; R.winners = nullptr; R.bounds = bounds; R.winner_mode = 2;
+    R.eta_c = 1.0f; R.prob_c = 0.0f; R.copy_c = 0.0f;
+    R.mutate = 1; R.eta_m = pm_eta; R.rate_m = pm_prob / static_cast<float>(d);
+    R.seed = seed; R.gen = static_cast<uint32_t>(generation); R.ox = x_out;
+    const long long total = static_cast<long long>(R.pairs) * d + ((n & 1) ? d : 0);
+    hipLaunchKernelGGL(sbx_pm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, R);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
+}
+
 int reproduce(const moea_reproduce_args& A)
 {
     if (A.n <= 0 || A.d <= 0 || A.active <= 0 || A.active > A.n)
@@ -274,16 +321,14 @@ int reproduce(const moea_reproduce_args& A)
         return fail(kInvalidArgument, "reproduce: null pointer");
     if (A.mating == MOEA_MATING_TOURNAMENT && (!A.parents_rank || !A.parents_cv))
         return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
-    if (A.mating != MOEA_MATING_TOURNAMENT && A.mating != MOEA_MATING_RANDOM)
+    if (A.mating != MOEA_MATING_TOURNAMENT && A.mating != MOEA_MATING_RANDOM && A.mating != MOEA_MATING_GIVEN)
         return fail(kInvalidArgument, "reproduce: unknown mating kind");
     hipStream_t st = static_cast<hipStream_t>(A.stream);
     const moea_draws* dr = A.draws;
     const uint32_t gen = static_cast<uint32_t>(A.generation);
-    hipLaunchKernelGGL(mating_kernel, dim3(ceil_div(A.n, 256)), dim3(256), 0, st, A.n, A.active,
-                       A.active_device, A.mating, A.parents_rank, A.parents_cv, dr ? dr->mate_a : nullptr,
-                       dr ? dr->mate_b : nullptr, A.seed, gen, A.parent_indices);
-    MOEA_HIP(hipGetLastError());
+    if (A.mating != MOEA_MATING_GIVEN) MOEA_CHECK(mate(A));
     RepArgs R{};
+    R.winner_mode = (A.mating == MOEA_MATING_GIVEN) ? 1 : 0;
     R.n = A.n; R.d = A.d; R.pairs = A.n / 2;
     R.dd = fast_div_make(static_cast<uint32_t>(A.d));
     R.small = (static_cast<long long>(R.pairs) + 1) * R.d < (1ll << 31);

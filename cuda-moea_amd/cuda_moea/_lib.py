@@ -13,7 +13,7 @@ _PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = _PKG_ROOT / "lib" / "libmoea_hip.so"
 
 MOEA_OK = 0
-MATING_TOURNAMENT, MATING_RANDOM = 0, 1
+MATING_TOURNAMENT, MATING_RANDOM, MATING_GIVEN = 0, 1, 2
 MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
 ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
 PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK, PROBLEM_DTLZ1, PROBLEM_DTLZ2 = 0, 1, 2, 3, 4
@@ -76,6 +76,23 @@ class ProblemDesc(ctypes.Structure):
     ]
 
 
+_POP_P = ctypes.POINTER(PopulationC)
+MATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                           c_int, c_int, c_void_p)
+CROSSOVER_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                c_int, c_void_p)
+MUTATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_int, c_int, c_void_p)
+DIRECTIONS_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                 c_int, c_int, c_void_p)
+
+
+class StrategyHooks(ctypes.Structure):
+    _fields_ = [
+        ("user", c_void_p), ("mate", MATE_FN), ("crossover", CROSSOVER_FN), ("mutate", MUTATE_FN),
+        ("update_directions", DIRECTIONS_FN),
+    ]
+
+
 class RunArgs(ctypes.Structure):
     _fields_ = [
         ("algorithm", c_int), ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int),
@@ -90,6 +107,7 @@ class RunArgs(ctypes.Structure):
         ("initial_x_given", c_int), ("problem", ProblemDesc), ("generation", c_int),
         ("directions_ready", c_int),
         ("workspace", c_void_p), ("workspace_bytes", c_size_t), ("stream", c_void_p),
+        ("hooks", ctypes.POINTER(StrategyHooks)),
     ]
 
 

@@ -21,8 +21,9 @@ from . import _lib
 from ._lib import check, lib
 from ._specs import (
     AdaptiveRVEADirections, CudaConfig, DasDennisDirections, NoMutation,
-    NSGA3EnvironmentSelector, PolynomialMutation, PythonProblem, RandomMating,
-    RVEAEnvironmentSelector, SBX, TournamentMating, UserDefinedDirections,
+    NSGA3EnvironmentSelector, PolynomialMutation, PythonCrossover, PythonMating, PythonMutation,
+    PythonProblem, PythonReferenceDirections, RandomMating, RVEAEnvironmentSelector, SBX,
+    TournamentMating, UserDefinedDirections,
 )
 from . import kernels as K
 from .problem import DTLZ2, DTLZProblem, QuadraticCenters, SeededMLP
@@ -41,6 +42,15 @@ class Result(Population):
     auxiliary_indices: Any
     active_count: int
     total_ms: float
+
+
+def _checked(value: Any, name: str, device: int, dtype: torch.dtype) -> torch.Tensor:
+    # adapters.cpp:101-112: a tensor on the run's device, converted to dtype
+    if not isinstance(value, torch.Tensor):
+        raise ValueError(f"{name} must be a torch.Tensor")
+    if not value.is_cuda or value.device.index != device:
+        raise ValueError(f"{name} must be on the algorithm CUDA device")
+    return value.to(dtype).contiguous()
 
 
 def _unsupported(obj: Any, what: str) -> NotImplementedError:
@@ -97,17 +107,19 @@ class Algorithm:
 
     # ------------------------------------------------------------ config ----
     def _validate_strategies(self) -> None:
-        if not isinstance(self.mating, (TournamentMating, RandomMating)):
+        if not isinstance(self.mating, (TournamentMating, RandomMating, PythonMating)):
             raise _unsupported(self.mating, "mating")
-        if not isinstance(self.crossover, SBX):
+        if isinstance(self.crossover, SBX):
+            self.crossover.validate()
+        elif not isinstance(self.crossover, PythonCrossover):
             raise _unsupported(self.crossover, "crossover")
-        self.crossover.validate()
         if isinstance(self.mutation, PolynomialMutation):
             self.mutation.validate()
-        elif not isinstance(self.mutation, NoMutation):
+        elif not isinstance(self.mutation, (NoMutation, PythonMutation)):
             raise _unsupported(self.mutation, "mutation")
         rd = self.reference_directions
-        if not isinstance(rd, (DasDennisDirections, AdaptiveRVEADirections, UserDefinedDirections)):
+        if not isinstance(rd, (DasDennisDirections, AdaptiveRVEADirections, UserDefinedDirections,
+                               PythonReferenceDirections)):
             raise _unsupported(rd, "reference directions")
         if isinstance(rd, DasDennisDirections) and rd.partitions < 0:
             raise ValueError("Das-Dennis partitions cannot be negative")
@@ -191,8 +203,9 @@ class Algorithm:
         else:
             a.mutation = _lib.MUTATION_NONE
         c = self.crossover
-        a.sbx_eta[0], a.sbx_eta[1] = c.eta_initial, c.eta_final
-        a.sbx_prob, a.sbx_copy_prob = c.probability, c.variable_copy_probability
+        if isinstance(c, SBX):
+            a.sbx_eta[0], a.sbx_eta[1] = c.eta_initial, c.eta_final
+            a.sbx_prob, a.sbx_copy_prob = c.probability, c.variable_copy_probability
         if self.kind == "RVEA":
             a.rvea_alpha = self.environment_selector.alpha
             freq = getattr(self.reference_directions, "frequency", 1.0)
@@ -201,7 +214,9 @@ class Algorithm:
             # fixed (user-defined) directions never adapt: an interval past the run
             a.rvea_frequency = freq if isinstance(self.reference_directions, AdaptiveRVEADirections) else 1e9
         a.seed = self._run_seed()
+        self._by_ptr = {pop.x.data_ptr(): pop for pop in self._pops}
         self._setup_problem(a)
+        self._setup_hooks(a)
         if self.initial_population is not None:
             init = torch.as_tensor(self.initial_population, dtype=torch.float32)
             if tuple(init.shape) != (n, d) and init.numel() != n * d:
@@ -235,6 +250,13 @@ class Algorithm:
 
     def _directions_host(self, n: int, m: int) -> np.ndarray:
         rd = self.reference_directions
+        if isinstance(rd, PythonReferenceDirections):
+            # adapters.cpp:294-301, 325-335: (count, objective_count) on the device
+            out = _checked(rd.initialize(n, m, {"device": self.cuda.device_id}), "reference directions",
+                           self.cuda.device_id, torch.float32)
+            if out.dim() != 2 or out.shape[1] != m or out.shape[0] == 0:
+                raise ValueError("reference directions must have shape (count, objective_count)")
+            return np.ascontiguousarray(out.t().cpu().numpy())
         if isinstance(rd, UserDefinedDirections):
             vals = np.asarray(rd.values, dtype=np.float32)
             if vals.ndim != 2 or vals.shape[0] != m or vals.size == 0:
@@ -270,14 +292,114 @@ class Algorithm:
             # built-in objective kernels, as the reference's DTLZ problems are native
             pd.kind = _lib.PROBLEM_DTLZ1 if p._kind == "DTLZ1" else _lib.PROBLEM_DTLZ2
         else:
-            self._by_ptr = {}
-            for pop in self._pops:
-                self._by_ptr[pop.x.data_ptr()] = pop
             if hasattr(p, "initialize"):
                 p.initialize({"device": self.cuda.device_id})
             self._callback = _lib.EVALUATE_FN(self._evaluate_python)
             pd.kind, pd.callback = _lib.PROBLEM_CALLBACK, self._callback
         a.problem = pd
+
+    def _context(self, generation: int, max_generations: int) -> dict[str, Any]:
+        return {"generation": generation, "max_generations": max_generations, "device": self.cuda.device_id}
+
+    def _pop_dict(self, pop_ptr) -> dict[str, Any]:
+        # adapters.cpp:80-88: objectives as an (n, m) view of the (m, n) storage
+        pop = self._by_ptr[pop_ptr.contents.x]
+        return {"variables": pop.x, "objectives": pop.f.t(), "constraints": pop.cv,
+                "bounds": self._bounds.view(self.d, 2)}
+
+    def _setup_hooks(self, a: _lib.RunArgs) -> None:
+        """Python strategies as native hooks of the generation loop
+        (adapters.cpp:196-338); built-in strategies stay on the GPU."""
+        info = {"name": "NSGA-III" if self.kind == "NSGA3" else "RVEA", "population_size": self.n,
+                "dimension": self.d, "objective_count": self.m, "max_generations": self.max_generations,
+                "device": self.cuda.device_id}
+        h = _lib.StrategyHooks()
+        used = False
+        if isinstance(self.mating, PythonMating):
+            h.mate, used = _lib.MATE_FN(self._mate_python), True
+        if isinstance(self.crossover, PythonCrossover):
+            h.crossover, used = _lib.CROSSOVER_FN(self._crossover_python), True
+        if isinstance(self.mutation, PythonMutation):
+            h.mutate, used = _lib.MUTATE_FN(self._mutate_python), True
+            a.mutation = _lib.MUTATION_NONE
+        rd = self.reference_directions
+        if isinstance(rd, PythonReferenceDirections) and hasattr(rd, "update"):
+            h.update_directions, used = _lib.DIRECTIONS_FN(self._directions_python), True
+        for strategy in (self.mating, self.crossover, self.mutation):
+            if isinstance(strategy, (PythonMating, PythonCrossover, PythonMutation)) and hasattr(strategy, "initialize"):
+                strategy.initialize(dict(info))
+        self._hooks = h
+        a.hooks = ctypes.pointer(h) if used else None
+
+    def _guarded(self, body) -> int:
+        try:
+            return body()
+        except BaseException as exc:  # surfaced after the native call returns
+            self._error = exc
+            return -1
+
+    def _mate_python(self, user, parents, n, d, m, rank, active_device, out, generation, max_generations,
+                     stream) -> int:
+        def body() -> int:
+            # adapters.cpp:207-224
+            state: dict[str, Any] = {}
+            if rank:
+                state["rank"] = next(t for t in self._fronts if t.data_ptr() == rank)
+            state["active_count"] = int(self._active.item()) if active_device else n
+            idx = _checked(self.mating.select(self._pop_dict(parents), state,
+                                              self._context(generation, max_generations)),
+                           "parent_indices", self.cuda.device_id, torch.int32)
+            if idx.numel() != n:
+                raise ValueError("parent_indices has the wrong number of elements")
+            idx = idx.reshape(n)
+            if bool(((idx < 0) | (idx >= n)).any()):
+                raise ValueError("parent_indices must lie in [0, population_size)")
+            self._parent_idx.copy_(idx)
+            return 0
+        return self._guarded(body)
+
+    def _crossover_python(self, user, parents, n, d, m, indices, offspring_x, generation, max_generations,
+                          stream) -> int:
+        def body() -> int:
+            # adapters.cpp:238-251
+            out = _checked(self.crossover.apply(self._pop_dict(parents), self._parent_idx,
+                                                self._context(generation, max_generations)),
+                           "offspring variables", self.cuda.device_id, torch.float32)
+            if tuple(out.shape) != (n, d):
+                raise ValueError("crossover output must have shape (population_size, dimension)")
+            self._by_ptr[offspring_x].x.copy_(out)
+            return 0
+        return self._guarded(body)
+
+    def _mutate_python(self, user, offspring, n, d, m, generation, max_generations, stream) -> int:
+        def body() -> int:
+            # adapters.cpp:268-281: None leaves the offspring as they are
+            out = self.mutation.mutate(self._pop_dict(offspring), self._context(generation, max_generations))
+            if out is not None:
+                out = _checked(out, "mutated variables", self.cuda.device_id, torch.float32)
+                if tuple(out.shape) != (n, d):
+                    raise ValueError("mutation output must have shape (population_size, dimension)")
+                self._by_ptr[offspring.contents.x].x.copy_(out)
+            return 0
+        return self._guarded(body)
+
+    def _directions_python(self, user, nxt, n, d, m, active_device, directions, k, generation,
+                           max_generations, stream) -> int:
+        def body() -> int:
+            # adapters.cpp:309-318, 325-335: the same count, (count, objective_count)
+            active = int(self._active.item()) if active_device else n
+            out = self.reference_directions.update(self._pop_dict(nxt), self._directions.t(), active,
+                                                   self._context(generation, max_generations))
+            if out is None:
+                return 0
+            out = _checked(out, "reference directions", self.cuda.device_id, torch.float32)
+            if out.dim() != 2 or out.shape[1] != m:
+                raise ValueError("reference directions must have shape (count, objective_count)")
+            if out.shape[0] != k:
+                raise ValueError("updated reference directions cannot change count")
+            self._directions.copy_(out.t())
+            return 1
+        return self._guarded(body)
 
     def _evaluate_python(self, user, x, n, d, m, f, cv, generation, max_generations, stream) -> int:
         # adapters.cpp:115-173: evaluate on the run's stream, copy (n, m) -> (m, n)
@@ -401,6 +523,9 @@ class Algorithm:
         self.synchronize()
         self._resets += 1
         self._directions.copy_(self._directions0)
+        rd = self.reference_directions
+        if isinstance(rd, PythonReferenceDirections) and hasattr(rd, "reset"):
+            rd.reset()                           # adapters.cpp:319-324
         self._args.seed = self._run_seed()
         self._args.current = 0
         self._args.fronts_current = 0

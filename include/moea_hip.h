@@ -190,6 +190,11 @@ int moea_ideal_point(const float* f, int ldf, int n, int m, float* ideal, void* 
 
 #define MOEA_MATING_TOURNAMENT 0
 #define MOEA_MATING_RANDOM 1
+/* The caller's mating: parent_indices is an input holding n indices in
+ * [0, n), read by SBX as they stand (PythonMating, adapters.cpp:196-227).
+ * An index outside [0, n) is clamped into it so that no read leaves the
+ * parents; the host adapter rejects such indices before the call. */
+#define MOEA_MATING_GIVEN 2
 #define MOEA_MUTATION_NONE 0
 #define MOEA_MUTATION_POLYNOMIAL 1
 
@@ -317,6 +322,41 @@ typedef struct moea_problem_desc {
     void* user;
 } moea_problem_desc;
 
+/* Strategy hooks: the C form of the reference's strategy plugins
+ * (IMatingSelector, ICrossoverOperator, IMutationOperator,
+ * IReferenceDirectionProvider; Python adapters in adapters.cpp:196-338).
+ * A NULL hook keeps the built-in strategy.  Every hook enqueues its work on
+ * `stream` and returns MOEA_OK or an error, which moea_run_step returns. */
+
+/* Mating: write n parent indices (pairs 2i, 2i+1 are crossed).  rank is the
+ * parents' NSGA-III front ranks (NULL under RVEA); active_device the RVEA
+ * active count (NULL under NSGA-III: all n). */
+typedef int (*moea_mate_fn)(void* user, const moea_population* parents, int n, int d, int m,
+                            const int32_t* rank, const int32_t* active_device, int32_t* parent_indices,
+                            int generation, int max_generations, void* stream);
+/* Crossover: write the (n, d) offspring variables from the parents and the
+ * mating's parent_indices. */
+typedef int (*moea_crossover_fn)(void* user, const moea_population* parents, int n, int d, int m,
+                                 const int32_t* parent_indices, float* offspring_x, int generation,
+                                 int max_generations, void* stream);
+/* Mutation: update offspring->x (n, d) in place, before evaluation. */
+typedef int (*moea_mutate_fn)(void* user, const moea_population* offspring, int n, int d, int m,
+                              int generation, int max_generations, void* stream);
+/* Reference-direction update after each selection (the survivors `next`):
+ * may rewrite directions (m, k) in place.  Returns 1 if it did, 0 if not, a
+ * negative error otherwise.  Replaces RVEA's built-in adaptation. */
+typedef int (*moea_directions_fn)(void* user, const moea_population* next, int n, int d, int m,
+                                  const int32_t* active_device, float* directions, int k, int generation,
+                                  int max_generations, void* stream);
+
+typedef struct moea_strategy_hooks {
+    void* user;
+    moea_mate_fn mate;                     /* NULL: args.mating */
+    moea_crossover_fn crossover;           /* NULL: SBX */
+    moea_mutate_fn mutate;                 /* NULL: args.mutation */
+    moea_directions_fn update_directions;  /* NULL: RVEA adaptation / fixed directions */
+} moea_strategy_hooks;
+
 /* State of one run.  The caller owns every buffer; moea_run_* update the
  * integer fields (current, fronts_current, generation). */
 typedef struct moea_run_args {
@@ -349,6 +389,7 @@ typedef struct moea_run_args {
     void* workspace;
     size_t workspace_bytes;
     void* stream;
+    const moea_strategy_hooks* hooks;  /* NULL: every strategy built in */
 } moea_run_args;
 
 size_t moea_run_workspace_bytes(int algorithm, int n, int m, int k);

@@ -38,7 +38,7 @@ def test_python_binding_covers_header():
 
 def test_host_side_calls():
     import cuda_moea._lib as L
-    assert L.lib.moea_abi_version() == 2
+    assert L.lib.moea_abi_version() == 3
     assert L.lib.moea_das_dennis(92, 3, 0, 0, None, None) == 91
     assert L.lib.moea_das_dennis(32768, 3, 0, 0, None, None) == 32640
     assert abs(L.lib.moea_annealed(30.0, 10.0, 50, 100) - 20.0) < 1e-6
@@ -50,7 +50,7 @@ def test_host_side_calls():
 STRUCTS = {"Draws": "moea_draws", "PopulationC": "moea_population", "NSGA3SelectArgs": "moea_nsga3_select_args",
            "ReproduceArgs": "moea_reproduce_args", "RVEASelectArgs": "moea_rvea_select_args",
            "ProblemDesc": "moea_problem_desc", "RunArgs": "moea_run_args", "BatchProblem": "moea_batch_problem",
-           "ServeConfig": "moea_serve_config"}
+           "ServeConfig": "moea_serve_config", "StrategyHooks": "moea_strategy_hooks"}
 
 
 def test_ctypes_structs_match_header_layout(tmp_path):
@@ -100,7 +100,11 @@ def test_api_validation_errors():
     with pytest.raises(ValueError):
         cm.RVEA(environment_selector=cm.RVEAEnvironmentSelector(alpha=0.0))
     with pytest.raises(NotImplementedError):
-        cm.NSGA3(mating=cm.PythonMating())
+        cm.NSGA3(environment_selector=cm.PythonEnvironmentSelector())
+    # the reproduction and direction strategies take Python objects (adapters.cpp:196-338)
+    alg = cm.NSGA3(mating=cm.PythonMating(), crossover=cm.PythonCrossover(), mutation=cm.PythonMutation(),
+                   reference_directions=cm.PythonReferenceDirections())
+    assert not alg.initialized
     b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(
         cm.QuadraticCenters(12, 3))
     alg = b.build()
