@@ -55,6 +55,8 @@ int check(const moea_run_args& a)
     if (a.algorithm == MOEA_ALGORITHM_RVEA &&
         (!a.gamma || !a.directions0 || !a.active_device || !a.adapt_scratch))
         return fail(kInvalidArgument, "moea_run: RVEA needs gamma, initial directions and active buffers");
+    if (a.hooks && a.hooks->select && (!a.active_device || !a.fronts[0] || !a.fronts[1]))
+        return fail(kInvalidArgument, "moea_run: a selection hook needs fronts and active buffers");
     return kOk;
 }
 
@@ -120,10 +122,13 @@ int run_step(moea_run_args& a, int generations)
         moea_reproduce_args r{};
         r.n = a.n; r.d = a.d;
         r.active = a.n;
-        r.active_device = (a.algorithm == MOEA_ALGORITHM_RVEA) ? a.active_device : nullptr;
+        const moea_strategy_hooks* H = a.hooks;
+        const bool hook_select = H && H->select;
+        // a caller's selector supplies the mating state (rank and active count)
+        r.active_device = (a.algorithm == MOEA_ALGORITHM_RVEA || hook_select) ? a.active_device : nullptr;
         r.mating = a.mating;
         r.parents_x = par.x;
-        r.parents_rank = (a.algorithm == MOEA_ALGORITHM_NSGA3) ? a.fronts[a.fronts_current] : nullptr;
+        r.parents_rank = (a.algorithm == MOEA_ALGORITHM_NSGA3 || hook_select) ? a.fronts[a.fronts_current] : nullptr;
         r.parents_cv = par.cv;
         r.bounds = a.bounds;
         r.sbx_eta = annealed(a.sbx_eta[0], a.sbx_eta[1], g, a.max_generations);
@@ -140,7 +145,6 @@ int run_step(moea_run_args& a, int generations)
         r.stream = a.stream;
         if (r.mating == MOEA_MATING_TOURNAMENT && !r.parents_rank)
             return fail(kInvalidArgument, "TournamentMating requires a rank array for every parent");
-        const moea_strategy_hooks* H = a.hooks;
         stage_begin(kStReproduce, st);
         if (!H || (!H->mate && !H->crossover && !H->mutate)) {
             MOEA_CHECK(reproduce(r));
@@ -169,7 +173,20 @@ int run_step(moea_run_args& a, int generations)
         stage_begin(kStEvaluate, st);
         MOEA_CHECK(evaluate(a, ofs, st, g));
         stage_end(kStEvaluate, st);
-        if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
+        if (hook_select) {
+            MOEA_CHECK(H->select(H->user, &par, &ofs, &nx, a.n, a.d, a.m, g, a.max_generations, a.stream));
+            if (H->update_directions) {
+                const int rc = H->update_directions(H->user, &nx, a.n, a.d, a.m, a.active_device,
+                                                    const_cast<float*>(a.directions), a.k, g, a.max_generations,
+                                                    a.stream);
+                if (rc < 0) return rc;
+                if (rc > 0) a.directions_ready = 0;
+            } else if (a.algorithm == MOEA_ALGORITHM_RVEA && adapt_now(a, g)) {
+                MOEA_CHECK(rvea_adapt(a.directions0, nx.f, a.n, a.active_device, a.m, a.k,
+                                      const_cast<float*>(a.directions), a.adapt_scratch, st));
+                MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
+            }
+        } else if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
             moea_nsga3_select_args s{};
             s.n = a.n; s.m = a.m; s.d = a.d; s.k = a.k;
             s.parents = par; s.offspring = ofs; s.next = nx;

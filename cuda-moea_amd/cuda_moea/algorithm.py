@@ -22,7 +22,7 @@ from ._lib import check, lib
 from ._specs import (
     AdaptiveRVEADirections, CudaConfig, DasDennisDirections, NoMutation,
     NSGA3EnvironmentSelector, PolynomialMutation, PythonCrossover, PythonMating, PythonMutation,
-    PythonProblem, PythonReferenceDirections, RandomMating, RVEAEnvironmentSelector, SBX,
+    PythonEnvironmentSelector, PythonProblem, PythonReferenceDirections, RandomMating, RVEAEnvironmentSelector, SBX,
     TournamentMating, UserDefinedDirections,
 )
 from . import kernels as K
@@ -124,7 +124,10 @@ class Algorithm:
         if isinstance(rd, DasDennisDirections) and rd.partitions < 0:
             raise ValueError("Das-Dennis partitions cannot be negative")
         sel = self.environment_selector
-        if self.kind == "NSGA3":
+        if isinstance(sel, PythonEnvironmentSelector):
+            if self.kind == "NSGA3" and isinstance(rd, AdaptiveRVEADirections):
+                raise ValueError("NSGA-III takes Das-Dennis or user-defined directions")
+        elif self.kind == "NSGA3":
             if not isinstance(sel, NSGA3EnvironmentSelector):
                 raise _unsupported(sel, "NSGA-III environment selector")
             if not sel.sparse_ratio > 0.0:
@@ -207,7 +210,9 @@ class Algorithm:
             a.sbx_eta[0], a.sbx_eta[1] = c.eta_initial, c.eta_final
             a.sbx_prob, a.sbx_copy_prob = c.probability, c.variable_copy_probability
         if self.kind == "RVEA":
-            a.rvea_alpha = self.environment_selector.alpha
+            sel = self.environment_selector
+            # the built-in selector's penalty exponent; a Python selector computes its own
+            a.rvea_alpha = sel.alpha if isinstance(sel, RVEAEnvironmentSelector) else 2.0
             freq = getattr(self.reference_directions, "frequency", 1.0)
             if isinstance(self.reference_directions, AdaptiveRVEADirections) and not 0.0 < freq <= 1.0:
                 raise ValueError("RVEA reference adaptation frequency must be in (0, 1]")
@@ -231,6 +236,8 @@ class Algorithm:
         a.stream = torch.cuda.current_stream(dev).cuda_stream
         self._args = a
         self._call(lib.moea_run_init)
+        if self._python_selector:
+            self._selector_start()
         self._initialized = True
         self._total_ms = 0.0
         self._front0 = None
@@ -302,8 +309,10 @@ class Algorithm:
         return {"generation": generation, "max_generations": max_generations, "device": self.cuda.device_id}
 
     def _pop_dict(self, pop_ptr) -> dict[str, Any]:
+        return self._pop_view(self._by_ptr[pop_ptr.contents.x])
+
+    def _pop_view(self, pop: K.Population) -> dict[str, Any]:
         # adapters.cpp:80-88: objectives as an (n, m) view of the (m, n) storage
-        pop = self._by_ptr[pop_ptr.contents.x]
         return {"variables": pop.x, "objectives": pop.f.t(), "constraints": pop.cv,
                 "bounds": self._bounds.view(self.d, 2)}
 
@@ -325,11 +334,108 @@ class Algorithm:
         rd = self.reference_directions
         if isinstance(rd, PythonReferenceDirections) and hasattr(rd, "update"):
             h.update_directions, used = _lib.DIRECTIONS_FN(self._directions_python), True
+        self._sel_state: dict[str, Any] = {}
+        if self._python_selector:
+            h.select, used = _lib.SELECT_FN(self._select_python), True
         for strategy in (self.mating, self.crossover, self.mutation):
             if isinstance(strategy, (PythonMating, PythonCrossover, PythonMutation)) and hasattr(strategy, "initialize"):
                 strategy.initialize(dict(info))
+        if self._python_selector:
+            # adapters.cpp:352-367
+            sel_info = {key: info[key] for key in ("name", "population_size", "dimension", "objective_count",
+                                                   "max_generations")}
+            self.environment_selector.initialize(sel_info, self._directions.t(), {"device": self.cuda.device_id})
         self._hooks = h
         a.hooks = ctypes.pointer(h) if used else None
+
+    @property
+    def _python_selector(self) -> bool:
+        return isinstance(self.environment_selector, PythonEnvironmentSelector)
+
+    def _selector_start(self) -> None:
+        """A Python selector's prepare() on the initial parents
+        (algorithm.cu:196-206, adapters.cpp:369-375)."""
+        self._sel_state = {}
+        self._active.fill_(self.n)
+        out = self.environment_selector.prepare(self._pop_view(self._current()), self._context(0, self.max_generations))
+        self._update_state(out)
+        self._check_mating_state()
+
+    def _update_state(self, out: Any) -> None:
+        # adapters.cpp:425-433: each key present replaces that part of the state
+        if not isinstance(out, dict):
+            return
+        dev = self.cuda.device_id
+        st = self._sel_state
+        for key, dtype in (("rank", torch.int32), ("reference_index", torch.int32), ("score", torch.float32),
+                           ("result_indices", torch.int32)):
+            if key in out:
+                st[key] = _checked(out[key], key, dev, dtype)
+        if "active_count" in out:
+            st["active_count"] = int(out["active_count"])
+            self._active.fill_(st["active_count"])
+        rank = st.get("rank")
+        if rank is not None and rank.numel() == self.n:
+            self._fronts[self._args.fronts_current].copy_(rank.reshape(self.n))
+
+    def _check_mating_state(self) -> None:
+        # mating_selection.cu: the tournament compares the selector's ranks
+        rank = self._sel_state.get("rank")
+        if isinstance(self.mating, TournamentMating) and (rank is None or rank.numel() != self.n):
+            raise ValueError("TournamentMating requires a rank array for every parent")
+
+    def _select_python(self, user, parents, offspring, nxt, n, d, m, generation, max_generations, stream) -> int:
+        def body() -> int:
+            # adapters.cpp:377-404
+            pd, od = self._pop_dict(parents), self._pop_dict(offspring)
+            out = self.environment_selector.select(pd, od, self._context(generation, max_generations))
+            if not isinstance(out, dict):
+                raise ValueError("environment select must return a dict")
+            dst = self._by_ptr[nxt.contents.x]
+            if "indices" in out:
+                idx = _checked(out["indices"], "selected indices", self.cuda.device_id, torch.int64)
+                if idx.numel() != n:
+                    raise ValueError("selected indices must contain population_size values")
+                idx = idx.reshape(n)
+                if bool(((idx < 0) | (idx >= 2 * n)).any()):
+                    raise ValueError("selected indices must lie in [0, 2 * population_size)")
+                dst.x.copy_(torch.cat([pd["variables"], od["variables"]]).index_select(0, idx))
+                dst.f.copy_(torch.cat([pd["objectives"], od["objectives"]]).index_select(0, idx).t())
+                dst.cv.copy_(torch.cat([pd["constraints"], od["constraints"]]).index_select(0, idx))
+            else:
+                x = _checked(out.get("variables"), "variables", self.cuda.device_id, torch.float32)
+                if tuple(x.shape) != (n, d):
+                    raise ValueError("variables must have shape (population_size, dimension)")
+                dst.x.copy_(x)
+                self._copy_evaluation(out, dst, n, m)
+            self._update_state(out)
+            if generation + 1 < max_generations:
+                self._check_mating_state()
+            return 0
+        return self._guarded(body)
+
+    def _copy_evaluation(self, out: Any, pop: K.Population, n: int, m: int) -> None:
+        # adapters.cpp:115-145: objectives (n, m) and optional constraints (n)
+        cons = None
+        if isinstance(out, dict):
+            obj, cons = out["objectives"], out.get("constraints")
+        elif isinstance(out, tuple):
+            obj = out[0]
+            cons = out[1] if len(out) > 1 else None
+        else:
+            obj = out
+        if not isinstance(obj, torch.Tensor) or not obj.is_cuda:
+            raise ValueError("objectives must be a torch.Tensor on the algorithm CUDA device")
+        if tuple(obj.shape) != (n, m):
+            raise ValueError("objectives must have shape (population_size, objective_count)")
+        pop.f.copy_(obj.to(torch.float32).t())
+        if cons is not None:
+            cons = torch.as_tensor(cons, device=self.device, dtype=torch.float32)
+            if cons.numel() != n:
+                raise ValueError("constraints must contain population_size values")
+            pop.cv.copy_(cons.reshape(n))
+        else:
+            pop.cv.zero_()
 
     def _guarded(self, body) -> int:
         try:
@@ -343,7 +449,11 @@ class Algorithm:
         def body() -> int:
             # adapters.cpp:207-224
             state: dict[str, Any] = {}
-            if rank:
+            if self._python_selector:
+                # adapters.cpp:412-417: the selector's mating state
+                state.update({k: v for k, v in self._sel_state.items()
+                              if k in ("rank", "reference_index", "score")})
+            elif rank:
                 state["rank"] = next(t for t in self._fronts if t.data_ptr() == rank)
             state["active_count"] = int(self._active.item()) if active_device else n
             idx = _checked(self.mating.select(self._pop_dict(parents), state,
@@ -408,26 +518,7 @@ class Algorithm:
             out = self.problem.evaluate(pop.x, {"generation": generation,
                                                 "max_generations": max_generations,
                                                 "device": self.cuda.device_id})
-            cons = None
-            if isinstance(out, dict):
-                obj, cons = out["objectives"], out.get("constraints")
-            elif isinstance(out, tuple):
-                obj = out[0]
-                cons = out[1] if len(out) > 1 else None
-            else:
-                obj = out
-            if not isinstance(obj, torch.Tensor) or not obj.is_cuda:
-                raise ValueError("objectives must be a torch.Tensor on the algorithm CUDA device")
-            if tuple(obj.shape) != (n, m):
-                raise ValueError("objectives must have shape (population_size, objective_count)")
-            pop.f.copy_(obj.to(torch.float32).t())
-            if cons is not None:
-                cons = torch.as_tensor(cons, device=self.device, dtype=torch.float32)
-                if cons.numel() != n:
-                    raise ValueError("constraints must contain population_size values")
-                pop.cv.copy_(cons.reshape(n))
-            else:
-                pop.cv.zero_()
+            self._copy_evaluation(out, pop, n, m)
             return 0
         except BaseException as exc:  # surfaced after the native call returns
             self._error = exc
@@ -460,7 +551,10 @@ class Algorithm:
             return
         torch.cuda.current_stream(self.device).synchronize()
         pop = self._current()
-        if self.kind == "NSGA3":
+        if self._python_selector:
+            res = self._sel_state.get("result_indices")
+            aux, active = (None if res is None else res.cpu().numpy()), self._sel_state.get("active_count", self.n)
+        elif self.kind == "NSGA3":
             aux, active = self._fronts[self._args.fronts_current].cpu().numpy(), self.n
         else:
             aux, active = self._front0, int(self._active.item())
@@ -503,7 +597,15 @@ class Algorithm:
             raise RuntimeError("Algorithm has not been initialized")
         pop = self._current()
         torch.cuda.current_stream(self.device).synchronize()
-        if self.kind == "NSGA3":
+        if self._python_selector:
+            sel = self.environment_selector
+            if hasattr(sel, "finalize"):         # adapters.cpp:405-411
+                self._update_state(sel.finalize(self._pop_view(pop), self._context(self.generation,
+                                                                                   self.max_generations)))
+            aux = self._sel_state.get("result_indices")
+            aux = torch.empty(0, dtype=torch.int32, device=self.device) if aux is None else aux
+            active = self._sel_state.get("active_count", self.n)
+        elif self.kind == "NSGA3":
             aux = self._fronts[self._args.fronts_current]
             active = self.n
         else:
@@ -530,7 +632,11 @@ class Algorithm:
         self._args.current = 0
         self._args.fronts_current = 0
         self._args.initial_x_given = 0
+        if self._python_selector and hasattr(self.environment_selector, "reset"):
+            self.environment_selector.reset()    # adapters.cpp:412-420
         self._call(lib.moea_run_init)
+        if self._python_selector:
+            self._selector_start()
         self._total_ms = 0.0
         self._front0 = None
         self._snapshot(0)                        # algorithm.cu:430

@@ -349,12 +349,22 @@ typedef int (*moea_directions_fn)(void* user, const moea_population* next, int n
                                   const int32_t* active_device, float* directions, int k, int generation,
                                   int max_generations, void* stream);
 
+/* Environmental selection (IEnvironmentSelector::select): write the n
+ * survivors into next (x, f, cv) from parents and offspring.  The hook keeps
+ * the selector's state: before the next generation it writes the survivors'
+ * ranks into args.fronts[args.fronts_current] (read by TournamentMating) and
+ * the active count into args.active_device (read by the mating draws). */
+typedef int (*moea_select_fn)(void* user, const moea_population* parents, const moea_population* offspring,
+                              const moea_population* next, int n, int d, int m, int generation,
+                              int max_generations, void* stream);
+
 typedef struct moea_strategy_hooks {
     void* user;
     moea_mate_fn mate;                     /* NULL: args.mating */
     moea_crossover_fn crossover;           /* NULL: SBX */
     moea_mutate_fn mutate;                 /* NULL: args.mutation */
     moea_directions_fn update_directions;  /* NULL: RVEA adaptation / fixed directions */
+    moea_select_fn select;                 /* NULL: args.algorithm's selection */
 } moea_strategy_hooks;
 
 /* State of one run.  The caller owns every buffer; moea_run_* update the

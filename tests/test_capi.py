@@ -100,10 +100,11 @@ def test_api_validation_errors():
     with pytest.raises(ValueError):
         cm.RVEA(environment_selector=cm.RVEAEnvironmentSelector(alpha=0.0))
     with pytest.raises(NotImplementedError):
-        cm.NSGA3(environment_selector=cm.PythonEnvironmentSelector())
-    # the reproduction and direction strategies take Python objects (adapters.cpp:196-338)
+        cm.NSGA3(environment_selector=cm.NSGA3EnvironmentSelector(cv_bins=4))
+    # every strategy takes a Python object (adapters.cpp:196-434)
     alg = cm.NSGA3(mating=cm.PythonMating(), crossover=cm.PythonCrossover(), mutation=cm.PythonMutation(),
-                   reference_directions=cm.PythonReferenceDirections())
+                   reference_directions=cm.PythonReferenceDirections(),
+                   environment_selector=cm.PythonEnvironmentSelector())
     assert not alg.initialized
     b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(
         cm.QuadraticCenters(12, 3))

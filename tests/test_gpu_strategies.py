@@ -245,3 +245,134 @@ def test_reproduce_given_indices_match_computed_winners():
     assert _lib.lib.moea_reproduce(ctypes.byref(a)) == 0
     torch.cuda.synchronize()
     assert torch.equal(off, off2)
+
+
+class PythonNsga3Selector(cm.PythonEnvironmentSelector):
+    """The built-in NSGA-III selection through the C ABI, returned as arrays
+    with the survivors' ranks as the mating state and the result indices."""
+
+    def initialize(self, info, references, context):
+        self.info, self.references = info, references
+
+    def prepare(self, population, context):
+        f = population["objectives"].t().contiguous()
+        self.ideal = K.ideal_point(f)
+        rank, _ = K.nds_rank(f, population["constraints"].contiguous())
+        return {"rank": rank, "result_indices": rank}
+
+    def select(self, parents, offspring, context):
+        def pop(d):
+            return K.Population(d["variables"], d["objectives"].t().contiguous(), d["constraints"])
+        w = self.references.t().contiguous()
+        nxt, nfr, _ = K.nsga3_select(pop(parents), pop(offspring), w, self.ideal, seed=SEED,
+                                     generation=context["generation"])
+        return {"variables": nxt.x, "objectives": nxt.f.t(), "constraints": nxt.cv, "rank": nfr,
+                "result_indices": nfr}
+
+
+def test_python_selector_equals_builtin_nsga3_selection():
+    print("strategies: nsga3 selector", flush=True)
+    _, ref = _run()
+    sel = PythonNsga3Selector()
+    _, got = _run(environment_selector=sel)
+    _same(ref, got)
+    assert got.active_count == 92 and sel.info["name"] == "NSGA-III" and sel.references.shape == (91, 3)
+
+
+class PythonRveaSelector(cm.PythonEnvironmentSelector):
+    """The built-in RVEA selection through the C ABI; the engine still adapts
+    the directions, with the active count this selector reports."""
+
+    def initialize(self, info, references, context):
+        self.references, self.max_gen = references, info["max_generations"]
+        self.active = info["population_size"]
+
+    def prepare(self, population, context):
+        self.active = population["variables"].shape[0]
+        return {"active_count": self.active}
+
+    def select(self, parents, offspring, context):
+        def pop(d):
+            return K.Population(d["variables"], d["objectives"].t().contiguous(), d["constraints"])
+        v = self.references.t().contiguous()
+        m = v.shape[0]
+        pen = K.rvea_penalty(m, context["generation"] + 1, self.max_gen, 2.0)
+        nxt, self.active = K.rvea_select(pop(parents), pop(offspring), v, K.rvea_gamma(v), self.active, pen)
+        return {"variables": nxt.x, "objectives": nxt.f.t(), "constraints": nxt.cv, "active_count": self.active}
+
+
+def test_python_selector_equals_builtin_rvea_selection():
+    print("strategies: rvea selector", flush=True)
+    a1, ref = _run(gens=12, n=100, kind="RVEA", reference_directions=cm.AdaptiveRVEADirections(frequency=0.25))
+    a2, got = _run(gens=12, n=100, kind="RVEA", reference_directions=cm.AdaptiveRVEADirections(frequency=0.25),
+                   environment_selector=PythonRveaSelector())
+    np.testing.assert_array_equal(ref.variables.cpu().numpy(), got.variables.cpu().numpy())
+    np.testing.assert_array_equal(ref.objectives.cpu().numpy(), got.objectives.cpu().numpy())
+    assert ref.active_count == got.active_count
+    np.testing.assert_array_equal(a1.directions.cpu().numpy(), a2.directions.cpu().numpy())
+
+
+class IndexSelector(cm.PythonEnvironmentSelector):
+    """The reference test's selector (tests/python/test_api.py:39-43) and its
+    offspring-only twin: survivors by index into parents ++ offspring."""
+
+    def __init__(self, offspring_only):
+        self.offspring_only = offspring_only
+        self.resets = 0
+        self.seen = None
+
+    def select(self, parents, offspring, context):
+        n = parents["variables"].shape[0]
+        self.seen = offspring["variables"].clone()
+        lo = n if self.offspring_only else 0
+        return {"indices": torch.arange(lo, lo + n, device=parents["variables"].device),
+                "score": torch.ones(n, device=parents["variables"].device), "active_count": n - 2}
+
+    def reset(self):
+        self.resets += 1
+
+
+class StateMating(cm.PythonMating):
+    def select(self, parents, state, context):
+        self.keys = sorted(state)
+        self.active = state["active_count"]
+        n = parents["variables"].shape[0]
+        return torch.arange(n, dtype=torch.int32, device=parents["variables"].device)
+
+
+@pytest.mark.parametrize("offspring_only", [False, True])
+def test_python_selector_indices_and_mating_state(offspring_only):
+    print(f"strategies: index selector offspring_only={offspring_only}", flush=True)
+    sel, mating = IndexSelector(offspring_only), StateMating()
+    alg = cm.NSGA3(population_size=92, max_generations=4, problem=cm.QuadraticCenters(12, 3, seed=1),
+                   mating=mating, environment_selector=sel, print_progress=False, seed=2)
+    alg.initialize()
+    x0 = alg.population.variables.clone()
+    alg.step()
+    want = sel.seen if offspring_only else x0
+    assert torch.equal(alg.population.variables, want)
+    alg.step()
+    assert mating.keys == ["active_count", "score"] and mating.active == 90
+    res = alg.run()
+    assert res.active_count == 90 and res.auxiliary_indices.numel() == 0
+    alg.reset()
+    assert sel.resets == 1 and alg.generation == 0
+
+
+def test_tournament_needs_the_selector_rank():
+    alg = cm.NSGA3(population_size=92, max_generations=3, problem=cm.QuadraticCenters(12, 3),
+                   environment_selector=IndexSelector(False), print_progress=False)
+    with pytest.raises(ValueError, match="rank array"):
+        alg.initialize()
+
+
+def test_selector_bad_indices_raise():
+    class Far(cm.PythonEnvironmentSelector):
+        def select(self, parents, offspring, context):
+            n = parents["variables"].shape[0]
+            return {"indices": torch.full((n,), 2 * n, device=parents["variables"].device)}
+
+    alg = cm.NSGA3(population_size=92, max_generations=3, problem=cm.QuadraticCenters(12, 3), mating=cm.RandomMating(),
+                   environment_selector=Far(), print_progress=False)
+    with pytest.raises(ValueError, match=r"\[0, 2 \* population_size\)"):
+        alg.step()
