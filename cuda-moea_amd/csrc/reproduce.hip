@@ -98,7 +98,6 @@ struct RepArgs {
     const float* bounds;
     float eta_c, prob_c, copy_c;
     int mutate;
-    int winner_mode;     // 0: winners as computed, 1: caller's winners clamped into [0, n), 2: row itself
     float eta_m, rate_m;
     const float* sbx_u; const float* sbx_p; const int32_t* sbx_sign; const float* sbx_copy;
     const float* pm_gate; const float* pm_delta;
@@ -107,16 +106,21 @@ struct RepArgs {
     float* ox;
 };
 
-// Parent row crossed into offspring row `row`.
+// Parent row crossed into offspring row `row`.  kWinners 0: the winners as
+// the mating kernel wrote them; 1: a caller's winners, clamped into [0, n);
+// 2: the row itself (mutation alone).
+template <int kWinners>
 __device__ __forceinline__ int winner_of(const RepArgs& R, int row)
 {
-    if (R.winner_mode == 2) return row;
+    if constexpr (kWinners == 2) return row;
     const int w = R.winners[row];
-    return R.winner_mode == 1 ? min(max(w, 0), R.n - 1) : w;
+    if constexpr (kWinners == 1) return min(max(w, 0), R.n - 1);
+    return w;
 }
 
 // Element e of the (pair, variable) space; the odd tail row (n odd) copies its
 // parent (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
+template <int kWinners>
 __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
 {
     const long long total_pair = static_cast<long long>(R.pairs) * R.d;
@@ -132,7 +136,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
             pair = static_cast<int>(e / R.d);
             j = static_cast<int>(e - static_cast<long long>(pair) * R.d);
         }
-        const int w0 = winner_of(R, 2 * pair), w1 = winner_of(R, 2 * pair + 1);
+        const int w0 = winner_of<kWinners>(R, 2 * pair), w1 = winner_of<kWinners>(R, 2 * pair + 1);
         const float x1 = R.px[static_cast<size_t>(w0) * R.d + j];
         const float x2 = R.px[static_cast<size_t>(w1) * R.d + j];
         const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
@@ -172,7 +176,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
         const int j = static_cast<int>(e - total_pair);
         const int row = R.n - 1;
         const float lo = R.bounds[2 * j], hi = R.bounds[2 * j + 1];
-        float v = R.px[static_cast<size_t>(winner_of(R, row)) * R.d + j];
+        float v = R.px[static_cast<size_t>(winner_of<kWinners>(R, row)) * R.d + j];
         if (R.mutate) {
             const size_t ra = static_cast<size_t>(row) * R.d + j;
             float g, dd = 0.0f;
@@ -190,9 +194,10 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
 }
 
 // One thread per (pair, variable).
+template <int kWinners>
 __global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
 {
-    sbx_pm_elem(R, static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+    sbx_pm_elem<kWinners>(R, static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // Population::initialize_random (core/population.cu:97-115):
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__
     R.eta_m = annealed_hd(I.pm_eta0, I.pm_eta1, gen, I.max_gen);
     R.rate_m = I.pm_prob / static_cast<float>(I.d);
     R.seed = I.seed; R.gen = static_cast<uint32_t>(gen); R.ox = I.x[off];
-    sbx_pm_elem(R, static_cast<long long>(local) * blockDim.x + threadIdx.x);
+    sbx_pm_elem<0>(R, static_cast<long long>(local) * blockDim.x + threadIdx.x);
 }
 
 }  // namespace
@@ -303,12 +308,12 @@ int mutate_rows(const float* x_in, float* x_out, int n, int d, const float* boun
     R.n = n; R.d = d; R.pairs = n / 2;
     R.dd = fast_div_make(static_cast<uint32_t>(d));
     R.small = (static_cast<long long>(R.pairs) + 1) * d < (1ll << 31);
-    R.px = x_in; R.winners = nullptr; R.bounds = bounds; R.winner_mode = 2;
+    R.px = x_in; R.winners = nullptr; R.bounds = bounds;
     R.eta_c = 1.0f; R.prob_c = 0.0f; R.copy_c = 0.0f;
     R.mutate = 1; R.eta_m = pm_eta; R.rate_m = pm_prob / static_cast<float>(d);
     R.seed = seed; R.gen = static_cast<uint32_t>(generation); R.ox = x_out;
     const long long total = static_cast<long long>(R.pairs) * d + ((n & 1) ? d : 0);
-    hipLaunchKernelGGL(sbx_pm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, R);
+    hipLaunchKernelGGL(sbx_pm_kernel<2>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, R);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
@@ -328,7 +333,6 @@ int reproduce(const moea_reproduce_args& A)
     const uint32_t gen = static_cast<uint32_t>(A.generation);
     if (A.mating != MOEA_MATING_GIVEN) MOEA_CHECK(mate(A));
     RepArgs R{};
-    R.winner_mode = (A.mating == MOEA_MATING_GIVEN) ? 1 : 0;
     R.n = A.n; R.d = A.d; R.pairs = A.n / 2;
     R.dd = fast_div_make(static_cast<uint32_t>(A.d));
     R.small = (static_cast<long long>(R.pairs) + 1) * R.d < (1ll << 31);
@@ -345,8 +349,9 @@ int reproduce(const moea_reproduce_args& A)
     }
     R.seed = A.seed; R.gen = gen; R.ox = A.offspring_x;
     const long long total = static_cast<long long>(R.pairs) * A.d + ((A.n & 1) ? A.d : 0);
-    hipLaunchKernelGGL(sbx_pm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
-                       st, R);
+    const dim3 grid(static_cast<unsigned>((total + 255) / 256));
+    if (A.mating == MOEA_MATING_GIVEN) hipLaunchKernelGGL(sbx_pm_kernel<1>, grid, dim3(256), 0, st, R);
+    else hipLaunchKernelGGL(sbx_pm_kernel<0>, grid, dim3(256), 0, st, R);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
