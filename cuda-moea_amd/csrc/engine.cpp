@@ -27,8 +27,10 @@ int evaluate(const moea_run_args& a, const moea_population& p, hipStream_t st, i
             return eval_mlp(p.x, a.n, a.d, pr.w1, pr.hidden, pr.w2, a.m, p.f, a.n, p.cv, pr.hwork, st);
         case MOEA_PROBLEM_DTLZ1:
             return eval_dtlz(1, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
-        case MOEA_PROBLEM_DTLZ2:
-            return eval_dtlz(2, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
+        case MOEA_PROBLEM_DTLZ2: case MOEA_PROBLEM_DTLZ3: case MOEA_PROBLEM_DTLZ4: case MOEA_PROBLEM_DTLZ5:
+        case MOEA_PROBLEM_DTLZ6: case MOEA_PROBLEM_DTLZ7: case MOEA_PROBLEM_CONVEX_DTLZ2:
+            // problem kinds 4..10 are DTLZ2..DTLZ7 and ConvexDTLZ2 (kernel kinds 2..8)
+            return eval_dtlz(pr.kind - MOEA_PROBLEM_DTLZ2 + 2, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
         case MOEA_PROBLEM_CALLBACK:
             if (!pr.callback) return fail(kInvalidArgument, "moea_run: callback problem without callback");
             return pr.callback(pr.user, p.x, a.n, a.d, a.m, p.f, p.cv, generation, a.max_generations, st);

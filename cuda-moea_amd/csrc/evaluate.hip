@@ -354,13 +354,34 @@ __global__ __launch_bounds__(256) void b_mlp_head_kernel(const BInst* __restrict
                     static_cast<int>(threadIdx.x & 63));
 }
 
-// DTLZ1 / DTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) with the reference's
-// protections: dtlz1_func_kernel / dtlz2_func_kernel
-// (problem/dtlz/dtlz_kernels.cu:28-112, 118-190).  g sums over the last
-// k = d - m + 1 variables; DTLZ1 clamps x_j to [0, 1] in the products and
-// 1 - x to >= 0; DTLZ2 uses max(0, cos).  The angle arguments are formed in
-// double and rounded to float, as the reference's M_PI expressions are.  One
-// wave per row; lanes own the objectives.
+// The DTLZ suite (Deb, Thiele, Laumanns, Zitzler 2005) with the reference's
+// protections (problem/dtlz/dtlz_kernels.cu:20-22, 28-717).  KIND is the
+// problem number; 8 is ConvexDTLZ2.  g runs over the last k = d - m + 1
+// variables:
+//   DTLZ1, 3: 100 (k + sum((x - 1/2)^2 - cos(20 pi (x - 1/2))))
+//   DTLZ2, 4, 5, ConvexDTLZ2: sum (x - 1/2)^2;  DTLZ6: sum x^0.1
+//   DTLZ7: 1 + 9/k sum x
+// DTLZ1 clamps x_j to [0, 1] in its products and 1 - x to >= 0; the
+// spherical kinds use max(0, cos); DTLZ4 maps x to x^100; DTLZ5/6 map
+// x_j, j >= 1, to theta = (1 + 2 g x_j) / (2 (1 + g)), clamped to [0, 1] in
+// DTLZ5; ConvexDTLZ2 raises the first m - 1 objectives to the 4th power and
+// the last to the 2nd.  The angle arguments are formed in double and rounded
+// to float, as the reference's M_PI expressions are.  One wave per row;
+// lanes own the objectives.
+template <int KIND>
+__device__ __forceinline__ float dtlz_angle(const float* __restrict__ xr, int j, float g)
+{
+    float t = xr[j];
+    if constexpr (KIND == 4) t = powf(t, 100.0f);
+    if constexpr (KIND == 5 || KIND == 6) {
+        if (j > 0) {
+            t = (1.0f + 2.0f * g * t) / (2.0f * (1.0f + g));
+            if constexpr (KIND == 5) t = fminf(fmaxf(t, 0.0f), 1.0f);
+        }
+    }
+    return static_cast<float>(static_cast<double>(t) * M_PI / 2.0);
+}
+
 template <int M, int KIND>
 __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf,
                                  float* __restrict__ cv)
@@ -371,56 +392,77 @@ __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, in
     const float* xr = x + static_cast<size_t>(row) * d;
     float s = 0.0f;
     for (int j = M - 1 + lane; j < d; j += 64) {
-        const float v = xr[j] - 0.5f;
-        if (KIND == 1) s = s + (v * v - cosf(static_cast<float>(20.0 * M_PI * static_cast<double>(v))));
-        else s = s + v * v;
+        if constexpr (KIND == 6) {
+            s = s + powf(xr[j], 0.1f);
+        } else if constexpr (KIND == 7) {
+            s = s + xr[j];
+        } else {
+            const float v = xr[j] - 0.5f;
+            if constexpr (KIND == 1 || KIND == 3) s = s + (v * v - cosf(static_cast<float>(20.0 * M_PI * static_cast<double>(v))));
+            else s = s + v * v;
+        }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s = s + __shfl_xor(s, o);
     if (lane < M) {
         const int k = d - (M - 1);
-        float out = (KIND == 1) ? 0.5f * (1.0f + 100.0f * (static_cast<float>(k) + s)) : 1.0f + s;
-        const int prod = M - 1 - lane;
-        for (int j = 0; j < prod; ++j) {
-            const float xj = xr[j];
-            if (KIND == 1) out *= fminf(fmaxf(xj, 0.0f), 1.0f);
-            else out *= fmaxf(0.0f, cosf(static_cast<float>(static_cast<double>(xj) * M_PI / 2.0)));
-        }
-        if (lane > 0) {
-            const float xi = xr[M - 1 - lane];
-            if (KIND == 1) out *= fmaxf(0.0f, 1.0f - xi);
-            else out *= sinf(static_cast<float>(static_cast<double>(xi) * M_PI / 2.0));
+        float out;
+        if constexpr (KIND == 7) {
+            const float g = 1.0f + (9.0f / static_cast<float>(k)) * s;
+            if (lane < M - 1) {
+                out = xr[lane];
+            } else {
+                float h = 0.0f;
+                for (int i = 0; i < M - 1; ++i) {
+                    const float fi = xr[i];
+                    h += (fi / (1.0f + g)) * (1.0f + sinf(static_cast<float>(3.0 * M_PI * static_cast<double>(fi))));
+                }
+                out = (1.0f + g) * (static_cast<float>(M) - h);
+            }
+        } else if constexpr (KIND == 1) {
+            out = 0.5f * (1.0f + 100.0f * (static_cast<float>(k) + s));
+            for (int j = 0; j < M - 1 - lane; ++j) out *= fminf(fmaxf(xr[j], 0.0f), 1.0f);
+            if (lane > 0) out *= fmaxf(0.0f, 1.0f - xr[M - 1 - lane]);
+        } else {
+            const float g = (KIND == 3) ? 100.0f * (static_cast<float>(k) + s) : s;
+            out = 1.0f + g;
+            for (int j = 0; j < M - 1 - lane; ++j) out *= fmaxf(0.0f, cosf(dtlz_angle<KIND>(xr, j, g)));
+            if (lane > 0) out *= sinf(dtlz_angle<KIND>(xr, M - 1 - lane, g));
+            if constexpr (KIND == 8) {
+                out = out * out;
+                if (lane < M - 1) out = out * out;
+            }
         }
         f[static_cast<size_t>(lane) * ldf + row] = out;
     }
     if (lane == 0 && cv) cv[row] = 0.0f;
 }
 
-template <int M>
-__global__ __launch_bounds__(256) void dtlz1_kernel(const float* __restrict__ x, int n, int d,
-                                                    float* __restrict__ f, int ldf, float* __restrict__ cv)
-{
-    dtlz_rows<M, 1>(x, n, d, f, ldf, cv);
-}
-
-template <int M>
-__global__ __launch_bounds__(256) void dtlz2_kernel(const float* __restrict__ x, int n, int d,
-                                                    float* __restrict__ f, int ldf, float* __restrict__ cv)
-{
-    dtlz_rows<M, 2>(x, n, d, f, ldf, cv);
-}
+template <int M> __global__ __launch_bounds__(256) void dtlz1_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 1>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz2_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 2>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz3_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 3>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz4_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 4>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz5_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 5>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz6_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 6>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void dtlz7_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 7>(x, n, d, f, ldf, cv); }
+template <int M> __global__ __launch_bounds__(256) void convex_dtlz2_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 8>(x, n, d, f, ldf, cv); }
 
 }  // namespace
 
 int eval_dtlz(int kind, const float* x, int n, int d, int m, float* f, int ldf, float* cv, hipStream_t stream)
 {
-    if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || (kind != 1 && kind != 2))
-        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (DTLZ1/2 need 2 <= m <= d)");
+    if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || kind < 1 || kind > 8)
+        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (kinds 1-8, 2 <= m <= d)");
     const dim3 grid(ceil_div(n, 4)), block(256);
-    if (kind == 1) {
-        MOEA_LAUNCH_M(m, dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv);
-    } else {
-        MOEA_LAUNCH_M(m, dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv);
+    switch (kind) {
+    case 1: MOEA_LAUNCH_M(m, dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 2: MOEA_LAUNCH_M(m, dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 3: MOEA_LAUNCH_M(m, dtlz3_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 4: MOEA_LAUNCH_M(m, dtlz4_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 5: MOEA_LAUNCH_M(m, dtlz5_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 6: MOEA_LAUNCH_M(m, dtlz6_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    case 7: MOEA_LAUNCH_M(m, dtlz7_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
+    default: MOEA_LAUNCH_M(m, convex_dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
     }
     MOEA_HIP(hipGetLastError());
     return kOk;

@@ -17,6 +17,11 @@ MATING_TOURNAMENT, MATING_RANDOM, MATING_GIVEN = 0, 1, 2
 MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
 ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
 PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK, PROBLEM_DTLZ1, PROBLEM_DTLZ2 = 0, 1, 2, 3, 4
+PROBLEM_DTLZ3, PROBLEM_DTLZ4, PROBLEM_DTLZ5, PROBLEM_DTLZ6, PROBLEM_DTLZ7, PROBLEM_CONVEX_DTLZ2 = 5, 6, 7, 8, 9, 10
+# native DTLZ problem kinds by name; moea_eval_dtlz numbers them 1-8 (ConvexDTLZ2 = 8)
+DTLZ_KINDS = {"DTLZ1": PROBLEM_DTLZ1, "DTLZ2": PROBLEM_DTLZ2, "DTLZ3": PROBLEM_DTLZ3, "DTLZ4": PROBLEM_DTLZ4,
+              "DTLZ5": PROBLEM_DTLZ5, "DTLZ6": PROBLEM_DTLZ6, "DTLZ7": PROBLEM_DTLZ7,
+              "ConvexDTLZ2": PROBLEM_CONVEX_DTLZ2}
 MAX_OBJECTIVES = 10
 
 

@@ -295,9 +295,9 @@ class Algorithm:
             self._hwork = torch.empty(self.n, p.hidden, dtype=torch.float32, device=self.device)
             pd.kind, pd.w1, pd.w2 = _lib.PROBLEM_MLP, self._w1.data_ptr(), self._w2.data_ptr()
             pd.hidden, pd.hwork = p.hidden, self._hwork.data_ptr()
-        elif isinstance(p, DTLZProblem) and p._kind in ("DTLZ1", "DTLZ2") and 2 <= self.m <= self.d:
+        elif isinstance(p, DTLZProblem) and p._kind in _lib.DTLZ_KINDS and 2 <= self.m <= self.d:
             # built-in objective kernels, as the reference's DTLZ problems are native
-            pd.kind = _lib.PROBLEM_DTLZ1 if p._kind == "DTLZ1" else _lib.PROBLEM_DTLZ2
+            pd.kind = _lib.DTLZ_KINDS[p._kind]
         else:
             if hasattr(p, "initialize"):
                 p.initialize({"device": self.cuda.device_id})

@@ -226,7 +226,7 @@ def eval_quadratic(x: torch.Tensor, centers: torch.Tensor):
 
 
 def eval_dtlz(kind: int, x: torch.Tensor, m: int):
-    """DTLZ1 (kind 1) or DTLZ2 (kind 2) objectives (m, n) and zero cv of x (n, d)."""
+    """DTLZ1-DTLZ7 (kind 1-7) or ConvexDTLZ2 (kind 8) objectives (m, n) and zero cv of x (n, d)."""
     _dev(x, _F32, "x")
     n, d = x.shape
     f = torch.empty(m, n, dtype=_F32, device=x.device)

@@ -240,10 +240,10 @@ int moea_init_population(float* x, int n, int d, const float* bounds, uint64_t s
 int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int m, float* f,
                         int ldf, float* cv, void* stream);
 
-/* DTLZ1 (problem 1) or DTLZ2 (problem 2), as the reference's built-in kernels
- * dtlz1_func_kernel / dtlz2_func_kernel (problem/dtlz/dtlz_kernels.cu:28-190),
- * with their clamping of x to [0, 1] (DTLZ1) and max(0, cos) (DTLZ2); cv = 0.
- * Needs 2 <= m <= d. */
+/* DTLZ1-DTLZ7 (problem 1-7) or ConvexDTLZ2 (problem 8), as the reference's
+ * built-in kernels (problem/dtlz/dtlz_kernels.cu:28-717): DTLZ1 clamps x to
+ * [0, 1], the spherical problems use max(0, cos), DTLZ4 maps x to x^100 and
+ * DTLZ5 clamps its theta to [0, 1]; cv = 0.  Needs 2 <= m <= d. */
 int moea_eval_dtlz(int problem, const float* x, int n, int d, int m, float* f, int ldf, float* cv,
                    void* stream);
 
@@ -303,6 +303,12 @@ int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_a
 #define MOEA_PROBLEM_CALLBACK 2
 #define MOEA_PROBLEM_DTLZ1 3
 #define MOEA_PROBLEM_DTLZ2 4
+#define MOEA_PROBLEM_DTLZ3 5
+#define MOEA_PROBLEM_DTLZ4 6
+#define MOEA_PROBLEM_DTLZ5 7
+#define MOEA_PROBLEM_DTLZ6 8
+#define MOEA_PROBLEM_DTLZ7 9
+#define MOEA_PROBLEM_CONVEX_DTLZ2 10
 
 /* Problem evaluator callback: the C form of IProblemEvaluator::evaluate
  * (problem/problem_evaluator.cuh:10-31).  Must write f (m, n) and cv (n) for

@@ -58,3 +58,104 @@ def test_dtlz_run_native(cls, kind):
     res = alg.run()
     x = res.variables.cpu().numpy()
     np.testing.assert_allclose(res.objectives.cpu().numpy().T, dtlz_fp64(kind, x, 3), rtol=2e-5, atol=2e-4)
+
+
+def dtlz_fp64_suite(kind, x, m):
+    """FP64 DTLZ3-7 (kinds 3-7) and ConvexDTLZ2 (kind 8) as the reference
+    defines them (dtlz_kernels.cu:194-717): DTLZ4 maps x to x^100, DTLZ5/6 map
+    x_j (j >= 1) to (1 + 2 g x_j) / (2 (1 + g)), DTLZ5 clamping it to [0, 1]."""
+    x = x.astype(np.float64)
+    n = x.shape[0]
+    xm, xk = x[:, : m - 1], x[:, m - 1:]
+    k = xk.shape[1]
+    f = np.empty((m, n))
+    if kind == 7:
+        g = 1.0 + 9.0 / k * xk.sum(1)
+        f[: m - 1] = xm.T
+        h = m - (xm / (1.0 + g)[:, None] * (1.0 + np.sin(3 * math.pi * xm))).sum(1)
+        f[m - 1] = (1.0 + g) * h
+        return f
+    if kind == 3:
+        g = 100.0 * (k + ((xk - 0.5) ** 2 - np.cos(20 * math.pi * (xk - 0.5))).sum(1))
+    elif kind == 6:
+        g = (xk ** 0.1).sum(1)
+    else:
+        g = ((xk - 0.5) ** 2).sum(1)
+    th = xm.copy()
+    if kind == 4:
+        th = th ** 100
+    if kind in (5, 6):
+        t = (1.0 + 2.0 * g[:, None] * xm) / (2.0 * (1.0 + g[:, None]))
+        th[:, 1:] = np.clip(t, 0.0, 1.0)[:, 1:] if kind == 5 else t[:, 1:]
+    ang = th * math.pi / 2
+    for i in range(m):
+        v = 1.0 + g
+        for j in range(m - 1 - i):
+            v = v * np.maximum(0.0, np.cos(ang[:, j]))
+        if i > 0:
+            v = v * np.sin(ang[:, m - 1 - i])
+        f[i] = v
+    if kind == 8:
+        f[: m - 1] = f[: m - 1] ** 4
+        f[m - 1] = f[m - 1] ** 2
+    return f
+
+
+SUITE = {3: cm.DTLZ3, 4: cm.DTLZ4, 5: cm.DTLZ5, 6: cm.DTLZ6, 7: cm.DTLZ7, 8: cm.ConvexDTLZ2}
+
+
+@pytest.mark.parametrize("kind", sorted(SUITE))
+@pytest.mark.parametrize("n,d,m", [(300, 12, 3), (513, 300, 5), (64, 20, 10)])
+def test_dtlz_suite_kernels_against_fp64(kind, n, d, m):
+    rng = np.random.default_rng(kind * 1000 + m)
+    x = rng.random((n, d), dtype=np.float32)
+    print(f"dtlz kind={kind} n={n} d={d} m={m}", flush=True)
+    f, cv = K.eval_dtlz(kind, torch.from_numpy(x).to(DEV), m)
+    _close_scaled(f.cpu().numpy(), dtlz_fp64_suite(kind, x, m), m)
+    assert not cv.cpu().numpy().any()
+
+
+def _close_scaled(got, ref, m):
+    # The reference's tolerance (rtol 2e-5, atol 2e-4) plus the float32
+    # conditioning of the formula: each angle is rounded to float32, which moves
+    # its cos / sin factor by up to about 1e-7, times the row's scale (1 + g, up
+    # to 3e4 for DTLZ3 at d = 300), for each of the m - 1 factors.
+    scale = np.abs(ref).max(axis=0, keepdims=True)
+    tol = 2e-5 * np.abs(ref) + 2e-4 + 4e-7 * m * scale
+    bad = np.abs(got - ref) > tol
+    assert not bad.any(), (np.argwhere(bad)[:5], got[bad][:5], ref[bad][:5])
+
+
+@pytest.mark.parametrize("kind", [2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("m", [3, 6])
+def test_dtlz_pareto_front_identities(kind, m):
+    """On the Pareto set (g at its minimum) the objectives satisfy the
+    problem's front equation, independent of any restatement: sum f^2 = 1 for
+    the spherical problems, f_M + sum sqrt(f_i) = 1 for ConvexDTLZ2, and
+    f_M = 2 (M - sum f_i / 2 (1 + sin 3 pi f_i)) with g = 1 for DTLZ7."""
+    rng = np.random.default_rng(7 + kind)
+    n, d = 400, m + 9
+    x = rng.random((n, d), dtype=np.float32)
+    x[:, m - 1:] = 0.0 if kind in (6, 7) else 0.5
+    f = K.eval_dtlz(kind, torch.from_numpy(x).to(DEV), m)[0].cpu().numpy().astype(np.float64)
+    if kind == 8:
+        np.testing.assert_allclose(f[m - 1] + np.sqrt(f[: m - 1]).sum(0), 1.0, atol=2e-5)
+    elif kind == 7:
+        xm = x[:, : m - 1].astype(np.float64)
+        np.testing.assert_array_equal(f[: m - 1], xm.T)
+        np.testing.assert_allclose(f[m - 1], 2.0 * (m - (xm / 2.0 * (1 + np.sin(3 * math.pi * xm))).sum(1)),
+                                   rtol=1e-5, atol=1e-5)
+    else:
+        np.testing.assert_allclose((f ** 2).sum(0), 1.0, atol=2e-5)
+        assert (f >= 0).all()
+
+
+@pytest.mark.parametrize("kind", sorted(SUITE))
+def test_dtlz_suite_runs_native(kind):
+    alg = cm.NSGA3(population_size=92, max_generations=10, problem=SUITE[kind](dimension=12, objectives=3),
+                   seed=3, print_progress=False)
+    alg.initialize()
+    assert alg._args.problem.kind == _lib.DTLZ_KINDS[SUITE[kind].__name__]
+    res = alg.run()
+    x = res.variables.cpu().numpy()
+    _close_scaled(res.objectives.cpu().numpy().T, dtlz_fp64_suite(kind, x, 3), 3)
