@@ -31,6 +31,10 @@ int evaluate(const moea_run_args& a, const moea_population& p, hipStream_t st, i
         case MOEA_PROBLEM_DTLZ6: case MOEA_PROBLEM_DTLZ7: case MOEA_PROBLEM_CONVEX_DTLZ2:
             // problem kinds 4..10 are DTLZ2..DTLZ7 and ConvexDTLZ2 (kernel kinds 2..8)
             return eval_dtlz(pr.kind - MOEA_PROBLEM_DTLZ2 + 2, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
+        case MOEA_PROBLEM_C1_DTLZ1: case MOEA_PROBLEM_C1_DTLZ3: case MOEA_PROBLEM_C2_DTLZ2:
+        case MOEA_PROBLEM_C2_CONVEX_DTLZ2: case MOEA_PROBLEM_C3_DTLZ1: case MOEA_PROBLEM_C3_DTLZ4:
+            // problem kinds 11..16 are the constrained variants (kernel kinds 9..14)
+            return eval_dtlz(pr.kind - MOEA_PROBLEM_C1_DTLZ1 + 9, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
         case MOEA_PROBLEM_CALLBACK:
             if (!pr.callback) return fail(kInvalidArgument, "moea_run: callback problem without callback");
             return pr.callback(pr.user, p.x, a.n, a.d, a.m, p.f, p.cv, generation, a.max_generations, st);

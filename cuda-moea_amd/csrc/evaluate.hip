@@ -382,9 +382,65 @@ __device__ __forceinline__ float dtlz_angle(const float* __restrict__ xr, int j,
     return static_cast<float>(static_cast<double>(t) * M_PI / 2.0);
 }
 
-template <int M, int KIND>
+// Constraint violation of the constrained variants from the row's objectives
+// fv (compute_cv, dtlz_kernels.cu:824-1044, 1097-1165).  CKIND 1: C1-DTLZ1,
+// 2: C1-DTLZ3, 3: C2-DTLZ2, 4: C2-ConvexDTLZ2, 5: C3-DTLZ1, 6: C3-DTLZ4;
+// R is the radius the reference picks for the objective count.
+template <int M, int CKIND>
+__device__ __forceinline__ float dtlz_cv(const float (&fv)[M], float R)
+{
+    if constexpr (CKIND == 1) {
+        float c = fv[M - 1] / 0.6f;
+        for (int i = 0; i < M - 1; ++i) c += fv[i] / 0.5f;
+        return fmaxf(c - 1.0f, 0.0f);
+    } else if constexpr (CKIND == 2) {
+        float s2 = 0.0f;
+        for (int i = 0; i < M; ++i) s2 += fv[i] * fv[i];
+        return fmaxf((s2 - 16.0f) * (R * R - s2), 0.0f);
+    } else if constexpr (CKIND == 3) {
+        const float inv_sqrt_m = 1.0f / sqrtf(static_cast<float>(M));
+        const float inv_r2 = 1.0f / (R * R);
+        float s2 = 0.0f;
+        for (int i = 0; i < M; ++i) s2 = fmaf(fv[i], fv[i], s2);
+        float lo = INFINITY;
+        for (int i = 0; i < M; ++i) {
+            const float diff = fv[i] - 1.0f;
+            lo = fminf(lo, fmaf(diff, diff, s2 - fv[i] * fv[i]) * inv_r2 - 1.0f);
+        }
+        float c2 = 0.0f;
+        for (int i = 0; i < M; ++i) {
+            const float df = fv[i] - inv_sqrt_m;
+            c2 = fmaf(df, df, c2);
+        }
+        return fmaxf(fminf(lo, c2 * inv_r2 - 1.0f), 0.0f);
+    } else if constexpr (CKIND == 4) {
+        float sum = 0.0f, sum_sq = 0.0f;
+        for (int i = 0; i < M; ++i) {
+            sum += fv[i];
+            sum_sq += fv[i] * fv[i];
+        }
+        const float mean = sum * (1.0f / static_cast<float>(M));
+        const float variance = sum_sq - static_cast<float>(M) * mean * mean;
+        return fmaxf(1.0f - variance * (1.0f / (R * R)), 0.0f);
+    } else if constexpr (CKIND == 5) {
+        float s1 = 0.0f;
+        for (int i = 0; i < M; ++i) s1 += fv[i];
+        s1 *= 2.0f;
+        float c = 0.0f;
+        for (int i = 0; i < M; ++i) c += fmaxf(fv[i] + 1.0f - s1, 0.0f);
+        return c;
+    } else {
+        float s2 = 0.0f;
+        for (int i = 0; i < M; ++i) s2 += fv[i] * fv[i];
+        float c = 0.0f;
+        for (int i = 0; i < M; ++i) c += fmaxf(0.75f * (fv[i] * fv[i]) - s2 + 1.0f, 0.0f);
+        return c;
+    }
+}
+
+template <int M, int KIND, int CKIND = 0>
 __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf,
-                                 float* __restrict__ cv)
+                                 float* __restrict__ cv, float R = 0.0f)
 {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -404,9 +460,9 @@ __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, in
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s = s + __shfl_xor(s, o);
+    float out = 0.0f;
     if (lane < M) {
         const int k = d - (M - 1);
-        float out;
         if constexpr (KIND == 7) {
             const float g = 1.0f + (9.0f / static_cast<float>(k)) * s;
             if (lane < M - 1) {
@@ -435,7 +491,14 @@ __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, in
         }
         f[static_cast<size_t>(lane) * ldf + row] = out;
     }
-    if (lane == 0 && cv) cv[row] = 0.0f;
+    if constexpr (CKIND != 0) {
+        float fv[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) fv[i] = __shfl(out, i);
+        if (lane == 0) cv[row] = dtlz_cv<M, CKIND>(fv, R);
+    } else {
+        if (lane == 0 && cv) cv[row] = 0.0f;
+    }
 }
 
 template <int M> __global__ __launch_bounds__(256) void dtlz1_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 1>(x, n, d, f, ldf, cv); }
@@ -446,15 +509,55 @@ template <int M> __global__ __launch_bounds__(256) void dtlz5_kernel(const float
 template <int M> __global__ __launch_bounds__(256) void dtlz6_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 6>(x, n, d, f, ldf, cv); }
 template <int M> __global__ __launch_bounds__(256) void dtlz7_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 7>(x, n, d, f, ldf, cv); }
 template <int M> __global__ __launch_bounds__(256) void convex_dtlz2_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv) { dtlz_rows<M, 8>(x, n, d, f, ldf, cv); }
+// constrained variants: objectives of their base problem, then the violation
+template <int M> __global__ __launch_bounds__(256) void c1_dtlz1_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 1, 1>(x, n, d, f, ldf, cv, R); }
+template <int M> __global__ __launch_bounds__(256) void c1_dtlz3_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 3, 2>(x, n, d, f, ldf, cv, R); }
+template <int M> __global__ __launch_bounds__(256) void c2_dtlz2_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 2, 3>(x, n, d, f, ldf, cv, R); }
+template <int M> __global__ __launch_bounds__(256) void c2_convex_dtlz2_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 8, 4>(x, n, d, f, ldf, cv, R); }
+template <int M> __global__ __launch_bounds__(256) void c3_dtlz1_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 1, 5>(x, n, d, f, ldf, cv, R); }
+template <int M> __global__ __launch_bounds__(256) void c3_dtlz4_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 4, 6>(x, n, d, f, ldf, cv, R); }
+
+// The radius of C1-DTLZ3, C2-DTLZ2 and C2-ConvexDTLZ2 by objective count
+// (compute_cv, dtlz_kernels.cu:1110-1158); 0 for a count the reference rejects.
+float dtlz_radius(int kind, int m)
+{
+    if (kind == 10) {
+        if (m == 2) return 6.0f;
+        if (m == 3) return 5.0f;
+        if (m == 5) return 12.5f;
+        return (m == 8 || m == 10 || m == 15) ? 15.0f : 0.0f;
+    }
+    if (kind == 11) {
+        if (m == 2) return 0.15f;
+        if (m == 3) return 0.40f;
+        return (m == 5 || m == 8 || m == 10 || m == 15) ? 1.0f : 0.0f;
+    }
+    if (kind == 12) {
+        if (m == 3) return 0.20f;
+        if (m == 5) return 0.225f;
+        if (m == 8 || m == 10) return 0.26f;
+        return m == 15 ? 0.27f : 0.0f;
+    }
+    return 1.0f;
+}
 
 }  // namespace
 
 int eval_dtlz(int kind, const float* x, int n, int d, int m, float* f, int ldf, float* cv, hipStream_t stream)
 {
-    if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || kind < 1 || kind > 8)
-        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (kinds 1-8, 2 <= m <= d)");
+    if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || kind < 1 || kind > 14)
+        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (kinds 1-14, 2 <= m <= d)");
+    if (kind > 8 && !cv) return fail(kInvalidArgument, "eval_dtlz: the constrained variants need a cv array");
+    const float R = dtlz_radius(kind, m);
+    if (R == 0.0f) return fail(kInvalidArgument, "eval_dtlz: objective count not defined for this constrained variant");
     const dim3 grid(ceil_div(n, 4)), block(256);
     switch (kind) {
+    case 9: MOEA_LAUNCH_M(m, c1_dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
+    case 10: MOEA_LAUNCH_M(m, c1_dtlz3_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
+    case 11: MOEA_LAUNCH_M(m, c2_dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
+    case 12: MOEA_LAUNCH_M(m, c2_convex_dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
+    case 13: MOEA_LAUNCH_M(m, c3_dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
+    case 14: MOEA_LAUNCH_M(m, c3_dtlz4_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv, R); break;
     case 1: MOEA_LAUNCH_M(m, dtlz1_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
     case 2: MOEA_LAUNCH_M(m, dtlz2_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;
     case 3: MOEA_LAUNCH_M(m, dtlz3_kernel, grid, block, 0, stream, x, n, d, f, ldf, cv); break;

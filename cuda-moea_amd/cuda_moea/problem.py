@@ -66,22 +66,33 @@ class SeededMLP:
 
 
 class DTLZProblem(PythonProblem):
-    """DTLZ1-7 and ConvexDTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) on device
-    tensors, with the reference's constructor (problem.py:9-47)."""
+    """DTLZ1-7 and ConvexDTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) and the
+    constrained C1/C2/C3 variants, with the reference's constructor
+    (problem.py:9-47).  The algorithm evaluates them with the built-in
+    kernels; ``evaluate`` is the FP64 torch form of the unconstrained ones,
+    used where the kernels do not apply (one objective, or more objectives
+    than variables)."""
 
     _kind = "DTLZ2"
+    _constrained = ("C1DTLZ1", "C1DTLZ3", "C2DTLZ2", "C2ConvexDTLZ2", "C3DTLZ1", "C3DTLZ4")
 
     def __init__(self, dimension: int = 12, objectives: int = 3,
                  constraint_activation_ratio: float = 0.0) -> None:
-        if constraint_activation_ratio != 0.0:
-            raise NotImplementedError("constraint activation is not supported by this engine")
-        super().__init__(dimension, objectives, 0.0, 1.0, 0, self._kind)
+        # dtlz_problem.cu:63-77: finite, clamped to [0, 1]; 0 keeps constraints on from the start
+        if not math.isfinite(constraint_activation_ratio):
+            raise ValueError("constraint_activation_ratio must be finite")
+        if constraint_activation_ratio > 0.0:
+            raise NotImplementedError("delayed constraint activation is not supported by this engine")
+        constrained = self._kind in self._constrained
+        super().__init__(dimension, objectives, 0.0, 1.0, 1 if constrained else 0, self._kind)
 
     def evaluate(self, variables: Any, context: dict[str, Any]) -> Any:
         x = variables.double()
         m = self.objectives
         xm, xk = x[:, : m - 1], x[:, m - 1:]
         kind = self._kind
+        if kind in self._constrained:
+            raise NotImplementedError(f"{kind} runs on the built-in kernels only (2 <= objectives <= dimension)")
         if kind in ("DTLZ1", "DTLZ3"):
             g = 100.0 * (xk.shape[1] + ((xk - 0.5) ** 2 - torch.cos(20 * math.pi * (xk - 0.5))).sum(1))
         elif kind in ("DTLZ2", "DTLZ4", "ConvexDTLZ2", "DTLZ5"):
@@ -149,12 +160,12 @@ DTLZ5 = _problem_class("DTLZ5")
 DTLZ6 = _problem_class("DTLZ6")
 DTLZ7 = _problem_class("DTLZ7")
 ConvexDTLZ2 = _problem_class("ConvexDTLZ2")
-C1DTLZ1 = _problem_class("C1DTLZ1", False)
-C1DTLZ3 = _problem_class("C1DTLZ3", False)
-C2DTLZ2 = _problem_class("C2DTLZ2", False)
-C2ConvexDTLZ2 = _problem_class("C2ConvexDTLZ2", False)
-C3DTLZ1 = _problem_class("C3DTLZ1", False)
-C3DTLZ4 = _problem_class("C3DTLZ4", False)
+C1DTLZ1 = _problem_class("C1DTLZ1")
+C1DTLZ3 = _problem_class("C1DTLZ3")
+C2DTLZ2 = _problem_class("C2DTLZ2")
+C2ConvexDTLZ2 = _problem_class("C2ConvexDTLZ2")
+C3DTLZ1 = _problem_class("C3DTLZ1")
+C3DTLZ4 = _problem_class("C3DTLZ4")
 CSDP = _problem_class("CSDP", False)
 
 __all__ = ["PythonProblem", "QuadraticCenters", "SeededMLP", "DTLZProblem", "DTLZ1", "DTLZ2",

@@ -243,7 +243,12 @@ int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int 
 /* DTLZ1-DTLZ7 (problem 1-7) or ConvexDTLZ2 (problem 8), as the reference's
  * built-in kernels (problem/dtlz/dtlz_kernels.cu:28-717): DTLZ1 clamps x to
  * [0, 1], the spherical problems use max(0, cos), DTLZ4 maps x to x^100 and
- * DTLZ5 clamps its theta to [0, 1]; cv = 0.  Needs 2 <= m <= d. */
+ * DTLZ5 clamps its theta to [0, 1]; cv = 0.  Problems 9-14 are the
+ * constrained variants C1-DTLZ1, C1-DTLZ3, C2-DTLZ2, C2-ConvexDTLZ2, C3-DTLZ1
+ * and C3-DTLZ4 (dtlz_kernels.cu:824-1165): their base problem's objectives
+ * and the violation in cv, constraints always active.  C1-DTLZ3 and C2-DTLZ2
+ * take m in {2, 3, 5, 8, 10}, C2-ConvexDTLZ2 m in {3, 5, 8, 10}.  Needs
+ * 2 <= m <= d. */
 int moea_eval_dtlz(int problem, const float* x, int n, int d, int m, float* f, int ldf, float* cv,
                    void* stream);
 
@@ -309,6 +314,12 @@ int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_a
 #define MOEA_PROBLEM_DTLZ6 8
 #define MOEA_PROBLEM_DTLZ7 9
 #define MOEA_PROBLEM_CONVEX_DTLZ2 10
+#define MOEA_PROBLEM_C1_DTLZ1 11
+#define MOEA_PROBLEM_C1_DTLZ3 12
+#define MOEA_PROBLEM_C2_DTLZ2 13
+#define MOEA_PROBLEM_C2_CONVEX_DTLZ2 14
+#define MOEA_PROBLEM_C3_DTLZ1 15
+#define MOEA_PROBLEM_C3_DTLZ4 16
 
 /* Problem evaluator callback: the C form of IProblemEvaluator::evaluate
  * (problem/problem_evaluator.cuh:10-31).  Must write f (m, n) and cv (n) for

@@ -106,6 +106,15 @@ def test_api_validation_errors():
                    reference_directions=cm.PythonReferenceDirections(),
                    environment_selector=cm.PythonEnvironmentSelector())
     assert not alg.initialized
+    # dtlz_problem.cu:63-77: a finite ratio, clamped to [0, 1]; constrained variants declare one constraint
+    assert cm.C1DTLZ1(12, 3).constraints == 1 and cm.DTLZ2(12, 3).constraints == 0
+    assert cm.C3DTLZ4(12, 3, constraint_activation_ratio=-1.0).constraints == 1
+    with pytest.raises(ValueError, match="finite"):
+        cm.C1DTLZ1(12, 3, constraint_activation_ratio=float("nan"))
+    with pytest.raises(NotImplementedError):
+        cm.C1DTLZ1(12, 3, constraint_activation_ratio=0.5)
+    with pytest.raises(NotImplementedError):
+        cm.CSDP(12, 3)
     b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(
         cm.QuadraticCenters(12, 3))
     alg = b.build()

@@ -159,3 +159,78 @@ def test_dtlz_suite_runs_native(kind):
     res = alg.run()
     x = res.variables.cpu().numpy()
     _close_scaled(res.objectives.cpu().numpy().T, dtlz_fp64_suite(kind, x, 3), 3)
+
+
+# constrained variant -> (moea_eval_dtlz kind, base kind); radii of dtlz_kernels.cu:1110-1158
+CONSTRAINED = {"C1DTLZ1": (9, 1), "C1DTLZ3": (10, 3), "C2DTLZ2": (11, 2), "C2ConvexDTLZ2": (12, 8),
+               "C3DTLZ1": (13, 1), "C3DTLZ4": (14, 4)}
+RADIUS = {"C1DTLZ3": {2: 6.0, 3: 5.0, 5: 12.5, 8: 15.0, 10: 15.0},
+          "C2DTLZ2": {2: 0.15, 3: 0.40, 5: 1.0, 8: 1.0, 10: 1.0},
+          "C2ConvexDTLZ2": {3: 0.20, 5: 0.225, 8: 0.26, 10: 0.26}}
+
+
+def cv_fp64(name, f):
+    """The constraint violations of C1/C2/C3 (Jain and Deb 2014, as
+    dtlz_kernels.cu:824-1044 writes them), in FP64 from objectives f (m, n)."""
+    f = f.astype(np.float64)
+    m = f.shape[0]
+    r = np.float64(np.float32(RADIUS.get(name, {}).get(m, 1.0)))
+    s2 = (f ** 2).sum(0)
+    if name == "C1DTLZ1":
+        return np.maximum(f[m - 1] / 0.6 + (f[: m - 1] / 0.5).sum(0) - 1.0, 0.0)
+    if name == "C1DTLZ3":
+        return np.maximum((s2 - 16.0) * (r * r - s2), 0.0)
+    if name == "C2DTLZ2":
+        lo = (((f - 1.0) ** 2 + s2 - f ** 2) / (r * r) - 1.0).min(0)
+        centre = ((f - 1.0 / math.sqrt(m)) ** 2).sum(0) / (r * r) - 1.0
+        return np.maximum(np.minimum(lo, centre), 0.0)
+    if name == "C2ConvexDTLZ2":
+        mean = f.mean(0)
+        return np.maximum(1.0 - (s2 - m * mean ** 2) / (r * r), 0.0)
+    if name == "C3DTLZ1":
+        return np.maximum(f + 1.0 - 2.0 * f.sum(0), 0.0).sum(0)
+    return np.maximum(0.75 * f ** 2 - s2 + 1.0, 0.0).sum(0)
+
+
+@pytest.mark.parametrize("name", sorted(CONSTRAINED))
+@pytest.mark.parametrize("m", [3, 5, 8])
+def test_constrained_dtlz_kernels(name, m):
+    kind, base = CONSTRAINED[name]
+    rng = np.random.default_rng(m * 31 + kind)
+    n, d = 700, m + 6
+    x = rng.random((n, d), dtype=np.float32)
+    # near the Pareto set, so that both feasible and infeasible rows occur
+    x[:, m - 1:] = (0.5 + 0.05 * (rng.random((n, d - m + 1)) - 0.5)).astype(np.float32)
+    print(f"constrained {name} m={m}", flush=True)
+    xt = torch.from_numpy(x).to(DEV)
+    f, cv = K.eval_dtlz(kind, xt, m)
+    fb, _ = K.eval_dtlz(base, xt, m)
+    assert torch.equal(f, fb)                      # the base problem's objectives, bit for bit
+    f64 = f.cpu().numpy()
+    ref = cv_fp64(name, f64)
+    scale = 1.0 + (f64.astype(np.float64) ** 2).sum(0) + np.abs(f64).sum(0)
+    got = cv.cpu().numpy()
+    assert (got >= 0).all()
+    bad = np.abs(got - ref) > 1e-5 * scale
+    assert not bad.any(), (name, m, got[bad][:5], ref[bad][:5])
+
+
+def test_constrained_dtlz_runs_native():
+    for name in ("C1DTLZ1", "C2DTLZ2", "C3DTLZ4"):
+        prob = getattr(cm, name)(dimension=12, objectives=3)
+        assert prob.constraints == 1
+        alg = cm.NSGA3(population_size=92, max_generations=10, problem=prob, seed=2, print_progress=False)
+        alg.initialize()
+        assert alg._args.problem.kind == _lib.DTLZ_KINDS[name]
+        res = alg.run()
+        f = res.objectives.cpu().numpy().T
+        ref = cv_fp64(name, f)
+        scale = 1.0 + (f.astype(np.float64) ** 2).sum(0) + np.abs(f).sum(0)
+        assert (np.abs(res.constraints.cpu().numpy() - ref) <= 1e-5 * scale).all()
+
+
+def test_constrained_dtlz_rejects_undefined_objective_counts():
+    alg = cm.NSGA3(population_size=92, max_generations=3, problem=cm.C2ConvexDTLZ2(dimension=12, objectives=2),
+                   print_progress=False)
+    with pytest.raises((ValueError, RuntimeError), match="objective count"):
+        alg.initialize()
