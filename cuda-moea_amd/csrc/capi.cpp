@@ -151,6 +151,12 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k)
 
 int moea_set_niching(int mode, int lds_cap) { return set_niching(mode, lds_cap); }
 
+int moea_quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization* q, void* stream)
+{
+    if (!q) return fail(kInvalidArgument, "moea_quantize_cv: null parameters");
+    return quantize_cv(cv, out, n, *q, static_cast<hipStream_t>(stream));
+}
+
 int moea_nsga3_select(const moea_nsga3_select_args* args)
 {
     if (!args) return fail(kInvalidArgument, "moea_nsga3_select: null args");

@@ -72,7 +72,8 @@ size_t run_workspace_bytes(int algorithm, int n, int m, int k)
 {
     if (algorithm == MOEA_ALGORITHM_NSGA3) {
         const size_t a = nsga3_workspace_bytes(n, m, k);
-        const size_t b = nds_workspace_bytes(n);
+        // the parents' sort, then room for their quantized violations
+        const size_t b = (nds_workspace_bytes(n) + 255) / 256 * 256 + sizeof(float) * static_cast<size_t>(n);
         return a > b ? a : b;
     }
     return rvea_workspace_bytes(n, m, k);
@@ -88,7 +89,18 @@ int run_init(moea_run_args& a)
     if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
         // NSGA3EnvironmentSelector::prepare: parent fronts over n with stop at n,
         // and the initial ideal point (nsga3_environment_selector.cu:236-287)
-        MOEA_CHECK(nds_run(par.f, a.n, par.cv, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init,
+        const float* cv_sort = par.cv;
+        if (a.cv_quant.bins > 0) {
+            // CV quantization of the parents' sort (nsga3_environment_selector.cu:257-262): the
+            // bins sit in the workspace past the sort's own part
+            const size_t nds_bytes = (nds_workspace_bytes(a.n) + 255) / 256 * 256;
+            if (nds_bytes + sizeof(float) * static_cast<size_t>(a.n) > a.workspace_bytes)
+                return fail(kWorkspaceTooSmall, "moea_run_init: workspace too small");
+            float* q = reinterpret_cast<float*>(static_cast<char*>(a.workspace) + nds_bytes);
+            MOEA_CHECK(quantize_cv(par.cv, q, a.n, a.cv_quant, st));
+            cv_sort = q;
+        }
+        MOEA_CHECK(nds_run(par.f, a.n, cv_sort, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init,
                            a.workspace, a.workspace_bytes, st));
         MOEA_CHECK(ideal_point(par.f, a.n, a.n, a.m, a.ideal, st));
     } else {
@@ -203,6 +215,7 @@ int run_step(moea_run_args& a, int generations)
             s.seed = a.seed;
             s.generation = g;
             s.directions_ready = a.directions_ready;
+            s.cv_quant = a.cv_quant;
             s.stats = a.stats;
             s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
             s.stream = a.stream;

@@ -20,6 +20,8 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 size_t nsga3_workspace_bytes(int n, int m, int k);
 int nsga3_select(const moea_nsga3_select_args& a);
 int set_niching(int mode, int lds_cap);
+bool cv_quant_valid(const moea_cv_quantization& q);
+int quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization& q, hipStream_t stream);
 // Cone tables of the pruned association, built once per set of directions.
 bool assoc_prunable(int k, int m);
 size_t cone_bytes(int m, int k);

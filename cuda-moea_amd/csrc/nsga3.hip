@@ -9,6 +9,8 @@
 //
 // Merged index space: s in [0, n) is parent s, s in [n, 2n) is offspring s-n.
 // Variables are never merged; the survivor gather reads them in place.
+#include <cmath>
+
 #include "batch_internal.hpp"
 #include "common.hpp"
 #include "moea_internal.hpp"
@@ -1809,8 +1811,24 @@ __global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restric
     if (threadIdx.x < kMetaLen) stats[threadIdx.x] = meta[threadIdx.x];
 }
 
+// quantize_cv / cv_quantize_kernel (selection/nsga3/kernels/cv_quantization.cu:8-113):
+// the bin of each violation, 0 when feasible.
+__global__ void quantize_cv_kernel(const float* __restrict__ cv, float* __restrict__ out, int n, int bins,
+                                   float clip, float inv_clip, float log_alpha, float inv_clip_log, float eps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float raw = cv[i];
+    const float u = fminf(fmaxf(raw, 0.0f), clip);
+    const float x = (log_alpha > 0.0f) ? log1pf(log_alpha * u) * inv_clip_log : u * inv_clip;
+    int bin = 1 + static_cast<int>(floorf(x * static_cast<float>(bins)));
+    bin = max(1, min(bins, bin));
+    const bool infeasible = (eps == 0.0f) ? (raw != 0.0f) : (raw > eps);
+    out[i] = infeasible ? static_cast<float>(bin) : 0.0f;
+}
+
 struct Layout {
-    float* fm; float* cvm; int32_t* fronts; int32_t* meta; int32_t* gidx; float* fn;
+    float* fm; float* cvm; float* cvq; int32_t* fronts; int32_t* meta; int32_t* gidx; float* fn;
     float* asf_val; int32_t* asf_idx; float* row_max; float* a; float* omin;
     float* part_d2; int32_t* part_idx; float* dist; int32_t* ref; int32_t* rho; ConeBufs cones;
     uint64_t* k0; uint64_t* k1; int32_t* v0; int32_t* v1; void* sort_tmp; size_t sort_bytes;
@@ -1825,6 +1843,7 @@ Layout carve(void* ws, size_t cap, int n, int m, int k, Carve& c)
     Layout L{};
     L.fm = c.take<float>(static_cast<size_t>(m) * nm);
     L.cvm = c.take<float>(nm);
+    L.cvq = c.take<float>(nm);
     L.fronts = c.take<int32_t>(nm);
     L.meta = c.take<int32_t>(kMetaLen);
     L.gidx = c.take<int32_t>(nm);
@@ -1886,6 +1905,32 @@ int cones_build(const float* w, int k, int m, void* buf, hipStream_t st)
         return fail(kInvalidArgument, "cones_build: invalid");
     Carve c(buf, ~size_t{0});
     return cones_prepare(w, k, m, carve_cones(c, m, k), st);
+}
+
+bool cv_quant_valid(const moea_cv_quantization& q)
+{
+    // validate_cv_quant_params (cv_quantization.cu:93-112)
+    if (q.bins <= 0) return false;
+    if (!(q.clip_upper > 0.0f) || !std::isfinite(q.clip_upper)) return false;
+    if (q.log_alpha < 0.0f || !std::isfinite(q.log_alpha)) return false;
+    if (q.feasibility_epsilon < 0.0f || !std::isfinite(q.feasibility_epsilon)) return false;
+    if (q.log_alpha > 0.0f) {
+        const float clip_log = std::log1p(q.log_alpha * q.clip_upper);
+        if (!(clip_log > 0.0f) || !std::isfinite(clip_log)) return false;
+    }
+    return true;
+}
+
+int quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization& q, hipStream_t st)
+{
+    if (!cv || !out || n <= 0) return fail(kInvalidArgument, "quantize_cv: invalid arguments");
+    if (!cv_quant_valid(q)) return fail(kInvalidArgument, "Invalid NSGA-III CV quantization configuration");
+    const float inv_clip = 1.0f / q.clip_upper;
+    const float inv_clip_log = (q.log_alpha > 0.0f) ? 1.0f / std::log1p(q.log_alpha * q.clip_upper) : 0.0f;
+    hipLaunchKernelGGL(quantize_cv_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, cv, out, n, q.bins,
+                       q.clip_upper, inv_clip, q.log_alpha, inv_clip_log, q.feasibility_epsilon);
+    MOEA_HIP(hipGetLastError());
+    return kOk;
 }
 
 size_t nsga3_workspace_bytes(int n, int m, int k)
@@ -2021,7 +2066,13 @@ int nsga3_select(const moea_nsga3_select_args& A)
     hipLaunchKernelGGL(merge_objectives, dim3(nb), dim3(256), 0, st, A.parents.f, A.offspring.f,
                        A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm, L.omin);
     MOEA_HIP(hipGetLastError());
-    MOEA_CHECK(nds_run(L.fm, nm, L.cvm, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
+    const float* cv_sort = L.cvm;
+    if (A.cv_quant.bins > 0) {
+        // the sort compares bins; the survivors keep their violations
+        MOEA_CHECK(quantize_cv(L.cvm, L.cvq, nm, A.cv_quant, st));
+        cv_sort = L.cvq;
+    }
+    MOEA_CHECK(nds_run(L.fm, nm, cv_sort, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
     stage_begin(kStSelectPrep, st);
     const int nsb = ceil_div(nm, kScanThreads);
     hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);

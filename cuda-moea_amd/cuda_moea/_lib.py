@@ -36,6 +36,11 @@ class PopulationC(ctypes.Structure):
     _fields_ = [("x", c_void_p), ("f", c_void_p), ("cv", c_void_p)]
 
 
+class CvQuantization(ctypes.Structure):
+    _fields_ = [("bins", c_int), ("clip_upper", c_float), ("log_alpha", c_float),
+                ("feasibility_epsilon", c_float)]
+
+
 class NSGA3SelectArgs(ctypes.Structure):
     _fields_ = [
         ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int),
@@ -44,7 +49,7 @@ class NSGA3SelectArgs(ctypes.Structure):
         ("draws", POINTER(Draws)), ("seed", c_uint64), ("generation", c_int),
         ("directions_ready", c_int),
         ("stats", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
-        ("stream", c_void_p),
+        ("stream", c_void_p), ("cv_quant", CvQuantization),
     ]
 
 
@@ -115,7 +120,7 @@ class RunArgs(ctypes.Structure):
         ("initial_x_given", c_int), ("problem", ProblemDesc), ("generation", c_int),
         ("directions_ready", c_int),
         ("workspace", c_void_p), ("workspace_bytes", c_size_t), ("stream", c_void_p),
-        ("hooks", ctypes.POINTER(StrategyHooks)),
+        ("hooks", ctypes.POINTER(StrategyHooks)), ("cv_quant", CvQuantization),
     ]
 
 
@@ -149,6 +154,7 @@ SIGNATURES = {
     "moea_associate": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                c_int, c_void_p, c_size_t, c_void_p]),
     "moea_reproduce": (c_int, [POINTER(ReproduceArgs)]),
+    "moea_quantize_cv": (c_int, [c_void_p, c_void_p, c_int, POINTER(CvQuantization), c_void_p]),
     "moea_annealed": (c_float, [c_float, c_float, c_int, c_int]),
     "moea_init_population": (c_int, [c_void_p, c_int, c_int, c_void_p, c_uint64, c_void_p]),
     "moea_eval_quadratic": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int,

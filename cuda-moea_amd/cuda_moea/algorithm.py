@@ -53,6 +53,16 @@ def _checked(value: Any, name: str, device: int, dtype: torch.dtype) -> torch.Te
     return value.to(dtype).contiguous()
 
 
+def _cv_quant_valid(sel: Any) -> bool:
+    # validate_cv_quant_params (cv_quantization.cu:93-112)
+    clip, alpha, eps = float(sel.cv_clip_upper), float(sel.cv_log_alpha), float(sel.feasibility_epsilon)
+    if not (clip > 0.0 and math.isfinite(clip)) or not (alpha >= 0.0 and math.isfinite(alpha)):
+        return False
+    if not (eps >= 0.0 and math.isfinite(eps)):
+        return False
+    return alpha == 0.0 or (math.log1p(alpha * clip) > 0.0 and math.isfinite(math.log1p(alpha * clip)))
+
+
 def _unsupported(obj: Any, what: str) -> NotImplementedError:
     return NotImplementedError(f"{type(obj).__name__} is not supported as {what} by this engine")
 
@@ -132,8 +142,9 @@ class Algorithm:
                 raise _unsupported(sel, "NSGA-III environment selector")
             if not sel.sparse_ratio > 0.0:
                 raise ValueError("NSGA-III sparse_ratio must be positive")
-            if sel.cv_bins > 0:
-                raise NotImplementedError("NSGA-III CV quantization (cv_bins > 0) is not supported")
+            if sel.cv_bins > 0 and not _cv_quant_valid(sel):
+                # nsga3_environment_selector.cu:216-220
+                raise ValueError("Invalid NSGA-III CV quantization configuration")
             if isinstance(rd, AdaptiveRVEADirections):
                 raise ValueError("NSGA-III takes Das-Dennis or user-defined directions")
         else:
@@ -219,6 +230,9 @@ class Algorithm:
             # fixed (user-defined) directions never adapt: an interval past the run
             a.rvea_frequency = freq if isinstance(self.reference_directions, AdaptiveRVEADirections) else 1e9
         a.seed = self._run_seed()
+        sel = self.environment_selector
+        if isinstance(sel, NSGA3EnvironmentSelector) and sel.cv_bins > 0:
+            a.cv_quant = K.cv_quantization(sel.cv_bins, sel.cv_clip_upper, sel.cv_log_alpha, sel.feasibility_epsilon)
         self._by_ptr = {pop.x.data_ptr(): pop for pop in self._pops}
         self._setup_problem(a)
         self._setup_hooks(a)

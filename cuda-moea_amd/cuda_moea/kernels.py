@@ -146,9 +146,23 @@ def associate(fn: torch.Tensor, a: torch.Tensor, directions: torch.Tensor, metho
     return dist, ref
 
 
+def cv_quantization(bins: int = 0, clip_upper: float = 0.0, log_alpha: float = 0.0,
+                    feasibility_epsilon: float = 0.0) -> _lib.CvQuantization:
+    return _lib.CvQuantization(int(bins), float(clip_upper), float(log_alpha), float(feasibility_epsilon))
+
+
+def quantize_cv(cv: torch.Tensor, **params) -> torch.Tensor:
+    """The sort's bins of the violations cv (n,): 0 feasible, else 1..bins."""
+    _dev(cv, _F32, "cv")
+    out = torch.empty_like(cv)
+    q = cv_quantization(**params)
+    check(lib.moea_quantize_cv(cv.data_ptr(), out.data_ptr(), cv.numel(), ctypes.byref(q), _stream(cv.device)))
+    return out
+
+
 def nsga3_select(parents: Population, offspring: Population, directions: torch.Tensor,
                  ideal: torch.Tensor, draws: Optional[dict] = None, seed: int = 0,
-                 generation: int = 0):
+                 generation: int = 0, cv_quant: Optional[dict] = None):
     """One NSGA-III selection.  ``ideal`` is updated in place.  Returns
     (next Population, next_fronts, stats int32[8] = max_front, n_prior, n_last,
     fallback, n_rem, ...)."""
@@ -173,6 +187,8 @@ def nsga3_select(parents: Population, offspring: Population, directions: torch.T
     a.seed, a.generation = seed, generation
     a.stats, a.workspace, a.workspace_bytes = stats.data_ptr(), ws.data_ptr(), ws.numel()
     a.stream = _stream(dev)
+    if cv_quant:
+        a.cv_quant = cv_quantization(**cv_quant)
     check(lib.moea_nsga3_select(ctypes.byref(a)))
     return nxt, nfr, stats
 

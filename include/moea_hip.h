@@ -119,6 +119,22 @@ typedef struct moea_population {
     float* cv;       /* (n,)   */
 } moea_population;
 
+/* NSGA-III CV quantization (selection/nsga3/kernels/cv_quantization.cu):
+ * before non-dominated sorting, each violation is replaced by its bin, 0 when
+ * feasible (cv == 0, or cv <= feasibility_epsilon when that is positive) and
+ * otherwise 1 + floor(B x) clamped to [1, B], where x = min(max(cv, 0),
+ * clip_upper) / clip_upper, or log1p(log_alpha u) / log1p(log_alpha
+ * clip_upper) when log_alpha > 0.  Only the sort sees the bins; survivors
+ * keep their violations.  bins = 0 turns it off. */
+typedef struct moea_cv_quantization {
+    int bins;
+    float clip_upper, log_alpha, feasibility_epsilon;
+} moea_cv_quantization;
+
+/* The bins of cv (n) into out (n); MOEA_E_INVALID for parameters
+ * the reference rejects (validate_cv_quant_params). */
+int moea_quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization* q, void* stream);
+
 typedef struct moea_nsga3_select_args {
     int n, m, d, k;
     moea_population parents;    /* input */
@@ -139,6 +155,7 @@ typedef struct moea_nsga3_select_args {
     void* workspace;
     size_t workspace_bytes;
     void* stream;
+    moea_cv_quantization cv_quant;   /* bins = 0: the sort compares raw violations */
 } moea_nsga3_select_args;
 
 size_t moea_nsga3_workspace_bytes(int n, int m, int k);
@@ -417,6 +434,7 @@ typedef struct moea_run_args {
     size_t workspace_bytes;
     void* stream;
     const moea_strategy_hooks* hooks;  /* NULL: every strategy built in */
+    moea_cv_quantization cv_quant;     /* NSGA-III; bins = 0: off */
 } moea_run_args;
 
 size_t moea_run_workspace_bytes(int algorithm, int n, int m, int k);

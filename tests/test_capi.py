@@ -50,7 +50,8 @@ def test_host_side_calls():
 STRUCTS = {"Draws": "moea_draws", "PopulationC": "moea_population", "NSGA3SelectArgs": "moea_nsga3_select_args",
            "ReproduceArgs": "moea_reproduce_args", "RVEASelectArgs": "moea_rvea_select_args",
            "ProblemDesc": "moea_problem_desc", "RunArgs": "moea_run_args", "BatchProblem": "moea_batch_problem",
-           "ServeConfig": "moea_serve_config", "StrategyHooks": "moea_strategy_hooks"}
+           "ServeConfig": "moea_serve_config", "StrategyHooks": "moea_strategy_hooks",
+           "CvQuantization": "moea_cv_quantization"}
 
 
 def test_ctypes_structs_match_header_layout(tmp_path):
@@ -99,8 +100,12 @@ def test_api_validation_errors():
         cm.NSGA3(crossover=cm.SBX(probability=2.0))
     with pytest.raises(ValueError):
         cm.RVEA(environment_selector=cm.RVEAEnvironmentSelector(alpha=0.0))
-    with pytest.raises(NotImplementedError):
-        cm.NSGA3(environment_selector=cm.NSGA3EnvironmentSelector(cv_bins=4))
+    # CV quantization parameters as validate_cv_quant_params checks them (cv_quantization.cu:93-112)
+    cm.NSGA3(environment_selector=cm.NSGA3EnvironmentSelector(cv_bins=4, cv_clip_upper=2.0, cv_log_alpha=3.0))
+    for bad in (dict(cv_clip_upper=0.0), dict(cv_clip_upper=1.0, cv_log_alpha=-1.0),
+                dict(cv_clip_upper=float("inf")), dict(cv_clip_upper=1.0, feasibility_epsilon=-0.1)):
+        with pytest.raises(ValueError, match="CV quantization"):
+            cm.NSGA3(environment_selector=cm.NSGA3EnvironmentSelector(cv_bins=4, **bad))
     # every strategy takes a Python object (adapters.cpp:196-434)
     alg = cm.NSGA3(mating=cm.PythonMating(), crossover=cm.PythonCrossover(), mutation=cm.PythonMutation(),
                    reference_directions=cm.PythonReferenceDirections(),
