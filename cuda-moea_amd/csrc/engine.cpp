@@ -17,7 +17,47 @@ namespace {
 
 __global__ void set_i32(int32_t* p, int v) { p[0] = v; }
 
+__global__ void zero_f32(float* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+bool constrained_dtlz(int kind) { return kind >= MOEA_PROBLEM_C1_DTLZ1 && kind <= MOEA_PROBLEM_C3_DTLZ4; }
+
+// MOEAStdTestEvaluator::is_cv_active (dtlz_kernels.cu:1300-1309): constraints of
+// an evaluation at generation g count once (g + 1) / max_generations reaches
+// the ratio; the initial population (g = -1) is evaluated without them.
+bool cv_active(const moea_run_args& a, int g)
+{
+    const float ratio = a.problem.cv_activation_ratio;
+    if (!constrained_dtlz(a.problem.kind) || !(ratio > 0.0f)) return true;
+    if (g < 0) return false;
+    const int total = a.max_generations > 0 ? a.max_generations : 1;
+    return static_cast<float>(g + 1) / static_cast<float>(total) >= ratio;
+}
+
+// is_cv_activation_switch_node (dtlz_kernels.cu:1311-1322)
+bool cv_switch(const moea_run_args& a, int g)
+{
+    if (!constrained_dtlz(a.problem.kind) || !(a.problem.cv_activation_ratio > 0.0f)) return false;
+    return g == 0 ? cv_active(a, 0) : (!cv_active(a, g - 1) && cv_active(a, g));
+}
+
+int evaluate_problem(const moea_run_args& a, const moea_population& p, hipStream_t st, int generation);
+
 int evaluate(const moea_run_args& a, const moea_population& p, hipStream_t st, int generation)
+{
+    MOEA_CHECK(evaluate_problem(a, p, st, generation));
+    if (!cv_active(a, generation)) {
+        // MOEAStdTestEvaluator::evaluate: violations count only once active
+        hipLaunchKernelGGL(zero_f32, dim3(ceil_div(a.n, 256)), dim3(256), 0, st, p.cv, a.n);
+        MOEA_HIP(hipGetLastError());
+    }
+    return kOk;
+}
+
+int evaluate_problem(const moea_run_args& a, const moea_population& p, hipStream_t st, int generation)
 {
     const moea_problem_desc& pr = a.problem;
     switch (pr.kind) {
@@ -66,6 +106,25 @@ int check(const moea_run_args& a)
     return kOk;
 }
 
+// NSGA3EnvironmentSelector::prepare's sort: the parents' fronts over n with
+// stop at n (nsga3_environment_selector.cu:236-275), on quantized violations
+// when CV quantization is on (the bins sit in the workspace past the sort's
+// own part).
+int parent_fronts(moea_run_args& a, const moea_population& par, hipStream_t st)
+{
+    const float* cv_sort = par.cv;
+    if (a.cv_quant.bins > 0) {
+        const size_t nds_bytes = (nds_workspace_bytes(a.n) + 255) / 256 * 256;
+        if (nds_bytes + sizeof(float) * static_cast<size_t>(a.n) > a.workspace_bytes)
+            return fail(kWorkspaceTooSmall, "moea_run: workspace too small");
+        float* q = reinterpret_cast<float*>(static_cast<char*>(a.workspace) + nds_bytes);
+        MOEA_CHECK(quantize_cv(par.cv, q, a.n, a.cv_quant, st));
+        cv_sort = q;
+    }
+    return nds_run(par.f, a.n, cv_sort, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init, a.workspace,
+                   a.workspace_bytes, st);
+}
+
 }  // namespace
 
 size_t run_workspace_bytes(int algorithm, int n, int m, int k)
@@ -89,19 +148,7 @@ int run_init(moea_run_args& a)
     if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
         // NSGA3EnvironmentSelector::prepare: parent fronts over n with stop at n,
         // and the initial ideal point (nsga3_environment_selector.cu:236-287)
-        const float* cv_sort = par.cv;
-        if (a.cv_quant.bins > 0) {
-            // CV quantization of the parents' sort (nsga3_environment_selector.cu:257-262): the
-            // bins sit in the workspace past the sort's own part
-            const size_t nds_bytes = (nds_workspace_bytes(a.n) + 255) / 256 * 256;
-            if (nds_bytes + sizeof(float) * static_cast<size_t>(a.n) > a.workspace_bytes)
-                return fail(kWorkspaceTooSmall, "moea_run_init: workspace too small");
-            float* q = reinterpret_cast<float*>(static_cast<char*>(a.workspace) + nds_bytes);
-            MOEA_CHECK(quantize_cv(par.cv, q, a.n, a.cv_quant, st));
-            cv_sort = q;
-        }
-        MOEA_CHECK(nds_run(par.f, a.n, cv_sort, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init,
-                           a.workspace, a.workspace_bytes, st));
+        MOEA_CHECK(parent_fronts(a, par, st));
         MOEA_CHECK(ideal_point(par.f, a.n, a.n, a.m, a.ideal, st));
     } else {
         MOEA_CHECK(rvea_gamma(a.directions, a.m, a.k, a.gamma, st));
@@ -137,6 +184,14 @@ int run_step(moea_run_args& a, int generations)
         moea_population& par = a.pops[a.current];
         moea_population& ofs = a.pops[off];
         moea_population& nx = a.pops[nxt];
+        if (cv_switch(a, g)) {
+            // DTLZProblem::prepare_parent (dtlz_kernels.cu:1363-1394, algorithm.cu:233-241): the
+            // constraints switch on at g, so the parents' violations are recomputed (a fresh
+            // evaluation gives the same objectives) and NSGA-III re-sorts its parent fronts
+            MOEA_CHECK(evaluate_problem(a, par, st, g));
+            if (a.algorithm == MOEA_ALGORITHM_NSGA3 && !(a.hooks && a.hooks->select))
+                MOEA_CHECK(parent_fronts(a, par, st));
+        }
         moea_reproduce_args r{};
         r.n = a.n; r.d = a.d;
         r.active = a.n;

@@ -84,6 +84,7 @@ class ProblemDesc(ctypes.Structure):
     _fields_ = [
         ("kind", c_int), ("centers", c_void_p), ("w1", c_void_p), ("w2", c_void_p),
         ("hidden", c_int), ("hwork", c_void_p), ("callback", EVALUATE_FN), ("user", c_void_p),
+        ("cv_activation_ratio", c_float),
     ]
 
 

@@ -312,6 +312,7 @@ class Algorithm:
         elif isinstance(p, DTLZProblem) and p._kind in _lib.DTLZ_KINDS and 2 <= self.m <= self.d:
             # built-in objective kernels, as the reference's DTLZ problems are native
             pd.kind = _lib.DTLZ_KINDS[p._kind]
+            pd.cv_activation_ratio = p.constraint_activation_ratio
         else:
             if hasattr(p, "initialize"):
                 p.initialize({"device": self.cuda.device_id})

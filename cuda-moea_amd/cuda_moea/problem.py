@@ -81,8 +81,7 @@ class DTLZProblem(PythonProblem):
         # dtlz_problem.cu:63-77: finite, clamped to [0, 1]; 0 keeps constraints on from the start
         if not math.isfinite(constraint_activation_ratio):
             raise ValueError("constraint_activation_ratio must be finite")
-        if constraint_activation_ratio > 0.0:
-            raise NotImplementedError("delayed constraint activation is not supported by this engine")
+        self.constraint_activation_ratio = min(max(float(constraint_activation_ratio), 0.0), 1.0)
         constrained = self._kind in self._constrained
         super().__init__(dimension, objectives, 0.0, 1.0, 1 if constrained else 0, self._kind)
 

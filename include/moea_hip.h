@@ -354,6 +354,12 @@ typedef struct moea_problem_desc {
     float* hwork;             /* mlp: (n, hidden) scratch */
     moea_evaluate_fn callback;
     void* user;
+    /* Constrained DTLZ variants (MOEA_PROBLEM_C*): constraints count from the
+     * generation g whose (g + 1) / max_generations reaches this ratio; before
+     * it cv = 0, and at that generation the parents' cv is recomputed and the
+     * NSGA-III parent fronts re-sorted (dtlz_kernels.cu:1295-1394).  <= 0:
+     * always on. */
+    float cv_activation_ratio;
 } moea_problem_desc;
 
 /* Strategy hooks: the C form of the reference's strategy plugins

@@ -116,8 +116,8 @@ def test_api_validation_errors():
     assert cm.C3DTLZ4(12, 3, constraint_activation_ratio=-1.0).constraints == 1
     with pytest.raises(ValueError, match="finite"):
         cm.C1DTLZ1(12, 3, constraint_activation_ratio=float("nan"))
-    with pytest.raises(NotImplementedError):
-        cm.C1DTLZ1(12, 3, constraint_activation_ratio=0.5)
+    assert cm.C1DTLZ1(12, 3, constraint_activation_ratio=0.5).constraint_activation_ratio == 0.5
+    assert cm.C1DTLZ1(12, 3, constraint_activation_ratio=7.0).constraint_activation_ratio == 1.0
     with pytest.raises(NotImplementedError):
         cm.CSDP(12, 3)
     b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(

@@ -234,3 +234,39 @@ def test_constrained_dtlz_rejects_undefined_objective_counts():
                    print_progress=False)
     with pytest.raises((ValueError, RuntimeError), match="objective count"):
         alg.initialize()
+
+
+@pytest.mark.parametrize("kind", ["NSGA3", "RVEA"])
+def test_constraint_activation_schedule(kind):
+    """constraint_activation_ratio = 0.5 over 10 generations: evaluations of
+    generations g with (g + 1) / 10 < 0.5 carry cv = 0; at g = 4 the parents'
+    violations are recomputed, and from then on every violation is the
+    problem's (dtlz_kernels.cu:1295-1394)."""
+    prob = cm.C1DTLZ1(dimension=12, objectives=3, constraint_activation_ratio=0.5)
+    alg = cm.Algorithm(kind, population_size=92 if kind == "NSGA3" else 100, max_generations=10, problem=prob,
+                       seed=4, print_progress=False)
+    alg.initialize()
+    print(f"activation {kind}", flush=True)
+
+    def rows():
+        # RVEA fills the slots past its active count with placeholders (cv 1e15)
+        return int(alg._active.item()) if kind == "RVEA" else 92
+
+    def cv_now():
+        return alg.population.constraints.cpu().numpy()[: rows()]
+
+    def true_cv():
+        x = alg.population.variables.contiguous()
+        return K.eval_dtlz(9, x, 3)[1].cpu().numpy()[: rows()]
+
+    assert not cv_now().any() and true_cv().any()
+    for _ in range(4):
+        alg.step()
+    assert not cv_now().any()                                         # generations 0-3: off
+    alg.step()                                                        # generation 4: on
+    np.testing.assert_array_equal(cv_now(), true_cv())
+    alg.run()
+    np.testing.assert_array_equal(cv_now(), true_cv())
+    # the schedule restarts with reset()
+    alg.reset()
+    assert not cv_now().any()
