@@ -23,7 +23,7 @@ __global__ void zero_f32(float* __restrict__ p, int n)
     if (i < n) p[i] = 0.0f;
 }
 
-bool constrained_dtlz(int kind) { return kind >= MOEA_PROBLEM_C1_DTLZ1 && kind <= MOEA_PROBLEM_C3_DTLZ4; }
+bool constrained_dtlz(int kind) { return kind >= MOEA_PROBLEM_C1_DTLZ1 && kind <= MOEA_PROBLEM_CSDP; }
 
 // MOEAStdTestEvaluator::is_cv_active (dtlz_kernels.cu:1300-1309): constraints of
 // an evaluation at generation g count once (g + 1) / max_generations reaches
@@ -75,6 +75,8 @@ int evaluate_problem(const moea_run_args& a, const moea_population& p, hipStream
         case MOEA_PROBLEM_C2_CONVEX_DTLZ2: case MOEA_PROBLEM_C3_DTLZ1: case MOEA_PROBLEM_C3_DTLZ4:
             // problem kinds 11..16 are the constrained variants (kernel kinds 9..14)
             return eval_dtlz(pr.kind - MOEA_PROBLEM_C1_DTLZ1 + 9, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
+        case MOEA_PROBLEM_CSDP:
+            return eval_dtlz(15, p.x, a.n, a.d, a.m, p.f, a.n, p.cv, st);
         case MOEA_PROBLEM_CALLBACK:
             if (!pr.callback) return fail(kInvalidArgument, "moea_run: callback problem without callback");
             return pr.callback(pr.user, p.x, a.n, a.d, a.m, p.f, p.cv, generation, a.max_generations, st);

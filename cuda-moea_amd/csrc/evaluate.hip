@@ -517,6 +517,46 @@ template <int M> __global__ __launch_bounds__(256) void c2_convex_dtlz2_kernel(c
 template <int M> __global__ __launch_bounds__(256) void c3_dtlz1_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 1, 5>(x, n, d, f, ldf, cv, R); }
 template <int M> __global__ __launch_bounds__(256) void c3_dtlz4_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ f, int ldf, float* __restrict__ cv, float R) { dtlz_rows<M, 4, 6>(x, n, d, f, ldf, cv, R); }
 
+// CSDP, the car side-impact design problem (csdp_func_kernel / csdp_cv_kernel,
+// dtlz_kernels.cu:719-738, 1046-1085): 7 variables, 3 objectives and the sum of
+// 10 constraint violations; one thread per row.  The objectives' constants are
+// double, as the reference writes them, and rounded to float once.
+__global__ __launch_bounds__(256) void csdp_kernel(const float* __restrict__ x, int n, float* __restrict__ f, int ldf,
+                                                   float* __restrict__ cv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = x + static_cast<size_t>(i) * 7;
+    const float x1 = r[0], x2 = r[1], x3 = r[2], x4 = r[3], x5 = r[4], x6 = r[5], x7 = r[6];
+    const float F = static_cast<float>(4.72 - 0.5 * x4 - 0.19 * x2 * x3);
+    const float vmbp = static_cast<float>(10.58 - 0.674 * x1 * x2 - 0.67275 * x2);
+    const float vfd = static_cast<float>(16.45 - 0.489 * x3 * x7 - 0.843 * x5 * x6);
+    f[i] = static_cast<float>(1.98 + 4.9 * x1 + 6.67 * x2 + 6.98 * x3 + 4.01 * x4 + 1.78 * x5 + 0.00001 * x6 +
+                              2.73 * x7);
+    f[static_cast<size_t>(ldf) + i] = F;
+    f[2 * static_cast<size_t>(ldf) + i] = 0.5f * (vmbp + vfd);
+    const float Ff = 4.72f - 0.5f * x4 - 0.19f * x2 * x3;
+    const float vmbpf = 10.58f - 0.674f * x1 * x2 - 0.67275f * x2;
+    const float vfdf = 16.45f - 0.489f * x3 * x7 - 0.843f * x5 * x6;
+    const float g[10] = {
+        1.16f - 0.3717f * x2 * x4 - 0.0092928f * x3 - 1.0f,
+        0.261f - 0.0159f * x1 * x2 - 0.06486f * x1 - 0.019f * x2 * x7 + 0.0144f * x3 * x5 + 0.0154464f * x6 - 0.32f,
+        0.214f + 0.00817f * x5 - 0.045195f * x1 - 0.0135168f * x1 + 0.03099f * x2 * x6 - 0.018f * x2 * x7 +
+            0.007176f * x3 + 0.023232f * x3 - 0.00364f * x5 * x6 - 0.018f * x2 * x2 - 0.32f,
+        0.74f - 0.61f * x2 - 0.031296f * x3 - 0.031872f * x7 + 0.227f * x2 * x2 - 0.32f,
+        28.98f + 3.818f * x3 - 4.2f * x1 * x2 + 1.27296f * x6 - 2.68065f * x7 - 32.0f,
+        33.86f + 2.95f * x3 - 5.057f * x1 * x2 - 3.795f * x2 - 3.4431f * x7 + 1.45728f - 32.0f,
+        46.36f - 9.9f * x2 - 4.4505f * x1 - 32.0f,
+        Ff - 4.0f,
+        vmbpf - 9.9f,
+        vfdf - 15.7f,
+    };
+    float total = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) total += fmaxf(g[j], 0.0f);
+    cv[i] = total;
+}
+
 // The radius of C1-DTLZ3, C2-DTLZ2 and C2-ConvexDTLZ2 by objective count
 // (compute_cv, dtlz_kernels.cu:1110-1158); 0 for a count the reference rejects.
 float dtlz_radius(int kind, int m)
@@ -545,8 +585,15 @@ float dtlz_radius(int kind, int m)
 
 int eval_dtlz(int kind, const float* x, int n, int d, int m, float* f, int ldf, float* cv, hipStream_t stream)
 {
+    if (kind == 15) {
+        if (!x || !f || !cv || n <= 0 || ldf < n) return fail(kInvalidArgument, "eval_dtlz: invalid arguments");
+        if (d != 7 || m != 3) return fail(kInvalidArgument, "CSDP needs 7 variables and 3 objectives");
+        hipLaunchKernelGGL(csdp_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, x, n, f, ldf, cv);
+        MOEA_HIP(hipGetLastError());
+        return kOk;
+    }
     if (!x || !f || n <= 0 || m <= 1 || m > kMaxM || d < m || ldf < n || kind < 1 || kind > 14)
-        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (kinds 1-14, 2 <= m <= d)");
+        return fail(kInvalidArgument, "eval_dtlz: invalid arguments (kinds 1-15, 2 <= m <= d)");
     if (kind > 8 && !cv) return fail(kInvalidArgument, "eval_dtlz: the constrained variants need a cv array");
     const float R = dtlz_radius(kind, m);
     if (R == 0.0f) return fail(kInvalidArgument, "eval_dtlz: objective count not defined for this constrained variant");

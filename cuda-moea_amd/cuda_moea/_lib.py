@@ -18,11 +18,11 @@ MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
 ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
 PROBLEM_QUADRATIC, PROBLEM_MLP, PROBLEM_CALLBACK, PROBLEM_DTLZ1, PROBLEM_DTLZ2 = 0, 1, 2, 3, 4
 PROBLEM_DTLZ3, PROBLEM_DTLZ4, PROBLEM_DTLZ5, PROBLEM_DTLZ6, PROBLEM_DTLZ7, PROBLEM_CONVEX_DTLZ2 = 5, 6, 7, 8, 9, 10
-# native DTLZ problem kinds by name (moea_eval_dtlz numbers the same problems 1-14)
+# native DTLZ problem kinds by name (moea_eval_dtlz numbers the same problems 1-15)
 DTLZ_KINDS = {"DTLZ1": PROBLEM_DTLZ1, "DTLZ2": PROBLEM_DTLZ2, "DTLZ3": PROBLEM_DTLZ3, "DTLZ4": PROBLEM_DTLZ4,
               "DTLZ5": PROBLEM_DTLZ5, "DTLZ6": PROBLEM_DTLZ6, "DTLZ7": PROBLEM_DTLZ7,
               "ConvexDTLZ2": PROBLEM_CONVEX_DTLZ2, "C1DTLZ1": 11, "C1DTLZ3": 12, "C2DTLZ2": 13,
-              "C2ConvexDTLZ2": 14, "C3DTLZ1": 15, "C3DTLZ4": 16}
+              "C2ConvexDTLZ2": 14, "C3DTLZ1": 15, "C3DTLZ4": 16, "CSDP": 17}
 MAX_OBJECTIVES = 10
 
 

@@ -74,7 +74,7 @@ class DTLZProblem(PythonProblem):
     than variables)."""
 
     _kind = "DTLZ2"
-    _constrained = ("C1DTLZ1", "C1DTLZ3", "C2DTLZ2", "C2ConvexDTLZ2", "C3DTLZ1", "C3DTLZ4")
+    _constrained = ("C1DTLZ1", "C1DTLZ3", "C2DTLZ2", "C2ConvexDTLZ2", "C3DTLZ1", "C3DTLZ4", "CSDP")
 
     def __init__(self, dimension: int = 12, objectives: int = 3,
                  constraint_activation_ratio: float = 0.0) -> None:
@@ -165,7 +165,7 @@ C2DTLZ2 = _problem_class("C2DTLZ2")
 C2ConvexDTLZ2 = _problem_class("C2ConvexDTLZ2")
 C3DTLZ1 = _problem_class("C3DTLZ1")
 C3DTLZ4 = _problem_class("C3DTLZ4")
-CSDP = _problem_class("CSDP", False)
+CSDP = _problem_class("CSDP")
 
 __all__ = ["PythonProblem", "QuadraticCenters", "SeededMLP", "DTLZProblem", "DTLZ1", "DTLZ2",
            "DTLZ3", "DTLZ4", "DTLZ5", "DTLZ6", "DTLZ7", "ConvexDTLZ2", "C1DTLZ1", "C1DTLZ3",

@@ -263,7 +263,9 @@ int moea_eval_quadratic(const float* x, int n, int d, const float* centers, int 
  * DTLZ5 clamps its theta to [0, 1]; cv = 0.  Problems 9-14 are the
  * constrained variants C1-DTLZ1, C1-DTLZ3, C2-DTLZ2, C2-ConvexDTLZ2, C3-DTLZ1
  * and C3-DTLZ4 (dtlz_kernels.cu:824-1165): their base problem's objectives
- * and the violation in cv, constraints always active.  C1-DTLZ3 and C2-DTLZ2
+ * and the violation in cv, constraints always active.  Problem 15 is CSDP,
+ * the car side-impact design problem (d = 7, m = 3; dtlz_kernels.cu:719-738,
+ * 1046-1085).  C1-DTLZ3 and C2-DTLZ2
  * take m in {2, 3, 5, 8, 10}, C2-ConvexDTLZ2 m in {3, 5, 8, 10}.  Needs
  * 2 <= m <= d. */
 int moea_eval_dtlz(int problem, const float* x, int n, int d, int m, float* f, int ldf, float* cv,
@@ -337,6 +339,7 @@ int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_a
 #define MOEA_PROBLEM_C2_CONVEX_DTLZ2 14
 #define MOEA_PROBLEM_C3_DTLZ1 15
 #define MOEA_PROBLEM_C3_DTLZ4 16
+#define MOEA_PROBLEM_CSDP 17
 
 /* Problem evaluator callback: the C form of IProblemEvaluator::evaluate
  * (problem/problem_evaluator.cuh:10-31).  Must write f (m, n) and cv (n) for

@@ -118,8 +118,7 @@ def test_api_validation_errors():
         cm.C1DTLZ1(12, 3, constraint_activation_ratio=float("nan"))
     assert cm.C1DTLZ1(12, 3, constraint_activation_ratio=0.5).constraint_activation_ratio == 0.5
     assert cm.C1DTLZ1(12, 3, constraint_activation_ratio=7.0).constraint_activation_ratio == 1.0
-    with pytest.raises(NotImplementedError):
-        cm.CSDP(12, 3)
+    assert cm.CSDP(7, 3).constraints == 1
     b = cm.AlgorithmBuilder("NSGA3").population_size(92).max_generations(10).problem(
         cm.QuadraticCenters(12, 3))
     alg = b.build()

@@ -270,3 +270,44 @@ def test_constraint_activation_schedule(kind):
     # the schedule restarts with reset()
     alg.reset()
     assert not cv_now().any()
+
+
+def csdp_fp64(x):
+    """The car side-impact design problem as dtlz_kernels.cu:719-738 and
+    1046-1085 define it: 3 objectives and the sum of 10 violations."""
+    x1, x2, x3, x4, x5, x6, x7 = x.astype(np.float64).T
+    F = 4.72 - 0.5 * x4 - 0.19 * x2 * x3
+    vmbp = 10.58 - 0.674 * x1 * x2 - 0.67275 * x2
+    vfd = 16.45 - 0.489 * x3 * x7 - 0.843 * x5 * x6
+    f = np.stack([1.98 + 4.9 * x1 + 6.67 * x2 + 6.98 * x3 + 4.01 * x4 + 1.78 * x5 + 0.00001 * x6 + 2.73 * x7,
+                  F, 0.5 * (vmbp + vfd)])
+    g = [1.16 - 0.3717 * x2 * x4 - 0.0092928 * x3 - 1.0,
+         0.261 - 0.0159 * x1 * x2 - 0.06486 * x1 - 0.019 * x2 * x7 + 0.0144 * x3 * x5 + 0.0154464 * x6 - 0.32,
+         0.214 + 0.00817 * x5 - 0.045195 * x1 - 0.0135168 * x1 + 0.03099 * x2 * x6 - 0.018 * x2 * x7
+         + 0.007176 * x3 + 0.023232 * x3 - 0.00364 * x5 * x6 - 0.018 * x2 * x2 - 0.32,
+         0.74 - 0.61 * x2 - 0.031296 * x3 - 0.031872 * x7 + 0.227 * x2 * x2 - 0.32,
+         28.98 + 3.818 * x3 - 4.2 * x1 * x2 + 1.27296 * x6 - 2.68065 * x7 - 32.0,
+         33.86 + 2.95 * x3 - 5.057 * x1 * x2 - 3.795 * x2 - 3.4431 * x7 + 1.45728 - 32.0,
+         46.36 - 9.9 * x2 - 4.4505 * x1 - 32.0,
+         F - 4.0, vmbp - 9.9, vfd - 15.7]
+    return f, sum(np.maximum(v, 0.0) for v in g)
+
+
+def test_csdp_kernel_and_run():
+    rng = np.random.default_rng(17)
+    x = rng.random((3000, 7), dtype=np.float32)
+    f, cv = K.eval_dtlz(15, torch.from_numpy(x).to(DEV), 3)
+    f64, cv64 = csdp_fp64(x)
+    np.testing.assert_allclose(f.cpu().numpy(), f64, rtol=2e-5, atol=2e-4)
+    np.testing.assert_allclose(cv.cpu().numpy(), cv64, rtol=2e-5, atol=2e-4)
+    assert (cv64 > 0).any()                          # the sample holds infeasible rows
+    alg = cm.NSGA3(population_size=92, max_generations=10, problem=cm.CSDP(7, 3), seed=5, print_progress=False)
+    alg.initialize()
+    assert alg._args.problem.kind == _lib.DTLZ_KINDS["CSDP"]
+    res = alg.run()
+    f64, cv64 = csdp_fp64(res.variables.cpu().numpy())
+    np.testing.assert_allclose(res.objectives.cpu().numpy().T, f64, rtol=2e-5, atol=2e-4)
+    np.testing.assert_allclose(res.constraints.cpu().numpy(), cv64, rtol=2e-5, atol=2e-4)
+    bad = cm.NSGA3(population_size=92, max_generations=3, problem=cm.CSDP(12, 3), print_progress=False)
+    with pytest.raises((ValueError, RuntimeError), match="7 variables"):
+        bad.initialize()
