@@ -66,8 +66,8 @@ class SeededMLP:
 
 
 class DTLZProblem(PythonProblem):
-    """DTLZ1-7 and ConvexDTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005) and the
-    constrained C1/C2/C3 variants, with the reference's constructor
+    """DTLZ1-7 and ConvexDTLZ2 (Deb, Thiele, Laumanns, Zitzler 2005), the
+    constrained C1/C2/C3 variants and CSDP, with the reference's constructor
     (problem.py:9-47).  The algorithm evaluates them with the built-in
     kernels; ``evaluate`` is the FP64 torch form of the unconstrained ones,
     used where the kernels do not apply (one objective, or more objectives
@@ -136,14 +136,12 @@ class DTLZProblem(PythonProblem):
         return f.float()
 
 
-def _problem_class(name: str, supported: bool = True):
+def _problem_class(name: str):
     class BuiltinProblem(DTLZProblem):
         _kind = name
 
         def __init__(self, dimension: int = 12, objectives: int = 3,
                      constraint_activation_ratio: float = 0.0) -> None:
-            if not supported:
-                raise NotImplementedError(f"{name} is not provided by this engine")
             super().__init__(dimension, objectives, constraint_activation_ratio)
 
     BuiltinProblem.__name__ = name
