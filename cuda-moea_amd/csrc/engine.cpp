@@ -191,8 +191,13 @@ int run_step(moea_run_args& a, int generations)
             // constraints switch on at g, so the parents' violations are recomputed (a fresh
             // evaluation gives the same objectives) and NSGA-III re-sorts its parent fronts
             MOEA_CHECK(evaluate_problem(a, par, st, g));
-            if (a.algorithm == MOEA_ALGORITHM_NSGA3 && !(a.hooks && a.hooks->select))
+            if (a.hooks && a.hooks->select) {
+                if (a.hooks->prepare)
+                    MOEA_CHECK(a.hooks->prepare(a.hooks->user, &par, a.n, a.d, a.m, g, a.max_generations,
+                                                a.stream));
+            } else if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
                 MOEA_CHECK(parent_fronts(a, par, st));
+            }
         }
         moea_reproduce_args r{};
         r.n = a.n; r.d = a.d;

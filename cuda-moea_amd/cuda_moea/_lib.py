@@ -96,6 +96,7 @@ CROSSOVER_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_
 MUTATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_int, c_int, c_void_p)
 SELECT_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, _POP_P, _POP_P, c_int, c_int, c_int, c_int, c_int,
                              c_void_p)
+PREPARE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_int, c_int, c_void_p)
 DIRECTIONS_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
                                  c_int, c_int, c_void_p)
 
@@ -103,7 +104,7 @@ DIRECTIONS_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c
 class StrategyHooks(ctypes.Structure):
     _fields_ = [
         ("user", c_void_p), ("mate", MATE_FN), ("crossover", CROSSOVER_FN), ("mutate", MUTATE_FN),
-        ("update_directions", DIRECTIONS_FN), ("select", SELECT_FN),
+        ("update_directions", DIRECTIONS_FN), ("select", SELECT_FN), ("prepare", PREPARE_FN),
     ]
 
 

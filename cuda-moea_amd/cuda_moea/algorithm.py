@@ -352,6 +352,7 @@ class Algorithm:
         self._sel_state: dict[str, Any] = {}
         if self._python_selector:
             h.select, used = _lib.SELECT_FN(self._select_python), True
+            h.prepare = _lib.PREPARE_FN(self._prepare_python)
         for strategy in (self.mating, self.crossover, self.mutation):
             if isinstance(strategy, (PythonMating, PythonCrossover, PythonMutation)) and hasattr(strategy, "initialize"):
                 strategy.initialize(dict(info))
@@ -398,6 +399,16 @@ class Algorithm:
         rank = self._sel_state.get("rank")
         if isinstance(self.mating, TournamentMating) and (rank is None or rank.numel() != self.n):
             raise ValueError("TournamentMating requires a rank array for every parent")
+
+    def _prepare_python(self, user, parents, n, d, m, generation, max_generations, stream) -> int:
+        def body() -> int:
+            # algorithm.cu:233-241: the parents changed, so the selector prepares again
+            out = self.environment_selector.prepare(self._pop_dict(parents),
+                                                    self._context(generation, max_generations))
+            self._update_state(out)
+            self._check_mating_state()
+            return 0
+        return self._guarded(body)
 
     def _select_python(self, user, parents, offspring, nxt, n, d, m, generation, max_generations, stream) -> int:
         def body() -> int:

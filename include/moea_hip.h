@@ -401,6 +401,12 @@ typedef int (*moea_select_fn)(void* user, const moea_population* parents, const 
                               const moea_population* next, int n, int d, int m, int generation,
                               int max_generations, void* stream);
 
+/* Re-preparation of a caller's selector (IEnvironmentSelector::prepare) on
+ * the parents, when a problem changes their evaluation during the run: the
+ * constrained DTLZ variants at the generation their constraints switch on. */
+typedef int (*moea_prepare_fn)(void* user, const moea_population* parents, int n, int d, int m, int generation,
+                               int max_generations, void* stream);
+
 typedef struct moea_strategy_hooks {
     void* user;
     moea_mate_fn mate;                     /* NULL: args.mating */
@@ -408,6 +414,7 @@ typedef struct moea_strategy_hooks {
     moea_mutate_fn mutate;                 /* NULL: args.mutation */
     moea_directions_fn update_directions;  /* NULL: RVEA adaptation / fixed directions */
     moea_select_fn select;                 /* NULL: args.algorithm's selection */
+    moea_prepare_fn prepare;               /* with select: re-prepare the caller's selector */
 } moea_strategy_hooks;
 
 /* State of one run.  The caller owns every buffer; moea_run_* update the

@@ -311,3 +311,27 @@ def test_csdp_kernel_and_run():
     bad = cm.NSGA3(population_size=92, max_generations=3, problem=cm.CSDP(12, 3), print_progress=False)
     with pytest.raises((ValueError, RuntimeError), match="7 variables"):
         bad.initialize()
+
+
+def test_activation_switch_prepares_a_python_selector():
+    """algorithm.cu:233-241: when the constraints switch on, the parents'
+    violations change and a Python selector's prepare() runs again on them."""
+    class Keep(cm.PythonEnvironmentSelector):
+        def __init__(self):
+            self.prepared = []
+
+        def prepare(self, population, context):
+            self.prepared.append((context["generation"], float(population["constraints"].abs().sum())))
+            return None
+
+        def select(self, parents, offspring, context):
+            n = parents["variables"].shape[0]
+            return {"indices": torch.arange(n, device=parents["variables"].device)}
+
+    sel = Keep()
+    prob = cm.C1DTLZ1(dimension=12, objectives=3, constraint_activation_ratio=0.5)
+    alg = cm.NSGA3(population_size=92, max_generations=10, problem=prob, mating=cm.RandomMating(),
+                   environment_selector=sel, seed=4, print_progress=False)
+    alg.run()
+    assert [g for g, _ in sel.prepared] == [0, 4]
+    assert sel.prepared[0][1] == 0.0 and sel.prepared[1][1] > 0.0    # violations off, then on
