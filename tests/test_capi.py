@@ -164,3 +164,43 @@ def test_serve_admission_host_logic():
     q.n, q.problem = 100, L.PROBLEM_MLP
     assert lib.moea_serve_admit(srv, ctypes.byref(q), ctypes.byref(slot)) == -1
     lib.moea_serve_destroy(srv)
+
+
+def test_reference_api_constructions():
+    """The construction cases of the reference's tests/python/test_api.py:46-110:
+    every built-in problem, every built-in strategy combination and every
+    Python strategy class construct without a GPU."""
+    import torch
+    import cuda_moea as cm
+
+    class Sphere(cm.PythonProblem):
+        def __init__(self):
+            super().__init__(4, 2)
+
+        def evaluate(self, variables, context):
+            return torch.stack((variables.square().sum(1), (variables - 1).square().sum(1)), 1)
+
+    class Selector(cm.PythonEnvironmentSelector):
+        def select(self, parents, offspring, context):
+            return {"indices": torch.arange(parents["variables"].shape[0], device=parents["variables"].device)}
+
+    crossover = cm.SBX(variable_copy_probability=0.5)
+    assert crossover._spec()["variable_copy_probability"] == 0.5
+    assert not cm.NSGA3(population_size=16, max_generations=1, crossover=crossover, enable_warmup=False).initialized
+    assert not cm.NSGA3(population_size=16, max_generations=1, problem=Sphere(),
+                        initial_population=torch.zeros((16, 4)), enable_warmup=False).initialized
+    for name in ["DTLZ1", "DTLZ2", "DTLZ3", "DTLZ4", "DTLZ5", "DTLZ6", "DTLZ7", "ConvexDTLZ2", "C1DTLZ1",
+                 "C1DTLZ3", "C2DTLZ2", "C2ConvexDTLZ2", "C3DTLZ1", "C3DTLZ4", "CSDP"]:
+        assert not cm.NSGA3(population_size=16, max_generations=1, problem=getattr(cm, name)(8, 3),
+                            enable_warmup=False).initialized
+    for options in [dict(mating=cm.RandomMating(), crossover=cm.SBX(), mutation=cm.NoMutation(),
+                         reference_directions=cm.DasDennisDirections(2),
+                         environment_selector=cm.NSGA3EnvironmentSelector()),
+                    dict(reference_directions=cm.UserDefinedDirections(torch.eye(3)))]:
+        assert not cm.Algorithm("NSGA3", population_size=16, max_generations=1, enable_warmup=False,
+                                **options).initialized
+    assert not cm.NSGA3(population_size=16, max_generations=1, problem=Sphere(), mating=cm.PythonMating(),
+                        crossover=cm.PythonCrossover(), mutation=cm.PythonMutation(),
+                        reference_directions=cm.PythonReferenceDirections(), environment_selector=Selector(),
+                        enable_warmup=False).initialized
+    assert cm.AlgorithmBuilder("RVEA").population_size(16).max_generations(2).problem(cm.DTLZ2()).build().generation == 0
