@@ -95,7 +95,6 @@ void carve_instance(Carve& c, const moea_batch_problem& p, BInst& I)
     I.perm = c.take<int32_t>(z.nm);
     I.hi = c.take<int32_t>(z.W);
     I.kind = c.take<uint8_t>(z.W);
-    I.cursor = c.take<int32_t>(z.nm);
     I.rend = c.take<int32_t>(z.nm);
     I.A = c.take<uint64_t>(2 * static_cast<size_t>(z.W));
     I.pstate = c.take<uint32_t>(kBatchPeelStateWords);
@@ -327,6 +326,7 @@ struct BatchCore {
     Grids run{}, init{};
     bool dirty = true;                        // membership changed since the last upload
     int peel_capacity = 0;                    // co-resident blocks of the batched peel
+    bool error_ready = false;                 // the device error word has been cleared
 };
 
 namespace {
@@ -348,13 +348,38 @@ size_t core_reserved(int slots, long long max_rows, Carve& c, BatchCore* C)
     for (int w = 34 + ib; w <= 64; ++w)
         sort_bytes = std::max(sort_bytes, sort_pairs_temp_bytes(static_cast<int>(max_rows), w));
     void* tmp = c.take_bytes(sort_bytes);
+    uint32_t* error = c.take<uint32_t>(1);
     if (C) {
         C->dev_insts = insts;
         C->dev_tables = tables;
         C->table_ints = table_ints;
-        C->scratch = BNdsScratch{k_in, k_out, v_in, v_out, tmp, sort_bytes, ib};
+        C->scratch = BNdsScratch{k_in, k_out, v_in, v_out, tmp, sort_bytes, ib, error};
     }
     return c.used;
+}
+
+// The sticky device error word (set when a peel's bounded wait gave up),
+// read at a point where the stream is already synchronized.
+int core_device_error(const BatchCore& C)
+{
+    if (!C.error_ready) return kOk;           // no device work queued yet
+    uint32_t err = 0;
+    MOEA_HIP(hipMemcpy(&err, C.scratch.error, sizeof(err), hipMemcpyDeviceToHost));
+    if (err != 0)
+        return fail(kDeviceFault, "moea_batch: a front-peeling launch gave up waiting at its grid barrier; "
+                                  "fronts of the affected generation are invalid");
+    return kOk;
+}
+
+// Clears the error word before the core's first device work (not at
+// creation, which touches no device memory).
+int core_clear_error(BatchCore& C)
+{
+    if (C.error_ready) return kOk;
+    const uint32_t zero = 0;
+    MOEA_HIP(hipMemcpy(C.scratch.error, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    C.error_ready = true;
+    return kOk;
 }
 
 // Rebuilds both grid sets and uploads the tables and the instance table
@@ -372,6 +397,8 @@ int core_upload(BatchCore& C)
         return fail(kWorkspaceTooSmall, "moea_batch: more rows than the sort scratch holds");
     if (T.t.size() > C.table_ints) return fail(kInvalidArgument, "moea_batch: table capacity exceeded");
     MOEA_HIP(hipStreamSynchronize(C.stream));
+    MOEA_CHECK(core_device_error(C));
+    MOEA_CHECK(core_clear_error(C));
     MOEA_HIP(hipMemcpy(C.dev_tables, T.t.data(), T.t.size() * sizeof(int), hipMemcpyHostToDevice));
     MOEA_HIP(hipMemcpy(C.dev_insts, C.host.data(), C.host.size() * sizeof(BInst), hipMemcpyHostToDevice));
     fix_grids(C.run, C.dev_tables);
@@ -668,6 +695,7 @@ int serve_release(ServeState& S, int slot)
         return fail(kInvalidArgument, "moea_serve_release: slot is not in use");
     // the device may still use the buffers: order the release after queued work
     MOEA_HIP(hipStreamSynchronize(C.stream));
+    MOEA_CHECK(core_device_error(C));
     arena_free(S, S.owned[slot].first, S.owned[slot].second);
     S.owned[slot] = {0, 0};
     C.host[slot] = BInst{};
@@ -732,6 +760,13 @@ int moea_batch_view(const moea_batch* batch, int index, moea_population* out, in
 {
     if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_view: null batch");
     return core_view(batch->state->core, index, out, fronts);
+}
+
+int moea_batch_check(const moea_batch* batch)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_check: null batch");
+    MOEA_HIP(hipStreamSynchronize(batch->state->core.stream));
+    return core_device_error(batch->state->core);
 }
 
 void moea_batch_destroy(moea_batch* batch)
@@ -806,6 +841,13 @@ int moea_serve_release(moea_serve* server, int slot)
 {
     if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_release: null server");
     return serve_release(*server->state, slot);
+}
+
+int moea_serve_check(const moea_serve* server)
+{
+    if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_check: null server");
+    MOEA_HIP(hipStreamSynchronize(server->state->core.stream));
+    return core_device_error(server->state->core);
 }
 
 void moea_serve_destroy(moea_serve* server)

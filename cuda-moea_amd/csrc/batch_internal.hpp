@@ -79,7 +79,6 @@ struct BInst {
     int32_t* perm;                // (nm) sorted position -> local row
     int32_t* hi;                  // (W)
     uint8_t* kind;                // (W)
-    int32_t* cursor;              // (nm)
     int32_t* rend;                // (nm) sorted row -> 1 + last word holding a dominator (0: none)
     uint64_t* A;                  // (2 W) assigned sets of the peel (this round, next round)
     uint32_t* pstate;             // (kBatchPeelStateWords) peel barrier and totals
@@ -158,6 +157,7 @@ struct BNdsScratch {
     void* sort_temp;
     size_t sort_bytes;
     int inst_bits;                // bits of the instance index above the 34-bit NDS key
+    uint32_t* error;              // (1) sticky: set when a peel's bounded wait gave up
 };
 
 // Evaluation grids: per objective count, the quadratic rows and the MLP

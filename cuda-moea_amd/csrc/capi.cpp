@@ -88,6 +88,8 @@ int moea_nds_rank(const float* f, int ldf, const float* cv, int n, int m, int st
                    static_cast<hipStream_t>(stream));
 }
 
+void moea_debug_set_spin_limit(uint32_t polls) { set_peel_spin_limit(polls); }
+
 int moea_das_dennis(int requested_count, int m, int partitions, int kind, float* out_host,
                     float* raw_host)
 {

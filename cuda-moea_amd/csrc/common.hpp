@@ -368,21 +368,38 @@ __device__ inline void st_relaxed_u64(uint64_t* p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier for cooperative launches (at most kBarMaxBlocks blocks).  Each
-// block publishes its arrival with a release store to its own flag word;
-// every thread of block 0 polls a share of the flags with relaxed loads and
-// takes an acquire fence, then block 0 releases the generation word that the
-// other blocks poll (relaxed loads, one acquire fence when it moves).  There
-// is no read-modify-write on a shared counter, and no acquire per poll (that
-// would invalidate the XCD's L2 on every poll).
+// Barrier of co-resident blocks for cooperative launches (at most
+// kBarMaxBlocks blocks) whose only data shared inside the launch is accessed
+// with device-scope atomics (coherent across the XCDs' L2 caches without
+// cache maintenance).  Each block publishes its arrival with a device-scope
+// store to its own flag word; every thread of block 0 polls a share of the
+// flags, then block 0 stores the generation word that the other blocks poll.
+// Ordering comes from waiting for each block's outstanding memory operations
+// (s_waitcnt 0) before its flag or generation store, and from loads issued
+// only after the polled value arrived; no L2 write-back or invalidate is
+// issued, so data the blocks read from the L2 stays cached across rounds.
 // Words of `state`, all zero at launch:
 constexpr int kBarMaxBlocks = 1024;
 constexpr int kBarGen = kBarMaxBlocks;        // generation
 constexpr int kBarError = kBarGen + 16;       // set when a spin gives up
 constexpr int kBarWords = kBarError + 16;     // first word free for the caller
-// The spin is bounded: after kSpinLimit polls a thread records an error and
+// The spin is bounded: after `spin_limit` polls a thread records an error and
 // the barrier returns false, so a broken launch ends instead of hanging.
 constexpr uint32_t kSpinLimit = 1u << 24;
+
+// Every memory operation this wave issued has completed (device-scope atomic
+// stores are then visible to every block); also a compiler barrier.
+__device__ inline void drain_memory()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ inline void st_relaxed_u32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Barrier of the `nblocks` co-resident blocks that share `state`; `rank` is
 // this block's index among them (0 .. nblocks-1).  With snap_src >= 0, block
@@ -390,59 +407,48 @@ constexpr uint32_t kSpinLimit = 1u << 24;
 // and before any leaves: a value accumulated before the barrier, read by
 // every block afterwards from snap_dst, is then the same for all of them even
 // if fast blocks start adding to snap_src for the next round.
-__device__ inline bool group_barrier(uint32_t* state, uint32_t rank, uint32_t nblocks, uint32_t& phase,
-                                     int snap_src = -1, int snap_dst = -1)
+__device__ inline bool coherent_barrier(uint32_t* state, uint32_t rank, uint32_t nblocks, uint32_t& phase,
+                                        uint32_t spin_limit, int snap_src = -1, int snap_dst = -1)
 {
     __shared__ int s_ok;
     const uint32_t target = phase + 1;
     if (threadIdx.x == 0) s_ok = 1;
+    drain_memory();
     __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&state[rank], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) st_relaxed_u32(&state[rank], target);
     if (rank == 0) {
         for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
             uint32_t spins = 0;
-            while (__hip_atomic_load(&state[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    __hip_atomic_store(&state[kBarError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (ld_relaxed_u32(&state[b]) != target) {
+                if (++spins > spin_limit) {
+                    st_relaxed_u32(&state[kBarError], 1u);
                     s_ok = 0;
                     break;
                 }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
         if (threadIdx.x == 0) {
-            if (snap_src >= 0)
-                __hip_atomic_store(&state[snap_dst],
-                                   __hip_atomic_load(&state[snap_src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&state[kBarGen], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (snap_src >= 0) st_relaxed_u32(&state[snap_dst], ld_relaxed_u32(&state[snap_src]));
+            drain_memory();
+            st_relaxed_u32(&state[kBarGen], target);
         }
     } else if (threadIdx.x == 0) {
         uint32_t spins = 0;
-        while (__hip_atomic_load(&state[kBarGen], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit) {
-                __hip_atomic_store(&state[kBarError], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (ld_relaxed_u32(&state[kBarGen]) != target) {
+            if (++spins > spin_limit) {
+                st_relaxed_u32(&state[kBarError], 1u);
                 s_ok = 0;
                 break;
             }
+            __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    if (threadIdx.x == 0 &&
-        __hip_atomic_load(&state[kBarError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-        s_ok = 0;
+    if (threadIdx.x == 0 && ld_relaxed_u32(&state[kBarError]) != 0) s_ok = 0;
     __syncthreads();
     phase = target;
     return s_ok != 0;
-}
-
-__device__ inline bool grid_barrier(uint32_t* state, uint32_t nblocks, uint32_t& phase)
-{
-    return group_barrier(state, blockIdx.x, nblocks, phase);
 }
 
 // ----------------------------------------------------------------------------

@@ -123,8 +123,9 @@ int parent_fronts(moea_run_args& a, const moea_population& par, hipStream_t st)
         MOEA_CHECK(quantize_cv(par.cv, q, a.n, a.cv_quant, st));
         cv_sort = q;
     }
+    // a failed sort is also flagged in the sticky device-error word of stats
     return nds_run(par.f, a.n, cv_sort, a.n, a.m, a.n, a.fronts[a.fronts_current], a.stats_init, a.workspace,
-                   a.workspace_bytes, st);
+                   a.workspace_bytes, st, a.stats ? reinterpret_cast<uint32_t*>(a.stats + 6) : nullptr);
 }
 
 }  // namespace

@@ -13,8 +13,11 @@ namespace moea {
 // nds.hip
 size_t nds_workspace_bytes(int n);
 int nds_max_n();
+// max_front receives -2 when the peel gave up waiting at a grid barrier (the
+// fronts are then invalid); `error` (optional) is a sticky flag also set then.
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
-            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream);
+            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error = nullptr);
+void set_peel_spin_limit(uint32_t limit);
 
 // nsga3.hip
 size_t nsga3_workspace_bytes(int n, int m, int k);

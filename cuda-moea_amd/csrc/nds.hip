@@ -36,6 +36,7 @@
 // A 64-row x 8-word chunk is 4 KiB contiguous: the dominance kernel writes it
 // as one 64-byte segment per lane, the peel reads a row's words 8 at a time.
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "batch_internal.hpp"
@@ -138,13 +139,16 @@ __device__ __forceinline__ void block_info_one(const uint64_t* __restrict__ keys
     hi[b] = end >> 6;
 }
 
-// zero (optional, `words` of them): cleared on the way, for a later launch.
+// zero (optional, `words` of them): cleared on the way, for a later launch;
+// mf_init (optional): set to -1, the value the peel replaces with its result.
 __global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
-                               int32_t* __restrict__ hi, uint32_t* __restrict__ zero, int words)
+                               int32_t* __restrict__ hi, uint32_t* __restrict__ zero, int words,
+                               int32_t* __restrict__ mf_init)
 {
     for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < words;
          j += static_cast<int>(gridDim.x * blockDim.x))
         zero[j] = 0u;
+    if (mf_init && blockIdx.x == 0 && threadIdx.x == 0) mf_init[0] = -1;
     block_info_one(keys, n, static_cast<int>(blockIdx.x), kind, hi);
 }
 
@@ -312,38 +316,49 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
 // Peel state (device), u32 words: [0, kBarWords) group barrier,
 // [kPeelTotal] total assigned, [kPeelTotal + 1] last round, [kPeelTotal + 2]
 // spare output word, [kPeelTotal + 4 + (k & 1)] the total as of the barrier
-// ending round k (snapshot read by every block).  A0 / A1:
-// assigned-set bitsets in sorted order (W words each), double-buffered across
-// rounds.
+// ending round k (snapshot read by every block).  A0 / A1: assigned-set
+// bitsets in sorted order (W words each), double-buffered across rounds.
 //
 // Workgroup g owns row blocks g, g + G, g + 2G, ... (G workgroups, spread over
-// the whole sort order).  Round k:
-//   A  the wave owning a block checks, lane per row, the chunk holding each
-//      live row's cursor against the set assigned before round k; rows that
-//      clear it check the next kAheadChunks chunks in one batch.  Rows whose
-//      dominator words run out are front k; rows still clear with words left
-//      go to the workgroup's task queue in LDS as (row, cursor).
+// the whole sort order).  Each owned row keeps, in LDS for the whole launch,
+// its blocking word c (the first word of its dominator row that held an
+// unassigned dominator when last looked at) and rem, the dominators of word c
+// that were still unassigned then.  Round k:
+//   A  lane per row: rem &= ~A[c] against the set assigned before round k.  A
+//      row with rem left stays blocked without touching HBM.  A row whose rem
+//      emptied reads on from c: the chunk holding c and, when that is clear,
+//      the next kAheadChunks chunks in one batch.  Rows whose dominator words
+//      run out are front k; rows still clear with words left go to the
+//      workgroup's task queue in LDS as (row, c).
 //   B  the workgroup's 16 waves take the queued rows and scan each one's
-//      remaining words (64 in the first round trip, then 512 per round trip).  A clear row joins front k, a
-//      blocked row keeps its cursor at the first blocking word.
+//      remaining words (64 in the first round trip, then 512 per round trip).
+//      A clear row joins front k, a blocked row keeps (c, rem).
 // Rows, cursors and tasks never leave their workgroup; only the assigned set
-// and the total cross workgroups, at the one grid barrier per round.  Each
-// word of a row is found clear at most once over all rounds.
+// and the total cross workgroups, at the one grid barrier per round.  Over
+// all rounds a row reads each of its words about once.
+//
+// Everything the workgroups exchange inside the launch (barrier words,
+// totals, the assigned sets) is accessed with device-scope atomics, which the
+// hardware keeps coherent across the XCDs' L2 caches; nothing else written in
+// the launch is read in it.  The barrier therefore orders only those
+// accesses (coherent_barrier) and does not write back or invalidate the L2.
 struct PeelArgs {
     const uint64_t* B;
     const int32_t* hi;       // (W,) last word block of each row block
     const int32_t* perm;     // (n,) sorted position -> input index
     int n, W, Wc, stop_at;
-    int blocks_per_group;    // ceil(W / gridDim.x)
+    int blocks_per_group;    // ceil(W / group size)
     int32_t* fronts;         // (n,) out, input order
-    int32_t* cursor;         // (n,) scratch, sorted order
     uint64_t* A0;
     uint64_t* A1;
     uint32_t* state;
-    int32_t* out_max_front;  // (1,) out
+    int32_t* out_max_front;  // (1,) out: last front, kPeelFailed when a bounded wait gave up
     const int32_t* rend;     // (n,) optional: one past each row's last non-zero word (sorted order)
+    uint32_t spin_limit;     // polls before a barrier wait gives up
+    uint32_t* error;         // optional sticky flag: set to 1 when a bounded wait gave up
 };
 
+constexpr int32_t kPeelFailed = -2;
 constexpr int kPeelTotal = kBarWords;
 constexpr int kPeelStateWords = kBarWords + 16;
 static_assert(kPeelStateWords <= kBatchPeelStateWords, "batched peel state too small");
@@ -352,12 +367,12 @@ constexpr int kPeelWaves = kPeelThreads / 64;
 constexpr int kAheadChunks = 3;     // phase A: chunks checked after a cleared cursor chunk
 constexpr int kScanLoads = 8;       // phase B: words per lane in flight
 
-// LDS layout of nds_peel (dynamic): s_A (W u64), s_next (bpg u64),
-// s_task (64 bpg u64), s_hi (W int).
+// LDS layout of the peel (dynamic): s_A (W u64), s_next (bpg u64), s_task
+// (64 bpg u64), s_rem (64 bpg u64), s_hi (W int), s_cur (64 bpg int).
 __host__ __device__ __forceinline__ size_t peel_lds_bytes(int W, int bpg)
 {
-    return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + static_cast<size_t>(bpg) * 64 * 8 +
-           static_cast<size_t>(W) * 4;
+    return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + static_cast<size_t>(bpg) * 64 * 16 +
+           static_cast<size_t>(W) * 4 + static_cast<size_t>(bpg) * 64 * 4;
 }
 
 __device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
@@ -366,9 +381,6 @@ __device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
     return (i & 1) ? h.y : h.x;
 }
 
-// Phase A for the lane's row p of block rb (live rows only): checks the
-// chunk holding the cursor, and the next kAheadChunks chunks if it is clear.
-// Updates cur; ready: no dominator word left; queued: clear so far, words left.
 __device__ __forceinline__ const uint64_t* peel_rowbase(const uint64_t* __restrict__ B, int Wc, int rb, int lane)
 {
     return B + (static_cast<size_t>(rb) * Wc * 64 + lane) * 8;
@@ -382,22 +394,25 @@ __device__ __forceinline__ void peel_load_chunk(const uint64_t* rowbase, int c, 
 }
 
 // The cursor chunk c (loaded in v): first word at or after cur holding an
-// unassigned dominator, or stop = end of the chunk (capped at end).
+// unassigned dominator (its unassigned bits in rem), or stop = end of the
+// chunk (capped at end).
 __device__ __forceinline__ int peel_first(const ulonglong2 (&v)[4], int c, int cur, int end, const uint64_t* s_A,
-                                          int& stop)
+                                          int& stop, uint64_t& rem)
 {
     stop = min(c * 8 + 8, end);
     int blocked = stop;
 #pragma unroll
     for (int i = 7; i >= 0; --i) {
         const int ww = c * 8 + i;
-        if (ww >= cur && ww < stop && (chunk_word(v, i) & ~s_A[ww]) != 0) blocked = ww;
+        const uint64_t x = (ww >= cur && ww < stop) ? (chunk_word(v, i) & ~s_A[ww]) : 0ull;
+        if (x != 0) { blocked = ww; rem = x; }
     }
     return blocked;
 }
 
 // The kAheadChunks chunks after chunk c, loaded in one batch.
-__device__ __forceinline__ int peel_ahead(const uint64_t* rowbase, int c, int end, const uint64_t* s_A, int& stop)
+__device__ __forceinline__ int peel_ahead(const uint64_t* rowbase, int c, int end, const uint64_t* s_A, int& stop,
+                                          uint64_t& rem)
 {
     ulonglong2 u[kAheadChunks][4];
 #pragma unroll
@@ -412,32 +427,37 @@ __device__ __forceinline__ int peel_ahead(const uint64_t* rowbase, int c, int en
 #pragma unroll
         for (int i = 7; i >= 0; --i) {
             const int ww = (c + 1 + a) * 8 + i;
-            if (ww < stop2 && (chunk_word(u[a], i) & ~s_A[ww]) != 0) blocked2 = ww;
+            const uint64_t x = (ww < stop2) ? (chunk_word(u[a], i) & ~s_A[ww]) : 0ull;
+            if (x != 0) { blocked2 = ww; rem = x; }
         }
     }
     stop = stop2;
     return blocked2;
 }
 
+// Phase A read for the lane's row (live, rem emptied): from word cur on.
+// ready: no dominator word left; queued: clear so far with words left;
+// otherwise blocked at the returned cur with its unassigned bits in rem.
 __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
-                                  const uint64_t* s_A, int& cur, bool& ready, bool& queued)
+                                           const uint64_t* s_A, int& cur, bool& ready, bool& queued, uint64_t& rem)
 {
     const uint64_t* rowbase = peel_rowbase(B, Wc, rb, lane);
     const int c = cur >> 3;
     ulonglong2 v[4];
     peel_load_chunk(rowbase, c, v);
     int stop;
-    cur = peel_first(v, c, cur, end, s_A, stop);
-    if (cur == stop && cur < end) cur = peel_ahead(rowbase, c, end, s_A, stop);
+    cur = peel_first(v, c, cur, end, s_A, stop, rem);
+    if (cur == stop && cur < end) cur = peel_ahead(rowbase, c, end, s_A, stop, rem);
     ready = (cur >= end);
     queued = !ready && (cur == stop);
 }
 
 // Phase B: the wave scans row pr from word c (wave-uniform) towards end, 64
 // words in the first round trip, then 512.  Returns the first word holding an
-// unassigned dominator, or end when the row is clear.
+// unassigned dominator (its unassigned bits in rem, wave-uniform), or end
+// when the row is clear.
 __device__ __forceinline__ int peel_scan(const uint64_t* __restrict__ B, int Wc, int pr, int c, int end,
-                                const uint64_t* s_A, int lane)
+                                         const uint64_t* s_A, int lane, uint64_t& rem)
 {
     int batch = 1;
     while (c < end) {
@@ -448,21 +468,43 @@ __device__ __forceinline__ int peel_scan(const uint64_t* __restrict__ B, int Wc,
             bl[i] = (i < batch && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
         }
         int first = -1;
+        uint64_t first_rem = 0;
 #pragma unroll
         for (int i = kScanLoads - 1; i >= 0; --i) {
             const int w = c + i * 64 + lane;
-            const uint64_t bm = ballot(i < batch && w < end && (bl[i] & ~s_A[w]) != 0);
-            if (bm) first = i * 64 + __builtin_ctzll(bm);
+            const uint64_t x = (i < batch && w < end) ? (bl[i] & ~s_A[w]) : 0ull;
+            const uint64_t bm = ballot(x != 0);
+            if (bm) {
+                const int l = __builtin_ctzll(bm);
+                first = i * 64 + l;
+                const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
+                const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), l));
+                first_rem = (static_cast<uint64_t>(hi) << 32) | lo;
+            }
         }
-        if (first >= 0) return c + first;
+        if (first >= 0) {
+            rem = first_rem;
+            return c + first;
+        }
         c += batch * 64;
         batch = kScanLoads;
     }
+    rem = 0;
     return end;
 }
 
+// Records a failed bounded wait: the run's front count becomes kPeelFailed
+// and the optional sticky flag is set.
+__device__ __forceinline__ void peel_fail(const PeelArgs& P)
+{
+    if (threadIdx.x != 0) return;
+    __hip_atomic_exchange(P.out_max_front, kPeelFailed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (P.error) __hip_atomic_fetch_or(P.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One peel over the G blocks of a group; g is this block's rank.  Returns
-// false when a barrier gave up (the launch is broken; the block exits).
+// false when a barrier gave up (the launch is broken; the block exits after
+// recording the failure).
 __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint64_t* s_dyn)
 {
     const int nblk = P.W;
@@ -470,7 +512,9 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
     uint64_t* s_A = s_dyn;                                          // (W) assigned before this round
     uint64_t* s_next = s_A + nblk;                                  // (bpg) next masks of own blocks
     uint64_t* s_task = s_next + bpg;                                // (64 bpg) queued (cursor << 32 | row)
-    int* s_hi = reinterpret_cast<int*>(s_task + static_cast<size_t>(bpg) * 64);   // (W)
+    uint64_t* s_rem = s_task + static_cast<size_t>(bpg) * 64;       // (64 bpg) unassigned bits of the blocking word
+    int* s_hi = reinterpret_cast<int*>(s_rem + static_cast<size_t>(bpg) * 64);   // (W)
+    int* s_cur = s_hi + nblk;                                       // (64 bpg) blocking word
     __shared__ int s_ntask;
     __shared__ int s_count;
     const int lane = lane_id();
@@ -480,12 +524,14 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
     for (int w = threadIdx.x; w < nblk; w += kPeelThreads) s_hi[w] = P.hi[w];
     for (int j = lw; j < bpg; j += kPeelWaves) {
         const int rb = g + j * G;
-        if (rb >= nblk) break;
+        s_cur[j * 64 + lane] = 0;
+        s_rem[j * 64 + lane] = 0ull;
+        if (rb >= nblk) continue;
         const int p = rb * 64 + lane;
-        if (p < P.n) { P.fronts[P.perm[p]] = -1; P.cursor[p] = 0; }
+        if (p < P.n) P.fronts[P.perm[p]] = -1;
         if (lane == 0) { st_relaxed_u64(&P.A0[rb], 0ull); st_relaxed_u64(&P.A1[rb], 0ull); }
     }
-    if (!group_barrier(P.state, g, G, phase)) return false;
+    if (!coherent_barrier(P.state, g, G, phase, P.spin_limit)) { peel_fail(P); return false; }
 
     int total_prev = 0;
     for (int k = 0;; ++k) {
@@ -502,15 +548,25 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
             if (rb >= nblk) break;
             const uint64_t amask = s_A[rb];
             const int p = rb * 64 + lane;
+            const int slot = j * 64 + lane;
             // words 0 .. end-1 can hold dominators
             const int end = (P.rend && p < P.n) ? P.rend[p] : s_hi[rb] + 1;
             const bool live = (p < P.n) && !((amask >> lane) & 1ull);
             int cur = end;
             bool ready = false, queued = false;
             if (live) {
-                cur = P.cursor[p];
-                peel_check(P.B, P.Wc, rb, lane, end, s_A, cur, ready, queued);
-                if (!ready) P.cursor[p] = cur;
+                cur = s_cur[slot];
+                if (cur >= end) {
+                    ready = true;
+                } else {
+                    // still blocked while dominators of its blocking word are unassigned
+                    uint64_t rem = s_rem[slot] & ~s_A[cur];
+                    if (rem == 0) {
+                        peel_check(P.B, P.Wc, rb, lane, end, s_A, cur, ready, queued, rem);
+                        s_cur[slot] = cur;
+                    }
+                    s_rem[slot] = rem;
+                }
             }
             const uint64_t ready_m = ballot(ready);
             const uint64_t task_m = ballot(queued);
@@ -540,8 +596,10 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
             const int pr = wave_uniform(static_cast<int>(static_cast<uint32_t>(rec)));
             const int c0 = wave_uniform(static_cast<int>(rec >> 32));
             const int rb = pr >> 6;
+            const int slot = ((rb - g) / G) * 64 + (pr & 63);
             const int end = P.rend ? wave_uniform(P.rend[pr]) : s_hi[rb] + 1;
-            const int c = peel_scan(P.B, P.Wc, pr, c0, end, s_A, lane);
+            uint64_t rem;
+            const int c = peel_scan(P.B, P.Wc, pr, c0, end, s_A, lane, rem);
             const bool clear = c >= end;
             if (lane == 0) {
                 if (clear) {
@@ -550,7 +608,8 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                              static_cast<unsigned long long>(1ull << (pr & 63)));
                     ++my_new;
                 } else {
-                    P.cursor[pr] = c;
+                    s_cur[slot] = c;
+                    s_rem[slot] = rem;
                 }
             }
         }
@@ -568,12 +627,17 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         // faster blocks already add for the next round, or the blocks could
         // disagree about stopping and strand each other at the next barrier
         const int snap = kPeelTotal + 4 + (k & 1);
-        if (!group_barrier(P.state, g, G, phase, kPeelTotal, snap)) return false;
+        if (!coherent_barrier(P.state, g, G, phase, P.spin_limit, kPeelTotal, snap)) { peel_fail(P); return false; }
         const int total = static_cast<int>(ld_relaxed_u32(&P.state[snap]));
         if (total >= P.stop_at || total == total_prev || total >= P.n) {
             if (g == 0 && threadIdx.x == 0) {
-                P.out_max_front[0] = (total == total_prev) ? k - 1 : k;
-                P.state[kPeelTotal + 1] = static_cast<uint32_t>(k);
+                // the launch before the peel set the output to -1; a block
+                // that gave up in this last barrier may already have marked
+                // the run failed, and that mark stays
+                int32_t expected = -1;
+                __hip_atomic_compare_exchange_strong(P.out_max_front, &expected,
+                                                     (total == total_prev) ? k - 1 : k, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             return true;
         }
@@ -699,6 +763,13 @@ __global__ __launch_bounds__(64) void b_nds_rowend(const BInst* __restrict__ ins
     if (p < R.rows) I.rend[p] = last + 1;
 }
 
+// Where a batched peel leaves the index of its last front: the selection's
+// meta word for the merged rows, a spare peel-state word for the parents.
+__device__ __forceinline__ int32_t* peel_max_front(const BInst& I, int mode)
+{
+    return mode == 1 ? I.meta : reinterpret_cast<int32_t*>(I.pstate + kPeelTotal + 2);
+}
+
 // Front peeling of every instance of a batch in one cooperative launch.  The
 // blocks form groups; each group peels its instances one after another
 // (peel_body, with the group barrier on the instance's own state words), so
@@ -706,7 +777,8 @@ __global__ __launch_bounds__(64) void b_nds_rowend(const BInst* __restrict__ ins
 // Jobs of block w: jobs[first[w] .. first[w + 1]) as (instance, rank, group size).
 __global__ __launch_bounds__(kPeelThreads) void b_nds_peel_groups(const BInst* __restrict__ insts, int mode,
                                                                  const int* __restrict__ first,
-                                                                 const int* __restrict__ jobs)
+                                                                 const int* __restrict__ jobs, uint32_t spin_limit,
+                                                                 uint32_t* error)
 {
     extern __shared__ uint64_t s_dyn[];
     const int w = static_cast<int>(blockIdx.x);
@@ -717,24 +789,31 @@ __global__ __launch_bounds__(kPeelThreads) void b_nds_peel_groups(const BInst* _
         const BRows R = brows(I, mode);
         const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
         PeelArgs P{I.B, I.hi, I.perm, R.rows, W, Wc, mode == 1 ? I.n : R.rows, (W + G - 1) / G, R.fronts,
-                   I.cursor, I.A, I.A + W, I.pstate,
-                   mode == 1 ? I.meta : reinterpret_cast<int32_t*>(I.pstate + kPeelTotal + 2), I.rend};
+                   I.A, I.A + W, I.pstate, peel_max_front(I, mode), I.rend, spin_limit, error};
         __syncthreads();
         if (!peel_body(P, g, G, s_dyn)) return;
     }
 }
 
 // Clears the peel state words of the instances in `state` (barrier flags and
-// totals start at zero).
-__global__ void b_peel_reset(const BInst* __restrict__ insts, int state)
+// totals start at zero) and sets their front-count outputs to -1.
+__global__ void b_peel_reset(const BInst* __restrict__ insts, int mode, int state)
 {
     const BInst& I = insts[blockIdx.x];
     if (I.state != state) return;
     for (int t = threadIdx.x; t < kPeelStateWords; t += blockDim.x) I.pstate[t] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) peel_max_front(I, mode)[0] = -1;
 }
 
 
 constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
+
+// Polls before a peel barrier gives up (kSpinLimit unless a test lowers it
+// with moea_debug_set_spin_limit to make the bounded wait give up on purpose).
+std::atomic<uint32_t> g_spin_limit{kSpinLimit};
+
+uint32_t peel_spin_limit() { return g_spin_limit.load(std::memory_order_relaxed); }
 
 struct NdsLayout {
     uint64_t* B;
@@ -748,7 +827,6 @@ struct NdsLayout {
     float* cvs;
     int32_t* hi;
     uint8_t* kind;
-    int32_t* cursor;
     uint64_t* A0;
     uint64_t* A1;
     uint32_t* state;
@@ -771,7 +849,6 @@ size_t carve_nds(void* ws, int n, NdsLayout& L)
     L.cvs = c.take<float>(npad);
     L.hi = c.take<int32_t>(W);
     L.kind = c.take<uint8_t>(W);
-    L.cursor = c.take<int32_t>(n);
     L.A0 = c.take<uint64_t>(W);
     L.A1 = c.take<uint64_t>(W);
     L.state = c.take<uint32_t>(kPeelStateWords);
@@ -812,7 +889,7 @@ size_t nds_workspace_bytes(int n)
 int nds_max_n() { return kMaxWords * 64; }
 
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
-            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream)
+            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error)
 {
     if (n <= 0 || m <= 0 || m > kMaxM || !f || !cv || !fronts || !max_front || ldf < n)
         return fail(kInvalidArgument, "nds: invalid arguments");
@@ -836,7 +913,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     MOEA_HIP(hipGetLastError());
     // also clears the peel's barrier state (not read before the peel)
     hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi, L.state,
-                       kPeelStateWords);
+                       kPeelStateWords, max_front);
     MOEA_HIP(hipGetLastError());
     stage_end(kStNdsSort, stream);
 
@@ -853,7 +930,8 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
     const int bpg = ceil_div(W, grid);
     const size_t lds = peel_lds_bytes(W, bpg);
-    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.cursor, L.A0, L.A1, L.state, max_front};
+    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.A0, L.A1, L.state, max_front, nullptr,
+                  peel_spin_limit(), error};
     void* kargs[] = {&args};
     stage_begin(kStPeel, stream);
     MOEA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(nds_peel), dim3(grid),
@@ -862,6 +940,8 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     stage_end(kStPeel, stream);
     return kOk;
 }
+
+void set_peel_spin_limit(uint32_t limit) { g_spin_limit.store(limit == 0 ? kSpinLimit : limit); }
 
 int batch_peel_capacity()
 {
@@ -970,13 +1050,15 @@ int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, in
     stage_begin(kStPeel, st);
     hipLaunchKernelGGL(b_nds_rowend, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map, mode);
     MOEA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(b_peel_reset, dim3(L.count), dim3(256), 0, st, L.insts, state);
+    hipLaunchKernelGGL(b_peel_reset, dim3(L.count), dim3(256), 0, st, L.insts, mode, state);
     MOEA_HIP(hipGetLastError());
     int mode_arg = mode;
     const BInst* insts_arg = L.insts;
     const int* first_arg = G.peel_first;
     const int* jobs_arg = G.peel_jobs;
-    void* kargs[] = {&insts_arg, &mode_arg, &first_arg, &jobs_arg};
+    uint32_t spin_arg = peel_spin_limit();
+    uint32_t* error_arg = S.error;
+    void* kargs[] = {&insts_arg, &mode_arg, &first_arg, &jobs_arg, &spin_arg, &error_arg};
     MOEA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(b_nds_peel_groups), dim3(G.peel_wgs),
                                         dim3(kPeelThreads), kargs, static_cast<unsigned>(G.peel_lds), st));
     MOEA_HIP(hipGetLastError());

@@ -20,7 +20,7 @@ namespace moea {
 namespace {
 
 // meta layout (int32, device)
-enum Meta : int { kMaxFront = 0, kNPrior = 1, kNLast = 2, kIll = 3, kNRem = 4, kMetaLen = 8 };
+enum Meta : int { kMaxFront = 0, kNPrior = 1, kNLast = 2, kIll = 3, kNRem = 4, kDevError = 6, kMetaLen = 8 };
 
 constexpr int kScanThreads = 1024;
 
@@ -91,6 +91,25 @@ __global__ __launch_bounds__(256) void merge_objectives(const float* __restrict_
     }
 }
 
+// The caller's stats words: the selection's meta words, except that the
+// device-error word only ever turns on (it stays set over later selections
+// until the caller clears it).
+__device__ __forceinline__ void copy_stats(const int32_t* __restrict__ meta, int32_t* __restrict__ stats)
+{
+    for (int j = 0; j < kMetaLen; ++j)
+        if (j != kDevError) stats[j] = meta[j];
+    if (meta[kDevError] != 0) stats[kDevError] = 1;
+}
+
+// Merged row s (front fr) is a prior-front survivor: fr < max_front.  A
+// negative max_front means the sort failed on the device (its peel gave up
+// waiting); then the n parents survive unchanged, so every later launch still
+// reads valid indices, and the selection's device-error word is set.
+__device__ __forceinline__ bool prior_survivor(int fr, int mf, int s, int n)
+{
+    return mf < 0 ? (s < n) : (fr >= 0 && fr < mf);
+}
+
 // prior = {front < max_front}, last = {front == max_front}, each ascending
 // (nds_frontmask_kernel + copy_if, non_dominated_sort.cu:1580-1629);
 // gidx = [prior; last].  Two passes over blocks of kScanThreads entries:
@@ -134,6 +153,19 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
     block_exclusive_scan(ap, s_wave, ap);
     block_exclusive_scan(al, s_wave, al);
     const int i = b * kScanThreads + threadIdx.x;
+    if (mf < 0) {
+        // the sort failed on the device (its peel gave up waiting): keep the
+        // parents, so every later launch reads valid indices, and flag it
+        if (i < n) gidx[i] = i;
+        if (b == 0 && threadIdx.x == 0) {
+            meta[kNPrior] = n;
+            meta[kNLast] = 0;
+            meta[kNRem] = 0;
+            meta[kIll] = -1;
+            meta[kDevError] = 1;
+        }
+        return;
+    }
     const int fr = i < nm ? fronts[i] : -1;
     const int fp = (fr >= 0 && fr < mf) ? 1 : 0, fl = (fr == mf) ? 1 : 0;
     const int op = block_exclusive_scan(fp, s_wave, t);
@@ -145,6 +177,7 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
         meta[kNLast] = al;
         meta[kNRem] = n - ap;
         meta[kIll] = -1;
+        meta[kDevError] = 0;
     }
 }
 
@@ -1079,8 +1112,7 @@ __global__ void mark_chosen(const int32_t* __restrict__ fronts, const int32_t* _
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nm) return;
-    const int fr = fronts[s];
-    flag[s] = (fr >= 0 && fr < meta[kMaxFront]) ? 1 : 0;
+    flag[s] = prior_survivor(fronts[s], meta[kMaxFront], s, nm / 2) ? 1 : 0;
 }
 __global__ void mark_selected(const int32_t* __restrict__ val3s, const int32_t* __restrict__ gidx,
                               const int32_t* __restrict__ meta, uint8_t* __restrict__ flag)
@@ -1638,8 +1670,10 @@ __device__ __forceinline__ void niche_select_body(const NicheArgs& A, int32_t* _
             const int q = q0 + u * kNicheThreads;
             const int4 f = fr[u];
             if (q < nq)
-                fw[q] = ((f.x >= 0 && f.x < mf) ? 1u : 0u) | ((f.y >= 0 && f.y < mf) ? 0x100u : 0u) |
-                        ((f.z >= 0 && f.z < mf) ? 0x10000u : 0u) | ((f.w >= 0 && f.w < mf) ? 0x1000000u : 0u);
+                fw[q] = (prior_survivor(f.x, mf, 4 * q, n) ? 1u : 0u) |
+                        (prior_survivor(f.y, mf, 4 * q + 1, n) ? 0x100u : 0u) |
+                        (prior_survivor(f.z, mf, 4 * q + 2, n) ? 0x10000u : 0u) |
+                        (prior_survivor(f.w, mf, 4 * q + 3, n) ? 0x1000000u : 0u);
         }
     }
     __syncthreads();
@@ -1712,8 +1746,7 @@ __global__ __launch_bounds__(kNicheThreads) void niche_block(NicheArgs A, int32_
     __shared__ __align__(16) unsigned char s_raw[kNicheLdsBytes];
     __shared__ int s_wave[64];
     niche_select_body(A, meta, nm, n, fronts, chosen, gk0, gk1, gv0, gv1, lds_cap, s_raw, s_wave);
-    if (stats && threadIdx.x == 0)
-        for (int j = 0; j < kMetaLen; ++j) stats[j] = meta[j];
+    if (stats && threadIdx.x == 0) copy_stats(meta, stats);
 }
 
 // Chosen indices in ascending merged order (thrust::sort of the selection,
@@ -1808,7 +1841,7 @@ __global__ __launch_bounds__(256) void gather_next(const int32_t* __restrict__ c
 
 __global__ void write_stats(const int32_t* __restrict__ meta, int32_t* __restrict__ stats)
 {
-    if (threadIdx.x < kMetaLen) stats[threadIdx.x] = meta[threadIdx.x];
+    if (threadIdx.x == 0) copy_stats(meta, stats);
 }
 
 // quantize_cv / cv_quantize_kernel (selection/nsga3/kernels/cv_quantization.cu:8-113):
@@ -2072,7 +2105,8 @@ int nsga3_select(const moea_nsga3_select_args& A)
         MOEA_CHECK(quantize_cv(L.cvm, L.cvq, nm, A.cv_quant, st));
         cv_sort = L.cvq;
     }
-    MOEA_CHECK(nds_run(L.fm, nm, cv_sort, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st));
+    MOEA_CHECK(nds_run(L.fm, nm, cv_sort, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st,
+                       A.stats ? reinterpret_cast<uint32_t*>(A.stats + kDevError) : nullptr));
     stage_begin(kStSelectPrep, st);
     const int nsb = ceil_div(nm, kScanThreads);
     hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);
@@ -2165,6 +2199,18 @@ __global__ __launch_bounds__(kScanThreads) void b_split(const BInst* __restrict_
     const BInst& I = insts[blockIdx.x];
     if (I.state != 1) return;
     const int mf = I.meta[kMaxFront];
+    if (mf < 0) {
+        // the sort failed on the device: keep the parents (prior_survivor)
+        for (int t = threadIdx.x; t < I.n; t += kScanThreads) I.gidx[t] = t;
+        if (threadIdx.x == 0) {
+            I.meta[kNPrior] = I.n;
+            I.meta[kNLast] = 0;
+            I.meta[kNRem] = 0;
+            I.meta[kIll] = -1;
+            I.meta[kDevError] = 1;
+        }
+        return;
+    }
     int tp = 0, tl = 0;
     for (int base = 0; base < I.nm; base += kScanThreads) {
         const int t = base + static_cast<int>(threadIdx.x);
@@ -2193,6 +2239,7 @@ __global__ __launch_bounds__(kScanThreads) void b_split(const BInst* __restrict_
         I.meta[kNLast] = tl;
         I.meta[kNRem] = I.n - tp;
         I.meta[kIll] = -1;
+        I.meta[kDevError] = 0;
     }
 }
 
@@ -2340,10 +2387,7 @@ __global__ __launch_bounds__(kScanThreads) void b_survivors(const BInst* __restr
     const BInst& I = insts[blockIdx.x];
     if (I.state != 1) return;
     const int mf = I.meta[kMaxFront];
-    for (int t = threadIdx.x; t < I.nm; t += kScanThreads) {
-        const int fr = I.fronts_mix[t];
-        I.flag[t] = (fr >= 0 && fr < mf) ? 1 : 0;
-    }
+    for (int t = threadIdx.x; t < I.nm; t += kScanThreads) I.flag[t] = prior_survivor(I.fronts_mix[t], mf, t, I.n) ? 1 : 0;
     __syncthreads();
     const int n_rem = I.meta[kNRem];
     if (n_rem > 0 && I.meta[kNLast] > 0) {

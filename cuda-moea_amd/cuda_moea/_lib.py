@@ -13,6 +13,10 @@ _PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = _PKG_ROOT / "lib" / "libmoea_hip.so"
 
 MOEA_OK = 0
+# max_front value of a sort whose front-peeling launch gave up waiting at its
+# grid barrier; stats word STAT_DEVICE_ERROR is the sticky flag of the same
+DEVICE_FAILED = -2
+STAT_DEVICE_ERROR = 6
 MATING_TOURNAMENT, MATING_RANDOM, MATING_GIVEN = 0, 1, 2
 MUTATION_NONE, MUTATION_POLYNOMIAL = 0, 1
 ALGORITHM_NSGA3, ALGORITHM_RVEA = 0, 1
@@ -147,6 +151,7 @@ SIGNATURES = {
     "moea_nds_workspace_bytes": (c_size_t, [c_int]),
     "moea_nds_rank": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                               c_void_p, c_size_t, c_void_p]),
+    "moea_debug_set_spin_limit": (None, [ctypes.c_uint32]),
     "moea_das_dennis": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "moea_set_niching": (c_int, [c_int, c_int]),
@@ -180,6 +185,7 @@ SIGNATURES = {
     "moea_batch_step": (c_int, [c_void_p, c_int]),
     "moea_batch_generation": (c_int, [c_void_p]),
     "moea_batch_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
+    "moea_batch_check": (c_int, [c_void_p]),
     "moea_batch_destroy": (None, [c_void_p]),
     "moea_serve_reserved_bytes": (c_size_t, [POINTER(ServeConfig)]),
     "moea_serve_instance_bytes": (c_size_t, [POINTER(BatchProblem)]),
@@ -189,6 +195,7 @@ SIGNATURES = {
     "moea_serve_generation": (c_int, [c_void_p, c_int]),
     "moea_serve_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
     "moea_serve_release": (c_int, [c_void_p, c_int]),
+    "moea_serve_check": (c_int, [c_void_p]),
     "moea_serve_destroy": (None, [c_void_p]),
 }
 

@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import MoeaError, check, lib
 from ._specs import (
     AdaptiveRVEADirections, CudaConfig, DasDennisDirections, NoMutation,
     NSGA3EnvironmentSelector, PolynomialMutation, PythonCrossover, PythonMating, PythonMutation,
@@ -623,6 +623,7 @@ class Algorithm:
             raise RuntimeError("Algorithm has not been initialized")
         pop = self._current()
         torch.cuda.current_stream(self.device).synchronize()
+        self._check_device()
         if self._python_selector:
             sel = self.environment_selector
             if hasattr(sel, "finalize"):         # adapters.cpp:405-411
@@ -648,7 +649,8 @@ class Algorithm:
     def reset(self) -> None:
         if not self._initialized:
             return
-        self.synchronize()
+        torch.cuda.current_stream(self.device).synchronize()
+        self._stats.zero_()                      # a new run starts with a clear device-error word
         self._resets += 1
         self._directions.copy_(self._directions0)
         rd = self.reference_directions
@@ -670,6 +672,15 @@ class Algorithm:
     def synchronize(self) -> None:
         if self._initialized:
             torch.cuda.current_stream(self.device).synchronize()
+            self._check_device()
+
+    def _check_device(self) -> None:
+        """Raises if a front-peeling launch of this run gave up waiting at its
+        grid barrier (the sticky device-error stats word; read after a
+        synchronization the caller already made)."""
+        if self.kind == "NSGA3" and int(self._stats[_lib.STAT_DEVICE_ERROR].item()) != 0:
+            raise MoeaError("a front-peeling launch gave up waiting at its grid barrier; the selections of "
+                            "the affected generations kept their parents")
 
     @property
     def population(self) -> Population:

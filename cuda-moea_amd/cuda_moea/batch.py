@@ -161,7 +161,8 @@ class BatchNSGA3:
         pop = _lib.PopulationC()
         fronts = ctypes.c_void_p()
         check(lib.moea_batch_view(self._handle, int(index), ctypes.byref(pop), ctypes.byref(fronts)))
-        torch.cuda.current_stream(self.device).synchronize()
+        # waits for the queued generations and raises if one of them failed on the device
+        check(lib.moea_batch_check(self._handle))
         return view_result(self._ws, pop, fronts.value, self.runs[index], self._total_ms, copy)
 
     def __del__(self) -> None:

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import DEVICE_FAILED, MoeaError, check, lib
 
 _F32, _I32 = torch.float32, torch.int32
 
@@ -99,7 +99,11 @@ def nds_rank(f: torch.Tensor, cv: torch.Tensor, stop_at: Optional[int] = None):
     check(lib.moea_nds_rank(f.data_ptr(), n, cv.data_ptr(), n, m, n if stop_at is None else stop_at,
                             fronts.data_ptr(), mf.data_ptr(), ws.data_ptr(), ws.numel(),
                             _stream(f.device)))
-    return fronts, int(mf.item())
+    max_front = int(mf.item())
+    if max_front == DEVICE_FAILED:
+        raise MoeaError("nds_rank: the front-peeling launch gave up waiting at its grid barrier; "
+                        "the fronts are invalid")
+    return fronts, max_front
 
 
 # -------------------------------------------------------------- directions --

@@ -90,7 +90,8 @@ class Server:
         pop = _lib.PopulationC()
         fronts = ctypes.c_void_p()
         check(lib.moea_serve_view(self._handle, int(slot), ctypes.byref(pop), ctypes.byref(fronts)))
-        torch.cuda.current_stream(self.device).synchronize()
+        # waits for the queued generations and raises if one of them failed on the device
+        check(lib.moea_serve_check(self._handle))
         return view_result(self._ws, pop, fronts.value, self._runs[slot], 0.0, copy)
 
     def release(self, slot: int) -> None:

@@ -91,10 +91,20 @@ size_t moea_nds_workspace_bytes(int n);
  * non_dominated_sort.cu:1333-1574).  Fronts are peeled until at least stop_at
  * individuals are assigned; the others get front -1.  Writes fronts (n,) and
  * max_front (1,) = index of the last peeled front, both on the device.
+ * max_front = -2 reports a device failure found after the call returned (the
+ * peeling launch gave up waiting at its grid barrier; fronts are invalid).
  * f: (m, ldf) objectives, cv: (n,) constraint violations. */
 int moea_nds_rank(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
                   int32_t* fronts, int32_t* max_front, void* workspace, size_t workspace_bytes,
                   void* stream);
+
+/* Test hook, no counterpart in the reference: the number of polls after which
+ * a grid barrier of the front-peeling launches gives up (0 restores the
+ * default, about 2^24).  A small value makes the bounded waits give up, so
+ * tests can check that the failure is reported (max_front = -2, the stats
+ * device-error word, MOEA_E_DEVICE from moea_batch_check / moea_serve_check)
+ * while every launch still ends.  Process-wide; not for production use. */
+void moea_debug_set_spin_limit(uint32_t polls);
 
 /* ------------------------------------------------------------------------ */
 /* Reference directions (host)                                               */
@@ -151,7 +161,11 @@ typedef struct moea_nsga3_select_args {
                                    workspace not used for anything else since); 0 builds
                                    them */
     int32_t* stats;             /* optional (8,) device: max_front, n_prior, n_last,
-                                   fallback flag, n_rem */
+                                   fallback flag, n_rem, -, device error (sticky:
+                                   set to 1 when the sort's front peeling gave up
+                                   waiting at its grid barrier; that selection
+                                   then keeps the parents; only the caller clears
+                                   it), - */
     void* workspace;
     size_t workspace_bytes;
     void* stream;
@@ -507,6 +521,11 @@ int moea_batch_generation(const moea_batch* batch);
 /* Device views of instance `index`'s current population (x (n, d), f (m, n),
  * cv (n)) and its fronts (n); valid until the next moea_batch_step. */
 int moea_batch_view(const moea_batch* batch, int index, moea_population* out, int32_t** fronts);
+/* Waits for the batch's queued work and returns MOEA_E_DEVICE if a
+ * front-peeling launch of any earlier generation gave up waiting at its grid
+ * barrier (the fronts of that generation were invalid; the affected
+ * instances kept their parents), else MOEA_OK. */
+int moea_batch_check(const moea_batch* batch);
 void moea_batch_destroy(moea_batch* batch);
 
 /* ------------------------------------------------------------------------ */
@@ -549,6 +568,10 @@ int moea_serve_generation(const moea_serve* server, int slot);
 int moea_serve_view(const moea_serve* server, int slot, moea_population* out, int32_t** fronts);
 /* Frees the slot and its arena space (synchronizes the stream first). */
 int moea_serve_release(moea_serve* server, int slot);
+/* As moea_batch_check, for a server.  moea_serve_step and
+ * moea_serve_release also return MOEA_E_DEVICE at the synchronizations they
+ * already make. */
+int moea_serve_check(const moea_serve* server);
 void moea_serve_destroy(moea_serve* server);
 
 /* ------------------------------------------------------------------------ */
