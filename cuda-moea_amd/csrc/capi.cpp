@@ -88,6 +88,15 @@ int moea_nds_rank(const float* f, int ldf, const float* cv, int n, int m, int st
                    static_cast<hipStream_t>(stream));
 }
 
+int moea_nds_rank_static(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
+                         int32_t* fronts, int32_t* max_front, void* stream)
+{
+    if (ldf < n) return fail(kInvalidArgument, "moea_nds_rank_static: ldf < n");
+    return nds_run_static(f, ldf, cv, n, m, stop_at, fronts, max_front, static_cast<hipStream_t>(stream));
+}
+
+int moea_nds_static_max_n(void) { return nds_static_max_n(); }
+
 void moea_debug_set_spin_limit(uint32_t polls) { set_peel_spin_limit(polls); }
 
 int moea_das_dennis(int requested_count, int m, int partitions, int kind, float* out_host,

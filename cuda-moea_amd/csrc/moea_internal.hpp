@@ -17,6 +17,9 @@ int nds_max_n();
 // fronts are then invalid); `error` (optional) is a sticky flag also set then.
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
             int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error = nullptr);
+int nds_run_static(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
+                   int32_t* max_front, hipStream_t stream);
+int nds_static_max_n();
 void set_peel_spin_limit(uint32_t limit);
 
 // nsga3.hip

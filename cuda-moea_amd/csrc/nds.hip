@@ -37,6 +37,8 @@
 // as one 64-byte segment per lane, the peel reads a row's words 8 at a time.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "batch_internal.hpp"
@@ -940,6 +942,56 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     stage_end(kStPeel, stream);
     return kOk;
 }
+
+// ------------------------------------------------------ allocation-free --
+// moea_nds_rank_static: the same sort with its workspace in this file-scope
+// device array instead of a caller's buffer, for callers that cannot allocate
+// (graph capture, serving loops).  Every GPU holds its own copy until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress reports for the current device (the array's name is a
+// host-side stand-in, not a device address).  Nothing in it needs clearing
+// between calls: nds_block_info zeroes the peel state and the peel
+// initializes cursors, fronts and the assigned sets.  Calls that overlap on
+// one GPU share the array and race.
+constexpr size_t kStaticScratchBytes = size_t{16} << 20;
+constexpr int kStaticMaxN = 8192;
+constexpr int kStaticMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kStaticMaxDevices] = {};
+}  // namespace
+
+int nds_run_static(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
+                   int32_t* max_front, hipStream_t stream)
+{
+    if (n > kStaticMaxN)
+        return fail(kInvalidArgument, "moea_nds_rank_static: n = " + std::to_string(n) + " exceeds the " +
+                                          std::to_string(kStaticMaxN) + " rows its static workspace holds");
+    if (n > 0 && nds_workspace_bytes(n) > kStaticScratchBytes)
+        return fail(kInvalidArgument, "moea_nds_rank_static: workspace for n = " + std::to_string(n) +
+                                          " exceeds the static scratch array");
+    int dev = 0;
+    MOEA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStaticMaxDevices) return fail(kUnsupported, "moea_nds_rank_static: device index too large");
+    void* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+            if (!p) return fail(kHipError, "hipGetSymbolAddress(scratch) returned a null address");
+            g_scratch_ptr[dev] = p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    return nds_run(f, ldf, cv, n, m, stop_at, fronts, max_front, ptr, kStaticScratchBytes, stream);
+}
+
+int nds_static_max_n() { return kStaticMaxN; }
 
 void set_peel_spin_limit(uint32_t limit) { g_spin_limit.store(limit == 0 ? kSpinLimit : limit); }
 

@@ -151,6 +151,9 @@ SIGNATURES = {
     "moea_nds_workspace_bytes": (c_size_t, [c_int]),
     "moea_nds_rank": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                               c_void_p, c_size_t, c_void_p]),
+    "moea_nds_rank_static": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p]),
+    "moea_nds_static_max_n": (c_int, []),
     "moea_debug_set_spin_limit": (None, [ctypes.c_uint32]),
     "moea_das_dennis": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),

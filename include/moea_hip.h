@@ -98,6 +98,21 @@ int moea_nds_rank(const float* f, int ldf, const float* cv, int n, int m, int st
                   int32_t* fronts, int32_t* max_front, void* workspace, size_t workspace_bytes,
                   void* stream);
 
+/* moea_nds_rank without a workspace argument and without any allocation.  It
+ * has no counterpart in the reference: it is for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  The workspace is a 16 MiB device array inside the
+ * library, one per GPU, held until the process ends; n is limited to
+ * moea_nds_static_max_n() (8192) rows and larger n returns MOEA_E_INVALID.
+ * Results equal moea_nds_rank's on the same inputs.  Every call on a GPU uses
+ * that GPU's one array: calls that overlap on one GPU (different streams or
+ * threads without ordering between them) race. */
+int moea_nds_rank_static(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
+                         int32_t* fronts, int32_t* max_front, void* stream);
+
+/* Largest n moea_nds_rank_static accepts. */
+int moea_nds_static_max_n(void);
+
 /* Test hook, no counterpart in the reference: the number of polls after which
  * a grid barrier of the front-peeling launches gives up (0 restores the
  * default, about 2^24).  A small value makes the bounded waits give up, so
