@@ -328,13 +328,14 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
 // that were still unassigned then.  Round k:
 //   A  lane per row: rem &= ~A[c] against the set assigned before round k.  A
 //      row with rem left stays blocked without touching HBM.  A row whose rem
-//      emptied reads on from c: the chunk holding c and, when that is clear,
-//      the next kAheadChunks chunks in one batch.  Rows whose dominator words
-//      run out are front k; rows still clear with words left go to the
-//      workgroup's task queue in LDS as (row, c).
-//   B  the workgroup's 16 waves take the queued rows and scan each one's
-//      remaining words (64 in the first round trip, then 512 per round trip).
-//      A clear row joins front k, a blocked row keeps (c, rem).
+//      emptied reads on from c: kCheckChunks chunks from the one holding c, in
+//      one batch.  Rows whose dominator words run out are front k; rows still
+//      clear with words left go to the workgroup's task queue in LDS as
+//      (row, c).
+//   B  the workgroup's 16 waves take the queued rows eight at a time (eight
+//      lanes a row, 64 words a round trip) and scan each on until it meets
+//      its next blocking word (the row keeps (c, rem)) or runs out of words
+//      (front k).
 // Rows, cursors and tasks never leave their workgroup; only the assigned set
 // and the total cross workgroups, at the one grid barrier per round.  Over
 // all rounds a row reads each of its words about once.
@@ -366,15 +367,15 @@ constexpr int kPeelStateWords = kBarWords + 16;
 static_assert(kPeelStateWords <= kBatchPeelStateWords, "batched peel state too small");
 constexpr int kPeelThreads = 1024;
 constexpr int kPeelWaves = kPeelThreads / 64;
-constexpr int kAheadChunks = 3;     // phase A: chunks checked after a cleared cursor chunk
-constexpr int kScanLoads = 8;       // phase B: words per lane in flight
+constexpr int kCheckChunks = 2;     // phase A: chunks read in one batch from the cursor's chunk
+constexpr int kScanLoads = 8;       // phase B: words per lane in flight (64 per row)
 
 // LDS layout of the peel (dynamic): s_A (W u64), s_next (bpg u64), s_task
 // (64 bpg u64), s_rem (64 bpg u64), s_hi (W int), s_cur (64 bpg int).
 __host__ __device__ __forceinline__ size_t peel_lds_bytes(int W, int bpg)
 {
-    return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + static_cast<size_t>(bpg) * 64 * 16 +
-           static_cast<size_t>(W) * 4 + static_cast<size_t>(bpg) * 64 * 4;
+    const size_t rows = static_cast<size_t>(bpg) * 64;
+    return static_cast<size_t>(W) * 12 + static_cast<size_t>(bpg) * 8 + rows * 20;
 }
 
 __device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
@@ -395,104 +396,90 @@ __device__ __forceinline__ void peel_load_chunk(const uint64_t* rowbase, int c, 
     for (int i = 0; i < 4; ++i) v[i] = src[i];
 }
 
-// The cursor chunk c (loaded in v): first word at or after cur holding an
-// unassigned dominator (its unassigned bits in rem), or stop = end of the
-// chunk (capped at end).
-__device__ __forceinline__ int peel_first(const ulonglong2 (&v)[4], int c, int cur, int end, const uint64_t* s_A,
-                                          int& stop, uint64_t& rem)
-{
-    stop = min(c * 8 + 8, end);
-    int blocked = stop;
-#pragma unroll
-    for (int i = 7; i >= 0; --i) {
-        const int ww = c * 8 + i;
-        const uint64_t x = (ww >= cur && ww < stop) ? (chunk_word(v, i) & ~s_A[ww]) : 0ull;
-        if (x != 0) { blocked = ww; rem = x; }
-    }
-    return blocked;
-}
-
-// The kAheadChunks chunks after chunk c, loaded in one batch.
-__device__ __forceinline__ int peel_ahead(const uint64_t* rowbase, int c, int end, const uint64_t* s_A, int& stop,
-                                          uint64_t& rem)
-{
-    ulonglong2 u[kAheadChunks][4];
-#pragma unroll
-    for (int a = 0; a < kAheadChunks; ++a) {
-        const int ca = c + 1 + a;
-        if (ca * 8 < end) peel_load_chunk(rowbase, ca, u[a]);
-    }
-    const int stop2 = min((c + 1 + kAheadChunks) * 8, end);
-    int blocked2 = stop2;
-#pragma unroll
-    for (int a = kAheadChunks - 1; a >= 0; --a) {
-#pragma unroll
-        for (int i = 7; i >= 0; --i) {
-            const int ww = (c + 1 + a) * 8 + i;
-            const uint64_t x = (ww < stop2) ? (chunk_word(u[a], i) & ~s_A[ww]) : 0ull;
-            if (x != 0) { blocked2 = ww; rem = x; }
-        }
-    }
-    stop = stop2;
-    return blocked2;
-}
-
-// Phase A read for the lane's row (live, rem emptied): from word cur on.
+// Phase A read for the lane's row (live, rem emptied): the kCheckChunks
+// chunks from the one holding cur, loaded in one batch (one HBM round trip).
 // ready: no dominator word left; queued: clear so far with words left;
-// otherwise blocked at the returned cur with its unassigned bits in rem.
+// otherwise blocked at the returned cur with the unassigned dominators of
+// that word in rem.
 __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
                                            const uint64_t* s_A, int& cur, bool& ready, bool& queued, uint64_t& rem)
 {
     const uint64_t* rowbase = peel_rowbase(B, Wc, rb, lane);
     const int c = cur >> 3;
-    ulonglong2 v[4];
-    peel_load_chunk(rowbase, c, v);
-    int stop;
-    cur = peel_first(v, c, cur, end, s_A, stop, rem);
-    if (cur == stop && cur < end) cur = peel_ahead(rowbase, c, end, s_A, stop, rem);
+    ulonglong2 v[kCheckChunks][4];
+#pragma unroll
+    for (int a = 0; a < kCheckChunks; ++a)
+        if ((c + a) * 8 < end) peel_load_chunk(rowbase, c + a, v[a]);
+    const int stop = min((c + kCheckChunks) * 8, end);
+    int blocked = stop;
+#pragma unroll
+    for (int a = kCheckChunks - 1; a >= 0; --a) {
+#pragma unroll
+        for (int i = 7; i >= 0; --i) {
+            const int ww = (c + a) * 8 + i;
+            const uint64_t x = (ww >= cur && ww < stop) ? (chunk_word(v[a], i) & ~s_A[ww]) : 0ull;
+            if (x != 0) { blocked = ww; rem = x; }
+        }
+    }
+    cur = blocked;
     ready = (cur >= end);
     queued = !ready && (cur == stop);
 }
 
-// Phase B: the wave scans row pr from word c (wave-uniform) towards end, 64
-// words in the first round trip, then 512.  Returns the first word holding an
-// unassigned dominator (its unassigned bits in rem, wave-uniform), or end
-// when the row is clear.
-__device__ __forceinline__ int peel_scan(const uint64_t* __restrict__ B, int Wc, int pr, int c, int end,
-                                         const uint64_t* s_A, int lane, uint64_t& rem)
+// Phase B: eight queued rows per wave, eight lanes each.  Lane sl of group gi
+// reads words base + 8i + sl (i < kScanLoads, base = c rounded down to a
+// multiple of 8) of its row: 64 words a round trip, each 8-word run one
+// 64-byte segment; words before c or from end on count as clear.  A group
+// loops until its row meets a word holding an unassigned dominator (returned
+// in c, its unassigned bits in rem, the same in every lane of the group) or
+// runs out of words (c = end).  Groups without a row pass done = true.
+__device__ __forceinline__ void peel_scan_groups(const uint64_t* __restrict__ B, int Wc, int pr, int& c, int end,
+                                                 const uint64_t* s_A, int lane, bool done, uint64_t& rem)
 {
-    int batch = 1;
-    while (c < end) {
-        uint64_t bl[kScanLoads];
+    const int gi = lane >> 3, sl = lane & 7;
+    rem = 0;
+    while (ballot(!done)) {
+        const int base = c & ~7;
+        uint64_t x[kScanLoads];
 #pragma unroll
         for (int i = 0; i < kScanLoads; ++i) {
-            const int w = c + i * 64 + lane;
-            bl[i] = (i < batch && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
+            const int w = base + i * 8 + sl;
+            x[i] = (!done && w >= c && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
         }
+#pragma unroll
+        for (int i = 0; i < kScanLoads; ++i) {
+            const int w = base + i * 8 + sl;
+            x[i] = (!done && w >= c && w < end) ? (x[i] & ~s_A[w]) : 0ull;
+        }
+        // first blocking word of this lane's group: the lowest (i, sl)
         int first = -1;
-        uint64_t first_rem = 0;
+        uint64_t hit = 0;
 #pragma unroll
         for (int i = kScanLoads - 1; i >= 0; --i) {
-            const int w = c + i * 64 + lane;
-            const uint64_t x = (i < batch && w < end) ? (bl[i] & ~s_A[w]) : 0ull;
-            const uint64_t bm = ballot(x != 0);
-            if (bm) {
-                const int l = __builtin_ctzll(bm);
-                first = i * 64 + l;
-                const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
-                const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), l));
-                first_rem = (static_cast<uint64_t>(hi) << 32) | lo;
+            const uint32_t gm = static_cast<uint32_t>(ballot(x[i] != 0) >> (8 * gi)) & 0xFFu;
+            if (gm) {
+                first = i * 8 + __builtin_ctz(gm);
+                hit = x[i];
             }
         }
-        if (first >= 0) {
-            rem = first_rem;
-            return c + first;
+        // the blocking word's bits, from the lane that read it
+        const int src = gi * 8 + (first & 7);
+        const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(hit), src));
+        const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(hit >> 32), src));
+        if (!done) {
+            if (first >= 0) {
+                c = base + first;
+                rem = (static_cast<uint64_t>(hi) << 32) | lo;
+                done = true;
+            } else {
+                c = base + 8 * kScanLoads;
+                if (c >= end) {
+                    c = end;
+                    done = true;
+                }
+            }
         }
-        c += batch * 64;
-        batch = kScanLoads;
     }
-    rem = 0;
-    return end;
 }
 
 // Records a failed bounded wait: the run's front count becomes kPeelFailed
@@ -591,20 +578,21 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         }
         __syncthreads();
 
-        // ---- phase B: the workgroup's waves scan the queued rows
+        // ---- phase B: the queued rows, eight a wave, each until it blocks or ends
         const int ntask = s_ntask;
-        for (int t = lw; t < ntask; t += kPeelWaves) {
-            const uint64_t rec = s_task[t];
-            const int pr = wave_uniform(static_cast<int>(static_cast<uint32_t>(rec)));
-            const int c0 = wave_uniform(static_cast<int>(rec >> 32));
+        for (int t0 = lw * 8; t0 < ntask; t0 += kPeelWaves * 8) {
+            const int t = t0 + (lane >> 3);
+            const bool has = t < ntask;
+            const uint64_t rec = has ? s_task[t] : 0ull;
+            const int pr = static_cast<int>(static_cast<uint32_t>(rec));
             const int rb = pr >> 6;
+            const int end = !has ? 0 : (P.rend ? P.rend[pr] : s_hi[rb] + 1);
             const int slot = ((rb - g) / G) * 64 + (pr & 63);
-            const int end = P.rend ? wave_uniform(P.rend[pr]) : s_hi[rb] + 1;
+            int c = static_cast<int>(rec >> 32);
             uint64_t rem;
-            const int c = peel_scan(P.B, P.Wc, pr, c0, end, s_A, lane, rem);
-            const bool clear = c >= end;
-            if (lane == 0) {
-                if (clear) {
+            peel_scan_groups(P.B, P.Wc, pr, c, end, s_A, lane, !has, rem);
+            if (has && (lane & 7) == 0) {
+                if (c >= end) {
                     P.fronts[P.perm[pr]] = k;
                     atomicOr(reinterpret_cast<unsigned long long*>(&s_next[(rb - g) / G]),
                              static_cast<unsigned long long>(1ull << (pr & 63)));
@@ -615,7 +603,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                 }
             }
         }
-        if (lane == 0 && my_new) atomicAdd(&s_count, my_new);
+        if (my_new) atomicAdd(&s_count, my_new);
         __syncthreads();
         for (int j = threadIdx.x; j < bpg; j += kPeelThreads) {
             const int rb = g + j * G;
