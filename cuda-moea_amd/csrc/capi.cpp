@@ -73,7 +73,7 @@ using namespace moea;
 
 extern "C" {
 
-int moea_abi_version(void) { return 3; }
+int moea_abi_version(void) { return 4; }
 
 const char* moea_last_error(void) { return g_last_error.c_str(); }
 

@@ -455,9 +455,14 @@ __device__ __forceinline__ int hyperplane_fp64_wave(int m, SolveLds& L, double& 
 
 // Wave 0 of the solve block: extreme-point matrix rows, the solve, the
 // row-max fallback and the float intercepts (lane k writes a_out[k]).
+// Optional outputs: out_ext (merged-row index of each axis's extreme point,
+// through gidx; -1 when none) and out_a (the intercepts before the float
+// rounding and sanitizing).
 __device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, int nm, int m, int n_nds,
                                                 const int (&ext)[kMaxM], const float (&rmax)[kMaxM],
-                                                SolveLds& L, int32_t* __restrict__ meta, float* __restrict__ a_out)
+                                                SolveLds& L, int32_t* __restrict__ meta, float* __restrict__ a_out,
+                                                const int32_t* __restrict__ gidx, int32_t* __restrict__ out_ext,
+                                                double* __restrict__ out_a)
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     int my_ext = -1;
@@ -475,6 +480,8 @@ __device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, in
     const int ill = hyperplane_fp64_wave(m, L, a);
     if (lane >= m) return;
     if (ill) a = (my_rmax > 1e-10f) ? 1.0 / static_cast<double>(fmaxf(my_rmax, 1e-6f)) : 1.0;
+    if (out_a) out_a[lane] = a;
+    if (out_ext) out_ext[lane] = (my_ext >= 0 && my_ext < n_nds) ? gidx[my_ext] : -1;
     float af = static_cast<float>(a);
     if (!isfinite(af) || af <= 0.0f) af = 1.0f;
     a_out[lane] = af;
@@ -484,7 +491,9 @@ __device__ __forceinline__ void norm_solve_tail(const float* __restrict__ fn, in
 __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, int nm, int m,
                                        const float* __restrict__ asf_val, const int32_t* __restrict__ asf_idx,
                                        const float* __restrict__ row_max, int32_t* __restrict__ meta,
-                                       float* __restrict__ a_out)
+                                       float* __restrict__ a_out, const int32_t* __restrict__ gidx = nullptr,
+                                       int32_t* __restrict__ out_ext = nullptr,
+                                       double* __restrict__ out_a = nullptr)
 {
     __shared__ float s_best[kSolveThreads / 64][kMaxM];
     __shared__ int s_bidx[kSolveThreads / 64][kMaxM];
@@ -547,7 +556,8 @@ __device__ __forceinline__ void norm_solve_body(const float* __restrict__ fn, in
         rmax[j] = rm;
     }
     __shared__ SolveLds s_solve;
-    if (m >= 1 && m <= kMaxM) norm_solve_tail(fn, nm, m, n_nds, ext, rmax, s_solve, meta, a_out);
+    if (m >= 1 && m <= kMaxM)
+        norm_solve_tail(fn, nm, m, n_nds, ext, rmax, s_solve, meta, a_out, gidx, out_ext, out_a);
 }
 
 __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restrict__ fn, int nm, int m,
@@ -555,9 +565,12 @@ __global__ __launch_bounds__(kSolveThreads) void norm_solve(const float* __restr
                                                             const int32_t* __restrict__ asf_idx,
                                                             const float* __restrict__ row_max,
                                                             int32_t* __restrict__ meta,
-                                                            float* __restrict__ a_out)
+                                                            float* __restrict__ a_out,
+                                                            const int32_t* __restrict__ gidx,
+                                                            int32_t* __restrict__ out_ext,
+                                                            double* __restrict__ out_a)
 {
-    norm_solve_body(fn, nm, m, asf_val, asf_idx, row_max, meta, a_out);
+    norm_solve_body(fn, nm, m, asf_val, asf_idx, row_max, meta, a_out, gidx, out_ext, out_a);
 }
 
 // Perpendicular-distance association, one individual per thread against a
@@ -2116,7 +2129,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
     MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, L.meta, L.fn, L.asf_val, L.asf_idx,
                   L.row_max, L.rho, k, L.omin, nb, A.ideal);
     hipLaunchKernelGGL(norm_solve, dim3(1), dim3(kSolveThreads), 0, st, L.fn, nm, m, L.asf_val,
-                       L.asf_idx, L.row_max, L.meta, L.a);
+                       L.asf_idx, L.row_max, L.meta, L.a, L.gidx, A.out_extremes, A.out_intercepts);
     stage_end(kStSelectPrep, st);
     stage_begin(kStAssociation, st);
     if (assoc_prunable(k, m)) {

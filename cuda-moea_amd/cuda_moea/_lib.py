@@ -54,6 +54,7 @@ class NSGA3SelectArgs(ctypes.Structure):
         ("directions_ready", c_int),
         ("stats", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
         ("stream", c_void_p), ("cv_quant", CvQuantization),
+        ("out_extremes", c_void_p), ("out_intercepts", c_void_p),
     ]
 
 

@@ -166,10 +166,12 @@ def quantize_cv(cv: torch.Tensor, **params) -> torch.Tensor:
 
 def nsga3_select(parents: Population, offspring: Population, directions: torch.Tensor,
                  ideal: torch.Tensor, draws: Optional[dict] = None, seed: int = 0,
-                 generation: int = 0, cv_quant: Optional[dict] = None):
+                 generation: int = 0, cv_quant: Optional[dict] = None, normalization: Optional[dict] = None):
     """One NSGA-III selection.  ``ideal`` is updated in place.  Returns
     (next Population, next_fronts, stats int32[8] = max_front, n_prior, n_last,
-    fallback, n_rem, ...)."""
+    fallback, n_rem, ...).  With a ``normalization`` dict, its "extremes"
+    (int32 (m,)) and "intercepts" (float64 (m,)) device tensors receive the
+    selection's extreme points and intercepts when it normalizes."""
     m, n = parents.f.shape
     d = parents.x.shape[1]
     k = directions.shape[1]
@@ -193,6 +195,13 @@ def nsga3_select(parents: Population, offspring: Population, directions: torch.T
     a.stream = _stream(dev)
     if cv_quant:
         a.cv_quant = cv_quantization(**cv_quant)
+    if normalization is not None:
+        ext = normalization.setdefault("extremes", torch.full((m,), -7, dtype=_I32, device=dev))
+        icp = normalization.setdefault("intercepts", torch.full((m,), -7.0, dtype=torch.float64, device=dev))
+        _dev(ext, _I32, "normalization extremes", (m,))
+        if icp.dtype != torch.float64 or icp.device != dev or tuple(icp.shape) != (m,):
+            raise ValueError("normalization intercepts must be a float64 (m,) tensor on the device")
+        a.out_extremes, a.out_intercepts = ext.data_ptr(), icp.data_ptr()
     check(lib.moea_nsga3_select(ctypes.byref(a)))
     return nxt, nfr, stats
 
@@ -200,10 +209,13 @@ def nsga3_select(parents: Population, offspring: Population, directions: torch.T
 # ----------------------------------------------------------- reproduction --
 def reproduce(parents_x: torch.Tensor, bounds: torch.Tensor, *, rank=None, cv=None,
               mating: int = _lib.MATING_TOURNAMENT, active: Optional[int] = None,
+              active_device: Optional[torch.Tensor] = None,
               sbx=(30.0, 1.0, 0.0), mutation: int = _lib.MUTATION_POLYNOMIAL, pm=(20.0, 1.0),
               draws: Optional[dict] = None, seed: int = 0, generation: int = 0):
     """Mating -> SBX -> PM.  sbx = (eta, probability, copy probability),
-    pm = (eta, probability).  Returns (offspring_x, parent_indices)."""
+    pm = (eta, probability).  Candidates come from the first ``active`` parents,
+    or from the first active_device[0] (an int32 device tensor, read by the
+    kernel) when given.  Returns (offspring_x, parent_indices)."""
     n, d = parents_x.shape
     dev = parents_x.device
     _dev(parents_x, _F32, "parents_x"); _dev(bounds, _F32, "bounds", (2 * d,))
@@ -212,6 +224,9 @@ def reproduce(parents_x: torch.Tensor, bounds: torch.Tensor, *, rank=None, cv=No
     dr, keep = make_draws(draws, dev)
     a = _lib.ReproduceArgs()
     a.n, a.d, a.active = n, d, n if active is None else active
+    if active_device is not None:
+        _dev(active_device, _I32, "active_device", (1,))
+        a.active_device = active_device.data_ptr()
     a.mating = mating
     a.parents_x, a.parents_rank, a.parents_cv = parents_x.data_ptr(), _ptr(rank), _ptr(cv)
     a.bounds = bounds.data_ptr()

@@ -185,6 +185,13 @@ typedef struct moea_nsga3_select_args {
     size_t workspace_bytes;
     void* stream;
     moea_cv_quantization cv_quant;   /* bins = 0: the sort compares raw violations */
+    /* optional normalization outputs, written when the selection normalizes
+     * (n_rem > 0; otherwise left unchanged); NULL skips them: */
+    int32_t* out_extremes;      /* (m,) merged-row index of each axis's extreme
+                                   point, -1 when no row is finite there */
+    double* out_intercepts;     /* (m,) hyperplane intercepts a (FP64 solve, or
+                                   the row-max fallback when stats[3] = 1)
+                                   before their float rounding and sanitizing */
 } moea_nsga3_select_args;
 
 size_t moea_nsga3_workspace_bytes(int n, int m, int k);

@@ -695,7 +695,7 @@ ORC_API int orc_nsga3_select(const float* fm, const float* cvm, int n, int m,
                              const float* w, int k, float* ideal_last,
                              const uint32_t* ua, const uint32_t* ub,
                              int32_t* chosen, int32_t* fronts_mix, int32_t* stats,
-                             float* a_out)
+                             float* a_out, int32_t* ext_out, double* a64_out)
 {
     const int nm = 2 * n;
     const int max_front = orc_nds(fm, cvm, nm, m, n, fronts_mix);
@@ -723,8 +723,13 @@ ORC_API int orc_nsga3_select(const float* fm, const float* cvm, int n, int m,
         for (int c = 0; c < m; ++c)
             for (int i = 0; i < n_nds; ++i) fn[(size_t)c * n_nds + i] = fm[(size_t)c * nm + gidx[i]];
         float a[16];
-        ill = orc_normalize(fn, n_nds, m, ideal_last, a, NULL, NULL);
+        int32_t ext[16];
+        double a64[16];
+        ill = orc_normalize(fn, n_nds, m, ideal_last, a, ext, a64);
         if (a_out) memcpy(a_out, a, sizeof(float) * (size_t)m);
+        if (a64_out) memcpy(a64_out, a64, sizeof(double) * (size_t)m);
+        if (ext_out)
+            for (int c = 0; c < m; ++c) ext_out[c] = (ext[c] >= 0 && ext[c] < n_nds) ? gidx[ext[c]] : -1;
         float* dist = (float*)malloc(sizeof(float) * (size_t)n_nds);
         int32_t* ref = (int32_t*)malloc(sizeof(int32_t) * (size_t)n_nds);
         orc_associate(fn, n_nds, m, w, k, dist, ref);
@@ -1038,7 +1043,7 @@ ORC_API double orc_nsga3_run(int n, int m, int d, int k_req, int partitions, int
         orc_eval_quadratic(off, n, d, centers, m, fm + n, nm, cvm + n);
         memcpy(xm, x, sizeof(float) * (size_t)n * d);
         orc_draw_niche(seed, (uint32_t)g, nm, ua, ubn);
-        orc_nsga3_select(fm, cvm, n, m, w, k, ideal, ua, ubn, chosen, fr_mix, NULL, NULL);
+        orc_nsga3_select(fm, cvm, n, m, w, k, ideal, ua, ubn, chosen, fr_mix, NULL, NULL, NULL, NULL);
         for (int i = 0; i < n; ++i) {
             const int s = chosen[i];
             memcpy(x + (size_t)i * d, xm + (size_t)s * d, sizeof(float) * d);
@@ -1250,6 +1255,86 @@ ORC_API void orc_rvea_adapt(const float* v0, const float* f, int n_active, int l
             v_out[(size_t)c * k + j] = o;
         }
     }
+}
+
+/* A full RVEA run (EvolutionaryAlgorithm::step, algorithm.cu:219-286, with the
+ * built-in RVEA strategies) on quadratic-center objectives, with the
+ * in-kernel draw generator: random mating over the live parents
+ * (mating_selection.cu:39-62, `lb + curand() % active`), SBX + PM as for
+ * NSGA-III, RVEA selection with t = generation + 1 in the penalty
+ * (rvea_environment_selector.cu:236-239), the next population's dead rows
+ * [active, n) holding NaN variables and 1e15 objectives and violations
+ * (rvea_population_update.cu:137-205), then the direction adaptation when
+ * generation + 1 is a multiple of max(1, round(frequency x gens)) over the
+ * live rows of the next population (reference_directions.cu:132-162) and a
+ * new gamma.  v0: the initial (m, k) directions.  Outputs x (n, d), f (m, n),
+ * cv (n), v (m, k) of the last generation; returns the live count. */
+ORC_API int orc_rvea_run(int n, int m, int d, const float* v0, int k, int gens, uint64_t seed,
+                         const float* bounds, const float* centers, float alpha, float frequency,
+                         float sbx_eta0, float sbx_eta1, float sbx_prob, float sbx_copy,
+                         float pm_eta0, float pm_eta1, float pm_prob,
+                         float* x, float* f, float* cv, float* v, int gen_count_limit)
+{
+    const int nm = 2 * n;
+    float* gamma = (float*)malloc(sizeof(float) * (size_t)k);
+    float* xm = (float*)malloc(sizeof(float) * (size_t)nm * d);
+    float* fm = (float*)malloc(sizeof(float) * (size_t)m * nm);
+    float* cvm = (float*)malloc(sizeof(float) * (size_t)nm);
+    int32_t* win = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    const int pairs = n / 2;
+    const size_t pd = (size_t)(pairs > 0 ? pairs : 1) * d;
+    float* ubeta = (float*)malloc(sizeof(float) * pd);
+    float* up = (float*)malloc(sizeof(float) * pd);
+    int32_t* sg = (int32_t*)malloc(sizeof(int32_t) * pd);
+    float* uc = (float*)malloc(sizeof(float) * pd);
+    float* ug = (float*)malloc(sizeof(float) * (size_t)n * d);
+    float* ud = (float*)malloc(sizeof(float) * (size_t)n * d);
+    int32_t* sel = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    float* off = xm + (size_t)n * d;
+
+    memcpy(v, v0, sizeof(float) * (size_t)m * k);
+    orc_rvea_gamma(v, m, k, gamma);
+    orc_draw_init(seed, n, d, bounds, x);
+    orc_eval_quadratic(x, n, d, centers, m, f, n, cv);
+    int active = n;
+    int interval = (int)lround((double)frequency * gens);
+    if (interval < 1) interval = 1;
+    const int g_end = (gen_count_limit > 0 && gen_count_limit < gens) ? gen_count_limit : gens;
+    for (int g = 0; g < g_end; ++g) {
+        const float eta_c = orc_annealed(sbx_eta0, sbx_eta1, g, gens);
+        const float eta_m = orc_annealed(pm_eta0, pm_eta1, g, gens);
+        orc_draw_mating(seed, (uint32_t)g, n, active, win, NULL);
+        orc_draw_sbx(seed, (uint32_t)g, pairs, d, ubeta, up, sg, uc);
+        orc_sbx(x, win, bounds, ubeta, up, sg, uc, off, eta_c, sbx_prob, sbx_copy, n, d);
+        orc_draw_pm(seed, (uint32_t)g, n, d, ug, ud);
+        orc_pm(ug, ud, off, bounds, eta_m, pm_prob, n, d);
+        for (int c = 0; c < m; ++c) memcpy(fm + (size_t)c * nm, f + (size_t)c * n, sizeof(float) * n);
+        memcpy(cvm, cv, sizeof(float) * n);
+        orc_eval_quadratic(off, n, d, centers, m, fm + n, nm, cvm + n);
+        memcpy(xm, x, sizeof(float) * (size_t)n * d);
+        const float pen = orc_rvea_penalty(m, g + 1, gens, alpha);
+        const int na = orc_rvea_select(fm, cvm, n, m, v, gamma, k, active, pen, sel, NULL, NULL, NULL);
+        for (int i = 0; i < n; ++i) {
+            if (i < na) {
+                const int s = sel[i];
+                memcpy(x + (size_t)i * d, xm + (size_t)s * d, sizeof(float) * d);
+                for (int c = 0; c < m; ++c) f[(size_t)c * n + i] = fm[(size_t)c * nm + s];
+                cv[i] = cvm[s];
+            } else {
+                for (int j = 0; j < d; ++j) x[(size_t)i * d + j] = NAN;
+                for (int c = 0; c < m; ++c) f[(size_t)c * n + i] = 1e15f;
+                cv[i] = 1e15f;
+            }
+        }
+        active = na;
+        if ((g + 1) % interval == 0) {
+            orc_rvea_adapt(v0, f, active, n, m, k, v);
+            orc_rvea_gamma(v, m, k, gamma);
+        }
+    }
+    free(sel); free(ud); free(ug); free(uc); free(sg); free(up); free(ubeta); free(win);
+    free(cvm); free(fm); free(xm); free(gamma);
+    return active;
 }
 
 ORC_API int orc_num_threads(void)

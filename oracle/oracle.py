@@ -59,7 +59,7 @@ def lib():
             "orc_associate": (None, [f32p, c_int, c_int, f32p, c_int, f32p, i32p]),
             "orc_niching": (None, [i32p, f32p, c_int, c_int, c_int, c_int, u32p, u32p, i32p, vp]),
             "orc_nsga3_select": (c_int, [f32p, f32p, c_int, c_int, f32p, c_int, f32p, u32p,
-                                         u32p, i32p, i32p, i32p, vp]),
+                                         u32p, i32p, i32p, i32p, vp, vp, vp]),
             "orc_annealed": (c_float, [c_float, c_float, c_int, c_int]),
             "orc_tournament": (None, [i32p, i32p, c_int, i32p, f32p, i32p]),
             "orc_sbx": (None, [f32p, i32p, f32p, f32p, f32p, i32p, f32p, f32p, c_float,
@@ -79,6 +79,9 @@ def lib():
             "orc_rvea_select": (c_int, [f32p, f32p, c_int, c_int, f32p, f32p, c_int, c_int,
                                         c_float, i32p, vp, vp, vp]),
             "orc_rvea_adapt": (None, [f32p, f32p, c_int, c_int, c_int, c_int, f32p]),
+            "orc_rvea_run": (c_int, [c_int, c_int, c_int, f32p, c_int, c_int, c_u64, f32p, f32p, c_float,
+                                     c_float, c_float, c_float, c_float, c_float, c_float, c_float,
+                                     c_float, f32p, f32p, f32p, f32p, c_int]),
             "orc_num_threads": (c_int, []),
             "orc_set_threads": (None, [c_int]),
         }
@@ -219,8 +222,11 @@ def niching(ref, dist, n_prior, n_last, k, n_rem, ua, ub):
     return out[:n_rem]
 
 
-def nsga3_select(f_mix, cv_mix, w, ideal_last, ua, ub):
-    """Returns (chosen sorted (n,), fronts_mix (2n,), stats, ideal_current)."""
+def nsga3_select(f_mix, cv_mix, w, ideal_last, ua, ub, normalization=None):
+    """Returns (chosen sorted (n,), fronts_mix (2n,), stats, ideal_current).
+    A ``normalization`` dict receives "extremes" (merged-row indices),
+    "intercepts" (float64, before rounding) and "a" (float32) when the
+    selection normalizes."""
     f_mix = _c(f_mix, np.float32); cv_mix = _c(cv_mix, np.float32); w = _c(w, np.float32)
     m, nm = f_mix.shape
     n = nm // 2
@@ -229,8 +235,11 @@ def nsga3_select(f_mix, cv_mix, w, ideal_last, ua, ub):
     ua = _c(ua, np.uint32); ub = _c(ub, np.uint32)
     if ua.size == 0:
         ua = np.zeros(1, np.uint32); ub = np.zeros(1, np.uint32)
+    ext = np.full(m, -7, np.int32); a64 = np.full(m, -7.0, np.float64); a32 = np.full(m, -7.0, np.float32)
     lib().orc_nsga3_select(f_mix, cv_mix, n, m, w, w.shape[1], ideal, ua, ub, chosen, fr, st,
-                           None)
+                           _ptr(a32), _ptr(ext), _ptr(a64))
+    if normalization is not None:
+        normalization.update(extremes=ext, intercepts=a64, a=a32)
     return chosen, fr, st, ideal
 
 
@@ -316,6 +325,20 @@ def rvea_adapt(v0, f, n_active):
     out = np.zeros((m, k), np.float32)
     lib().orc_rvea_adapt(v0, f, n_active, f.shape[1], m, k, out)
     return out
+
+
+def rvea_run(n, m, d, v0, gens, seed, bounds, centers, alpha=2.0, frequency=0.1,
+             sbx=(30.0, 30.0, 1.0, 0.0), pm=(20.0, 20.0, 1.0), limit=0):
+    """Full RVEA run (random mating, SBX, PM, RVEA selection, adaptation) with
+    the in-kernel draw generator.  Returns (x, f, cv, v, active)."""
+    v0 = _c(v0, np.float32)
+    k = v0.shape[1]
+    x = np.zeros((n, d), np.float32); f = np.zeros((m, n), np.float32)
+    cv = np.zeros(n, np.float32); v = np.zeros((m, k), np.float32)
+    na = lib().orc_rvea_run(n, m, d, v0, k, gens, seed, _c(bounds, np.float32), _c(centers, np.float32),
+                            alpha, frequency, sbx[0], sbx[1], sbx[2], sbx[3], pm[0], pm[1], pm[2],
+                            x, f, cv, v, limit)
+    return x, f, cv, v, int(na)
 
 
 def num_threads() -> int:

@@ -38,7 +38,7 @@ def test_python_binding_covers_header():
 
 def test_host_side_calls():
     import cuda_moea._lib as L
-    assert L.lib.moea_abi_version() == 3
+    assert L.lib.moea_abi_version() == 4
     assert L.lib.moea_das_dennis(92, 3, 0, 0, None, None) == 91
     assert L.lib.moea_das_dennis(32768, 3, 0, 0, None, None) == 32640
     assert abs(L.lib.moea_annealed(30.0, 10.0, 50, 100) - 20.0) < 1e-6

@@ -291,3 +291,20 @@ def test_nsga3_select_generated_draws():
     ofs = K.Population(_f(xm[n:]), _f(fm[:, n:]), _f(cvm[n:]))
     nxt, nfr, _ = K.nsga3_select(par, ofs, _f(w), _f(ideal0), seed=seed, generation=gen)
     np.testing.assert_array_equal(nxt.x[:, 0].cpu().numpy().astype(np.int64), chosen)
+
+
+@pytest.mark.parametrize("n,active,gen", [(92, 92, 0), (120, 37, 3), (256, 1, 5), (1001, 640, 9)])
+def test_random_mating_matches_oracle(n, active, gen):
+    """RandomMating::select (mating_selection.cu:39-62): candidate t is
+    draw % active over the live parents, with the live count read on the
+    device (the RVEA path), bit for bit against the oracle's draw."""
+    d = 4
+    rng = np.random.default_rng(n + gen)
+    x = torch.from_numpy(rng.random((n, d), dtype=np.float32)).to(DEV)
+    bounds = torch.from_numpy(np.tile(np.array([0.0, 1.0], np.float32), d)).to(DEV)
+    act = torch.tensor([active], dtype=torch.int32, device=DEV)
+    print(f"random mating n={n} active={active} gen={gen}", flush=True)
+    _, idx = K.reproduce(x, bounds, mating=1, active=n, active_device=act, seed=11, generation=gen)
+    want = O.draw_mating(11, gen, n, active, pair=False)
+    np.testing.assert_array_equal(idx.cpu().numpy(), want)
+    assert idx.max().item() < active

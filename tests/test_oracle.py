@@ -432,3 +432,211 @@ def test_rvea_adapt_identity_range():
     f = np.stack([np.linspace(0, 1, 50)] * 3).astype(np.float32)
     out = O.rvea_adapt(v0, f, 50)
     np.testing.assert_allclose(out, v0, atol=1e-7)
+
+
+# --------------------------------------------- RVEA against its definition ----
+def _rvea_fp64(fm, cvm, v, gam, active, penalty):
+    """The published RVEA selection (Cheng et al., IEEE TEVC 2016, with the
+    reference's conventions: translation by the minimum over all 2n columns,
+    floor 1e-32, dead parent slots excluded, feasible rows by APD, one
+    survivor per direction, truncation to n by APD) in float64, written from
+    the definition: per direction, the member with the smallest APD survives."""
+    m, nm = fm.shape
+    n = nm // 2
+    F = fm.astype(np.float64)
+    zmin = np.min(F, axis=1, keepdims=True)
+    T = np.maximum(F - zmin, 1e-32)
+    V = v.astype(np.float64)
+    nrm = np.sqrt((T * T).sum(0))
+    cos = (V.T @ T) / np.sqrt((T * T).sum(0) + 1e-12)          # (k, 2n)
+    ref = np.argmax(cos, axis=0)
+    c = np.clip(cos[ref, np.arange(nm)], 0.0, 1.0)
+    theta = np.arccos(c)
+    ref = ref.astype(np.int64)
+    ref[active:n] = -1                                          # dead parent slots
+    apd = (1.0 + penalty * np.clip(theta, 0, np.pi) / np.maximum(gam.astype(np.float64)[np.maximum(ref, 0)],
+                                                                     1e-6)) * nrm
+    winners = []
+    for j in range(v.shape[1]):
+        members = np.where(ref == j)[0]
+        if len(members):
+            winners.append(members[np.argmin(apd[members])])
+    winners = np.array(sorted(winners, key=lambda i: apd[i]))[:n]
+    return np.sort(winners), ref, theta, c, apd
+
+
+@pytest.mark.parametrize("n,m,active,seed", [(30, 2, 30, 0), (60, 3, 60, 1), (60, 3, 41, 2), (105, 5, 105, 3),
+                                             (120, 8, 77, 4), (50, 3, 50, 5), (200, 4, 180, 6)])
+def test_rvea_select_against_fp64_definition(n, m, active, seed):
+    rng = np.random.default_rng(seed)
+    v = O.das_dennis(n, m, kind=2)
+    gam = O.rvea_gamma(v)
+    fm = (rng.random((m, 2 * n)) * 3 + 0.25).astype(np.float32)
+    fm[:, active:n] = 1e15
+    cvm = np.zeros(2 * n, np.float32)
+    cvm[active:n] = 1e15
+    pen = O.rvea_penalty(m, 37, 100, 2.0)
+    sel, na, ref, th, apd = O.rvea_select(fm, cvm, v, gam, active, pen)
+    w64, ref64, th64, c64, apd64 = _rvea_fp64(fm, cvm, v, gam, active, pen)
+    live = np.r_[0:active, n:2 * n]
+    np.testing.assert_array_equal(ref[live], ref64[live])
+    assert np.all(ref[active:n] == v.shape[1])
+    # theta: the float32 cosine carries a few ulp of 1 (an M-term fma chain and
+    # a reciprocal square root), which acos amplifies by 1 / sqrt(1 - c^2)
+    ulp1 = 2.0 ** -24
+    tol_th = 4 * ulp1 * np.pi + 8 * m * ulp1 / np.sqrt(np.maximum(1 - c64[live] ** 2, 1e-12))
+    assert np.all(np.abs(th[live] - th64[live]) <= tol_th)
+    # APD = (1 + P theta / gamma) ||f'||: relative error of the norm plus the
+    # theta error scaled by P / gamma
+    g = np.maximum(gam.astype(np.float64)[ref64[live]], 1e-6)
+    nrm = apd64[live] / (1 + pen * np.clip(th64[live], 0, np.pi) / g)
+    tol_apd = 8 * ulp1 * apd64[live] + pen / g * tol_th * nrm
+    assert np.all(np.abs(apd[live] - apd64[live]) <= tol_apd)
+    np.testing.assert_array_equal(sel, w64)
+    assert na == len(w64)
+
+
+def test_rvea_penalty_closed_forms():
+    # at t = G the penalty is exactly M; at t = 0 it is 0; alpha = 1 is linear
+    for m in (2, 3, 5, 10):
+        assert O.rvea_penalty(m, 100, 100, 2.0) == float(m)
+        assert O.rvea_penalty(m, 0, 100, 2.0) == 0.0
+        assert O.rvea_penalty(m, 250, 100, 2.0) == float(m)          # progress clamps at 1
+    assert O.rvea_penalty(3, 25, 100, 1.0) == np.float32(0.75)
+    assert O.rvea_penalty(4, 50, 100, 2.0) == 1.0
+
+
+def test_rvea_two_directions_known_angle():
+    """Two directions 60 degrees apart, one individual 10 degrees from the
+    first: it belongs to the first, theta = 10 degrees, and the APD is
+    (1 + P theta / gamma) |f'| with gamma = 60 degrees."""
+    phi, psi = math.radians(60), math.radians(10)
+    v = np.array([[1.0, math.cos(phi)], [0.0, math.sin(phi)]], np.float32)
+    gam = O.rvea_gamma(v)
+    np.testing.assert_allclose(gam, [phi, phi], rtol=1e-6)
+    # merged population of 2 (n = 1): the anchor at the origin after translation,
+    # and the individual at radius 2 and angle psi from direction 0
+    fm = np.array([[0.0, 2 * math.cos(psi)], [0.0, 2 * math.sin(psi)]], np.float32)
+    pen = 2.0
+    sel, na, ref, th, apd = O.rvea_select(fm, np.zeros(2, np.float32), v, gam, 1, pen)
+    assert ref[1] == 0
+    np.testing.assert_allclose(th[1], psi, rtol=1e-5)
+    np.testing.assert_allclose(apd[1], (1 + pen * psi / phi) * 2.0, rtol=1e-5)
+    assert apd[0] == 0.0                                       # zero norm after translation
+
+
+def test_rvea_adapt_shifted_objectives_and_dead_rows():
+    """V = normalize(V0 * (zmax - zmin)) over the live rows only: objectives
+    shifted by +5 (so zmin != 0 matters) and dead rows at 1e15 past n_active."""
+    rng = np.random.default_rng(7)
+    m, n, active = 3, 80, 53
+    v0 = O.das_dennis(91, m, kind=2)
+    f = (rng.random((m, n)) * np.array([[1.0], [3.0], [0.5]]) + 5.0).astype(np.float32)
+    f[:, active:] = 1e15
+    out = O.rvea_adapt(v0, f, active)
+    rng_ = f[:, :active].astype(np.float64).max(1) - f[:, :active].astype(np.float64).min(1)
+    s = v0.astype(np.float64) * rng_[:, None]
+    ref = s / np.linalg.norm(s, axis=0, keepdims=True)
+    np.testing.assert_allclose(out, ref, rtol=0, atol=8 * 2.0 ** -24)
+    # a dropped "- zmin" or a range over all n rows would be far off
+    wrong = v0.astype(np.float64) * f[:, :active].max(1)[:, None]
+    assert np.abs(out - wrong / np.linalg.norm(wrong, axis=0)).max() > 1e-2
+
+
+# ---------------------------------------------- PM against Deb's formula ----
+def test_pm_against_fp64_deb_formula():
+    """Polynomial mutation (Deb & Goyal 1996, in the bounded form the reference
+    uses): with x' = (x - lo) / (hi - lo), for u <= 1/2
+    delta = (2u + (1 - 2u)(1 - x')^(eta+1))^(1/(eta+1)) - 1, else
+    delta = 1 - (2(1 - u) + 2(u - 1/2) x'^(eta+1))^(1/(eta+1)); the child is
+    clamp(x + delta (hi - lo)).  Both branches, values near both bounds."""
+    rng = np.random.default_rng(8)
+    n, d, eta = 64, 8, 20.0
+    lo = np.array([-1.0, 0.0, 2.0, -5.0, 0.0, 0.0, 10.0, -0.5], np.float32)
+    hi = np.array([1.0, 1.0, 3.0, 5.0, 0.001, 100.0, 10.5, 0.5], np.float32)
+    bounds = np.stack([lo, hi], 1).reshape(-1).astype(np.float32)
+    t = rng.random((n, d))
+    t[:8] = 1e-4          # near the lower bound
+    t[8:16] = 1 - 1e-4    # near the upper bound
+    x = (lo + t * (hi - lo)).astype(np.float32)
+    ud = rng.random(n * d).astype(np.float32)
+    ud[::3] = np.float32(0.5)                               # the branch point itself
+    out = O.pm(x, bounds, np.zeros(n * d, np.float32), ud, eta, 1.0)
+    for r in range(n):
+        for j in range(d):
+            u = float(ud[r * d + j])
+            xv, a, b = float(x[r, j]), float(lo[j]), float(hi[j])
+            xn = (xv - a) / (b - a)
+            e1 = eta + 1.0
+            if u <= 0.5:
+                delta = (2 * u + (1 - 2 * u) * (1 - xn) ** e1) ** (1 / e1) - 1
+            else:
+                delta = 1 - (2 * (1 - u) + 2 * (u - 0.5) * xn ** e1) ** (1 / e1)
+            ref = min(max(xv + delta * (b - a), a), b)
+            # float32 evaluation: the normalized value, two powf (within 1 ulp
+            # each) and the fused update, a few ulp of the range
+            assert abs(out[r, j] - ref) <= 16 * 2.0 ** -24 * (b - a) + _ulp32(ref), (r, j, u, out[r, j], ref)
+
+
+# ------------------------------------------ niching against its definition ----
+def _niching_definition(ref, dist, n_prior, n_last, k, n_rem, ua, ub):
+    """NSGA-III niching with the reference's random tie-breaks, written from
+    the definition (niching.cu:114-178) rather than as three sorts: list the
+    last front by (direction, distance, index) and number that order p1;
+    inside each direction, a direction no prior-front member uses (rho = 0)
+    takes its closest member first (distance, then index), and every other
+    member follows by the draw ua[p1] | 1 (then p1); a member's round is rho
+    plus its place in that order.  Number the members by (direction, place)
+    as p2; the survivors are the first n_rem by (round, ub[p2], p2)."""
+    rho = np.zeros(k, np.int64)
+    for r in ref[:n_prior]:
+        if 0 <= r < k:
+            rho[r] += 1
+    members = list(range(n_prior, n_prior + n_last))
+    dbits = lambda i: int(np.float32(dist[i]).view(np.uint32))
+    p1_order = sorted(members, key=lambda i: (int(np.uint32(np.int64(ref[i]) & 0xFFFFFFFF)), dbits(i), i))
+    p1 = {i: t for t, i in enumerate(p1_order)}
+    by_dir = {}
+    for i in p1_order:
+        by_dir.setdefault(int(ref[i]), []).append(i)
+    placed = []
+    for r in sorted(by_dir, key=lambda r: r & 0xFFFFFFFF):
+        lst = by_dir[r]
+        rh = rho[r] if 0 <= r < k else 0
+        first = [lst[0]] if rh == 0 else []          # lst is already by (distance, index)
+        rest = sorted([i for i in lst if i not in first], key=lambda i: (int(ua[p1[i]]) | 1, p1[i]))
+        for place, i in enumerate(first + rest):
+            placed.append((r, place, i, rh + place))
+    p2 = {i: t for t, (r, place, i, rnd) in enumerate(placed)}
+    final = sorted(placed, key=lambda e: (e[3], int(ub[p2[e[2]]]), p2[e[2]]))
+    return np.array([e[2] for e in final[:n_rem]], np.int32)
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_niching_against_definition_with_draws(case):
+    rng = np.random.default_rng(100 + case)
+    k = int(rng.integers(2, 12))
+    n_prior = int(rng.integers(0, 15))
+    n_last = int(rng.integers(1, 25))
+    n_rem = int(rng.integers(1, n_last + 1))
+    ref = rng.integers(0, k, n_prior + n_last).astype(np.int32)
+    # coarse distances make exact ties on purpose (the index breaks them)
+    dist = (rng.integers(0, 6, n_prior + n_last) / 4.0).astype(np.float32)
+    ua = rng.integers(0, 2 ** 32, n_last, dtype=np.uint64).astype(np.uint32)
+    ub = rng.integers(0, 2 ** 32, n_last, dtype=np.uint64).astype(np.uint32)
+    if case % 4 == 0:
+        ub[:] = rng.integers(0, 3, n_last).astype(np.uint32)   # ties in the last key too
+    got = O.niching(ref, dist, n_prior, n_last, k, n_rem, ua, ub)
+    want = _niching_definition(ref, dist, n_prior, n_last, k, n_rem, ua, ub)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_rvea_partition_ties_take_smallest_index():
+    """Equal cosines to two directions (the bisector of the axes; a duplicated
+    direction) go to the smaller direction index."""
+    v = np.array([[1.0, 0.0, 1.0], [0.0, 1.0, 0.0]], np.float32)       # directions 0 and 2 identical
+    gam = np.full(3, 0.5, np.float32)
+    fm = np.array([[0.0, 3.0, 1.0, 2.0], [0.0, 3.0, 0.0, 0.0]], np.float32)
+    sel, na, ref, th, apd = O.rvea_select(fm, np.zeros(4, np.float32), v, gam, 2, 1.0)
+    assert ref[1] == 0            # (3, 3) after translation: cos 1/sqrt2 to directions 0 and 1
+    assert ref[2] == 0 and ref[3] == 0   # on the axis shared by directions 0 and 2
