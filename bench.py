@@ -428,7 +428,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-rvea", action="store_true")
     ap.add_argument("--rvea-generations", type=int, default=5)
-    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-seconds", type=float, default=120.0)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-plain-sample", type=int, default=24)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)

@@ -327,6 +327,7 @@ struct BatchCore {
     bool dirty = true;                        // membership changed since the last upload
     int peel_capacity = 0;                    // co-resident blocks of the batched peel
     bool error_ready = false;                 // the device error word has been cleared
+    moea_niching niching{};                   // niching path of every selection
 };
 
 namespace {
@@ -420,7 +421,7 @@ int core_init_admitted(BatchCore& C)
 {
     if (!any_state(C, kStateAdmitted)) return kOk;
     if (C.dirty) MOEA_CHECK(core_upload(C));
-    BatchLaunch L{C.dev_insts, C.slots, C.stream};
+    BatchLaunch L{C.dev_insts, C.slots, C.stream, C.niching};
     for (const BInst& I : C.host)
         if (I.state == kStateAdmitted && I.cone_buf) MOEA_CHECK(cones_build(I.dirs, I.k, I.m, I.cone_buf, C.stream));
     MOEA_CHECK(batch_init_population(L, C.init.init256.map, C.init.init256.blocks));
@@ -440,7 +441,7 @@ int core_step(BatchCore& C, std::vector<int>* done)
 {
     if (!any_state(C, kStateRunning)) return kOk;
     if (C.dirty) MOEA_CHECK(core_upload(C));
-    BatchLaunch L{C.dev_insts, C.slots, C.stream};
+    BatchLaunch L{C.dev_insts, C.slots, C.stream, C.niching};
     stage_begin(kStReproduce, C.stream);
     MOEA_CHECK(batch_reproduce(L, C.run.rows256.map, C.run.rows256.blocks, C.run.sbx256.map, C.run.sbx256.blocks));
     stage_end(kStReproduce, C.stream);
@@ -493,7 +494,10 @@ int validate_batch(const moea_batch_problem* probs, int count)
         MOEA_CHECK(validate_problem(probs[i]));
         if (probs[i].max_generations != probs[0].max_generations)
             return fail(kInvalidArgument, "moea_batch: every instance of a batch runs the same number of generations");
+        if (probs[i].niching.mode != probs[0].niching.mode || probs[i].niching.lds_cap != probs[0].niching.lds_cap)
+            return fail(kInvalidArgument, "moea_batch: every instance of a batch uses the same niching path");
     }
+    if (!niching_valid(probs[0].niching)) return fail(kInvalidArgument, "moea_batch: invalid niching mode or capacity");
     return kOk;
 }
 
@@ -538,6 +542,7 @@ int batch_create(const moea_batch_problem* probs, int count, void* ws, size_t ws
     C.stream = stream;
     for (int i = 0; i < count; ++i) C.max_rows += 2ll * probs[i].n;
     batch_layout(probs, count, static_cast<char*>(ws), &C);
+    C.niching = probs[0].niching;
     B->max_generations = probs[0].max_generations;
     *out = B;
     return kOk;
@@ -629,7 +634,7 @@ size_t serve_reserved_bytes(int max_instances, long long max_rows)
 int serve_create(const moea_serve_config& cfg, void* ws, size_t ws_bytes, hipStream_t stream, ServeState** out)
 {
     if (!out) return fail(kInvalidArgument, "moea_serve_create: null output handle");
-    if (cfg.max_instances <= 0 || cfg.max_rows <= 0 || cfg.max_rows > (1ll << 30))
+    if (cfg.max_instances <= 0 || cfg.max_rows <= 0 || cfg.max_rows > (1ll << 30) || !niching_valid(cfg.niching))
         return fail(kInvalidArgument, "moea_serve_create: invalid configuration");
     const size_t reserved = serve_reserved_bytes(cfg.max_instances, cfg.max_rows);
     if (!ws || ws_bytes <= reserved) return fail(kWorkspaceTooSmall, "moea_serve_create: workspace too small");
@@ -641,6 +646,7 @@ int serve_create(const moea_serve_config& cfg, void* ws, size_t ws_bytes, hipStr
     C.probs.assign(C.slots, moea_batch_problem{});
     C.stream = stream;
     C.max_rows = cfg.max_rows;
+    C.niching = cfg.niching;
     Carve c(ws, ws_bytes);
     core_reserved(C.slots, C.max_rows, c, &C);
     S->base = static_cast<char*>(ws);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "../../include/moea_hip.h"
 
 namespace moea {
 
@@ -117,6 +118,7 @@ struct BatchLaunch {
     BInst* insts;                 // device instance table
     int count;
     hipStream_t stream;
+    moea_niching niching;         // niching path of the batch's selections
 };
 
 // Grids of one batched NDS call; mode 0 ranks the n parents of a slot, mode 1

@@ -160,8 +160,6 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k)
     return nsga3_workspace_bytes(n, m, k);
 }
 
-int moea_set_niching(int mode, int lds_cap) { return set_niching(mode, lds_cap); }
-
 int moea_quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization* q, void* stream)
 {
     if (!q) return fail(kInvalidArgument, "moea_quantize_cv: null parameters");

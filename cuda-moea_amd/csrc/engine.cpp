@@ -37,11 +37,24 @@ bool cv_active(const moea_run_args& a, int g)
     return static_cast<float>(g + 1) / static_cast<float>(total) >= ratio;
 }
 
-// is_cv_activation_switch_node (dtlz_kernels.cu:1311-1322)
+// is_cv_activation_switch_node (dtlz_kernels.cu:1311-1322): generation 0 when
+// the constraints are on from the start (ratio <= 0 included), else the
+// generation at which they switch on
 bool cv_switch(const moea_run_args& a, int g)
 {
-    if (!constrained_dtlz(a.problem.kind) || !(a.problem.cv_activation_ratio > 0.0f)) return false;
+    if (!constrained_dtlz(a.problem.kind)) return false;
     return g == 0 ? cv_active(a, 0) : (!cv_active(a, g - 1) && cv_active(a, g));
+}
+
+// RVEA rows past the live count keep their 1e15 objectives and violations
+// (and NaN variables) when a switch re-evaluates the parents.
+__global__ void rvea_dead_sentinels(float* __restrict__ f, float* __restrict__ cv, int n, int m,
+                                    const int32_t* __restrict__ active)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || i < active[0]) return;
+    for (int k = 0; k < m; ++k) f[static_cast<size_t>(k) * n + i] = 1e15f;
+    cv[i] = 1e15f;
 }
 
 int evaluate_problem(const moea_run_args& a, const moea_population& p, hipStream_t st, int generation);
@@ -187,11 +200,28 @@ int run_step(moea_run_args& a, int generations)
         moea_population& par = a.pops[a.current];
         moea_population& ofs = a.pops[off];
         moea_population& nx = a.pops[nxt];
+        bool parents_changed = false;
         if (cv_switch(a, g)) {
             // DTLZProblem::prepare_parent (dtlz_kernels.cu:1363-1394, algorithm.cu:233-241): the
             // constraints switch on at g, so the parents' violations are recomputed (a fresh
-            // evaluation gives the same objectives) and NSGA-III re-sorts its parent fronts
+            // evaluation gives the same objectives)
             MOEA_CHECK(evaluate_problem(a, par, st, g));
+            if (a.algorithm == MOEA_ALGORITHM_RVEA && a.active_device) {
+                hipLaunchKernelGGL(rvea_dead_sentinels, dim3(ceil_div(a.n, 256)), dim3(256), 0, st, par.f, par.cv,
+                                   a.n, a.m, a.active_device);
+                MOEA_HIP(hipGetLastError());
+            }
+            parents_changed = true;
+        } else if (a.problem.kind == MOEA_PROBLEM_CALLBACK && a.problem.prepare_parent) {
+            // IProblemEvaluator::prepare_parent of a callback problem (algorithm.cu:230-241)
+            const int rc = a.problem.prepare_parent(a.problem.user, &par, a.n, a.d, a.m, g, a.max_generations,
+                                                    a.stream);
+            if (rc < 0) return rc;
+            parents_changed = rc > 0;
+        }
+        if (parents_changed) {
+            // IEnvironmentSelector::prepare: NSGA-III re-sorts its parent fronts; the built-in
+            // RVEA selector keeps its state (rvea_environment_selector.cu:155-161)
             if (a.hooks && a.hooks->select) {
                 if (a.hooks->prepare)
                     MOEA_CHECK(a.hooks->prepare(a.hooks->user, &par, a.n, a.d, a.m, g, a.max_generations,
@@ -279,6 +309,7 @@ int run_step(moea_run_args& a, int generations)
             s.generation = g;
             s.directions_ready = a.directions_ready;
             s.cv_quant = a.cv_quant;
+            s.niching = a.niching;
             s.stats = a.stats;
             s.workspace = a.workspace; s.workspace_bytes = a.workspace_bytes;
             s.stream = a.stream;

@@ -25,7 +25,7 @@ void set_peel_spin_limit(uint32_t limit);
 // nsga3.hip
 size_t nsga3_workspace_bytes(int n, int m, int k);
 int nsga3_select(const moea_nsga3_select_args& a);
-int set_niching(int mode, int lds_cap);
+bool niching_valid(const moea_niching& c);
 bool cv_quant_valid(const moea_cv_quantization& q);
 int quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization& q, hipStream_t stream);
 // Cone tables of the pruned association, built once per set of directions.

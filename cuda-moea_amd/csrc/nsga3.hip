@@ -2062,36 +2062,24 @@ int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t
     return kOk;
 }
 
+// Niching path (moea_niching): 0 automatic (one block), 1 one block, 2
+// device-wide sorts, 3 one block with its radix sorts only.
+bool niching_valid(const moea_niching& c) { return c.mode >= 0 && c.mode <= 3 && c.lds_cap >= 0; }
+
 namespace {
-// Niching path: 0 automatic (one block), 1 one block, 2 device-wide sorts,
-// 3 one block with its radix sorts only.
-int g_niching_mode = 0;
-int g_niching_lds_cap = 1 << 30;   // each path also stops at its own LDS capacity
-
-bool niching_one_block(int nm)
-{
-    (void)nm;
-    return g_niching_mode != 2;
-}
-int niching_buckets() { return g_niching_mode == 3 ? 0 : 1; }
-int niching_mode() { return g_niching_mode; }
+bool niching_one_block(const moea_niching& c) { return c.mode != 2; }
+int niching_buckets(const moea_niching& c) { return c.mode == 3 ? 0 : 1; }
+// each path also stops at its own LDS capacity
+int niching_lds_cap(const moea_niching& c) { return c.lds_cap > 0 ? c.lds_cap : (1 << 30); }
 constexpr int kBatchNicheMaxNm = 8192;
-int niching_lds_cap() { return g_niching_lds_cap; }
 }  // namespace
-
-int set_niching(int mode, int lds_cap)
-{
-    if (mode < 0 || mode > 3 || lds_cap < 0) return fail(kInvalidArgument, "set_niching: invalid mode or capacity");
-    g_niching_mode = mode;
-    g_niching_lds_cap = lds_cap;
-    return kOk;
-}
 
 int nsga3_select(const moea_nsga3_select_args& A)
 {
     const int n = A.n, m = A.m, d = A.d, k = A.k;
     if (n <= 0 || (n & 1) || m <= 0 || m > kMaxM || d <= 0 || k <= 0)
         return fail(kInvalidArgument, "nsga3_select: invalid sizes (n must be even and positive)");
+    if (!niching_valid(A.niching)) return fail(kInvalidArgument, "nsga3_select: invalid niching mode or capacity");
     if (!A.parents.x || !A.parents.f || !A.parents.cv || !A.offspring.x || !A.offspring.f ||
         !A.offspring.cv || !A.next.x || !A.next.f || !A.next.cv || !A.next_fronts ||
         !A.directions || !A.ideal || !A.workspace)
@@ -2146,11 +2134,11 @@ int nsga3_select(const moea_nsga3_select_args& A)
     stage_begin(kStNiching, st);
     const int bits_ref = 32 + (32 - __builtin_clz(static_cast<unsigned>(k + 1)));
     const int bits_rank = 32 + (32 - __builtin_clz(static_cast<unsigned>(nm)));
-    if (niching_one_block(nm)) {
+    if (niching_one_block(A.niching)) {
         NicheArgs na{L.ref, L.dist, L.rho, k, bits_ref, bits_rank, dr ? dr->niche_a : nullptr,
-                     dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets()};
+                     dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets(A.niching)};
         hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, n, L.fronts, L.chosen,
-                           L.k0, L.k1, L.v0, L.v1, niching_lds_cap(), A.stats);
+                           L.k0, L.k1, L.v0, L.v1, niching_lds_cap(A.niching), A.stats);
     } else {
         hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
         MOEA_HIP(hipGetLastError());
@@ -2176,7 +2164,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
                        A.parents.x, A.offspring.x, L.fm, L.cvm, L.fronts, A.next.x, A.next.f,
                        A.next.cv, A.next_fronts);
     stage_end(kStGather, st);
-    if (A.stats && !niching_one_block(nm)) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
+    if (A.stats && !niching_one_block(A.niching)) hipLaunchKernelGGL(write_stats, dim3(1), dim3(64), 0, st, L.meta, A.stats);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
@@ -2508,9 +2496,9 @@ int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, 
     // one workgroup per instance while every instance is small (2n <= 8192);
     // larger runs (many objectives put most of 2n in the last front) sort
     // faster device-wide
-    if (niching_one_block(total) && (niching_mode() == 1 || G.max_nm <= kBatchNicheMaxNm)) {
+    if (niching_one_block(L.niching) && (L.niching.mode == 1 || G.max_nm <= kBatchNicheMaxNm)) {
         hipLaunchKernelGGL(b_niche, dim3(L.count), dim3(kNicheThreads), 0, st, L.insts, S.keys_in, S.keys_out,
-                           S.vals_in, S.vals_out, niching_buckets(), niching_lds_cap());
+                           S.vals_in, S.vals_out, niching_buckets(L.niching), niching_lds_cap(L.niching));
         MOEA_HIP(hipGetLastError());
         stage_end(kStNiching, st);
         stage_begin(kStGather, st);

@@ -45,6 +45,10 @@ class CvQuantization(ctypes.Structure):
                 ("feasibility_epsilon", c_float)]
 
 
+class Niching(ctypes.Structure):
+    _fields_ = [("mode", c_int), ("lds_cap", c_int)]
+
+
 class NSGA3SelectArgs(ctypes.Structure):
     _fields_ = [
         ("n", c_int), ("m", c_int), ("d", c_int), ("k", c_int),
@@ -54,7 +58,7 @@ class NSGA3SelectArgs(ctypes.Structure):
         ("directions_ready", c_int),
         ("stats", c_void_p), ("workspace", c_void_p), ("workspace_bytes", c_size_t),
         ("stream", c_void_p), ("cv_quant", CvQuantization),
-        ("out_extremes", c_void_p), ("out_intercepts", c_void_p),
+        ("out_extremes", c_void_p), ("out_intercepts", c_void_p), ("niching", Niching),
     ]
 
 
@@ -85,15 +89,16 @@ EVALUATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_void_p, c_int, c_int, c_int, c
                                c_void_p, c_int, c_int, c_void_p)
 
 
+_POP_P = ctypes.POINTER(PopulationC)
+PREPARE_PARENT_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_int, c_int, c_void_p)
+
+
 class ProblemDesc(ctypes.Structure):
     _fields_ = [
         ("kind", c_int), ("centers", c_void_p), ("w1", c_void_p), ("w2", c_void_p),
         ("hidden", c_int), ("hwork", c_void_p), ("callback", EVALUATE_FN), ("user", c_void_p),
-        ("cv_activation_ratio", c_float),
+        ("cv_activation_ratio", c_float), ("prepare_parent", PREPARE_PARENT_FN),
     ]
-
-
-_POP_P = ctypes.POINTER(PopulationC)
 MATE_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                            c_int, c_int, c_void_p)
 CROSSOVER_FN = ctypes.CFUNCTYPE(c_int, c_void_p, _POP_P, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
@@ -127,7 +132,7 @@ class RunArgs(ctypes.Structure):
         ("initial_x_given", c_int), ("problem", ProblemDesc), ("generation", c_int),
         ("directions_ready", c_int),
         ("workspace", c_void_p), ("workspace_bytes", c_size_t), ("stream", c_void_p),
-        ("hooks", ctypes.POINTER(StrategyHooks)), ("cv_quant", CvQuantization),
+        ("hooks", ctypes.POINTER(StrategyHooks)), ("cv_quant", CvQuantization), ("niching", Niching),
     ]
 
 
@@ -137,12 +142,12 @@ class BatchProblem(ctypes.Structure):
         ("seed", c_uint64), ("problem", c_int), ("centers", c_void_p), ("w1", c_void_p),
         ("w2", c_void_p), ("hidden", c_int), ("bounds", c_void_p), ("directions", c_void_p),
         ("sbx_eta", c_float * 2), ("sbx_prob", c_float), ("sbx_copy_prob", c_float),
-        ("pm_eta", c_float * 2), ("pm_prob", c_float),
+        ("pm_eta", c_float * 2), ("pm_prob", c_float), ("niching", Niching),
     ]
 
 
 class ServeConfig(ctypes.Structure):
-    _fields_ = [("max_instances", c_int), ("max_rows", ctypes.c_longlong)]
+    _fields_ = [("max_instances", c_int), ("max_rows", ctypes.c_longlong), ("niching", Niching)]
 
 
 # name -> (restype, argtypes); every symbol include/moea_hip.h declares
@@ -158,7 +163,6 @@ SIGNATURES = {
     "moea_debug_set_spin_limit": (None, [ctypes.c_uint32]),
     "moea_das_dennis": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_nsga3_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
-    "moea_set_niching": (c_int, [c_int, c_int]),
     "moea_nsga3_select": (c_int, [POINTER(NSGA3SelectArgs)]),
     "moea_ideal_point": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     "moea_associate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),

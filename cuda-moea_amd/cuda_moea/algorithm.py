@@ -318,6 +318,9 @@ class Algorithm:
                 p.initialize({"device": self.cuda.device_id})
             self._callback = _lib.EVALUATE_FN(self._evaluate_python)
             pd.kind, pd.callback = _lib.PROBLEM_CALLBACK, self._callback
+            if hasattr(p, "prepare_parent"):
+                self._prepare_cb = _lib.PREPARE_PARENT_FN(self._prepare_parent_python)
+                pd.prepare_parent = self._prepare_cb
         a.problem = pd
 
     def _context(self, generation: int, max_generations: int) -> dict[str, Any]:
@@ -550,6 +553,25 @@ class Algorithm:
             self._error = exc
             return -1
 
+    def _prepare_parent_python(self, user, parents, n, d, m, generation, max_generations, stream) -> int:
+        # adapters.cpp:175-186: None / False -> unchanged; True -> changed in
+        # place; an evaluation result replaces the parents' objectives and
+        # constraints
+        try:
+            pop = self._by_ptr[parents.contents.x]
+            out = self.problem.prepare_parent(self._pop_view(pop), {"generation": generation,
+                                                                   "max_generations": max_generations,
+                                                                   "device": self.cuda.device_id})
+            if out is None:
+                return 0
+            if isinstance(out, bool):
+                return 1 if out else 0
+            self._copy_evaluation(out, pop, n, m)
+            return 1
+        except BaseException as exc:  # surfaced after the native call returns
+            self._error = exc
+            return -1
+
     def _call(self, fn, *extra) -> None:
         self._error = None
         status = fn(ctypes.byref(self._args), *extra)
@@ -656,6 +678,8 @@ class Algorithm:
         rd = self.reference_directions
         if isinstance(rd, PythonReferenceDirections) and hasattr(rd, "reset"):
             rd.reset()                           # adapters.cpp:319-324
+        if self._args.problem.kind == _lib.PROBLEM_CALLBACK and hasattr(self.problem, "reset"):
+            self.problem.reset()                 # algorithm.cu:413, adapters.cpp:187-190
         self._args.seed = self._run_seed()
         self._args.current = 0
         self._args.fronts_current = 0

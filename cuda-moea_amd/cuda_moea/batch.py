@@ -91,8 +91,11 @@ def view_result(ws: torch.Tensor, pop, fronts: int, r, total_ms: float, copy: bo
 
 
 class BatchNSGA3:
-    def __init__(self, runs: Sequence[NSGA3]) -> None:
+    def __init__(self, runs: Sequence[NSGA3], *, niching: tuple[int, int] = (0, 0)) -> None:
+        """``niching`` = (mode, lds_cap) of moea_niching: the niching path of
+        every selection of the batch ((0, 0): automatic)."""
         runs = list(runs)
+        self._niching = (int(niching[0]), int(niching[1]))
         if not runs:
             raise ValueError("BatchNSGA3 needs at least one run")
         for r in runs:
@@ -119,6 +122,7 @@ class BatchNSGA3:
         self._keep = []
         for i, r in enumerate(self.runs):
             fill_problem(arr[i], r, dev, self._keep)
+            arr[i].niching.mode, arr[i].niching.lds_cap = self._niching
         self._arr = arr
         nbytes = lib.moea_batch_workspace_bytes(arr, count)
         if nbytes == 0:

@@ -166,12 +166,14 @@ def quantize_cv(cv: torch.Tensor, **params) -> torch.Tensor:
 
 def nsga3_select(parents: Population, offspring: Population, directions: torch.Tensor,
                  ideal: torch.Tensor, draws: Optional[dict] = None, seed: int = 0,
-                 generation: int = 0, cv_quant: Optional[dict] = None, normalization: Optional[dict] = None):
+                 generation: int = 0, cv_quant: Optional[dict] = None, normalization: Optional[dict] = None,
+                 niching: tuple = (0, 0)):
     """One NSGA-III selection.  ``ideal`` is updated in place.  Returns
     (next Population, next_fronts, stats int32[8] = max_front, n_prior, n_last,
     fallback, n_rem, ...).  With a ``normalization`` dict, its "extremes"
     (int32 (m,)) and "intercepts" (float64 (m,)) device tensors receive the
-    selection's extreme points and intercepts when it normalizes."""
+    selection's extreme points and intercepts when it normalizes.  ``niching``
+    = (mode, lds_cap) of moea_niching ((0, 0): automatic)."""
     m, n = parents.f.shape
     d = parents.x.shape[1]
     k = directions.shape[1]
@@ -191,6 +193,7 @@ def nsga3_select(parents: Population, offspring: Population, directions: torch.T
     a.next_fronts, a.directions, a.ideal = nfr.data_ptr(), directions.data_ptr(), ideal.data_ptr()
     a.draws = ctypes.pointer(dr) if dr is not None else None
     a.seed, a.generation = seed, generation
+    a.niching.mode, a.niching.lds_cap = int(niching[0]), int(niching[1])
     a.stats, a.workspace, a.workspace_bytes = stats.data_ptr(), ws.data_ptr(), ws.numel()
     a.stream = _stream(dev)
     if cv_quant:

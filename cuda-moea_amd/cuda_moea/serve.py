@@ -25,13 +25,14 @@ from .batch import check_run, fill_problem, view_result
 
 class Server:
     def __init__(self, budget_bytes: int, *, max_instances: int = 256, max_rows: int | None = None,
-                 device: int | torch.device = 0) -> None:
+                 device: int | torch.device = 0, niching: tuple[int, int] = (0, 0)) -> None:
         if not torch.cuda.is_available():
             raise RuntimeError("cuda_moea requires a ROCm GPU")
         self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
         cfg = _lib.ServeConfig()
         cfg.max_instances = int(max_instances)
         cfg.max_rows = int(max_rows if max_rows is not None else 2 * 16384 * max_instances)
+        cfg.niching.mode, cfg.niching.lds_cap = int(niching[0]), int(niching[1])
         reserved = lib.moea_serve_reserved_bytes(ctypes.byref(cfg))
         if reserved == 0:
             raise ValueError("invalid server configuration")

@@ -160,6 +160,24 @@ typedef struct moea_cv_quantization {
  * the reference rejects (validate_cv_quant_params). */
 int moea_quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization* q, void* stream);
 
+/* Niching path of an NSGA-III selection.  Every path selects the same
+ * survivors; they differ in speed and workspace use.  All zero: automatic.
+ * The reference has one path (niching.cu:250-400); this choice has no
+ * counterpart there. */
+typedef struct moea_niching {
+    int mode;      /* 0 automatic, 1 one workgroup over only the last front's
+                      members, grouped by direction (a niche of more than 128
+                      members falls back to that workgroup's radix sorts),
+                      2 three device-wide sorts of all 2n slots, 3 the one
+                      workgroup's radix sorts only.  The batched and served
+                      engines take one workgroup per instance in mode 1, and
+                      in mode 0 while every instance has 2n <= 8192. */
+    int lds_cap;   /* one-workgroup paths: last-front members kept in LDS at
+                      most (0: as many as the path's LDS holds, about 9500 for
+                      the grouping at K = 32640, 5888 for the radix sorts);
+                      the others go to the workspace */
+} moea_niching;
+
 typedef struct moea_nsga3_select_args {
     int n, m, d, k;
     moea_population parents;    /* input */
@@ -192,6 +210,7 @@ typedef struct moea_nsga3_select_args {
     double* out_intercepts;     /* (m,) hyperplane intercepts a (FP64 solve, or
                                    the row-max fallback when stats[3] = 1)
                                    before their float rounding and sanitizing */
+    moea_niching niching;       /* niching path; all zero: automatic */
 } moea_nsga3_select_args;
 
 size_t moea_nsga3_workspace_bytes(int n, int m, int k);
@@ -204,18 +223,7 @@ size_t moea_nsga3_workspace_bytes(int n, int m, int k);
  * and the survivor gather in ascending merged-index order. */
 int moea_nsga3_select(const moea_nsga3_select_args* args);
 
-/* Niching path of moea_nsga3_select, process-wide (set it while no selection
- * runs): mode 0 automatic, 1 one workgroup over only the last front's
- * members, grouped by direction (a niche of more than 128 members falls back
- * to radix sorts), 2 three device-wide sorts of all 2n slots, 3 the one
- * workgroup's radix sorts only.  The one-workgroup paths keep the members in
- * LDS when there are at most lds_cap of them and they fit (about 9500 for the
- * grouping at K = 32640, 5888 for the radix sorts), else in the workspace;
- * mode 0 with a large lds_cap is the default.  The batched and served
- * engines take one workgroup per instance in mode 1, and in mode 0 while
- * every instance has 2n <= 8192 (device-wide sorts otherwise).  Every path
- * selects the same survivors. */
-int moea_set_niching(int mode, int lds_cap);
+
 
 /* Perpendicular-distance association of n normalized individuals: for
  * individual i, x = fn[:, i] * a (a: (m,) intercepts) and ref[i] = the
@@ -384,6 +392,15 @@ int moea_rvea_adapt(const float* v0, const float* f, int ldf, const int32_t* n_a
 typedef int (*moea_evaluate_fn)(void* user, const float* x, int n, int d, int m, float* f,
                                 float* cv, int generation, int max_generations, void* stream);
 
+/* Optional companion of a callback problem: IProblemEvaluator::prepare_parent,
+ * which EvolutionaryAlgorithm::step calls before reproduction
+ * (algorithm.cu:230-241; the Python form in adapters.cpp:175-186).  It may
+ * rewrite the parents' f and cv, enqueued on `stream`.  Returns 1 when the
+ * parents changed (the selector then prepares again: NSGA-III re-sorts the
+ * parent fronts), 0 when not, or a negative MOEA_E_* code. */
+typedef int (*moea_prepare_parent_fn)(void* user, moea_population* parents, int n, int d, int m,
+                                      int generation, int max_generations, void* stream);
+
 typedef struct moea_problem_desc {
     int kind;                 /* MOEA_PROBLEM_* */
     const float* centers;     /* quadratic: (m, d) */
@@ -399,6 +416,7 @@ typedef struct moea_problem_desc {
      * NSGA-III parent fronts re-sorted (dtlz_kernels.cu:1295-1394).  <= 0:
      * always on. */
     float cv_activation_ratio;
+    moea_prepare_parent_fn prepare_parent;   /* callback problems: optional (NULL: none) */
 } moea_problem_desc;
 
 /* Strategy hooks: the C form of the reference's strategy plugins
@@ -487,6 +505,7 @@ typedef struct moea_run_args {
     void* stream;
     const moea_strategy_hooks* hooks;  /* NULL: every strategy built in */
     moea_cv_quantization cv_quant;     /* NSGA-III; bins = 0: off */
+    moea_niching niching;              /* NSGA-III niching path; all zero: automatic */
 } moea_run_args;
 
 size_t moea_run_workspace_bytes(int algorithm, int n, int m, int k);
@@ -522,6 +541,8 @@ typedef struct moea_batch_problem {
     const float* directions;   /* (m, k) device, unit length */
     float sbx_eta[2], sbx_prob, sbx_copy_prob;
     float pm_eta[2], pm_prob;
+    moea_niching niching;      /* batch: the same in every problem of the batch;
+                                  serve: ignored (moea_serve_config's applies) */
 } moea_batch_problem;
 
 typedef struct moea_batch moea_batch;   /* host handle */
@@ -565,6 +586,7 @@ void moea_batch_destroy(moea_batch* batch);
 typedef struct moea_serve_config {
     int max_instances;         /* slots */
     long long max_rows;        /* bound on the sum of 2n over live runs (sort scratch) */
+    moea_niching niching;      /* niching path of every served run; all zero: automatic */
 } moea_serve_config;
 
 typedef struct moea_serve moea_serve;   /* host handle */

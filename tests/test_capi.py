@@ -51,7 +51,7 @@ STRUCTS = {"Draws": "moea_draws", "PopulationC": "moea_population", "NSGA3Select
            "ReproduceArgs": "moea_reproduce_args", "RVEASelectArgs": "moea_rvea_select_args",
            "ProblemDesc": "moea_problem_desc", "RunArgs": "moea_run_args", "BatchProblem": "moea_batch_problem",
            "ServeConfig": "moea_serve_config", "StrategyHooks": "moea_strategy_hooks",
-           "CvQuantization": "moea_cv_quantization"}
+           "CvQuantization": "moea_cv_quantization", "Niching": "moea_niching"}
 
 
 def test_ctypes_structs_match_header_layout(tmp_path):

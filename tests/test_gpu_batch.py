@@ -67,16 +67,11 @@ def test_batch_niching_paths_equal_single_runs(mode):
     """The batch's niching paths (one workgroup per instance, chosen while every
     2n <= 8192; device-wide sorts otherwise or in mode 2), and its pruned
     association for instances with K >= 2048, against single runs."""
-    from cuda_moea import _lib
     gens = 6
     # 2n = 8400: device-wide niching in mode 0; K >= 2048: pruned association
     cases = MIX[:3] + [(4200, 3, 20, 0, 11), (2600, 5, 15, 0, 12)]
     print(f"batch niching mode {mode}", flush=True)
-    try:
-        assert _lib.lib.moea_set_niching(mode, 1 << 30) == 0
-        results = cm.BatchNSGA3([_run(*c, gens) for c in cases]).run()
-    finally:
-        assert _lib.lib.moea_set_niching(0, 1 << 30) == 0
+    results = cm.BatchNSGA3([_run(*c, gens) for c in cases], niching=(mode, 0)).run()
     for c, res in zip(cases, results):
         print(f"  instance {c}", flush=True)
         _same(res, _run(*c, gens).run())

@@ -265,6 +265,10 @@ def test_constraint_activation_schedule(kind):
     assert not cv_now().any()                                         # generations 0-3: off
     alg.step()                                                        # generation 4: on
     np.testing.assert_array_equal(cv_now(), true_cv())
+    if kind == "RVEA":
+        # rows past the live count keep their placeholders through the switch
+        pop, live = alg.population, rows()
+        assert bool((pop.objectives[live:] == 1e15).all()) and bool((pop.constraints[live:] == 1e15).all())
     alg.run()
     np.testing.assert_array_equal(cv_now(), true_cv())
     # the schedule restarts with reset()
@@ -335,3 +339,13 @@ def test_activation_switch_prepares_a_python_selector():
     alg.run()
     assert [g for g, _ in sel.prepared] == [0, 4]
     assert sel.prepared[0][1] == 0.0 and sel.prepared[1][1] > 0.0    # violations off, then on
+    # ratio 0: the constraints count from the start, and generation 0 is the
+    # switch node (is_cv_activation_switch_node(0) = is_cv_active(0)): the
+    # selector prepares at initialization and again at generation 0
+    sel0 = Keep()
+    alg = cm.NSGA3(population_size=92, max_generations=3,
+                   problem=cm.C1DTLZ1(dimension=12, objectives=3, constraint_activation_ratio=0.0),
+                   mating=cm.RandomMating(), environment_selector=sel0, seed=4, print_progress=False)
+    alg.run()
+    assert [g for g, _ in sel0.prepared] == [0, 0]
+    assert sel0.prepared[0][1] > 0.0 and sel0.prepared[0][1] == sel0.prepared[1][1]

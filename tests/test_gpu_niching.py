@@ -11,18 +11,13 @@ if not torch.cuda.is_available():
     pytest.skip("needs a ROCm GPU", allow_module_level=True)
 
 import oracle as O  # noqa: E402
-from cuda_moea import _lib  # noqa: E402
 from cuda_moea import kernels as K  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
-PATHS = [(1, 1 << 30), (1, 0), (3, 1 << 30), (3, 0), (2, 0)]     # (mode, LDS capacity)
-
-
-@pytest.fixture(autouse=True)
-def _restore_niching():
-    yield
-    assert _lib.lib.moea_set_niching(0, 1 << 30) == 0
+# (mode, LDS capacity): each path in LDS, the one-workgroup paths also with a
+# capacity of one member (the rest in the workspace), the device-wide sorts
+PATHS = [(1, 0), (1, 1), (3, 0), (3, 1), (2, 0)]
 
 
 def _f(a):
@@ -62,10 +57,10 @@ def test_niching_paths_equal_oracle(n, m, kind, seed):
     chosen, fr_mix, st, _ = O.nsga3_select(fm, cvm, w, ideal0, ua, ub)
     print(f"niching n={n} m={m} kind={kind}: n_prior={st[1]} n_last={st[2]}", flush=True)
     for mode, cap in PATHS:
-        assert _lib.lib.moea_set_niching(mode, cap) == 0
         par = K.Population(_f(xm[:n]), _f(fm[:, :n]), _f(cvm[:n]))
         ofs = K.Population(_f(xm[n:]), _f(fm[:, n:]), _f(cvm[n:]))
-        nxt, nfr, stg = K.nsga3_select(par, ofs, _f(w), _f(ideal0), draws={"niche_a": ua, "niche_b": ub})
+        nxt, nfr, stg = K.nsga3_select(par, ofs, _f(w), _f(ideal0), draws={"niche_a": ua, "niche_b": ub},
+                                       niching=(mode, cap))
         stg = stg.cpu().numpy()
         assert stg[1] == st[1] and stg[2] == st[2], (mode, cap, stg, st)
         np.testing.assert_array_equal(nxt.x[:, 0].cpu().numpy().astype(np.int64), chosen, err_msg=f"{mode} {cap}")
@@ -84,13 +79,21 @@ def test_niching_generated_draws_paths_agree():
     ua, ub = O.draw_niche(seed, gen, 2 * n)
     chosen, _, _, _ = O.nsga3_select(fm, cvm, w, ideal0, ua, ub)
     for mode, cap in PATHS:
-        assert _lib.lib.moea_set_niching(mode, cap) == 0
         par = K.Population(_f(xm[:n]), _f(fm[:, :n]), _f(cvm[:n]))
         ofs = K.Population(_f(xm[n:]), _f(fm[:, n:]), _f(cvm[n:]))
-        nxt, _, _ = K.nsga3_select(par, ofs, _f(w), _f(ideal0), seed=seed, generation=gen)
+        nxt, _, _ = K.nsga3_select(par, ofs, _f(w), _f(ideal0), seed=seed, generation=gen, niching=(mode, cap))
         np.testing.assert_array_equal(nxt.x[:, 0].cpu().numpy().astype(np.int64), chosen, err_msg=f"{mode} {cap}")
 
 
-def test_set_niching_rejects_bad_arguments():
-    assert _lib.lib.moea_set_niching(4, 100) != 0
-    assert _lib.lib.moea_set_niching(1, -1) != 0
+def test_niching_rejects_bad_arguments():
+    n, m = 46, 3
+    rng = np.random.default_rng(0)
+    fm = _objectives(rng, "sphere", m, 2 * n)
+    z = np.zeros(2 * n, np.float32)
+    x = np.zeros((2 * n, 2), np.float32)
+    par = K.Population(_f(x[:n]), _f(fm[:, :n]), _f(z[:n]))
+    ofs = K.Population(_f(x[n:]), _f(fm[:, n:]), _f(z[n:]))
+    w = _f(O.das_dennis(n, m))
+    for bad in ((4, 0), (1, -1), (-1, 0)):
+        with pytest.raises(ValueError, match="niching"):
+            K.nsga3_select(par, ofs, w, _f(fm[:, :n].min(axis=1)), niching=bad)

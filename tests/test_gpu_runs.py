@@ -93,3 +93,46 @@ def test_invalid_configurations():
     with pytest.raises(ValueError):
         cm.NSGA3(population_size=20, problem=cm.QuadraticCenters(4, 3), print_progress=False,
                  initial_population=np.full((20, 4), 2.0, np.float32)).initialize()
+
+
+def test_python_problem_prepare_parent_and_reset():
+    """IProblemEvaluator::prepare_parent runs before every reproduction
+    (algorithm.cu:230-241): an evaluation it returns replaces the parents'
+    objectives and the NSGA-III parent fronts are re-sorted; True reports an
+    in-place change; reset() reaches the problem (algorithm.cu:413)."""
+    class Dynamic(cm.PythonProblem):
+        def __init__(self, in_place):
+            super().__init__(dimension=6, objectives=2)
+            self.calls, self.resets, self.in_place = [], 0, in_place
+
+        def evaluate(self, x, context):
+            return torch.stack([(x ** 2).sum(1), ((x - 1) ** 2).sum(1)], 1)
+
+        def prepare_parent(self, population, context):
+            self.calls.append(context["generation"])
+            f = self.evaluate(population["variables"], context)
+            f[0] = -1e6                     # parent 0 now dominates every other row
+            if self.in_place:
+                population["objectives"].copy_(f)
+                return True
+            return {"objectives": f, "constraints": torch.zeros_like(f[:, 0])}
+
+        def reset(self):
+            self.resets += 1
+
+    for in_place in (False, True):
+        prob = Dynamic(in_place)
+        alg = cm.NSGA3(population_size=40, max_generations=5, problem=prob, seed=2, print_progress=False)
+        print(f"prepare_parent in_place={in_place}", flush=True)
+        for g in range(5):
+            alg.step()
+            # merged row 0 (parent 0) is the only first-front row: it survives
+            # as row 0 of the next population with its rewritten objectives, and
+            # the re-sorted parent fronts ranked it alone in front 0
+            f = alg.population.objectives.cpu().numpy()
+            assert np.all(f[0] == -1e6) and np.all(f[1:] > -1e6)
+        assert prob.calls == [0, 1, 2, 3, 4]
+        res = alg.result()
+        assert int((res.auxiliary_indices == 0).sum()) == 1
+        alg.reset()
+        assert prob.resets == 1

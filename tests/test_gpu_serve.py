@@ -120,3 +120,65 @@ def test_serve_budget_admission_and_release():
         srv.admit(_run("quad", 91, 3, 12, 12, 0, 5))
     with pytest.raises(Exception):
         srv.release(sa + 1 if sa + 1 != sb else sa + 2)
+
+
+def test_served_quadratic_runs_match_oracle():
+    """Every quadratic-center run served in a staggered mix (slots reused,
+    MLP runs beside them) ends exactly where the oracle's restatement of the
+    reference NSGA-III loop ends: variables, objectives and fronts."""
+    import oracle as O
+    print("serve staggered vs oracle", flush=True)
+    arrivals = [0, 0, 1, 2, 2, 5, 6, 9]
+    out = _serve(list(zip(arrivals, CASES)), max_instances=3)
+    for i, (kind, n, m, d, parts, seed, gens) in enumerate(CASES):
+        if kind != "quad":
+            continue
+        print(f"  oracle case n={n} m={m} d={d} parts={parts} seed={seed} gens={gens}", flush=True)
+        prob = cm.QuadraticCenters(d, m, seed=seed + 100)
+        bounds = np.tile(np.array([0.0, 1.0], np.float32), d)
+        x, f, cv, fr, _ = O.nsga3_run(n, m, d, gens, seed, bounds, prob.centers.numpy(), partitions=parts)
+        np.testing.assert_array_equal(out[i][0], x)
+        np.testing.assert_array_equal(out[i][1], f.T)
+        np.testing.assert_array_equal(out[i][2], cv)
+        np.testing.assert_array_equal(out[i][3], fr)
+
+
+def test_serve_full_size_instance_alone_equals_mixed():
+    """The largest served shape (N = 8192, M = 10, D = 4096, seeded MLP
+    D -> 256 -> 10, directions sized from N) served beside small runs that
+    arrive and leave around it equals the same run served alone."""
+    big = ("mlp_big", 8192, 10, 4096, 0, 9, 3)
+
+    def make(case):
+        if case[0] == "mlp_big":
+            _, n, m, d, parts, seed, gens = case
+            prob = cm.SeededMLP(d, m, hidden=256, seed=seed + 200, device=torch.device("cuda", 0))
+            return cm.NSGA3(population_size=n, max_generations=gens, problem=prob, seed=seed,
+                            print_progress=False)
+        return _run(*case)
+
+    def serve(schedule):
+        srv = cm.Server(8 << 30, max_instances=6, max_rows=2 * 8192 + 2 * 256 * 5)
+        pending = sorted(range(len(schedule)), key=lambda i: schedule[i][0])
+        owner, out, step = {}, {}, 0
+        while len(out) < len(schedule):
+            while pending and schedule[pending[0]][0] <= step:
+                slot = srv.admit(make(schedule[pending[0]][1]))
+                assert slot is not None
+                owner[slot] = pending.pop(0)
+            for slot in srv.step():
+                i = owner.pop(slot)
+                out[i] = _arrays(srv.result(slot))
+                srv.release(slot)
+            step += 1
+            assert step < 100
+        return out
+
+    print("serve full-size instance alone", flush=True)
+    alone = serve([(0, big)])[0]
+    print("serve full-size instance mixed", flush=True)
+    mixed = serve([(1, big), (0, CASES[0]), (0, CASES[1]), (2, CASES[2]), (3, CASES[3])])[0]
+    for u, v in zip(alone, mixed):
+        np.testing.assert_array_equal(u, v)
+    f = alone[1]
+    assert f.shape == (8192, 10) and np.all(np.isfinite(f))
