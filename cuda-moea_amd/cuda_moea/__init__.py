@@ -7,6 +7,7 @@ from .algorithm import Algorithm, AlgorithmBuilder, NSGA3, Population, RVEA, Res
 from .batch import BatchNSGA3
 from .serve import Server
 from .record import SnapshotRecorder, load_snapshot
+from .native import NativeProblem
 from . import problem as _problem
 from .problem import *  # noqa: F401,F403
 from ._specs import (
@@ -20,7 +21,7 @@ from ._specs import (
 __version__ = "0.2.0"
 
 __all__ = [
-    "BatchNSGA3", "Server", "SnapshotRecorder", "load_snapshot",
+    "BatchNSGA3", "Server", "SnapshotRecorder", "load_snapshot", "NativeProblem",
     "Algorithm", "AlgorithmBuilder", "NSGA3", "RVEA", "Population", "Result",
     "CudaConfig", "SBX", "PolynomialMutation", "NoMutation",
     "RandomMating", "TournamentMating", "DasDennisDirections",

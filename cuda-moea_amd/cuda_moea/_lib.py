@@ -150,6 +150,16 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("max_instances", c_int), ("max_rows", ctypes.c_longlong), ("niching", Niching)]
 
 
+PARAM_INT, PARAM_FLOAT, PARAM_BOOL, PARAM_STR, PARAM_FLOAT_LIST = 0, 1, 2, 3, 4
+PARAM_TYPES = ("int", "float", "bool", "str", "float_list")
+
+
+class ParamValue(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("type", c_int), ("list_length", c_int),
+                ("int_value", ctypes.c_longlong), ("float_value", ctypes.c_double),
+                ("str_value", ctypes.c_char_p), ("list", POINTER(ctypes.c_double))]
+
+
 # name -> (restype, argtypes); every symbol include/moea_hip.h declares
 SIGNATURES = {
     "moea_abi_version": (c_int, []),
@@ -205,6 +215,22 @@ SIGNATURES = {
     "moea_serve_release": (c_int, [c_void_p, c_int]),
     "moea_serve_check": (c_int, [c_void_p]),
     "moea_serve_destroy": (None, [c_void_p]),
+    "moea_native_build_id": (ctypes.c_char_p, []),
+    "moea_plugin_open": (c_int, [ctypes.c_char_p, POINTER(c_void_p)]),
+    "moea_plugin_close": (None, [c_void_p]),
+    "moea_plugin_name": (ctypes.c_char_p, [c_void_p]),
+    "moea_plugin_build_id": (ctypes.c_char_p, [c_void_p]),
+    "moea_plugin_param_count": (c_int, [c_void_p]),
+    "moea_plugin_param": (c_int, [c_void_p, c_int, POINTER(ParamValue), POINTER(ctypes.c_char_p)]),
+    "moea_plugin_create": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, POINTER(ParamValue), c_int,
+                                   POINTER(c_void_p)]),
+    "moea_plugin_problem_constraints": (c_int, [c_void_p]),
+    "moea_plugin_problem_initialize": (c_int, [c_void_p, c_void_p, c_uint64, c_void_p]),
+    "moea_plugin_problem_destroy": (None, [c_void_p]),
+    "moea_plugin_evaluate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
+                                     c_void_p]),
+    "moea_plugin_prepare_parent": (c_int, [c_void_p, _POP_P, c_int, c_int, c_int, c_int, c_int, c_void_p]),
+    "moea_plugin_reset": (c_int, [c_void_p]),
 }
 
 

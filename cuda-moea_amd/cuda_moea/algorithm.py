@@ -26,6 +26,7 @@ from ._specs import (
     TournamentMating, UserDefinedDirections,
 )
 from . import kernels as K
+from .native import EVALUATE as _NATIVE_EVALUATE, PREPARE_PARENT as _NATIVE_PREPARE_PARENT, NativeProblem
 from .problem import DTLZ2, DTLZProblem, QuadraticCenters, SeededMLP
 from .record import SnapshotRecorder
 
@@ -110,6 +111,9 @@ class Algorithm:
         self.progress_interval = progress_interval
         self.print_progress = print_progress
         self._validate_strategies()
+        # a native problem is created now, as the reference's algorithm
+        # constructor does, so that its factory's checks raise here
+        self._native = self.problem._instantiate() if isinstance(self.problem, NativeProblem) else None
         self._initialized = False
         self._total_ms = 0.0
         self._resets = 0
@@ -152,7 +156,7 @@ class Algorithm:
                 raise _unsupported(sel, "RVEA environment selector")
             if not sel.alpha > 0.0:
                 raise ValueError("RVEA alpha must be positive")
-        if not isinstance(self.problem, (QuadraticCenters, SeededMLP, PythonProblem)):
+        if not isinstance(self.problem, (QuadraticCenters, SeededMLP, PythonProblem, NativeProblem)):
             raise _unsupported(self.problem, "problem")
 
     # ------------------------------------------------------------ device ----
@@ -248,6 +252,8 @@ class Algorithm:
             a.initial_x_given = 1
         a.workspace, a.workspace_bytes = self._ws.data_ptr(), self._ws.numel()
         a.stream = torch.cuda.current_stream(dev).cuda_stream
+        if self._native is not None:
+            self._native.initialize(a.stream, a.seed, self._bounds.data_ptr())
         self._args = a
         self._call(lib.moea_run_init)
         if self._python_selector:
@@ -309,6 +315,11 @@ class Algorithm:
             self._hwork = torch.empty(self.n, p.hidden, dtype=torch.float32, device=self.device)
             pd.kind, pd.w1, pd.w2 = _lib.PROBLEM_MLP, self._w1.data_ptr(), self._w2.data_ptr()
             pd.hidden, pd.hwork = p.hidden, self._hwork.data_ptr()
+        elif isinstance(p, NativeProblem):
+            # the plugin's evaluate and prepare_parent are called by the run
+            # loop through the loader's C entry points
+            pd.kind, pd.callback, pd.user = _lib.PROBLEM_CALLBACK, _NATIVE_EVALUATE, self._native.handle
+            pd.prepare_parent = _NATIVE_PREPARE_PARENT
         elif isinstance(p, DTLZProblem) and p._kind in _lib.DTLZ_KINDS and 2 <= self.m <= self.d:
             # built-in objective kernels, as the reference's DTLZ problems are native
             pd.kind = _lib.DTLZ_KINDS[p._kind]
@@ -678,7 +689,9 @@ class Algorithm:
         rd = self.reference_directions
         if isinstance(rd, PythonReferenceDirections) and hasattr(rd, "reset"):
             rd.reset()                           # adapters.cpp:319-324
-        if self._args.problem.kind == _lib.PROBLEM_CALLBACK and hasattr(self.problem, "reset"):
+        if self._native is not None:
+            self._native.reset()                 # algorithm.cu:413, native_problem.cpp:142
+        elif self._args.problem.kind == _lib.PROBLEM_CALLBACK and hasattr(self.problem, "reset"):
             self.problem.reset()                 # algorithm.cu:413, adapters.cpp:187-190
         self._args.seed = self._run_seed()
         self._args.current = 0

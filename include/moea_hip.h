@@ -619,6 +619,88 @@ int moea_serve_check(const moea_serve* server);
 void moea_serve_destroy(moea_serve* server);
 
 /* ------------------------------------------------------------------------ */
+/* Native problem plugins                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* A plugin is a shared library built with hipcc against
+ * include/cuda_moea/native/plugin.hpp: it exports
+ * `const cuda_moea::native::Plugin* cuda_moea_problem_v1()` returning
+ * {abi = 1, build_id, name, parameter schema, create, destroy}
+ * (python/csrc/native/include/cuda_moea/native/plugin.cuh:35-58).  These
+ * functions are the reference's loader (python/csrc/native_problem.cpp:13-170)
+ * behind the C ABI.  A created problem plugs into a run as a callback
+ * problem: moea_problem_desc{kind = MOEA_PROBLEM_CALLBACK,
+ * callback = moea_plugin_evaluate, prepare_parent = moea_plugin_prepare_parent,
+ * user = the moea_plugin_problem*}. */
+typedef struct moea_plugin moea_plugin;
+typedef struct moea_plugin_problem moea_plugin_problem;
+
+/* Parameter types, numbered as the schema's variant alternatives. */
+#define MOEA_PARAM_INT 0
+#define MOEA_PARAM_FLOAT 1
+#define MOEA_PARAM_BOOL 2
+#define MOEA_PARAM_STR 3
+#define MOEA_PARAM_FLOAT_LIST 4
+
+typedef struct moea_param_value {
+    const char* name;
+    int type;                 /* MOEA_PARAM_* */
+    int list_length;          /* float_list */
+    long long int_value;      /* int; bool as 0 / 1 */
+    double float_value;       /* float */
+    const char* str_value;    /* str */
+    const double* list;       /* float_list */
+} moea_param_value;
+
+/* The SDK build id this library accepts (CUDA_MOEA_NATIVE_BUILD_ID). */
+const char* moea_native_build_id(void);
+
+/* dlopens `path` and checks its registration, in the reference's order
+ * (native_problem.cpp:16-34): the entry symbol ("Missing
+ * cuda_moea_problem_v1 registration"), abi == 1 and an exact build_id ("SDK
+ * mismatch"; nothing past those two fields is read before), a complete record,
+ * unique non-null parameter names.  MOEA_E_INVALID with the message on
+ * failure (the library is closed again). */
+int moea_plugin_open(const char* path, moea_plugin** out);
+/* Releases the caller's handle; the library stays loaded while problems
+ * created from it live. */
+void moea_plugin_close(moea_plugin* plugin);
+const char* moea_plugin_name(const moea_plugin* plugin);
+const char* moea_plugin_build_id(const moea_plugin* plugin);
+/* The parameter schema (native_problem.cpp:155-170): count, then per index
+ * the name, type and default (pointers valid while the plugin is open) and
+ * the description. */
+int moea_plugin_param_count(const moea_plugin* plugin);
+int moea_plugin_param(const moea_plugin* plugin, int index, moea_param_value* out, const char** description);
+
+/* Builds a problem (native_problem.cpp:75-109): every schema parameter takes
+ * the given value or its default; unknown names are rejected; values are
+ * checked against the default's type (an int is accepted for a float; floats
+ * finite; bool and int strict); the factory's exceptions become
+ * MOEA_E_INVALID "Native problem construction failed: ..."; the instance's
+ * ProblemInfo must equal the requested dimension, objectives and bounds. */
+int moea_plugin_create(moea_plugin* plugin, int dimension, int objectives, const float* lower_bounds,
+                       const float* upper_bounds, const moea_param_value* params, int param_count,
+                       moea_plugin_problem** out);
+int moea_plugin_problem_constraints(const moea_plugin_problem* problem);
+/* IProblemEvaluator::initialize with a CudaContext on `stream` (the run's
+ * stream) with `seed`; `bounds` is the run's device bounds array
+ * (interleaved lb, ub), handed to evaluate in PopulationView::bounds. */
+int moea_plugin_problem_initialize(moea_plugin_problem* problem, void* stream, uint64_t seed,
+                                   const float* bounds);
+/* Waits for the problem's stream, destroys the instance, and unloads the
+ * library when no handle or problem still uses it (native_problem.cpp:110-126). */
+void moea_plugin_problem_destroy(moea_plugin_problem* problem);
+/* moea_evaluate_fn with user = the moea_plugin_problem*.  Violations of a
+ * problem with no constraints are written as zero before evaluate runs. */
+int moea_plugin_evaluate(void* user, const float* x, int n, int d, int m, float* f, float* cv, int generation,
+                         int max_generations, void* stream);
+/* moea_prepare_parent_fn with user = the moea_plugin_problem*. */
+int moea_plugin_prepare_parent(void* user, moea_population* parents, int n, int d, int m, int generation,
+                               int max_generations, void* stream);
+int moea_plugin_reset(moea_plugin_problem* problem);
+
+/* ------------------------------------------------------------------------ */
 /* Profiling                                                                 */
 /* ------------------------------------------------------------------------ */
 
