@@ -285,10 +285,17 @@ class Algorithm:
                 raise ValueError("reference directions must have shape (count, objective_count)")
             return np.ascontiguousarray(out.t().cpu().numpy())
         if isinstance(rd, UserDefinedDirections):
-            vals = np.asarray(rd.values, dtype=np.float32)
-            if vals.ndim != 2 or vals.shape[0] != m or vals.size == 0:
+            # adapters.cpp:557-562: a (count, objective_count) tensor, stored
+            # objective-major; reference_directions.cu:189-206 checks the sizes
+            vals = rd.values.detach().cpu() if isinstance(rd.values, torch.Tensor) else rd.values
+            vals = np.asarray(vals, dtype=np.float32)
+            if vals.ndim != 2:
+                raise ValueError("user directions must be a 2-D tensor")
+            if vals.size == 0:
                 raise ValueError("User directions must have shape (objective_count, count)")
-            vals = np.array(vals, dtype=np.float32, copy=True)
+            if vals.shape[1] != m:
+                raise ValueError("User direction objective count does not match the problem")
+            vals = np.array(vals.T, dtype=np.float32, copy=True)
             if rd.normalize:
                 for j in range(vals.shape[1]):
                     ss = np.float32(0.0)

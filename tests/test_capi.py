@@ -3,6 +3,8 @@ declares (no compute calls; CPU only)."""
 import ctypes
 import re
 import subprocess
+
+import pytest
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -204,3 +206,25 @@ def test_reference_api_constructions():
                         reference_directions=cm.PythonReferenceDirections(), environment_selector=Selector(),
                         enable_warmup=False).initialized
     assert cm.AlgorithmBuilder("RVEA").population_size(16).max_generations(2).problem(cm.DTLZ2()).build().generation == 0
+
+
+def test_user_directions_are_count_by_objectives():
+    """UserDefinedDirections takes a (count, objective_count) tensor and
+    stores it objective-major (adapters.cpp:557-562 of the reference), with
+    each direction normalised; a wrong objective count is refused."""
+    import numpy as np
+    import torch
+    import cuda_moea as cm
+    vals = torch.tensor([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 3.0], [1.0, 1.0, 0.0]])
+    alg = cm.NSGA3(population_size=16, max_generations=1, problem=cm.DTLZ2(8, 3),
+                   reference_directions=cm.UserDefinedDirections(vals))
+    v = alg._directions_host(16, 3)
+    assert v.shape == (3, 4)
+    np.testing.assert_allclose(v[:, :3], np.eye(3), atol=0)
+    np.testing.assert_allclose(v[:, 3], [2 ** -0.5, 2 ** -0.5, 0.0], rtol=1e-7)
+    raw = cm.NSGA3(population_size=16, max_generations=1, problem=cm.DTLZ2(8, 3),
+                   reference_directions=cm.UserDefinedDirections(vals, normalize=False))._directions_host(16, 3)
+    np.testing.assert_array_equal(raw, vals.numpy().T)
+    with pytest.raises(ValueError, match="objective count"):
+        cm.NSGA3(population_size=16, max_generations=1, problem=cm.DTLZ2(8, 3),
+                 reference_directions=cm.UserDefinedDirections(vals.T.contiguous()))._directions_host(16, 3)
