@@ -105,8 +105,11 @@ def _tolerance(ref):
 # penalty weight M (t/T)^2 grows while each adaptation of the directions to the
 # population's objective ranges re-assigns the solutions).  The same drift
 # shows in a CPU restatement of the reference's RVEA generation and in an
-# independent textbook RVEA loop; the mechanism that keeps the reference's own
-# runs on the front has not been identified.
+# independent textbook RVEA loop.  The published runs were made with release
+# 0.1.0 of the reference (their framework_version column), the restated code
+# is release 0.2.0 (python/cuda_moea/__init__.py:15), and the published IGD
+# behaves like a far weaker late penalty (a fixed M * 0.04 reproduces DTLZ2's
+# 0.01699 on the CPU).
 RVEA_MISSES = {"DTLZ2", "DTLZ3", "DTLZ4", "DTLZ5", "DTLZ6", "DTLZ7", "ConvexDTLZ2"}
 
 
@@ -114,7 +117,7 @@ RVEA_MISSES = {"DTLZ2", "DTLZ3", "DTLZ4", "DTLZ5", "DTLZ6", "DTLZ7", "ConvexDTLZ
 @pytest.mark.parametrize("algorithm", ["nsga3", "rvea"])
 def test_final_igd_median_matches_published(problem, algorithm, request):
     if algorithm == "rvea" and problem in RVEA_MISSES:
-        request.applymarker(pytest.mark.xfail(reason="RVEA final IGD differs from the published runs", strict=False))
+        request.applymarker(pytest.mark.xfail(reason="RVEA final IGD differs from the published release-0.1.0 runs", strict=False))
     print(f"quality {algorithm} {problem} seeds {SEEDS.start}-{SEEDS.stop - 1}", flush=True)
     igd, hv = quality_table(problem, algorithm)
     ref = reference_igd(problem, algorithm)
