@@ -102,7 +102,11 @@ def batched_config(generations: int, count: int, plain_sample: int) -> dict:
     even; M in {3, 5, 8} with Das-Dennis H = 12 / 6 / 3; D log-uniform in
     [10, 1000]; per-run seeded quadratic centers) advanced together by one
     BatchNSGA3, against the plain one-run-at-a-time GPU path (cuda_moea.NSGA3)
-    timed on the first `plain_sample` runs.  Runs/s count whole runs of
+    timed on the first `plain_sample` runs.  The batch is timed three ways on
+    the same runs: direct launches, one replayed hipGraph per generation, and
+    graphs with the peel plan autotuned on the runs' own first generations (the
+    configured path, which `batched_runs_per_s` reports; the tuning
+    generations are inside the timing).  Runs/s count whole runs of
     `generations` generations; the initial evaluation is outside the timing,
     as in Result.total_ms."""
     import math
@@ -128,15 +132,30 @@ def batched_config(generations: int, count: int, plain_sample: int) -> dict:
 
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(dev)
-    batch = cm.BatchNSGA3([make(i) for i in range(count)])
-    batch.initialize()
-    stream.synchronize()
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    batch.step(generations)
-    t1.record(stream)
-    t1.synchronize()
-    batched_s = t0.elapsed_time(t1) / 1e3
+
+    def timed_batch(graph: bool, tune: bool):
+        batch = cm.BatchNSGA3([make(i) for i in range(count)], graph=graph)
+        batch.initialize()
+        stream.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        report = batch.autotune(trials=3) if tune else None      # the tuning generations are the runs' own
+        batch.step(generations - batch.generation)
+        t1.record(stream)
+        t1.synchronize()
+        first = batch.result(0).objectives.cpu()
+        info = batch.graph_info()
+        del batch
+        torch.cuda.empty_cache()
+        return t0.elapsed_time(t1) / 1e3, report, info, first
+
+    direct_s, _, _, f_direct = timed_batch(False, False)
+    graph_s, _, graph_info, f_graph = timed_batch(True, False)
+    tuned_s, tune, _, f_tuned = timed_batch(True, True)
+    if not (torch.equal(f_direct, f_graph) and torch.equal(f_direct, f_tuned)):
+        raise RuntimeError("batched paths disagree: direct, graph and tuned results differ")
+    # the configured path: one captured graph per generation, peel plan autotuned
+    batched_s = tuned_s
     plain_s = 0.0
     for i in range(plain_sample):
         alg = make(i)
@@ -153,8 +172,14 @@ def batched_config(generations: int, count: int, plain_sample: int) -> dict:
     return {"workload": "config 3: NSGA-III, N log-uniform [64,4096], M in {3,5,8}, D log-uniform [10,1000], "
                         "quadratic-center",
             "instances": count, "generations": generations,
+            "path": "hipGraph replay per generation, peel plan autotuned on the runs' own first 24 generations",
             "batched_runs_per_s": round(batched_rps, 2), "batched_s": round(batched_s, 3),
             "ms_per_batched_generation": round(batched_s / generations * 1e3, 4),
+            "ms_per_batched_generation_direct": round(direct_s / generations * 1e3, 4),
+            "ms_per_batched_generation_graph": round(graph_s / generations * 1e3, 4),
+            "graph_kernel_nodes": graph_info["kernel_nodes"], "graph_memset_nodes": graph_info["memset_nodes"],
+            "autotune": {"chosen_plan": list(tune["plan"]),
+                         "ms_per_generation": [round(x, 4) for x in tune["ms_per_generation"]]},
             "plain_runs_per_s": round(plain_rps, 3), "plain_sample_runs": plain_sample,
             "speedup_vs_plain": round(batched_rps / plain_rps, 2),
             "total_individuals": sum(s[0] for s in specs)}

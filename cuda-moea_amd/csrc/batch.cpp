@@ -209,7 +209,8 @@ void fix_grids(Grids& G, const int* base)
 // Builds the grids for the instances in `state` and assigns their sort-key
 // offsets (contiguous in slot order).  Returns the rows sorted (sum of 2n),
 // or a negative status when the peel cannot be planned.
-long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, TableBuilder& T, Grids& G)
+long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, int peel_rows, TableBuilder& T,
+                      Grids& G)
 {
     const int count = static_cast<int>(host.size());
     std::vector<long long> init256(count, 0), rows256(count, 0), sbx256(count, 0), merged256(count, 0),
@@ -295,7 +296,7 @@ long long build_grids(std::vector<BInst>& host, int state, int peel_capacity, Ta
         BNdsGrids& N = mode ? G.nds1 : G.nds0;
         std::vector<int> first, jobs;
         size_t lds = 0;
-        const int st = batch_peel_plan(peel_inst, mode ? peel_W1 : peel_W0, peel_capacity, first, jobs, lds);
+        const int st = batch_peel_plan(peel_inst, mode ? peel_W1 : peel_W0, peel_capacity, peel_rows, first, jobs, lds);
         if (st != kOk) return st;
         N.peel_wgs = static_cast<int>(first.size()) - 1;
         N.peel_lds = lds;
@@ -328,6 +329,27 @@ struct BatchCore {
     int peel_capacity = 0;                    // co-resident blocks of the batched peel
     bool error_ready = false;                 // the device error word has been cleared
     moea_niching niching{};                   // niching path of every selection
+    // front-peeling plan: workgroups used = co-resident capacity / peel_cap_div,
+    // one workgroup per peel_rows row blocks (chosen by moea_batch_autotune)
+    int peel_cap_div = 1, peel_rows = 16;
+    // generation graph: one generation's launches captured from the stream and
+    // replayed while the set of running instances is unchanged
+    bool graph_on = false, graph_valid = false;
+    hipGraph_t graph{};
+    hipGraphExec_t graph_exec{};
+    hipStream_t capture_stream{};             // own stream to capture on when `stream` is the null stream
+    int graph_nodes = 0, graph_kernels = 0, graph_memsets = 0, graph_captures = 0;
+
+    BatchCore() = default;
+    BatchCore(const BatchCore&) = delete;
+    BatchCore& operator=(const BatchCore&) = delete;
+    ~BatchCore()
+    {
+        if (graph_exec || graph) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+    }
 };
 
 namespace {
@@ -389,10 +411,12 @@ int core_upload(BatchCore& C)
 {
     if (C.peel_capacity == 0) C.peel_capacity = batch_peel_capacity();
     if (C.peel_capacity <= 0) return fail(kUnsupported, "moea_batch: the peel kernel cannot be resident");
+    C.graph_valid = false;                    // the grids and tables change: recapture
+    const int cap = std::max(1, C.peel_capacity / std::max(1, C.peel_cap_div));
     TableBuilder T;
-    const long long run_rows = build_grids(C.host, kStateRunning, C.peel_capacity, T, C.run);
+    const long long run_rows = build_grids(C.host, kStateRunning, cap, C.peel_rows, T, C.run);
     if (run_rows < 0) return static_cast<int>(run_rows);
-    const long long init_rows = build_grids(C.host, kStateAdmitted, C.peel_capacity, T, C.init);
+    const long long init_rows = build_grids(C.host, kStateAdmitted, cap, C.peel_rows, T, C.init);
     if (init_rows < 0) return static_cast<int>(init_rows);
     if (run_rows > C.max_rows || init_rows > C.max_rows)
         return fail(kWorkspaceTooSmall, "moea_batch: more rows than the sort scratch holds");
@@ -435,21 +459,104 @@ int core_init_admitted(BatchCore& C)
     return kOk;
 }
 
+// The launches of one generation of every running instance.  They take
+// every per-generation quantity (generation, buffer rotation, annealed
+// operator parameters, draw keys) from the device instance table, which the
+// last launch advances, so the same sequence serves every generation until
+// the set of running instances changes.
+int core_step_launch(BatchCore& C, hipStream_t st)
+{
+    BatchLaunch L{C.dev_insts, C.slots, st, C.niching};
+    stage_begin(kStReproduce, st);
+    MOEA_CHECK(batch_reproduce(L, C.run.rows256.map, C.run.rows256.blocks, C.run.sbx256.map, C.run.sbx256.blocks));
+    stage_end(kStReproduce, st);
+    stage_begin(kStEvaluate, st);
+    MOEA_CHECK(batch_evaluate(L, C.run.eval, 1));
+    stage_end(kStEvaluate, st);
+    MOEA_CHECK(batch_select(L, C.run.sel, C.run.nds1, C.scratch));
+    MOEA_CHECK(batch_advance(L, kStateRunning, kStateRunning, 1));
+    return kOk;
+}
+
+void graph_release(BatchCore& C)
+{
+    if (C.graph_exec) (void)hipGraphExecDestroy(C.graph_exec);
+    if (C.graph) (void)hipGraphDestroy(C.graph);
+    C.graph_exec = nullptr;
+    C.graph = nullptr;
+    C.graph_valid = false;
+}
+
+// Captures core_step_launch into a graph.  Only kernel nodes are accepted: a
+// memset, copy or host node would need its own replay semantics.  The null
+// stream cannot be captured, so a batch launched on it captures on a stream
+// of its own (capture runs nothing) and replays on its stream.
+int core_capture(BatchCore& C)
+{
+    MOEA_HIP(hipStreamSynchronize(C.stream));   // the old executable may be in flight
+    graph_release(C);
+    hipStream_t cs = C.stream;
+    if (!cs) {
+        if (!C.capture_stream) MOEA_HIP(hipStreamCreateWithFlags(&C.capture_stream, hipStreamNonBlocking));
+        cs = C.capture_stream;
+    }
+    MOEA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int st = core_step_launch(C, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    if (st != kOk) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ec != hipSuccess || !g) return hip_fail(ec != hipSuccess ? ec : hipErrorUnknown, "hipStreamEndCapture");
+    size_t count = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &count);
+    std::vector<hipGraphNode_t> nodes(count);
+    if (e == hipSuccess && count) e = hipGraphGetNodes(g, nodes.data(), &count);
+    int kernels = 0, memsets = 0, other = 0;
+    for (size_t i = 0; e == hipSuccess && i < count; ++i) {
+        hipGraphNodeType t{};
+        e = hipGraphNodeGetType(nodes[i], &t);
+        if (t == hipGraphNodeTypeKernel) ++kernels;
+        else if (t == hipGraphNodeTypeMemset) ++memsets;
+        else ++other;
+    }
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphGetNodes");
+    }
+    C.graph_nodes = static_cast<int>(count);
+    C.graph_kernels = kernels;
+    C.graph_memsets = memsets;
+    if (memsets || other) {
+        (void)hipGraphDestroy(g);
+        return fail(kUnsupported, "moea_batch: the captured generation holds non-kernel nodes");
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    C.graph = g;
+    C.graph_exec = x;
+    C.graph_valid = true;
+    C.graph_captures += 1;
+    return kOk;
+}
+
 // One generation of every running instance; instances reaching their
 // generation count become finished.  Appends finished slots to `done`.
 int core_step(BatchCore& C, std::vector<int>* done)
 {
     if (!any_state(C, kStateRunning)) return kOk;
     if (C.dirty) MOEA_CHECK(core_upload(C));
-    BatchLaunch L{C.dev_insts, C.slots, C.stream, C.niching};
-    stage_begin(kStReproduce, C.stream);
-    MOEA_CHECK(batch_reproduce(L, C.run.rows256.map, C.run.rows256.blocks, C.run.sbx256.map, C.run.sbx256.blocks));
-    stage_end(kStReproduce, C.stream);
-    stage_begin(kStEvaluate, C.stream);
-    MOEA_CHECK(batch_evaluate(L, C.run.eval, 1));
-    stage_end(kStEvaluate, C.stream);
-    MOEA_CHECK(batch_select(L, C.run.sel, C.run.nds1, C.scratch));
-    MOEA_CHECK(batch_advance(L, kStateRunning, kStateRunning, 1));
+    if (C.graph_on) {
+        if (!C.graph_valid) MOEA_CHECK(core_capture(C));
+        MOEA_HIP(hipGraphLaunch(C.graph_exec, C.stream));
+    } else {
+        MOEA_CHECK(core_step_launch(C, C.stream));
+    }
     for (int i = 0; i < C.slots; ++i) {
         BInst& I = C.host[i];
         if (I.state != kStateRunning) continue;
@@ -571,6 +678,86 @@ int batch_step(BatchState& B, int generations)
         B.generation += 1;
     }
     return kOk;
+}
+
+// Peel plans the tuner tries: (capacity divisor, row blocks per workgroup).
+constexpr int kTuneCandidates = 4;
+constexpr int kTunePlan[kTuneCandidates][2] = {{1, 16}, {1, 32}, {2, 16}, {2, 32}};
+
+// Times each peel plan on the batch's own next generations (two passes over
+// the candidates, forward then backward, `trials` generations per visit,
+// the first of each visit untimed: it uploads the plan and recaptures) and
+// keeps the fastest.  The plan only decides which workgroup peels which rows,
+// so the generations it runs are the run's own.  Returns the generations run.
+int batch_autotune(BatchState& B, int trials, int* chosen, double* ms_out, int max_out)
+{
+    BatchCore& C = B.core;
+    if (trials < 2) return fail(kInvalidArgument, "moea_batch_autotune: trials must be at least 2");
+    if (!any_state(C, kStateRunning)) return fail(kInvalidArgument, "moea_batch_autotune: call moea_batch_init first");
+    double total[kTuneCandidates] = {};
+    int visits[kTuneCandidates] = {};
+    bool valid[kTuneCandidates];
+    const int keep_div = C.peel_cap_div, keep_rows = C.peel_rows;
+    for (int c = 0; c < kTuneCandidates; ++c) {
+        // a plan that cannot hold the largest instance is skipped
+        C.peel_cap_div = kTunePlan[c][0];
+        C.peel_rows = kTunePlan[c][1];
+        C.dirty = true;
+        valid[c] = core_upload(C) == kOk;
+    }
+    C.peel_cap_div = keep_div;
+    C.peel_rows = keep_rows;
+    C.dirty = true;
+    hipEvent_t a = nullptr, b = nullptr;
+    MOEA_HIP(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(kHipError, "moea_batch_autotune: hipEventCreate failed");
+    }
+    int ran = 0, status = kOk;
+    for (int pass = 0; pass < 2 && status == kOk; ++pass) {
+        for (int v = 0; v < kTuneCandidates && status == kOk; ++v) {
+            const int c = pass == 0 ? v : kTuneCandidates - 1 - v;
+            if (!valid[c] || B.generation + trials > B.max_generations) continue;
+            C.peel_cap_div = kTunePlan[c][0];
+            C.peel_rows = kTunePlan[c][1];
+            C.dirty = true;
+            status = core_step(C, nullptr);
+            if (status != kOk) break;
+            B.generation += 1;
+            if (hipEventRecord(a, C.stream) != hipSuccess) { status = fail(kHipError, "hipEventRecord"); break; }
+            for (int t = 1; t < trials && status == kOk; ++t) {
+                status = core_step(C, nullptr);
+                B.generation += 1;
+            }
+            if (status != kOk) break;
+            float ms = 0.0f;
+            if (hipEventRecord(b, C.stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                status = fail(kHipError, "moea_batch_autotune: event timing failed");
+                break;
+            }
+            total[c] += ms / (trials - 1);
+            visits[c] += 1;
+            ran += trials;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (status != kOk) return status;
+    int best = -1;
+    for (int c = 0; c < kTuneCandidates; ++c) {
+        const double ms = visits[c] ? total[c] / visits[c] : -1.0;
+        if (ms_out && c < max_out) ms_out[c] = ms;
+        if (visits[c] && (best < 0 || ms < total[best] / visits[best])) best = c;
+    }
+    if (best >= 0) {
+        C.peel_cap_div = kTunePlan[best][0];
+        C.peel_rows = kTunePlan[best][1];
+        C.dirty = true;
+    }
+    if (chosen) *chosen = best;
+    return ran;
 }
 
 // ------------------------------------------------------------------ serve --
@@ -775,6 +962,50 @@ int moea_batch_check(const moea_batch* batch)
     return core_device_error(batch->state->core);
 }
 
+int moea_batch_set_graph(moea_batch* batch, int enable)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_set_graph: null batch");
+    batch->state->core.graph_on = enable != 0;
+    return kOk;
+}
+
+int moea_batch_graph_info(const moea_batch* batch, int* kernel_nodes, int* memset_nodes, int* captures)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_graph_info: null batch");
+    const BatchCore& C = batch->state->core;
+    if (kernel_nodes) *kernel_nodes = C.graph_kernels;
+    if (memset_nodes) *memset_nodes = C.graph_memsets;
+    if (captures) *captures = C.graph_captures;
+    return C.graph_nodes;
+}
+
+int moea_batch_autotune(moea_batch* batch, int trials, int* chosen, double* ms_per_generation, int max_candidates)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_autotune: null batch");
+    return batch_autotune(*batch->state, trials, chosen, ms_per_generation, max_candidates);
+}
+
+int moea_batch_peel_plan(const moea_batch* batch, int* capacity_divisor, int* rows_per_workgroup)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_peel_plan: null batch");
+    const BatchCore& C = batch->state->core;
+    if (capacity_divisor) *capacity_divisor = C.peel_cap_div;
+    if (rows_per_workgroup) *rows_per_workgroup = C.peel_rows;
+    return kOk;
+}
+
+int moea_batch_set_peel_plan(moea_batch* batch, int capacity_divisor, int rows_per_workgroup)
+{
+    if (!batch || !batch->state) return fail(kInvalidArgument, "moea_batch_set_peel_plan: null batch");
+    if (capacity_divisor < 1 || capacity_divisor > 64 || rows_per_workgroup < 1 || rows_per_workgroup > 32)
+        return fail(kInvalidArgument, "moea_batch_set_peel_plan: divisor in [1, 64], rows in [1, 32]");
+    BatchCore& C = batch->state->core;
+    C.peel_cap_div = capacity_divisor;
+    C.peel_rows = rows_per_workgroup;
+    C.dirty = true;
+    return kOk;
+}
+
 void moea_batch_destroy(moea_batch* batch)
 {
     if (!batch) return;
@@ -854,6 +1085,13 @@ int moea_serve_check(const moea_serve* server)
     if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_check: null server");
     MOEA_HIP(hipStreamSynchronize(server->state->core.stream));
     return core_device_error(server->state->core);
+}
+
+int moea_serve_set_graph(moea_serve* server, int enable)
+{
+    if (!server || !server->state) return fail(kInvalidArgument, "moea_serve_set_graph: null server");
+    server->state->core.graph_on = enable != 0;
+    return kOk;
 }
 
 void moea_serve_destroy(moea_serve* server)

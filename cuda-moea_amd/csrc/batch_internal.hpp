@@ -182,8 +182,8 @@ int mlp_tile_count(int n, int hidden);            // blocks of one MLP hidden-la
 size_t batch_nds_sort_bytes(long long total_rows, int inst_bits);                             // nds.hip
 int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, int mode, int state);  // nds.hip
 int batch_peel_capacity();                                                                    // nds.hip
-int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, std::vector<int>& first,
-                    std::vector<int>& jobs, size_t& lds);                                     // nds.hip
+int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, int rows_per_block,
+                    std::vector<int>& first, std::vector<int>& jobs, size_t& lds);           // nds.hip
 int batch_ideal(const BatchLaunch& L, const BSelGrids& G, int mode);                          // nsga3.hip
 int batch_select(const BatchLaunch& L, const BSelGrids& G, const BNdsGrids& NG, const BNdsScratch& S);  // nsga3.hip
 int batch_advance(const BatchLaunch& L, int from_state, int to_state, int rotate);              // nsga3.hip

@@ -996,14 +996,15 @@ int batch_peel_capacity()
 }
 
 // Groups for the batched peel.  An instance of W row blocks wants one block
-// per 16 row blocks (one row block per wave) and needs at least one per 32
-// (the per-block task queue is sized for 32).  When the wanted blocks exceed
-// the capacity, the largest instances still get groups of their own (scaled
-// down) and the rest join the least-loaded group large enough for them; a
-// group's blocks run its instances in the same order, so no two groups wait
-// on each other.
-int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, std::vector<int>& first,
-                    std::vector<int>& jobs, size_t& lds)
+// per `rows_per_block` row blocks (16: one row block per wave; 32: two) and
+// needs at least one per 32 (the per-block task queue is sized for 32).  When
+// the wanted blocks exceed the capacity, the largest instances still get
+// groups of their own (scaled down) and the rest join the least-loaded group
+// large enough for them; a group's blocks run its instances in the same
+// order, so no two groups wait on each other.  The plan decides only which
+// workgroup peels which rows: the fronts do not depend on it.
+int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int capacity, int rows_per_block,
+                    std::vector<int>& first, std::vector<int>& jobs, size_t& lds)
 {
     const int count = static_cast<int>(inst.size());
     first.assign(1, 0);
@@ -1011,10 +1012,11 @@ int batch_peel_plan(const std::vector<int>& inst, const std::vector<int>& W, int
     lds = 0;
     if (count == 0) return 0;
     if (capacity < 1) return fail(kUnsupported, "batched peel cannot be resident on this device");
+    if (rows_per_block < 1 || rows_per_block > 32) return fail(kInvalidArgument, "batched peel: rows per block out of range");
     std::vector<int> want(count), need(count), order(count);
     long long demand = 0;
     for (int i = 0; i < count; ++i) {
-        want[i] = std::max(1, std::min(ceil_div(W[i], 16), capacity));
+        want[i] = std::max(1, std::min(ceil_div(W[i], rows_per_block), capacity));
         need[i] = std::max(1, ceil_div(W[i], 32));
         if (need[i] > capacity) return fail(kUnsupported, "batched peel: population too large for one device");
         demand += want[i];

@@ -60,28 +60,36 @@ void drain(Timer& t)
     t.done.clear();
 }
 
+// Launches being captured into a graph are not timed: an event recorded
+// during capture would become a graph node.
+bool capturing(hipStream_t stream)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 }  // namespace
 
 void stage_begin(int stage, hipStream_t stream)
 {
     Timer& t = timer();
-    if (!t.enabled) return;
+    if (!t.enabled || capturing(stream)) return;
     std::lock_guard<std::mutex> lk(t.mu);
     if (t.free.size() < 2) drain(t);
     if (t.free.size() < 2) return;
     hipEvent_t a = take(t), b = take(t);
-    hipEventRecord(a, stream);
+    (void)hipEventRecord(a, stream);
     t.open.push_back({stage, a, b});
 }
 
 void stage_end(int stage, hipStream_t stream)
 {
     Timer& t = timer();
-    if (!t.enabled) return;
+    if (!t.enabled || capturing(stream)) return;
     std::lock_guard<std::mutex> lk(t.mu);
     for (size_t i = t.open.size(); i-- > 0;) {
         if (t.open[i].stage == stage) {
-            hipEventRecord(t.open[i].b, stream);
+            (void)hipEventRecord(t.open[i].b, stream);
             t.done.push_back(t.open[i]);
             t.open.erase(t.open.begin() + static_cast<long>(i));
             return;

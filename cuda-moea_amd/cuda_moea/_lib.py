@@ -204,6 +204,11 @@ SIGNATURES = {
     "moea_batch_generation": (c_int, [c_void_p]),
     "moea_batch_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
     "moea_batch_check": (c_int, [c_void_p]),
+    "moea_batch_set_graph": (c_int, [c_void_p, c_int]),
+    "moea_batch_graph_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "moea_batch_peel_plan": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int)]),
+    "moea_batch_set_peel_plan": (c_int, [c_void_p, c_int, c_int]),
+    "moea_batch_autotune": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(ctypes.c_double), c_int]),
     "moea_batch_destroy": (None, [c_void_p]),
     "moea_serve_reserved_bytes": (c_size_t, [POINTER(ServeConfig)]),
     "moea_serve_instance_bytes": (c_size_t, [POINTER(BatchProblem)]),
@@ -214,6 +219,7 @@ SIGNATURES = {
     "moea_serve_view": (c_int, [c_void_p, c_int, POINTER(PopulationC), POINTER(c_void_p)]),
     "moea_serve_release": (c_int, [c_void_p, c_int]),
     "moea_serve_check": (c_int, [c_void_p]),
+    "moea_serve_set_graph": (c_int, [c_void_p, c_int]),
     "moea_serve_destroy": (None, [c_void_p]),
     "moea_native_build_id": (ctypes.c_char_p, []),
     "moea_plugin_open": (c_int, [ctypes.c_char_p, POINTER(c_void_p)]),

@@ -91,11 +91,13 @@ def view_result(ws: torch.Tensor, pop, fronts: int, r, total_ms: float, copy: bo
 
 
 class BatchNSGA3:
-    def __init__(self, runs: Sequence[NSGA3], *, niching: tuple[int, int] = (0, 0)) -> None:
+    def __init__(self, runs: Sequence[NSGA3], *, niching: tuple[int, int] = (0, 0), graph: bool = False) -> None:
         """``niching`` = (mode, lds_cap) of moea_niching: the niching path of
-        every selection of the batch ((0, 0): automatic)."""
+        every selection of the batch ((0, 0): automatic).  ``graph``: replay
+        each generation from a captured HIP graph (moea_batch_set_graph)."""
         runs = list(runs)
         self._niching = (int(niching[0]), int(niching[1]))
+        self._graph = bool(graph)
         if not runs:
             raise ValueError("BatchNSGA3 needs at least one run")
         for r in runs:
@@ -132,12 +134,39 @@ class BatchNSGA3:
         check(lib.moea_batch_create(arr, count, self._ws.data_ptr(), self._ws.numel(), stream,
                                     ctypes.byref(self._handle)))
         check(lib.moea_batch_init(self._handle))
+        check(lib.moea_batch_set_graph(self._handle, 1 if self._graph else 0))
         self._initialized = True
 
     def step(self, generations: int = 1) -> None:
         if not self._initialized:
             self.initialize()
         check(lib.moea_batch_step(self._handle, int(generations)))
+
+    def autotune(self, trials: int = 3) -> dict:
+        """Choose the front-peeling plan by timing this batch's own next
+        generations under each candidate (moea_batch_autotune); the results
+        equal an untuned run.  Returns the chosen (capacity divisor, rows per
+        workgroup), the ms per generation of every candidate and the
+        generations used."""
+        if not self._initialized:
+            self.initialize()
+        chosen = ctypes.c_int(-1)
+        ms = (ctypes.c_double * 8)()
+        ran = lib.moea_batch_autotune(self._handle, int(trials), ctypes.byref(chosen), ms, 8)
+        if ran < 0:
+            check(ran)
+        div, rows = ctypes.c_int(), ctypes.c_int()
+        check(lib.moea_batch_peel_plan(self._handle, ctypes.byref(div), ctypes.byref(rows)))
+        return {"chosen": chosen.value, "plan": (div.value, rows.value), "ms_per_generation": list(ms)[:4],
+                "generations": ran}
+
+    def graph_info(self) -> dict:
+        """Nodes of the captured generation graph (zeros before a capture)."""
+        k, ms, caps = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        n = lib.moea_batch_graph_info(self._handle, ctypes.byref(k), ctypes.byref(ms), ctypes.byref(caps))
+        if n < 0:
+            check(n)
+        return {"nodes": n, "kernel_nodes": k.value, "memset_nodes": ms.value, "captures": caps.value}
 
     @property
     def generation(self) -> int:

@@ -25,7 +25,9 @@ from .batch import check_run, fill_problem, view_result
 
 class Server:
     def __init__(self, budget_bytes: int, *, max_instances: int = 256, max_rows: int | None = None,
-                 device: int | torch.device = 0, niching: tuple[int, int] = (0, 0)) -> None:
+                 device: int | torch.device = 0, niching: tuple[int, int] = (0, 0), graph: bool = False) -> None:
+        """``graph``: steps between changes of the set of runs replay one
+        captured generation (moea_serve_set_graph)."""
         if not torch.cuda.is_available():
             raise RuntimeError("cuda_moea requires a ROCm GPU")
         self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
@@ -44,6 +46,7 @@ class Server:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(lib.moea_serve_create(ctypes.byref(cfg), self._ws.data_ptr(), self._ws.numel(), stream,
                                     ctypes.byref(self._handle)))
+        check(lib.moea_serve_set_graph(self._handle, 1 if graph else 0))
         self._runs: dict[int, NSGA3] = {}
         self._keep: dict[int, list] = {}
         self._finished_buf = (ctypes.c_int32 * cfg.max_instances)()

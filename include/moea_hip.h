@@ -16,8 +16,10 @@
  *   - directions (m, k)  objective-major, unit length
  * All pointers are device pointers unless stated; `stream` is a hipStream_t.
  * The caller owns every buffer, including the workspace whose size the
- * *_workspace_bytes functions report; nothing here allocates device memory,
- * so every call can be captured into a HIP graph.
+ * *_workspace_bytes functions report; the compute entry points allocate no
+ * device memory and make no host synchronization unless stated, so their
+ * launches can be captured into a HIP graph (the batched engine does that
+ * itself: moea_batch_set_graph).
  * Every function returns MOEA_OK (0) or a negative MOEA_E_* code; the message
  * of the last failure on the calling thread is moea_last_error().
  */
@@ -569,6 +571,32 @@ int moea_batch_view(const moea_batch* batch, int index, moea_population* out, in
  * barrier (the fronts of that generation were invalid; the affected
  * instances kept their parents), else MOEA_OK. */
 int moea_batch_check(const moea_batch* batch);
+/* Generation graphs (no reference counterpart; the batched engine is this
+ * library's own).  With enable != 0, moea_batch_step replays one HIP graph
+ * per generation, captured from the batch's own launch sequence the first
+ * time it is needed and again whenever the set of running instances changed.
+ * Every per-generation quantity lives in the device instance table, so the
+ * replays are the same launches as the direct path and give the same bits.
+ * The captured graph holds kernel nodes only (a capture with any other node
+ * fails with MOEA_E_UNSUPPORTED).  0 switches back to direct launches. */
+int moea_batch_set_graph(moea_batch* batch, int enable);
+/* Nodes of the current generation graph (0 before the first capture); the
+ * kernel and memset node counts and the number of captures so far. */
+int moea_batch_graph_info(const moea_batch* batch, int* kernel_nodes, int* memset_nodes, int* captures);
+/* The batched front-peeling plan: workgroups used = the peel kernel's
+ * co-resident capacity / capacity_divisor, one workgroup per
+ * rows_per_workgroup 64-row blocks of an instance.  The plan decides only
+ * which workgroup peels which rows; the fronts do not depend on it. */
+int moea_batch_peel_plan(const moea_batch* batch, int* capacity_divisor, int* rows_per_workgroup);
+int moea_batch_set_peel_plan(moea_batch* batch, int capacity_divisor, int rows_per_workgroup);
+/* Autotuner: runs the batch's next generations under each candidate peel
+ * plan ((1, 16), (1, 32), (2, 16), (2, 32)), twice each (forward, then
+ * backward order), `trials` >= 2 generations per visit with the first of each
+ * visit untimed, and keeps the fastest.  The generations are the run's own:
+ * the results equal an untuned run.  Writes the chosen candidate index and
+ * up to max_candidates mean ms per generation (-1 for a candidate that could
+ * not hold the batch or did not run); returns the generations it ran. */
+int moea_batch_autotune(moea_batch* batch, int trials, int* chosen, double* ms_per_generation, int max_candidates);
 void moea_batch_destroy(moea_batch* batch);
 
 /* ------------------------------------------------------------------------ */
@@ -616,6 +644,9 @@ int moea_serve_release(moea_serve* server, int slot);
  * moea_serve_release also return MOEA_E_DEVICE at the synchronizations they
  * already make. */
 int moea_serve_check(const moea_serve* server);
+/* As moea_batch_set_graph, for a server: steps between changes of the set
+ * of runs replay one captured generation. */
+int moea_serve_set_graph(moea_serve* server, int enable);
 void moea_serve_destroy(moea_serve* server);
 
 /* ------------------------------------------------------------------------ */
