@@ -132,3 +132,114 @@ def test_served_graph_equals_served_direct():
     for i in range(len(cases)):
         for u, v in zip(a[i], b[i]):
             np.testing.assert_array_equal(u, v)
+
+
+def _hip():
+    """The HIP runtime this process already loaded (torch's and the
+    library's), for the graph API calls of the test."""
+    import ctypes
+    with open("/proc/self/maps") as f:
+        paths = sorted({ln.split()[-1] for ln in f if "libamdhip64" in ln and "/" in ln})
+    assert paths, "HIP runtime not loaded"
+    hip = ctypes.CDLL(paths[0])
+    vp = ctypes.c_void_p
+    for name, args in (("hipStreamCreateWithFlags", [ctypes.POINTER(vp), ctypes.c_uint]),
+                       ("hipStreamBeginCapture", [vp, ctypes.c_int]),
+                       ("hipStreamEndCapture", [vp, ctypes.POINTER(vp)]),
+                       ("hipGraphGetNodes", [vp, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]),
+                       ("hipGraphNodeGetType", [vp, ctypes.POINTER(ctypes.c_int)]),
+                       ("hipGraphInstantiate", [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_size_t]),
+                       ("hipGraphLaunch", [vp, vp]), ("hipStreamSynchronize", [vp]),
+                       ("hipGraphExecDestroy", [vp]), ("hipGraphDestroy", [vp]), ("hipStreamDestroy", [vp])):
+        fn = getattr(hip, name)
+        fn.restype, fn.argtypes = ctypes.c_int, args
+    return hip
+
+
+def test_single_population_generation_replays_from_a_graph():
+    """One NSGA-III generation through the C ABI (moea_reproduce,
+    moea_eval_quadratic, moea_nsga3_select, whose sort includes the
+    cooperative front-peeling launch), captured from a stream into a graph:
+    only kernel nodes, and every replay equals the direct calls."""
+    import ctypes
+    from cuda_moea import kernels as K
+    hip = _hip()
+    n, m, d = 2048, 3, 30
+    v = torch.from_numpy(K.das_dennis(n, m)).to(DEV)
+    k = v.shape[1]
+    bounds = torch.tensor([0.0, 1.0] * d, device=DEV)
+    centers = torch.rand(m, d, generator=torch.Generator().manual_seed(3)).to(DEV)
+    par = K.Population(K.init_population(n, d, bounds, 11), None, None)
+    par.f, par.cv = K.eval_quadratic(par.x, centers)
+    ideal0 = K.ideal_point(par.f)
+    off = K.Population.empty(n, d, m, DEV)
+    nxt = K.Population.empty(n, d, m, DEV)
+    idx = torch.empty(n, dtype=torch.int32, device=DEV)
+    nfr = torch.empty(n, dtype=torch.int32, device=DEV)
+    stats = torch.zeros(8, dtype=torch.int32, device=DEV)
+    ideal = ideal0.clone()
+    ws = K.workspace(_lib.lib.moea_nsga3_workspace_bytes(n, m, k), DEV)
+    torch.cuda.synchronize()
+
+    def generation(stream):
+        r = _lib.ReproduceArgs()
+        r.n, r.d, r.active, r.mating = n, d, n, _lib.MATING_RANDOM
+        r.parents_x, r.bounds = par.x.data_ptr(), bounds.data_ptr()
+        r.sbx_eta, r.sbx_prob, r.sbx_copy_prob = 30.0, 1.0, 0.5
+        r.mutation, r.pm_eta, r.pm_prob = _lib.MUTATION_POLYNOMIAL, 20.0, 1.0
+        r.seed, r.generation = 5, 7
+        r.offspring_x, r.parent_indices, r.stream = off.x.data_ptr(), idx.data_ptr(), stream
+        _lib.check(_lib.lib.moea_reproduce(ctypes.byref(r)))
+        _lib.check(_lib.lib.moea_eval_quadratic(off.x.data_ptr(), n, d, centers.data_ptr(), m, off.f.data_ptr(), n,
+                                                off.cv.data_ptr(), stream))
+        a = _lib.NSGA3SelectArgs()
+        a.n, a.m, a.d, a.k = n, m, d, k
+        a.parents, a.offspring, a.next = par.c(), off.c(), nxt.c()
+        a.next_fronts, a.directions, a.ideal = nfr.data_ptr(), v.data_ptr(), ideal.data_ptr()
+        a.seed, a.generation = 5, 7
+        a.stats, a.workspace, a.workspace_bytes, a.stream = stats.data_ptr(), ws.data_ptr(), ws.numel(), stream
+        _lib.check(_lib.lib.moea_nsga3_select(ctypes.byref(a)))
+
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreateWithFlags(ctypes.byref(s), 1) == 0
+    print("single generation: direct", flush=True)
+    generation(s.value)
+    assert hip.hipStreamSynchronize(s) == 0
+    want = [t.clone() for t in (off.x, off.f, nxt.x, nxt.f, nxt.cv, nfr, stats[:3])]
+    ideal.copy_(ideal0)
+    torch.cuda.synchronize()
+    print("single generation: capture", flush=True)
+    assert hip.hipStreamBeginCapture(s, 1) == 0
+    generation(s.value)
+    g = ctypes.c_void_p()
+    assert hip.hipStreamEndCapture(s, ctypes.byref(g)) == 0 and g.value
+    count = ctypes.c_size_t(0)
+    assert hip.hipGraphGetNodes(g, None, ctypes.byref(count)) == 0 and count.value > 0
+    nodes = (ctypes.c_void_p * count.value)()
+    assert hip.hipGraphGetNodes(g, nodes, ctypes.byref(count)) == 0
+    types = []
+    for nd in nodes:
+        t = ctypes.c_int(-1)
+        assert hip.hipGraphNodeGetType(ctypes.c_void_p(nd), ctypes.byref(t)) == 0
+        types.append(t.value)
+    print(f"  {len(types)} nodes, types {sorted(set(types))}", flush=True)
+    assert set(types) == {0}                       # hipGraphNodeTypeKernel only: no memset nodes
+    x = ctypes.c_void_p()
+    assert hip.hipGraphInstantiate(ctypes.byref(x), g, None, None, 0) == 0
+    scratch = torch.zeros(1 << 18, device=DEV)
+    try:
+        for rep in range(3):
+            print(f"single generation: replay {rep}", flush=True)
+            for t in (off.x, off.f, nxt.x, nxt.f, nxt.cv):
+                t.fill_(-1.0)
+            nfr.fill_(-9)
+            scratch.add_(1.0)
+            torch.cuda.synchronize()
+            assert hip.hipGraphLaunch(x, s) == 0
+            assert hip.hipStreamSynchronize(s) == 0
+            for a_, b_ in zip([off.x, off.f, nxt.x, nxt.f, nxt.cv, nfr, stats[:3]], want):
+                assert torch.equal(a_, b_)
+    finally:
+        hip.hipGraphExecDestroy(x)
+        hip.hipGraphDestroy(g)
+        hip.hipStreamDestroy(s)
