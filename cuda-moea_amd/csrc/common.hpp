@@ -232,6 +232,72 @@ __host__ __device__ inline float powf_det(float x, float y)
     return static_cast<float>(exp2_d(static_cast<double>(y) * log2_d(static_cast<double>(x))));
 }
 
+// x^y for 0 <= x < 2^24 and 0 < y <= 1 (the SBX spread factor
+// (2u)^(1/(1+eta)) or (2 - 2u)^(-1/(1+eta)), and the polynomial-mutation root
+// vm^(1/(eta+1))) in float32 arithmetic only, one fixed operation sequence,
+// within 1.05 ulp of the exact value over those domains:
+//   x = 2^e m with m in [sqrt(1/2), sqrt(2)];
+//   log2 m = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1), by the odd series
+//     atanh s = s + s^3/3 + ... + s^11/11 (Abramowitz & Stegun 4.6.22), with
+//     s and the (2 / ln 2) product carried to double-float precision;
+//   z = y (e + log2 m) as an unevaluated sum hi + lo;
+//   2^z = 2^k e^w, k = rint(hi), w = ((hi - k) + lo) ln 2, e^w by its Taylor
+//     polynomial to w^8 / 8! (A&S 4.2.1), |w| <= 0.35.
+// Identical on host and device (fmaf, IEEE division, no contraction).
+__host__ __device__ inline float powf_root(float x, float y)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
+    if (x == 1.0f) return 1.0f;
+    if (x == INFINITY) return INFINITY;
+    int e = 0;
+    if (x < 1.17549435e-38f) { x = x * 16777216.0f; e = -24; }      // subnormal: scale by 2^24
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    e += static_cast<int>((b >> 23) & 0xffu) - 127;
+    float m = __builtin_bit_cast(float, (b & 0x007fffffu) | 0x3f800000u);   // [1, 2)
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    const float num = m - 1.0f;                           // exact
+    const float den = m + 1.0f;
+    const float den_lo = m - (den - 1.0f);                // den + den_lo = m + 1 exactly
+    const float s = num / den;
+    float rem = fmaf(-s, den, num);                       // exact remainder
+    rem = fmaf(-s, den_lo, rem);
+    const float s_lo = rem / den;                         // s + s_lo = (m - 1) / (m + 1)
+    const float s2 = s * s;
+    float p = 1.0f / 11.0f;
+    p = fmaf(p, s2, 1.0f / 9.0f);
+    p = fmaf(p, s2, 1.0f / 7.0f);
+    p = fmaf(p, s2, 1.0f / 5.0f);
+    p = fmaf(p, s2, 1.0f / 3.0f);
+    p = p * s2;                                           // atanh(s) / s - 1
+    const float kC_hi = 2.88539004f, kC_lo = 3.851926e-08f;           // 2 / ln 2
+    const float tail = fmaf(s, p, s_lo);
+    const float lm = s * kC_hi;
+    const float lm_err = fmaf(s, kC_hi, -lm);
+    const float lm_lo = fmaf(tail, kC_hi, fmaf(s, kC_lo, lm_err));   // log2 m = lm + lm_lo
+    const float ef = static_cast<float>(e);
+    const float hi0 = y * ef;
+    const float hi0_err = fmaf(y, ef, -hi0);
+    const float t1 = y * lm;
+    const float t1_err = fmaf(y, lm, -t1);
+    const float hi = hi0 + t1;
+    const float hi_err = (hi0 - hi) + t1;                 // |hi0| >= 2 |t1| or hi0 = 0
+    const float lo = fmaf(y, lm_lo, hi0_err + t1_err) + hi_err;
+    const float k = rintf(hi);
+    const float r = (hi - k) + lo;
+    const float kL_hi = 0.693147182f, kL_lo = -1.9046542e-09f;        // ln 2
+    const float w = fmaf(r, kL_hi, r * kL_lo);
+    float q = 1.0f / 40320.0f;
+    q = fmaf(q, w, 1.0f / 5040.0f);
+    q = fmaf(q, w, 1.0f / 720.0f);
+    q = fmaf(q, w, 1.0f / 120.0f);
+    q = fmaf(q, w, 1.0f / 24.0f);
+    q = fmaf(q, w, 1.0f / 6.0f);
+    q = fmaf(q, w, 0.5f);
+    q = fmaf(q, w, 1.0f);
+    q = fmaf(q, w, 1.0f);
+    return ldexpf(q, static_cast<int>(k));
+}
+
 __host__ __device__ inline double atan01_d(double t)
 {
     const int i = static_cast<int>(nearbyint(t * 4.0));

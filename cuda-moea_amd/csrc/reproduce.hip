@@ -59,9 +59,9 @@ __device__ __forceinline__ void sbx_one(float x1, float x2, float ub, float up, 
 {
     if (up >= prob || uc < copy_prob) { ca = x1; cb = x2; return; }
     const float power = 1.0f / (1.0f + eta);
-    // one powf_det on the selected base (a wave holds lanes of both branches)
+    // one powf_root on the selected base (a wave holds lanes of both branches)
     const float base = (ub < 0.5f) ? 2.0f * ub : 1.0f / (2.0f - 2.0f * ub);
-    float beta = powf_det(base, power);
+    float beta = powf_root(base, power);
     beta = beta * static_cast<float>(2 * sign - 1);
     const float mid = 0.5f * (x1 + x2);
     const float delta = 0.5f * beta * (x1 - x2);
@@ -76,15 +76,16 @@ __device__ __forceinline__ float pm_one(float value, float ug, float ud, float l
     const float range = hi - lo;
     const float normalized = (value - lo) / range;
     const float exponent = eta + 1.0f;
-    // both branches' operands selected first, then one pair of powf_det calls:
+    // both branches' operands selected first, then one power and one root:
     //   ud <= 0.5: vm = 2 ud + (1 - 2 ud) (1 - x)^e,          delta = vm^(1/e) - 1
     //   else:      vm = 2 (1 - ud) + 2 (ud - 0.5) x^e,        delta = 1 - vm^(1/e)
+    // (x^e with e > 1 can reach far below 2^-126: evaluated in double)
     const bool lower = ud <= 0.5f;
     const float xy = lower ? 1.0f - normalized : normalized;
     const float a = lower ? 2.0f * ud : 2.0f * (1.0f - ud);
     const float b = lower ? 1.0f - 2.0f * ud : 2.0f * (ud - 0.5f);
     const float vm = a + b * powf_det(xy, exponent);
-    const float pw = powf_det(vm, 1.0f / exponent);
+    const float pw = powf_root(vm, 1.0f / exponent);
     const float delta = lower ? pw - 1.0f : 1.0f - pw;
     return fminf(fmaxf(fmaf(delta, range, value), lo), hi);
 }

@@ -13,9 +13,12 @@
  *     fmaf()/fma() call placed where the reference's nvcc build contracts one
  *     (nvcc defaults to -fmad=true, CMakeLists.txt:115-123 sets no override);
  *   - sqrtf / float division are IEEE correctly rounded;
- *   - powf / acosf are replaced by orc_powf / orc_acosf, which evaluate in
- *     double with a fixed operation sequence and round once to float, because
- *     CUDA's powf/acosf last-bit behaviour cannot be reproduced;
+ *   - powf / acosf are replaced by deterministic functions, because CUDA's
+ *     powf/acosf last-bit behaviour cannot be reproduced: the SBX spread
+ *     factor and the mutation root (exponent <= 1) by orc_powf_root, in float32
+ *     with a fixed operation sequence; the mutation power (exponent > 1, whose
+ *     results reach far below 2^-126) by orc_powf, and acosf by orc_acosf,
+ *     which evaluate in double and round once to float;
  *   - rsqrtf (RVEA partition and adaptation) is replaced by 1/sqrtf.
  *
  * Layouts follow the reference (NI/core/types.cuh:21-51): objectives (M, n)
@@ -172,6 +175,78 @@ ORC_API float orc_powf(float x, float y)
     if (x == 1.0f || y == 0.0f) return 1.0f;
     if (isinf(x)) return y > 0.0f ? INFINITY : 0.0f;
     return (float)orc_exp2_d((double)y * orc_log2_d((double)x));
+}
+
+/* x^y for 0 <= x < 2^24, 0 < y <= 1 in float32 arithmetic: the SBX spread
+ * factor (sbx_crossover.cu:47-49) and the polynomial-mutation root
+ * (polynomial_mutation.cu:43,49).  x = 2^e m, m in [sqrt(1/2), sqrt(2)];
+ * log2 m = (2 / ln 2) atanh((m - 1) / (m + 1)) by the odd atanh series to
+ * s^11 / 11 (Abramowitz & Stegun 4.6.22), the quotient and the (2 / ln 2)
+ * product carried to double-float precision; z = y (e + log2 m) as hi + lo;
+ * 2^z = 2^rint(hi) e^w with e^w by its Taylor polynomial to w^8 / 8!
+ * (A&S 4.2.1).  Within 1.05 ulp of the exact power on these domains. */
+static inline uint32_t f2u(float x) { uint32_t u; memcpy(&u, &x, sizeof u); return u; }
+static inline float u2f(uint32_t u) { float x; memcpy(&x, &u, sizeof x); return x; }
+
+ORC_API float orc_powf_root(float x, float y)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? 0.0f : NAN;
+    if (x == 1.0f) return 1.0f;
+    if (x == INFINITY) return INFINITY;
+    int e = 0;
+    if (x < 1.17549435e-38f) { x = x * 16777216.0f; e = -24; }
+    const uint32_t b = f2u(x);
+    e += (int)((b >> 23) & 0xffu) - 127;
+    float m = u2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    const float num = m - 1.0f;
+    const float den = m + 1.0f;
+    const float den_lo = m - (den - 1.0f);
+    const float s = num / den;
+    float rem = fmaf(-s, den, num);
+    rem = fmaf(-s, den_lo, rem);
+    const float s_lo = rem / den;
+    const float s2 = s * s;
+    float p = 1.0f / 11.0f;
+    p = fmaf(p, s2, 1.0f / 9.0f);
+    p = fmaf(p, s2, 1.0f / 7.0f);
+    p = fmaf(p, s2, 1.0f / 5.0f);
+    p = fmaf(p, s2, 1.0f / 3.0f);
+    p = p * s2;
+    const float c_hi = 2.88539004f, c_lo = 3.851926e-08f;
+    const float tail = fmaf(s, p, s_lo);
+    const float lm = s * c_hi;
+    const float lm_err = fmaf(s, c_hi, -lm);
+    const float lm_lo = fmaf(tail, c_hi, fmaf(s, c_lo, lm_err));
+    const float ef = (float)e;
+    const float hi0 = y * ef;
+    const float hi0_err = fmaf(y, ef, -hi0);
+    const float t1 = y * lm;
+    const float t1_err = fmaf(y, lm, -t1);
+    const float hi = hi0 + t1;
+    const float hi_err = (hi0 - hi) + t1;
+    const float lo = fmaf(y, lm_lo, hi0_err + t1_err) + hi_err;
+    const float k = rintf(hi);
+    const float r = (hi - k) + lo;
+    const float l_hi = 0.693147182f, l_lo = -1.9046542e-09f;
+    const float w = fmaf(r, l_hi, r * l_lo);
+    float q = 1.0f / 40320.0f;
+    q = fmaf(q, w, 1.0f / 5040.0f);
+    q = fmaf(q, w, 1.0f / 720.0f);
+    q = fmaf(q, w, 1.0f / 120.0f);
+    q = fmaf(q, w, 1.0f / 24.0f);
+    q = fmaf(q, w, 1.0f / 6.0f);
+    q = fmaf(q, w, 0.5f);
+    q = fmaf(q, w, 1.0f);
+    q = fmaf(q, w, 1.0f);
+    return ldexpf(q, (int)k);
+}
+
+/* orc_powf_root over arrays (tests). */
+ORC_API void orc_powf_root_array(const float* x, const float* y, float* out, long long n)
+{
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) out[i] = orc_powf_root(x[i], y[i]);
 }
 
 /* atan(t) for t in [0, 1]: reduce by the nearest c in {0, 1/4, 1/2, 3/4, 1},
@@ -789,8 +864,8 @@ static inline void sbx_one(float x1, float x2, float ub_, float up, int sign, fl
 {
     if (up >= prob || uc < copy_prob) { *ca = x1; *cb = x2; return; }
     const float power = 1.0f / (1.0f + eta);
-    float beta = (ub_ < 0.5f) ? orc_powf(2.0f * ub_, power)
-                              : orc_powf(1.0f / (2.0f - 2.0f * ub_), power);
+    float beta = (ub_ < 0.5f) ? orc_powf_root(2.0f * ub_, power)
+                              : orc_powf_root(1.0f / (2.0f - 2.0f * ub_), power);
     beta = beta * (float)(2 * sign - 1);
     const float mid = 0.5f * (x1 + x2);
     const float delta = 0.5f * beta * (x1 - x2);
@@ -813,11 +888,11 @@ static inline float pm_one(float value, float ug, float ud, float lo, float hi, 
     if (ud <= 0.5f) {
         const float xy = 1.0f - normalized;
         const float vm = 2.0f * ud + (1.0f - 2.0f * ud) * orc_powf(xy, exponent);
-        delta = orc_powf(vm, 1.0f / exponent) - 1.0f;
+        delta = orc_powf_root(vm, 1.0f / exponent) - 1.0f;
     } else {
         const float xy = normalized;
         const float vm = 2.0f * (1.0f - ud) + 2.0f * (ud - 0.5f) * orc_powf(xy, exponent);
-        delta = 1.0f - orc_powf(vm, 1.0f / exponent);
+        delta = 1.0f - orc_powf_root(vm, 1.0f / exponent);
     }
     return fminf(fmaxf(fmaf(delta, range, value), lo), hi);
 }

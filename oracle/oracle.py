@@ -48,6 +48,8 @@ def lib():
             "orc_log2_d": (c_double, [c_double]),
             "orc_exp2_d": (c_double, [c_double]),
             "orc_powf": (c_float, [c_float, c_float]),
+            "orc_powf_root": (c_float, [c_float, c_float]),
+            "orc_powf_root_array": (None, [vp, vp, vp, ctypes.c_longlong]),
             "orc_acosf": (c_float, [c_float]),
             "orc_das_dennis": (c_int, [c_int, c_int, c_int, c_int, vp, vp]),
             "orc_normalize_directions": (c_int, [f32p, c_int, c_int]),
@@ -155,6 +157,18 @@ def draw_niche(seed, gen, count):
 # ------------------------------------------------------------------ math ----
 def powf(x, y):
     return float(lib().orc_powf(x, y))
+
+
+def powf_root(x, y):
+    """x^y (0 <= x < 2^24, 0 < y <= 1) in float32: scalar, or elementwise
+    over equal-length arrays."""
+    if np.isscalar(x) and np.isscalar(y):
+        return float(lib().orc_powf_root(x, y))
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float32), x.shape), np.float32)
+    out = np.empty_like(x)
+    lib().orc_powf_root_array(x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
 
 
 def acosf(x):

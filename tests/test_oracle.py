@@ -92,6 +92,33 @@ def test_powf_close_to_fp64():
     assert O.powf(0.0, 0.5) == 0.0 and O.powf(1.0, 7.0) == 1.0
 
 
+def test_powf_root_within_two_ulp_of_fp64():
+    """The float32 root used by SBX and the mutation root, against the
+    double-precision power: every SBX base 2u / (2 - 2u)^-1 for u on the
+    2^-24 draw grid, and a dense sweep of x in [2^-30, 2] (the mutation root's
+    vm), for several distribution indices."""
+    def ulps(got, x, y):
+        exact = x.astype(np.float64) ** np.float64(y)
+        ref = exact.astype(np.float32)
+        spacing = (np.nextafter(ref, np.float32(np.inf)) - ref).astype(np.float64)
+        return np.abs(got.astype(np.float64) - exact) / spacing
+
+    u = np.arange(1 << 24, dtype=np.float64) * 2.0 ** -24
+    base = np.where(u < 0.5, 2.0 * u, 1.0 / (2.0 - 2.0 * u)).astype(np.float32)
+    base = base[base > 0]
+    sweep = np.arange(np.float32(2.0 ** -30).view(np.int32), np.float32(2.0).view(np.int32), 13,
+                      dtype=np.int32).view(np.float32)
+    worst = 0.0
+    for eta in (0.5, 1.0, 20.0, 30.0):
+        y = np.float32(1.0) / np.float32(1.0 + eta)
+        for x in (base, sweep):
+            worst = max(worst, float(ulps(O.powf_root(x, y), x, y).max()))
+    assert worst <= 2.0, worst
+    assert O.powf_root(0.0, 0.5) == 0.0 and O.powf_root(1.0, 0.25) == 1.0
+    assert O.powf_root(2.0 ** -140, 0.5) == pytest.approx(2.0 ** -70, rel=2 ** -22)   # subnormal input
+    assert np.isnan(O.powf_root(-1.0, 0.5)) and O.powf_root(float("inf"), 0.5) == float("inf")
+
+
 def test_acosf_close_to_fp64():
     for x in np.linspace(0, 1, 5001, dtype=np.float32):
         exact = math.acos(float(x))
