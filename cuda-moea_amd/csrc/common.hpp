@@ -298,6 +298,47 @@ __host__ __device__ inline float powf_root(float x, float y)
     return ldexpf(q, static_cast<int>(k));
 }
 
+// cos(t) in double for |t| <= 2^20 with one fixed operation sequence:
+// t = k (pi / 2) + r, k = rint(t 2 / pi), pi / 2 as a two-term sum, then the
+// Taylor series of sin r or cos r (|r| <= pi / 4, to r^17 / r^18) by the
+// quadrant of k.
+__host__ __device__ inline double cos_d(double t)
+{
+    const double k = nearbyint(t * 6.36619772367581382433e-01);
+    double r = fma(-k, 1.57079632679489655800e+00, t);
+    r = fma(-k, 6.12323399573676603587e-17, r);
+    const double r2 = r * r;
+    const int q = static_cast<int>(k) & 3;
+    double v;
+    if (q & 1) {                 // +-sin r
+        double p = 1.0 / 355687428096000.0;                    // 1/17!
+        p = fma(p, -r2, 1.0 / 1307674368000.0);                // 1/15!
+        p = fma(p, -r2, 1.0 / 6227020800.0);                   // 1/13!
+        p = fma(p, -r2, 1.0 / 39916800.0);                     // 1/11!
+        p = fma(p, -r2, 1.0 / 362880.0);                       // 1/9!
+        p = fma(p, -r2, 1.0 / 5040.0);                         // 1/7!
+        p = fma(p, -r2, 1.0 / 120.0);                          // 1/5!
+        p = fma(p, -r2, 1.0 / 6.0);                            // 1/3!
+        v = fma(-r * r2, p, r);
+    } else {                     // +-cos r
+        double p = 1.0 / 6402373705728000.0;                   // 1/18!
+        p = fma(p, -r2, 1.0 / 20922789888000.0);               // 1/16!
+        p = fma(p, -r2, 1.0 / 87178291200.0);                  // 1/14!
+        p = fma(p, -r2, 1.0 / 479001600.0);                    // 1/12!
+        p = fma(p, -r2, 1.0 / 3628800.0);                      // 1/10!
+        p = fma(p, -r2, 1.0 / 40320.0);                        // 1/8!
+        p = fma(p, -r2, 1.0 / 720.0);                          // 1/6!
+        p = fma(p, -r2, 1.0 / 24.0);                           // 1/4!
+        p = fma(p, -r2, 0.5);                                  // 1/2!
+        v = fma(-r2, p, 1.0);
+    }
+    // quadrants: 0 cos r, 1 -sin r, 2 -cos r, 3 sin r
+    return (q == 1 || q == 2) ? -v : v;
+}
+
+// cosf of the DTLZ1 / DTLZ3 distance terms, rounded once from double.
+__host__ __device__ inline float cosf_det(float x) { return static_cast<float>(cos_d(static_cast<double>(x))); }
+
 __host__ __device__ inline double atan01_d(double t)
 {
     const int i = static_cast<int>(nearbyint(t * 4.0));

@@ -454,7 +454,7 @@ __device__ __forceinline__ void dtlz_rows(const float* __restrict__ x, int n, in
             s = s + xr[j];
         } else {
             const float v = xr[j] - 0.5f;
-            if constexpr (KIND == 1 || KIND == 3) s = s + (v * v - cosf(static_cast<float>(20.0 * M_PI * static_cast<double>(v))));
+            if constexpr (KIND == 1 || KIND == 3) s = s + (v * v - cosf_det(static_cast<float>(20.0 * M_PI * static_cast<double>(v))));
             else s = s + v * v;
         }
     }

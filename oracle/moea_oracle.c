@@ -966,6 +966,84 @@ ORC_API void orc_eval_quadratic(const float* x, int n, int d, const float* cente
     }
 }
 
+/* cos t in double with one fixed operation sequence (the library's cos_d):
+ * t = k (pi / 2) + r with k = rint(t 2 / pi) and pi / 2 as a two-term sum, then
+ * the Taylor series of sin r (to r^17) or cos r (to r^18) by k's quadrant. */
+static double orc_cos_d(double t)
+{
+    const double k = nearbyint(t * 6.36619772367581382433e-01);
+    double r = fma(-k, 1.57079632679489655800e+00, t);
+    r = fma(-k, 6.12323399573676603587e-17, r);
+    const double r2 = r * r;
+    const int q = (int)k & 3;
+    double v;
+    if (q & 1) {
+        double p = 1.0 / 355687428096000.0;
+        p = fma(p, -r2, 1.0 / 1307674368000.0);
+        p = fma(p, -r2, 1.0 / 6227020800.0);
+        p = fma(p, -r2, 1.0 / 39916800.0);
+        p = fma(p, -r2, 1.0 / 362880.0);
+        p = fma(p, -r2, 1.0 / 5040.0);
+        p = fma(p, -r2, 1.0 / 120.0);
+        p = fma(p, -r2, 1.0 / 6.0);
+        v = fma(-r * r2, p, r);
+    } else {
+        double p = 1.0 / 6402373705728000.0;
+        p = fma(p, -r2, 1.0 / 20922789888000.0);
+        p = fma(p, -r2, 1.0 / 87178291200.0);
+        p = fma(p, -r2, 1.0 / 479001600.0);
+        p = fma(p, -r2, 1.0 / 3628800.0);
+        p = fma(p, -r2, 1.0 / 40320.0);
+        p = fma(p, -r2, 1.0 / 720.0);
+        p = fma(p, -r2, 1.0 / 24.0);
+        p = fma(p, -r2, 0.5);
+        v = fma(-r2, p, 1.0);
+    }
+    return (q == 1 || q == 2) ? -v : v;
+}
+
+ORC_API float orc_cosf_det(float x) { return (float)orc_cos_d((double)x); }
+
+/* DTLZ1 (problem = 0) or C1-DTLZ1 (problem = 1) of n rows of x (n, d) into f
+ * (m, ldf) and cv (dtlz1_func_kernel, dtlz_kernels.cu:17-80, and the C1 rule
+ * of compute_cv, :824-860), in the library's summation order: 64 strided
+ * partial sums of g (partial j takes i = m-1+j, m-1+j+64, ... ascending), a
+ * pairwise tree of the partials, cosf of the distance term as orc_cosf_det.
+ * cv = 0 for DTLZ1. */
+ORC_API void orc_eval_dtlz1(const float* x, int n, int d, int m, float* f, int ldf, float* cv, int problem)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const float* xr = x + (size_t)i * d;
+        float acc[64];
+        for (int j = 0; j < 64; ++j) acc[j] = 0.0f;
+        for (int j = 0; j < 64; ++j)
+            for (int t = m - 1 + j; t < d; t += 64) {
+                const float v = xr[t] - 0.5f;
+                acc[j] = acc[j] + (v * v - orc_cosf_det((float)(20.0 * 3.14159265358979323846 * (double)v)));
+            }
+        for (int sft = 32; sft >= 1; sft >>= 1)
+            for (int j = 0; j < sft; ++j) acc[j] = acc[j] + acc[j + sft];
+        const float s = acc[0];
+        const int k = d - (m - 1);
+        float fv[16];
+        for (int c = 0; c < m; ++c) {
+            float out = 0.5f * (1.0f + 100.0f * ((float)k + s));
+            for (int j = 0; j < m - 1 - c; ++j) out *= fminf(fmaxf(xr[j], 0.0f), 1.0f);
+            if (c > 0) out *= fmaxf(0.0f, 1.0f - xr[m - 1 - c]);
+            f[(size_t)c * ldf + i] = out;
+            fv[c] = out;
+        }
+        if (problem == 1) {
+            float c = fv[m - 1] / 0.6f;
+            for (int j = 0; j < m - 1; ++j) c += fv[j] / 0.5f;
+            cv[i] = fmaxf(c - 1.0f, 0.0f);
+        } else if (cv) {
+            cv[i] = 0.0f;
+        }
+    }
+}
+
 /* ======================================================================== */
 /* In-kernel draw generator, restated for the CPU                            */
 /* ======================================================================== */
@@ -1061,8 +1139,68 @@ ORC_API void orc_draw_niche(uint64_t seed, uint32_t gen, int count, uint32_t* ua
  * unless init_x is given; on return x/f/cv/fronts hold the final population.
  * Returns the wall time in ms of the generation loop (after the initial
  * evaluation, the reference's timing boundary algorithm.cu:313-329). */
+/* Objectives of the run loops: 0 quadratic centers, 1 DTLZ1, 2 C1-DTLZ1 whose
+ * constraints count from the generation g with (g + 1) / gens >= ratio
+ * (MOEAStdTestEvaluator::is_cv_active, dtlz_kernels.cu:1300-1309; ratio <= 0:
+ * always, the initial population included). */
+typedef struct OrcProblem { int kind; const float* centers; float ratio; } OrcProblem;
+
+static int orc_cv_active(const OrcProblem* pb, int g, int gens)
+{
+    if (pb->kind != 2 || !(pb->ratio > 0.0f)) return 1;
+    if (g < 0) return 0;
+    return (float)(g + 1) / (float)(gens > 0 ? gens : 1) >= pb->ratio;
+}
+
+static void orc_evaluate(const OrcProblem* pb, const float* x, int n, int d, int m, float* f, int ldf,
+                         float* cv, int g, int gens)
+{
+    if (pb->kind == 0) {
+        orc_eval_quadratic(x, n, d, pb->centers, m, f, ldf, cv);
+        return;
+    }
+    orc_eval_dtlz1(x, n, d, m, f, ldf, cv, pb->kind == 2 ? 1 : 0);
+    if (!orc_cv_active(pb, g, gens))
+        for (int i = 0; i < n; ++i) cv[i] = 0.0f;
+}
+
+static double nsga3_run_impl(int n, int m, int d, int k_req, int partitions, int gens,
+                             uint64_t seed, const float* bounds, const OrcProblem* pb,
+                             float sbx_eta0, float sbx_eta1, float sbx_prob, float sbx_copy,
+                             float pm_eta0, float pm_eta1, float pm_prob,
+                             const float* init_x, float* x, float* f, float* cv,
+                             int32_t* fronts, int gen_count_limit);
+
 ORC_API double orc_nsga3_run(int n, int m, int d, int k_req, int partitions, int gens,
                              uint64_t seed, const float* bounds, const float* centers,
+                             float sbx_eta0, float sbx_eta1, float sbx_prob, float sbx_copy,
+                             float pm_eta0, float pm_eta1, float pm_prob,
+                             const float* init_x, float* x, float* f, float* cv,
+                             int32_t* fronts, int gen_count_limit)
+{
+    const OrcProblem pb = {0, centers, 0.0f};
+    return nsga3_run_impl(n, m, d, k_req, partitions, gens, seed, bounds, &pb, sbx_eta0, sbx_eta1, sbx_prob,
+                          sbx_copy, pm_eta0, pm_eta1, pm_prob, init_x, x, f, cv, fronts, gen_count_limit);
+}
+
+/* NSGA-III on DTLZ1 (problem 1) or C1-DTLZ1 (problem 2) with the constraint
+ * activation ratio: before the switch the violations are 0; at the switch
+ * generation (0 when they count from the start) the parents are evaluated
+ * again and their fronts re-sorted before reproduction
+ * (DTLZProblem::prepare_parent, dtlz_kernels.cu:1311-1394; algorithm.cu:230-241). */
+ORC_API double orc_nsga3_run_dtlz1(int n, int m, int d, int k_req, int partitions, int gens,
+                                   uint64_t seed, const float* bounds, int problem, float ratio,
+                                   float sbx_eta0, float sbx_eta1, float sbx_prob, float sbx_copy,
+                                   float pm_eta0, float pm_eta1, float pm_prob,
+                                   float* x, float* f, float* cv, int32_t* fronts, int gen_count_limit)
+{
+    const OrcProblem pb = {problem, NULL, ratio};
+    return nsga3_run_impl(n, m, d, k_req, partitions, gens, seed, bounds, &pb, sbx_eta0, sbx_eta1, sbx_prob,
+                          sbx_copy, pm_eta0, pm_eta1, pm_prob, NULL, x, f, cv, fronts, gen_count_limit);
+}
+
+static double nsga3_run_impl(int n, int m, int d, int k_req, int partitions, int gens,
+                             uint64_t seed, const float* bounds, const OrcProblem* pb,
                              float sbx_eta0, float sbx_eta1, float sbx_prob, float sbx_copy,
                              float pm_eta0, float pm_eta1, float pm_prob,
                              const float* init_x, float* x, float* f, float* cv,
@@ -1093,7 +1231,7 @@ ORC_API double orc_nsga3_run(int n, int m, int d, int k_req, int partitions, int
 
     if (init_x) memcpy(x, init_x, sizeof(float) * (size_t)n * d);
     else orc_draw_init(seed, n, d, bounds, x);
-    orc_eval_quadratic(x, n, d, centers, m, f, n, cv);
+    orc_evaluate(pb, x, n, d, m, f, n, cv, -1, gens);
     /* prepare: parent fronts over N with stop at N and the initial ideal point
      * (nsga3_environment_selector.cu:236-287) */
     orc_nds(f, cv, n, m, n, fronts);
@@ -1104,6 +1242,12 @@ ORC_API double orc_nsga3_run(int n, int m, int d, int k_req, int partitions, int
     clock_gettime(CLOCK_MONOTONIC, &t0);
     const int g_end = (gen_count_limit > 0 && gen_count_limit < gens) ? gen_count_limit : gens;
     for (int g = 0; g < g_end; ++g) {
+        const int sw = pb->kind == 2 && (g == 0 ? orc_cv_active(pb, 0, gens)
+                                                : (!orc_cv_active(pb, g - 1, gens) && orc_cv_active(pb, g, gens)));
+        if (sw) {
+            orc_evaluate(pb, x, n, d, m, f, n, cv, g, gens);
+            orc_nds(f, cv, n, m, n, fronts);
+        }
         const float eta_c = orc_annealed(sbx_eta0, sbx_eta1, g, gens);
         const float eta_m = orc_annealed(pm_eta0, pm_eta1, g, gens);
         orc_draw_mating(seed, (uint32_t)g, n, n, ca, cb);
@@ -1115,7 +1259,7 @@ ORC_API double orc_nsga3_run(int n, int m, int d, int k_req, int partitions, int
         /* merged objectives: parents then offspring */
         for (int c = 0; c < m; ++c) memcpy(fm + (size_t)c * nm, f + (size_t)c * n, sizeof(float) * n);
         memcpy(cvm, cv, sizeof(float) * n);
-        orc_eval_quadratic(off, n, d, centers, m, fm + n, nm, cvm + n);
+        orc_evaluate(pb, off, n, d, m, fm + n, nm, cvm + n, g, gens);
         memcpy(xm, x, sizeof(float) * (size_t)n * d);
         orc_draw_niche(seed, (uint32_t)g, nm, ua, ubn);
         orc_nsga3_select(fm, cvm, n, m, w, k, ideal, ua, ubn, chosen, fr_mix, NULL, NULL, NULL, NULL);

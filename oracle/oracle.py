@@ -76,6 +76,12 @@ def lib():
             "orc_nsga3_run": (c_double, [c_int, c_int, c_int, c_int, c_int, c_int, c_u64, f32p,
                                          f32p, c_float, c_float, c_float, c_float, c_float,
                                          c_float, c_float, vp, f32p, f32p, f32p, i32p, c_int]),
+            "orc_cosf_det": (c_float, [c_float]),
+            "orc_eval_dtlz1": (None, [f32p, c_int, c_int, c_int, f32p, c_int, vp, c_int]),
+            "orc_nsga3_run_dtlz1": (c_double, [c_int, c_int, c_int, c_int, c_int, c_int, c_u64, f32p,
+                                               c_int, c_float, c_float, c_float, c_float, c_float,
+                                               c_float, c_float, c_float, f32p, f32p, f32p, i32p,
+                                               c_int]),
             "orc_rvea_gamma": (None, [f32p, c_int, c_int, f32p]),
             "orc_rvea_penalty": (c_float, [c_int, c_int, c_int, c_float]),
             "orc_rvea_select": (c_int, [f32p, f32p, c_int, c_int, f32p, f32p, c_int, c_int,
@@ -306,6 +312,38 @@ def nsga3_run(n, m, d, gens, seed, bounds, centers, k_req=None, partitions=0,
     ms = lib().orc_nsga3_run(n, m, d, n if k_req is None else k_req, partitions, gens, seed,
                              _c(bounds, np.float32), _c(centers, np.float32), sbx[0], sbx[1],
                              sbx[2], sbx[3], pm[0], pm[1], pm[2], _ptr(ix), x, f, cv, fr, limit)
+    return x, f, cv, fr, float(ms)
+
+
+def cosf_det(x):
+    """cos of a float32 through the fixed double sequence (scalar or array)."""
+    if np.ndim(x) == 0:
+        return np.float32(lib().orc_cosf_det(float(np.float32(x))))
+    fn = lib().orc_cosf_det
+    a = np.asarray(x, np.float32)
+    return np.array([fn(float(v)) for v in a.ravel()], np.float32).reshape(a.shape)
+
+
+def eval_dtlz1(x, m, constrained=False):
+    """DTLZ1 (or C1-DTLZ1's objectives and violation).  Returns (f (m, n), cv)."""
+    x = _c(x, np.float32)
+    n, d = x.shape
+    f = np.zeros((m, n), np.float32)
+    cv = np.zeros(n, np.float32)
+    lib().orc_eval_dtlz1(x, n, d, m, f, n, _ptr(cv), 1 if constrained else 0)
+    return f, cv
+
+
+def nsga3_run_dtlz1(n, m, d, gens, seed, constrained=False, ratio=0.0, k_req=None, partitions=0,
+                    sbx=(30.0, 30.0, 1.0, 0.0), pm=(20.0, 20.0, 1.0), limit=0):
+    """NSGA-III on DTLZ1 or C1-DTLZ1 (bounds [0, 1]) with the constraint
+    activation ratio.  Returns (x, f, cv, fronts, loop_ms)."""
+    bounds = np.stack([np.zeros(d, np.float32), np.ones(d, np.float32)], axis=1)
+    x = np.zeros((n, d), np.float32); f = np.zeros((m, n), np.float32)
+    cv = np.zeros(n, np.float32); fr = np.zeros(n, np.int32)
+    ms = lib().orc_nsga3_run_dtlz1(n, m, d, n if k_req is None else k_req, partitions, gens, seed,
+                                   bounds, 2 if constrained else 1, ratio, sbx[0], sbx[1], sbx[2],
+                                   sbx[3], pm[0], pm[1], pm[2], x, f, cv, fr, limit)
     return x, f, cv, fr, float(ms)
 
 

@@ -349,3 +349,38 @@ def test_activation_switch_prepares_a_python_selector():
     alg.run()
     assert [g for g, _ in sel0.prepared] == [0, 0]
     assert sel0.prepared[0][1] > 0.0 and sel0.prepared[0][1] == sel0.prepared[1][1]
+
+
+@pytest.mark.parametrize("n,d,m", [(300, 12, 3), (1000, 500, 3), (77, 40, 8), (64, 3, 3)])
+def test_dtlz1_kernels_match_oracle_bitexact(n, d, m):
+    """DTLZ1 and C1-DTLZ1 evaluated on the GPU equal the oracle's float32
+    restatement of the same summation order, bit for bit."""
+    import oracle as O
+    rng = np.random.default_rng(n + m)
+    x = rng.random((n, d), dtype=np.float32)
+    x[: n // 2, m - 1:] = (0.5 + 0.05 * (rng.random((n // 2, d - m + 1)) - 0.5)).astype(np.float32)
+    print(f"dtlz1 bit-exact n={n} d={d} m={m}", flush=True)
+    xt = torch.from_numpy(x).to(DEV)
+    for kind, constrained in ((1, False), (9, True)):
+        f, cv = K.eval_dtlz(kind, xt, m)
+        fo, cvo = O.eval_dtlz1(x, m, constrained=constrained)
+        np.testing.assert_array_equal(f.cpu().numpy(), fo)
+        np.testing.assert_array_equal(cv.cpu().numpy(), cvo)
+
+
+@pytest.mark.parametrize("name,ratio,gens,seed", [("DTLZ1", 0.0, 15, 0), ("C1DTLZ1", 0.0, 15, 1),
+                                                  ("C1DTLZ1", 0.5, 20, 2), ("C1DTLZ1", 0.9, 10, 3)])
+def test_dtlz1_run_matches_oracle(name, ratio, gens, seed):
+    """NSGA-III trajectories on DTLZ1 and on C1-DTLZ1 with the constraint
+    activation ratio (violations zero until (g + 1) / gens reaches it, then the
+    parents are evaluated again and re-sorted) equal the oracle's, bit for bit."""
+    import oracle as O
+    prob = getattr(cm, name)(dimension=12, objectives=3, constraint_activation_ratio=ratio)
+    alg = cm.NSGA3(population_size=92, max_generations=gens, problem=prob, seed=seed, print_progress=False)
+    print(f"dtlz1 run {name} ratio={ratio} gens={gens} seed={seed}", flush=True)
+    res = alg.run()
+    x, f, cv, fr, _ = O.nsga3_run_dtlz1(92, 3, 12, gens, seed, constrained=name != "DTLZ1", ratio=ratio)
+    np.testing.assert_array_equal(res.variables.cpu().numpy(), x)
+    np.testing.assert_array_equal(res.objectives.cpu().numpy(), f.T)
+    np.testing.assert_array_equal(res.constraints.cpu().numpy().reshape(-1), cv)
+    np.testing.assert_array_equal(res.auxiliary_indices.cpu().numpy(), fr)

@@ -119,6 +119,77 @@ def test_powf_root_within_two_ulp_of_fp64():
     assert np.isnan(O.powf_root(-1.0, 0.5)) and O.powf_root(float("inf"), 0.5) == float("inf")
 
 
+def test_cosf_det_is_rounded_fp64_cos():
+    """cos of a float32 through the fixed double sequence rounds the double
+    cosine: equal to float32(cos(x)) (at most 1 ulp where the double result
+    sits at a float32 rounding tie), over the DTLZ distance-term range
+    |20 pi v| <= 10 pi, the float32 neighbours of k pi / 2 and a wider band."""
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([
+        rng.uniform(-10 * math.pi, 10 * math.pi, 200000),
+        rng.uniform(-400.0, 400.0, 20000),
+        [k * math.pi / 2 for k in range(-64, 65)],
+        [0.0, -0.0, 1e-30, 2.0 ** -24, 1.0, 31.41592653589793],
+    ]).astype(np.float32)
+    got = O.cosf_det(xs).astype(np.float64)
+    ref = np.cos(xs.astype(np.float64))
+    refs = ref.astype(np.float32)
+    spacing = np.abs(np.nextafter(refs, np.float32(np.inf)) - refs).astype(np.float64)
+    spacing = np.maximum(spacing, 2.0 ** -149)
+    err = np.abs(got - ref) / spacing
+    assert err.max() <= 1.0, float(err.max())
+    assert np.count_nonzero(got != refs.astype(np.float64)) <= 3
+    assert O.cosf_det(0.0) == 1.0 and O.cosf_det(np.float32(math.pi)) == np.float32(-1.0)
+
+
+def _dtlz1_fp64(x, m):
+    x = x.astype(np.float64)
+    xm, xk = x[:, : m - 1], x[:, m - 1:]
+    g = 100.0 * (xk.shape[1] + ((xk - 0.5) ** 2 - np.cos(20 * math.pi * (xk - 0.5))).sum(1))
+    f = np.empty((m, x.shape[0]))
+    for i in range(m):
+        v = 0.5 * (1 + g) * np.prod(xm[:, : m - 1 - i], axis=1)
+        if i > 0:
+            v = v * (1 - xm[:, m - 1 - i])
+        f[i] = v
+    return f, g
+
+
+@pytest.mark.parametrize("n,d,m", [(300, 12, 3), (200, 140, 5), (64, 3, 3)])
+def test_dtlz1_against_fp64_definition(n, d, m):
+    """DTLZ1 (Deb, Thiele, Laumanns, Zitzler 2002) evaluated in float32 in the
+    library's summation order, against the definition in double."""
+    x = np.random.default_rng(n + d).random((n, d), dtype=np.float32)
+    f, cv = O.eval_dtlz1(x, m)
+    ref, _ = _dtlz1_fp64(x, m)
+    np.testing.assert_allclose(f, ref, rtol=2e-5, atol=1e-5)
+    assert not cv.any()
+
+
+def test_dtlz1_front_and_c1_constraint():
+    """On the Pareto front (tail variables 0.5) the objectives sum to 0.5 and
+    C1-DTLZ1 is feasible there; at the corner x1 = x2 = 1 (f = (0.5(1+g), 0, 0))
+    its violation is g, at x1 = 0 (f = (0, 0, 0.5(1+g))) it is
+    max(0.5(1+g)/0.6 - 1, 0) (Jain and Deb 2014: the feasible band between the
+    front and the plane through 0.5 on the first M-1 axes and 0.6 on the last)."""
+    rng = np.random.default_rng(3)
+    n, d, m = 256, 10, 3
+    x = rng.random((n, d), dtype=np.float32)
+    x[:, m - 1:] = 0.5
+    f, cv = O.eval_dtlz1(x, m, constrained=True)
+    np.testing.assert_allclose(f.sum(0), 0.5, rtol=1e-6)
+    assert not cv.any()
+    x = rng.random((n, d), dtype=np.float32) * np.float32(0.02) + np.float32(0.49)
+    x[: n // 2, :2] = 1.0
+    x[n // 2:, 0] = 0.0
+    f, cv = O.eval_dtlz1(x, m, constrained=True)
+    _, g = _dtlz1_fp64(x, m)
+    np.testing.assert_allclose(cv[: n // 2], g[: n // 2], rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(cv[n // 2:], np.maximum(0.5 * (1 + g[n // 2:]) / 0.6 - 1, 0), rtol=1e-4,
+                               atol=1e-4)
+    assert (cv[: n // 2] > 0).all()
+
+
 def test_acosf_close_to_fp64():
     for x in np.linspace(0, 1, 5001, dtype=np.float32):
         exact = math.acos(float(x))
