@@ -20,7 +20,7 @@
 namespace moea {
 
 constexpr int kBatchPeelStateWords = 1024 + 64;   // per instance: peel group barrier flags and totals
-constexpr int kBatchPeelMaxLds = 72 * 1024;        // LDS bound of one batched peel block
+constexpr int kBatchPeelMaxLds = 80 * 1024;        // LDS bound of one batched peel block
 
 struct BInst {
     // lifecycle: 0 free, 1 running, 2 admitted (initialization pending)

@@ -142,15 +142,18 @@ __device__ __forceinline__ void block_info_one(const uint64_t* __restrict__ keys
 }
 
 // zero (optional, `words` of them): cleared on the way, for a later launch;
-// mf_init (optional): set to -1, the value the peel replaces with its result.
+// mf_init (optional): set to -1, the value the peel replaces with its result;
+// rend (optional, one per padded row, 64-thread blocks): set to 0, the row
+// ends the dominance kernel raises.
 __global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
                                int32_t* __restrict__ hi, uint32_t* __restrict__ zero, int words,
-                               int32_t* __restrict__ mf_init)
+                               int32_t* __restrict__ mf_init, int32_t* __restrict__ rend)
 {
     for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < words;
          j += static_cast<int>(gridDim.x * blockDim.x))
         zero[j] = 0u;
     if (mf_init && blockIdx.x == 0 && threadIdx.x == 0) mf_init[0] = -1;
+    if (rend) rend[blockIdx.x * 64 + threadIdx.x] = 0;
     block_info_one(keys, n, static_cast<int>(blockIdx.x), kind, hi);
 }
 
@@ -189,7 +192,8 @@ __device__ __forceinline__ uint64_t tile_fast(const float* __restrict__ fs, int 
 template <int M>
 __device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
                                         int npad, int n, int qb, int pb, const float (&pv)[M],
-                                        float cp, bool backward, uint64_t* __restrict__ B, int Wc)
+                                        float cp, bool backward, uint64_t* __restrict__ B, int Wc,
+                                        int32_t* __restrict__ rend)
 {
     const int lane = lane_id();
     const int p = pb * 64 + lane;
@@ -228,7 +232,10 @@ __device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, c
     }, std::make_integer_sequence<int, 64>{});
     if (backward) {
         const int q = q0 + lane;
-        if (q < n) B[bword_index(q, pb, Wc)] = (static_cast<uint64_t>(col_hi) << 32) | col_lo;
+        if (q < n) {
+            B[bword_index(q, pb, Wc)] = (static_cast<uint64_t>(col_hi) << 32) | col_lo;
+            if (rend && (col_hi | col_lo)) atomicMax(&rend[q], pb + 1);
+        }
     }
     return (static_cast<uint64_t>(acc_hi) << 32) | acc_lo;
 }
@@ -249,11 +256,13 @@ __device__ __forceinline__ void decode_unit(long long u, int& pb, int& qc)
 // block pb, accumulated in registers and written as one 64-byte segment per
 // lane.  In the diagonal chunk only words <= pb are written; words past pb
 // belong to tied tiles, whose backward direction the wave of the later block
-// stores.
+// stores.  rend (optional): each row's end, one past its last non-zero word,
+// raised with atomicMax from every unit that holds a non-zero word of it.
 template <int M>
 __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const float* __restrict__ cvs, int npad,
                                 int n, int W, int Wc, const uint8_t* __restrict__ kind,
-                                const int32_t* __restrict__ hi, uint64_t* __restrict__ B, long long u)
+                                const int32_t* __restrict__ hi, uint64_t* __restrict__ B, long long u,
+                                int32_t* __restrict__ rend = nullptr)
 {
     int pb, qc;
     decode_unit(u, pb, qc);
@@ -276,9 +285,9 @@ __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const flo
         if (qb < pb) {
             const bool tied = ld_uniform(hi, qb) >= pb;
             if (!tied && p_plain && ld_uniform(kind, qb) == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
-            else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc);
+            else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc, rend);
         } else if (qb == pb) {
-            word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc);
+            word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
         }
         // rotate: after 8 steps w0..w7 hold tiles 0..7
         w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7; w7 = word;
@@ -301,18 +310,22 @@ __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const flo
         if (last >= 6) dst[6] = w6;
         if (last >= 7) dst[7] = w7;
     }
+    if (rend && p < n) {
+        const int top = w7 ? 8 : w6 ? 7 : w5 ? 6 : w4 ? 5 : w3 ? 4 : w2 ? 3 : w1 ? 2 : w0 ? 1 : 0;
+        if (top) atomicMax(&rend[p], qc * 8 + top);
+    }
 }
 
 template <int M>
 __global__ __launch_bounds__(kDomThreads) void nds_dominance(
     const float* __restrict__ fs, const float* __restrict__ cvs, int npad, int n, int W, int Wc,
     const uint8_t* __restrict__ kind, const int32_t* __restrict__ hi, uint64_t* __restrict__ B,
-    long long units)
+    long long units, int32_t* __restrict__ rend)
 {
     const int wave = wave_uniform(static_cast<int>(threadIdx.x >> 6));
     const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
     if (u >= units) return;
-    dom_unit<M>(fs, cvs, npad, n, W, Wc, kind, hi, B, u);
+    dom_unit<M>(fs, cvs, npad, n, W, Wc, kind, hi, B, u, rend);
 }
 
 // Peel state (device), u32 words: [0, kBarWords) group barrier,
@@ -336,9 +349,10 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
 //      lanes a row, 64 words a round trip) and scan each on until it meets
 //      its next blocking word (the row keeps (c, rem)) or runs out of words
 //      (front k).
-// Rows, cursors and tasks never leave their workgroup; only the assigned set
-// and the total cross workgroups, at the one grid barrier per round.  Over
-// all rounds a row reads each of its words about once.
+// Rows, cursors, fronts and tasks never leave their workgroup (the fronts go
+// out once, when the peel stops); only the assigned set and the total cross
+// workgroups, at the one grid barrier per round.  Over all rounds a row reads
+// each of its words about once.
 //
 // Everything the workgroups exchange inside the launch (barrier words,
 // totals, the assigned sets) is accessed with device-scope atomics, which the
@@ -368,14 +382,16 @@ static_assert(kPeelStateWords <= kBatchPeelStateWords, "batched peel state too s
 constexpr int kPeelThreads = 1024;
 constexpr int kPeelWaves = kPeelThreads / 64;
 constexpr int kCheckChunks = 2;     // phase A: chunks read in one batch from the cursor's chunk
-constexpr int kScanLoads = 8;       // phase B: words per lane in flight (64 per row)
+constexpr int kScanLoads = 8;       // phase B: words per lane in flight (8 per lane of a row)
+constexpr int kMinLanes = 8;        // phase B: fewest lanes per row (64 words a trip)
 
 // LDS layout of the peel (dynamic): s_A (W u64), s_next (bpg u64), s_task
-// (64 bpg u64), s_rem (64 bpg u64), s_hi (W int), s_cur (64 bpg int).
+// (2 x 64 bpg u64), s_rem (64 bpg u64), s_cur, s_front and s_end (64 bpg int
+// each).
 __host__ __device__ __forceinline__ size_t peel_lds_bytes(int W, int bpg)
 {
     const size_t rows = static_cast<size_t>(bpg) * 64;
-    return static_cast<size_t>(W) * 12 + static_cast<size_t>(bpg) * 8 + rows * 20;
+    return static_cast<size_t>(W) * 8 + static_cast<size_t>(bpg) * 8 + rows * 36;
 }
 
 __device__ __forceinline__ uint64_t chunk_word(const ulonglong2 (&v)[4], int i)
@@ -426,60 +442,55 @@ __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int W
     queued = !ready && (cur == stop);
 }
 
-// Phase B: eight queued rows per wave, eight lanes each.  Lane sl of group gi
-// reads words base + 8i + sl (i < kScanLoads, base = c rounded down to a
-// multiple of 8) of its row: 64 words a round trip, each 8-word run one
-// 64-byte segment; words before c or from end on count as clear.  A group
-// loops until its row meets a word holding an unassigned dominator (returned
-// in c, its unassigned bits in rem, the same in every lane of the group) or
-// runs out of words (c = end).  Groups without a row pass done = true.
-__device__ __forceinline__ void peel_scan_groups(const uint64_t* __restrict__ B, int Wc, int pr, int& c, int end,
-                                                 const uint64_t* s_A, int lane, bool done, uint64_t& rem)
+// Phase B, one trip: groups of L lanes (L = 8, 16, 32 or 64) of a wave, one
+// queued row each.  Lane sl of group gi reads words base + iL + sl
+// (i < kScanLoads, base = c rounded down to a multiple of 8) of its row: 8L
+// words a trip, each 8-word run one 64-byte segment; words before c or from
+// end on count as clear.  Returns true when the row meets a word holding an
+// unassigned dominator (its index in c, its unassigned bits in rem, the same
+// in every lane of the group); otherwise c moves past the words read (to end
+// when they run out).  Every lane of the wave calls it; lanes of a group
+// without a row pass has = false.
+__device__ __forceinline__ bool peel_trip(const uint64_t* __restrict__ B, int Wc, int pr, int& c, int end,
+                                          const uint64_t* s_A, int lane, int L, bool has, uint64_t& rem)
 {
-    const int gi = lane >> 3, sl = lane & 7;
-    rem = 0;
-    while (ballot(!done)) {
-        const int base = c & ~7;
-        uint64_t x[kScanLoads];
+    const int sl = lane & (L - 1);
+    const int shift = lane - sl;
+    const uint64_t gmask = (L == 64) ? ~0ull : ((1ull << L) - 1);
+    const int base = c & ~7;
+    uint64_t x[kScanLoads];
 #pragma unroll
-        for (int i = 0; i < kScanLoads; ++i) {
-            const int w = base + i * 8 + sl;
-            x[i] = (!done && w >= c && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
-        }
+    for (int i = 0; i < kScanLoads; ++i) {
+        const int w = base + i * L + sl;
+        x[i] = (has && w >= c && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
+    }
 #pragma unroll
-        for (int i = 0; i < kScanLoads; ++i) {
-            const int w = base + i * 8 + sl;
-            x[i] = (!done && w >= c && w < end) ? (x[i] & ~s_A[w]) : 0ull;
-        }
-        // first blocking word of this lane's group: the lowest (i, sl)
-        int first = -1;
-        uint64_t hit = 0;
+    for (int i = 0; i < kScanLoads; ++i) {
+        const int w = base + i * L + sl;
+        x[i] = (has && w >= c && w < end) ? (x[i] & ~s_A[w]) : 0ull;
+    }
+    // first blocking word of this lane's group: the lowest (i, sl)
+    int first = -1;
+    uint64_t hit = 0;
 #pragma unroll
-        for (int i = kScanLoads - 1; i >= 0; --i) {
-            const uint32_t gm = static_cast<uint32_t>(ballot(x[i] != 0) >> (8 * gi)) & 0xFFu;
-            if (gm) {
-                first = i * 8 + __builtin_ctz(gm);
-                hit = x[i];
-            }
-        }
-        // the blocking word's bits, from the lane that read it
-        const int src = gi * 8 + (first & 7);
-        const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(hit), src));
-        const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(hit >> 32), src));
-        if (!done) {
-            if (first >= 0) {
-                c = base + first;
-                rem = (static_cast<uint64_t>(hi) << 32) | lo;
-                done = true;
-            } else {
-                c = base + 8 * kScanLoads;
-                if (c >= end) {
-                    c = end;
-                    done = true;
-                }
-            }
+    for (int i = kScanLoads - 1; i >= 0; --i) {
+        const uint64_t gm = (ballot(x[i] != 0) >> shift) & gmask;
+        if (gm) {
+            first = i * L + __builtin_ctzll(gm);
+            hit = x[i];
         }
     }
+    // the blocking word's bits, from the lane that read it
+    const int src = shift + (first & (L - 1));
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(hit), src));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(hit >> 32), src));
+    if (first >= 0) {
+        c = base + first;
+        rem = (static_cast<uint64_t>(hi) << 32) | lo;
+        return true;
+    }
+    c = min(base + kScanLoads * L, end);
+    return false;
 }
 
 // Records a failed bounded wait: the run's front count becomes kPeelFailed
@@ -500,24 +511,26 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
     const int bpg = P.blocks_per_group;
     uint64_t* s_A = s_dyn;                                          // (W) assigned before this round
     uint64_t* s_next = s_A + nblk;                                  // (bpg) next masks of own blocks
-    uint64_t* s_task = s_next + bpg;                                // (64 bpg) queued (cursor << 32 | row)
-    uint64_t* s_rem = s_task + static_cast<size_t>(bpg) * 64;       // (64 bpg) unassigned bits of the blocking word
-    int* s_hi = reinterpret_cast<int*>(s_rem + static_cast<size_t>(bpg) * 64);   // (W)
-    int* s_cur = s_hi + nblk;                                       // (64 bpg) blocking word
-    __shared__ int s_ntask;
+    const int rows = bpg * 64;
+    uint64_t* s_task = s_next + bpg;                                // (2 x 64 bpg) queued (cursor << 32 | row)
+    uint64_t* s_rem = s_task + 2 * static_cast<size_t>(rows);       // (64 bpg) unassigned bits of the blocking word
+    int* s_cur = reinterpret_cast<int*>(s_rem + rows);              // (64 bpg) blocking word
+    int* s_front = s_cur + rows;                                    // (64 bpg) front of each owned row
+    int* s_end = s_front + rows;                                    // (64 bpg) words 0 .. end-1 can hold dominators
+    __shared__ int s_nq[3];         // queue lengths, rotating over the sweeps of phase B
     __shared__ int s_count;
     const int lane = lane_id();
     const int lw = wave_uniform(static_cast<int>(threadIdx.x >> 6));
     uint32_t phase = 0;
 
-    for (int w = threadIdx.x; w < nblk; w += kPeelThreads) s_hi[w] = P.hi[w];
     for (int j = lw; j < bpg; j += kPeelWaves) {
         const int rb = g + j * G;
         s_cur[j * 64 + lane] = 0;
         s_rem[j * 64 + lane] = 0ull;
+        s_front[j * 64 + lane] = -1;
         if (rb >= nblk) continue;
         const int p = rb * 64 + lane;
-        if (p < P.n) P.fronts[P.perm[p]] = -1;
+        s_end[j * 64 + lane] = (P.rend && p < P.n) ? P.rend[p] : P.hi[rb] + 1;
         if (lane == 0) { st_relaxed_u64(&P.A0[rb], 0ull); st_relaxed_u64(&P.A1[rb], 0ull); }
     }
     if (!coherent_barrier(P.state, g, G, phase, P.spin_limit)) { peel_fail(P); return false; }
@@ -527,7 +540,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         const uint64_t* Acur = (k & 1) ? P.A1 : P.A0;
         uint64_t* Anext = (k & 1) ? P.A0 : P.A1;
         for (int w = threadIdx.x; w < nblk; w += kPeelThreads) s_A[w] = ld_relaxed_u64(&Acur[w]);
-        if (threadIdx.x == 0) { s_count = 0; s_ntask = 0; }
+        if (threadIdx.x == 0) { s_count = 0; s_nq[0] = 0; s_nq[1] = 0; }
         __syncthreads();
 
         // ---- phase A: lane per row
@@ -538,8 +551,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
             const uint64_t amask = s_A[rb];
             const int p = rb * 64 + lane;
             const int slot = j * 64 + lane;
-            // words 0 .. end-1 can hold dominators
-            const int end = (P.rend && p < P.n) ? P.rend[p] : s_hi[rb] + 1;
+            const int end = s_end[slot];
             const bool live = (p < P.n) && !((amask >> lane) & 1ull);
             int cur = end;
             bool ready = false, queued = false;
@@ -559,10 +571,10 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
             }
             const uint64_t ready_m = ballot(ready);
             const uint64_t task_m = ballot(queued);
-            if (ready) P.fronts[P.perm[p]] = k;
+            if (ready) s_front[slot] = k;
             if (task_m) {
                 int base = 0;
-                if (lane == 0) base = atomicAdd(&s_ntask, __builtin_popcountll(task_m));
+                if (lane == 0) base = atomicAdd(&s_nq[0], __builtin_popcountll(task_m));
                 base = __builtin_amdgcn_readfirstlane(base);
                 if (queued) {
                     const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
@@ -576,30 +588,47 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                 my_new += __builtin_popcountll(ready_m);
             }
         }
-        __syncthreads();
 
-        // ---- phase B: the queued rows, eight a wave, each until it blocks or ends
-        const int ntask = s_ntask;
-        for (int t0 = lw * 8; t0 < ntask; t0 += kPeelWaves * 8) {
-            const int t = t0 + (lane >> 3);
-            const bool has = t < ntask;
-            const uint64_t rec = has ? s_task[t] : 0ull;
-            const int pr = static_cast<int>(static_cast<uint32_t>(rec));
-            const int rb = pr >> 6;
-            const int end = !has ? 0 : (P.rend ? P.rend[pr] : s_hi[rb] + 1);
-            const int slot = ((rb - g) / G) * 64 + (pr & 63);
-            int c = static_cast<int>(rec >> 32);
-            uint64_t rem;
-            peel_scan_groups(P.B, P.Wc, pr, c, end, s_A, lane, !has, rem);
-            if (has && (lane & 7) == 0) {
-                if (c >= end) {
-                    P.fronts[P.perm[pr]] = k;
-                    atomicOr(reinterpret_cast<unsigned long long*>(&s_next[(rb - g) / G]),
-                             static_cast<unsigned long long>(1ull << (pr & 63)));
-                    ++my_new;
-                } else {
-                    s_cur[slot] = c;
-                    s_rem[slot] = rem;
+        // ---- phase B: the queued rows, in sweeps of one trip per row, each
+        // row until it blocks or runs out of words.  Sweep s takes queue s % 2
+        // (length s_nq[s % 3]) and queues the rows still clear in the other;
+        // the fewer rows are left, the more lanes each gets (8 lanes for more
+        // than 64 rows, up to a wave for 16 or fewer), so long scans take few
+        // trips.
+        for (int sw = 0;; ++sw) {
+            __syncthreads();
+            const int nq = s_nq[sw % 3];
+            if (nq == 0) break;
+            if (threadIdx.x == 0) s_nq[(sw + 2) % 3] = 0;
+            int L = 64;
+            while (L > kMinLanes && nq > kPeelWaves * (64 / L)) L >>= 1;
+            const int gpw = 64 / L;
+            const uint64_t* q_in = s_task + (sw & 1) * static_cast<size_t>(rows);
+            uint64_t* q_out = s_task + ((sw + 1) & 1) * static_cast<size_t>(rows);
+            for (int t0 = lw * gpw; t0 < nq; t0 += kPeelWaves * gpw) {
+                const int t = t0 + lane / L;
+                const bool has = t < nq;
+                const uint64_t rec = has ? q_in[t] : 0ull;
+                const int pr = static_cast<int>(static_cast<uint32_t>(rec));
+                const int rb = pr >> 6;
+                const int slot = ((rb - g) / G) * 64 + (pr & 63);
+                const int end = has ? s_end[slot] : 0;
+                int c = static_cast<int>(rec >> 32);
+                uint64_t rem = 0;
+                const bool blocked = peel_trip(P.B, P.Wc, pr, c, end, s_A, lane, L, has, rem);
+                if (has && (lane & (L - 1)) == 0) {
+                    if (blocked) {
+                        s_cur[slot] = c;
+                        s_rem[slot] = rem;
+                    } else if (c >= end) {
+                        s_front[slot] = k;
+                        atomicOr(reinterpret_cast<unsigned long long*>(&s_next[(rb - g) / G]),
+                                 static_cast<unsigned long long>(1ull << (pr & 63)));
+                        ++my_new;
+                    } else {
+                        q_out[atomicAdd(&s_nq[(sw + 1) % 3], 1)] =
+                            (static_cast<uint64_t>(c) << 32) | static_cast<uint32_t>(pr);
+                    }
                 }
             }
         }
@@ -628,6 +657,11 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                 __hip_atomic_compare_exchange_strong(P.out_max_front, &expected,
                                                      (total == total_prev) ? k - 1 : k, __ATOMIC_RELAXED,
                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            // the owned rows' fronts (-1: not assigned), in input order
+            for (int j = lw; j < bpg; j += kPeelWaves) {
+                const int p = (g + j * G) * 64 + lane;
+                if (p < P.n) P.fronts[P.perm[p]] = s_front[j * 64 + lane];
             }
             return true;
         }
@@ -820,6 +854,7 @@ struct NdsLayout {
     uint64_t* A0;
     uint64_t* A1;
     uint32_t* state;
+    int32_t* rend;
 };
 
 size_t carve_nds(void* ws, int n, NdsLayout& L)
@@ -842,6 +877,7 @@ size_t carve_nds(void* ws, int n, NdsLayout& L)
     L.A0 = c.take<uint64_t>(W);
     L.A1 = c.take<uint64_t>(W);
     L.state = c.take<uint32_t>(kPeelStateWords);
+    L.rend = c.take<int32_t>(npad);
     return c.used;
 }
 
@@ -903,7 +939,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     MOEA_HIP(hipGetLastError());
     // also clears the peel's barrier state (not read before the peel)
     hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi, L.state,
-                       kPeelStateWords, max_front);
+                       kPeelStateWords, max_front, L.rend);
     MOEA_HIP(hipGetLastError());
     stage_end(kStNdsSort, stream);
 
@@ -912,7 +948,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     const dim3 dgrid(static_cast<unsigned>((units + kDomWaves - 1) / kDomWaves));
     const dim3 dblock(kDomThreads);
     MOEA_LAUNCH_M(m, nds_dominance, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
-                  L.hi, L.B, units);
+                  L.hi, L.B, units, L.rend);
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
@@ -920,7 +956,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
     const int bpg = ceil_div(W, grid);
     const size_t lds = peel_lds_bytes(W, bpg);
-    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.A0, L.A1, L.state, max_front, nullptr,
+    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.A0, L.A1, L.state, max_front, L.rend,
                   peel_spin_limit(), error};
     void* kargs[] = {&args};
     stage_begin(kStPeel, stream);
