@@ -189,8 +189,11 @@ __device__ __forceinline__ uint64_t tile_fast(const float* __restrict__ fs, int 
 // Any tile with the full constrained rule.  Row word of lane p over the rows q
 // of block qb (q < n, q != p); when `backward` is set, the words "p dominates
 // q" of the rows q are stored at word pb of their rows (tied tiles only).
+// Kept out of line: inlined, its 64 unrolled steps raised the SGPR count of
+// the dominance kernel past what fits without spilling, and the spill code
+// ran inside the fast tiles' loop; it runs only on diagonal and tied tiles.
 template <int M>
-__device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
+__device__ __attribute__((noinline)) uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
                                         int npad, int n, int qb, int pb, const float (&pv)[M],
                                         float cp, bool backward, uint64_t* __restrict__ B, int Wc,
                                         int32_t* __restrict__ rend)
@@ -417,7 +420,7 @@ __device__ __forceinline__ void peel_load_chunk(const uint64_t* rowbase, int c, 
 // ready: no dominator word left; queued: clear so far with words left;
 // otherwise blocked at the returned cur with the unassigned dominators of
 // that word in rem.
-__device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int rb, int lane, int end,
+__device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int Wc, int W, int rb, int lane, int end,
                                            const uint64_t* s_A, int& cur, bool& ready, bool& queued, uint64_t& rem)
 {
     const uint64_t* rowbase = peel_rowbase(B, Wc, rb, lane);
@@ -426,6 +429,11 @@ __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int W
 #pragma unroll
     for (int a = 0; a < kCheckChunks; ++a)
         if ((c + a) * 8 < end) peel_load_chunk(rowbase, c + a, v[a]);
+    // the assigned-set words of the same span, read while the row's words
+    // are in flight (indices clamped to the set; words past stop are unused)
+    uint64_t am[kCheckChunks * 8];
+#pragma unroll
+    for (int j = 0; j < kCheckChunks * 8; ++j) am[j] = s_A[min(c * 8 + j, W - 1)];
     const int stop = min((c + kCheckChunks) * 8, end);
     int blocked = stop;
 #pragma unroll
@@ -433,8 +441,8 @@ __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int W
 #pragma unroll
         for (int i = 7; i >= 0; --i) {
             const int ww = (c + a) * 8 + i;
-            const uint64_t x = (ww >= cur && ww < stop) ? (chunk_word(v[a], i) & ~s_A[ww]) : 0ull;
-            if (x != 0) { blocked = ww; rem = x; }
+            const uint64_t x = chunk_word(v[a], i) & ~am[a * 8 + i];
+            if (ww >= cur && ww < stop && x != 0) { blocked = ww; rem = x; }
         }
     }
     cur = blocked;
@@ -451,7 +459,7 @@ __device__ __forceinline__ void peel_check(const uint64_t* __restrict__ B, int W
 // in every lane of the group); otherwise c moves past the words read (to end
 // when they run out).  Every lane of the wave calls it; lanes of a group
 // without a row pass has = false.
-__device__ __forceinline__ bool peel_trip(const uint64_t* __restrict__ B, int Wc, int pr, int& c, int end,
+__device__ __forceinline__ bool peel_trip(const uint64_t* __restrict__ B, int Wc, int W, int pr, int& c, int end,
                                           const uint64_t* s_A, int lane, int L, bool has, uint64_t& rem)
 {
     const int sl = lane & (L - 1);
@@ -464,10 +472,14 @@ __device__ __forceinline__ bool peel_trip(const uint64_t* __restrict__ B, int Wc
         const int w = base + i * L + sl;
         x[i] = (has && w >= c && w < end) ? B[bword_index(pr, w, Wc)] : 0ull;
     }
+    // the assigned-set words, read while the row's words are in flight
+    uint64_t am[kScanLoads];
+#pragma unroll
+    for (int i = 0; i < kScanLoads; ++i) am[i] = s_A[min(base + i * L + sl, W - 1)];
 #pragma unroll
     for (int i = 0; i < kScanLoads; ++i) {
         const int w = base + i * L + sl;
-        x[i] = (has && w >= c && w < end) ? (x[i] & ~s_A[w]) : 0ull;
+        x[i] = (has && w >= c && w < end) ? (x[i] & ~am[i]) : 0ull;
     }
     // first blocking word of this lane's group: the lowest (i, sl)
     int first = -1;
@@ -563,7 +575,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                     // still blocked while dominators of its blocking word are unassigned
                     uint64_t rem = s_rem[slot] & ~s_A[cur];
                     if (rem == 0) {
-                        peel_check(P.B, P.Wc, rb, lane, end, s_A, cur, ready, queued, rem);
+                        peel_check(P.B, P.Wc, P.W, rb, lane, end, s_A, cur, ready, queued, rem);
                         s_cur[slot] = cur;
                     }
                     s_rem[slot] = rem;
@@ -615,7 +627,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
                 const int end = has ? s_end[slot] : 0;
                 int c = static_cast<int>(rec >> 32);
                 uint64_t rem = 0;
-                const bool blocked = peel_trip(P.B, P.Wc, pr, c, end, s_A, lane, L, has, rem);
+                const bool blocked = peel_trip(P.B, P.Wc, P.W, pr, c, end, s_A, lane, L, has, rem);
                 if (has && (lane & (L - 1)) == 0) {
                     if (blocked) {
                         s_cur[slot] = c;
