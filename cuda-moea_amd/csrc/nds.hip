@@ -844,6 +844,7 @@ __global__ void b_peel_reset(const BInst* __restrict__ insts, int mode, int stat
 
 
 constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
+constexpr int kRowEndMinM = 4;    // fewest objectives for which the sort computes row ends
 
 // Polls before a peel barrier gives up (kSpinLimit unless a test lowers it
 // with moea_debug_set_spin_limit to make the bounded wait give up on purpose).
@@ -950,8 +951,11 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
                        npad, L.perm, L.fs, L.cvs);
     MOEA_HIP(hipGetLastError());
     // also clears the peel's barrier state (not read before the peel)
+    // row ends pay for their atomics when many rows have no dominator near
+    // their own block, which is the case for many objectives
+    int32_t* rend = m >= kRowEndMinM ? L.rend : nullptr;
     hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi, L.state,
-                       kPeelStateWords, max_front, L.rend);
+                       kPeelStateWords, max_front, rend);
     MOEA_HIP(hipGetLastError());
     stage_end(kStNdsSort, stream);
 
@@ -960,7 +964,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     const dim3 dgrid(static_cast<unsigned>((units + kDomWaves - 1) / kDomWaves));
     const dim3 dblock(kDomThreads);
     MOEA_LAUNCH_M(m, nds_dominance, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
-                  L.hi, L.B, units, L.rend);
+                  L.hi, L.B, units, rend);
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
@@ -968,7 +972,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     if (grid <= 0) return fail(kUnsupported, "nds: peel kernel cannot be resident on this device");
     const int bpg = ceil_div(W, grid);
     const size_t lds = peel_lds_bytes(W, bpg);
-    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.A0, L.A1, L.state, max_front, L.rend,
+    PeelArgs args{L.B, L.hi, L.perm, n, W, Wc, stop_at, bpg, fronts, L.A0, L.A1, L.state, max_front, rend,
                   peel_spin_limit(), error};
     void* kargs[] = {&args};
     stage_begin(kStPeel, stream);
