@@ -189,11 +189,8 @@ __device__ __forceinline__ uint64_t tile_fast(const float* __restrict__ fs, int 
 // Any tile with the full constrained rule.  Row word of lane p over the rows q
 // of block qb (q < n, q != p); when `backward` is set, the words "p dominates
 // q" of the rows q are stored at word pb of their rows (tied tiles only).
-// Kept out of line: inlined, its 64 unrolled steps raised the SGPR count of
-// the dominance kernel past what fits without spilling, and the spill code
-// ran inside the fast tiles' loop; it runs only on diagonal and tied tiles.
 template <int M>
-__device__ __attribute__((noinline)) uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
+__device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, const float* __restrict__ cvs,
                                         int npad, int n, int qb, int pb, const float (&pv)[M],
                                         float cp, bool backward, uint64_t* __restrict__ B, int Wc,
                                         int32_t* __restrict__ rend)
@@ -243,6 +240,20 @@ __device__ __attribute__((noinline)) uint64_t tile_general(const float* __restri
     return (static_cast<uint64_t>(acc_hi) << 32) | acc_lo;
 }
 
+// The same, out of line.  Inlined, the 64 unrolled steps of tile_general raise
+// the SGPR count of the dominance kernel past what fits without spilling, and
+// the spill code runs inside the fast tiles' loop; out of line, the call costs
+// more than the spills where diagonal tiles are common (a few row blocks).
+template <int M>
+__device__ __attribute__((noinline)) uint64_t tile_general_call(const float* __restrict__ fs,
+                                                                const float* __restrict__ cvs, int npad, int n, int qb,
+                                                                int pb, const float (&pv)[M], float cp, bool backward,
+                                                                uint64_t* __restrict__ B, int Wc,
+                                                                int32_t* __restrict__ rend)
+{
+    return tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, backward, B, Wc, rend);
+}
+
 // Work unit u -> (row block pb, chunk qc), qc <= pb / 8.  Units are grouped by
 // a = pb / 8: group a holds 8 row blocks x (a + 1) chunks.
 __device__ __forceinline__ void decode_unit(long long u, int& pb, int& qc)
@@ -261,7 +272,7 @@ __device__ __forceinline__ void decode_unit(long long u, int& pb, int& qc)
 // belong to tied tiles, whose backward direction the wave of the later block
 // stores.  rend (optional): each row's end, one past its last non-zero word,
 // raised with atomicMax from every unit that holds a non-zero word of it.
-template <int M>
+template <int M, bool kOutline = false>
 __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const float* __restrict__ cvs, int npad,
                                 int n, int W, int Wc, const uint8_t* __restrict__ kind,
                                 const int32_t* __restrict__ hi, uint64_t* __restrict__ B, long long u,
@@ -288,9 +299,11 @@ __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const flo
         if (qb < pb) {
             const bool tied = ld_uniform(hi, qb) >= pb;
             if (!tied && p_plain && ld_uniform(kind, qb) == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
+            else if constexpr (kOutline) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc, rend);
             else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc, rend);
         } else if (qb == pb) {
-            word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
+            if constexpr (kOutline) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
+            else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
         }
         // rotate: after 8 steps w0..w7 hold tiles 0..7
         w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6; w6 = w7; w7 = word;
@@ -329,6 +342,20 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance(
     const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
     if (u >= units) return;
     dom_unit<M>(fs, cvs, npad, n, W, Wc, kind, hi, B, u, rend);
+}
+
+// The same for many row blocks, where nearly every tile is a fast one: the
+// general tiles out of line keep the fast loop free of spill code.
+template <int M>
+__global__ __launch_bounds__(kDomThreads) void nds_dominance_wide(
+    const float* __restrict__ fs, const float* __restrict__ cvs, int npad, int n, int W, int Wc,
+    const uint8_t* __restrict__ kind, const int32_t* __restrict__ hi, uint64_t* __restrict__ B,
+    long long units, int32_t* __restrict__ rend)
+{
+    const int wave = wave_uniform(static_cast<int>(threadIdx.x >> 6));
+    const long long u = static_cast<long long>(blockIdx.x) * kDomWaves + wave;
+    if (u >= units) return;
+    dom_unit<M, true>(fs, cvs, npad, n, W, Wc, kind, hi, B, u, rend);
 }
 
 // Peel state (device), u32 words: [0, kBarWords) group barrier,
@@ -845,6 +872,7 @@ __global__ void b_peel_reset(const BInst* __restrict__ insts, int mode, int stat
 
 constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
 constexpr int kRowEndMinM = 4;    // fewest objectives for which the sort computes row ends
+constexpr int kWideBlocks = 64;   // row blocks (n >= 4033) from which dominance uses nds_dominance_wide
 
 // Polls before a peel barrier gives up (kSpinLimit unless a test lowers it
 // with moea_debug_set_spin_limit to make the bounded wait give up on purpose).
@@ -963,8 +991,13 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     const long long units = 4ll * Wc * (Wc + 1);
     const dim3 dgrid(static_cast<unsigned>((units + kDomWaves - 1) / kDomWaves));
     const dim3 dblock(kDomThreads);
-    MOEA_LAUNCH_M(m, nds_dominance, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
-                  L.hi, L.B, units, rend);
+    if (W >= kWideBlocks) {
+        MOEA_LAUNCH_M(m, nds_dominance_wide, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
+                      L.hi, L.B, units, rend);
+    } else {
+        MOEA_LAUNCH_M(m, nds_dominance, dgrid, dblock, 0, stream, L.fs, L.cvs, npad, n, W, Wc, L.kind,
+                      L.hi, L.B, units, rend);
+    }
     MOEA_HIP(hipGetLastError());
     stage_end(kStDominance, stream);
 
