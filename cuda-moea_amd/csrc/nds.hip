@@ -244,14 +244,33 @@ __device__ __forceinline__ uint64_t tile_general(const float* __restrict__ fs, c
 // the SGPR count of the dominance kernel past what fits without spilling, and
 // the spill code runs inside the fast tiles' loop; out of line, the call costs
 // more than the spills where diagonal tiles are common (a few row blocks).
+// The row's objectives go by value, in registers (by reference they would be
+// stored to scratch in every unit).  Up to kOutlineMaxM objectives: from 7 the
+// out-of-line form lowers the occupancy and the inlined one measured faster
+// (M = 8, n = 65536: 0.97 against 1.00 ms).
+constexpr int kOutlineMaxM = 6;
+template <int M>
+struct RowObjectives {
+    float v[M];
+};
+
 template <int M>
 __device__ __attribute__((noinline)) uint64_t tile_general_call(const float* __restrict__ fs,
                                                                 const float* __restrict__ cvs, int npad, int n, int qb,
-                                                                int pb, const float (&pv)[M], float cp, bool backward,
+                                                                int pb, RowObjectives<M> pv, float cp, bool backward,
                                                                 uint64_t* __restrict__ B, int Wc,
                                                                 int32_t* __restrict__ rend)
 {
-    return tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, backward, B, Wc, rend);
+    return tile_general<M>(fs, cvs, npad, n, qb, pb, pv.v, cp, backward, B, Wc, rend);
+}
+
+template <int M>
+__device__ __forceinline__ RowObjectives<M> row_pv(const float (&pv)[M])
+{
+    RowObjectives<M> r;
+#pragma unroll
+    for (int k = 0; k < M; ++k) r.v[k] = pv[k];
+    return r;
 }
 
 // Work unit u -> (row block pb, chunk qc), qc <= pb / 8.  Units are grouped by
@@ -299,10 +318,10 @@ __device__ __forceinline__ void dom_unit(const float* __restrict__ fs, const flo
         if (qb < pb) {
             const bool tied = ld_uniform(hi, qb) >= pb;
             if (!tied && p_plain && ld_uniform(kind, qb) == kPlain) word = tile_fast<M>(fs, npad, qb * 64, pv);
-            else if constexpr (kOutline) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc, rend);
+            else if constexpr (kOutline && M <= kOutlineMaxM) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, row_pv(pv), cp, tied, B, Wc, rend);
             else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, tied, B, Wc, rend);
         } else if (qb == pb) {
-            if constexpr (kOutline) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
+            if constexpr (kOutline && M <= kOutlineMaxM) word = tile_general_call<M>(fs, cvs, npad, n, qb, pb, row_pv(pv), cp, false, B, Wc, rend);
             else word = tile_general<M>(fs, cvs, npad, n, qb, pb, pv, cp, false, B, Wc, rend);
         }
         // rotate: after 8 steps w0..w7 hold tiles 0..7
@@ -630,10 +649,12 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
 
         // ---- phase B: the queued rows, in sweeps of one trip per row, each
         // row until it blocks or runs out of words.  Sweep s takes queue s % 2
-        // (length s_nq[s % 3]) and queues the rows still clear in the other;
-        // the fewer rows are left, the more lanes each gets (8 lanes for more
-        // than 64 rows, up to a wave for 16 or fewer), so long scans take few
-        // trips.
+        // (length s_nq[s % 3]) and queues the rows still clear in the other.
+        // A row gets 8 lanes (64 words a trip) in the first sweep and twice as
+        // many in each later one, up to a wave, as far as the queue leaves
+        // lanes free (8 lanes for more than 64 rows, a wave for 16 or fewer):
+        // long scans take few trips, and a row reads at most about twice the
+        // words it needs.
         for (int sw = 0;; ++sw) {
             __syncthreads();
             const int nq = s_nq[sw % 3];
@@ -641,6 +662,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
             if (threadIdx.x == 0) s_nq[(sw + 2) % 3] = 0;
             int L = 64;
             while (L > kMinLanes && nq > kPeelWaves * (64 / L)) L >>= 1;
+            L = min(L, kMinLanes << min(sw, 3));
             const int gpw = 64 / L;
             const uint64_t* q_in = s_task + (sw & 1) * static_cast<size_t>(rows);
             uint64_t* q_out = s_task + ((sw + 1) & 1) * static_cast<size_t>(rows);
