@@ -394,14 +394,16 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance_wide(
 //      one batch.  Rows whose dominator words run out are front k; rows still
 //      clear with words left go to the workgroup's task queue in LDS as
 //      (row, c).
-//   B  the workgroup's 16 waves take the queued rows eight at a time (eight
-//      lanes a row, 64 words a round trip) and scan each on until it meets
-//      its next blocking word (the row keeps (c, rem)) or runs out of words
+//   B  the workgroup's 16 waves scan the queued rows in sweeps of one round
+//      trip per row (8 lanes and 64 words a row in the first sweep, twice as
+//      many in each later one while lanes are free), each until it meets its
+//      next blocking word (the row keeps (c, rem)) or runs out of words
 //      (front k).
 // Rows, cursors, fronts and tasks never leave their workgroup (the fronts go
 // out once, when the peel stops); only the assigned set and the total cross
 // workgroups, at the one grid barrier per round.  Over all rounds a row reads
-// each of its words about once.
+// each of its words about once, plus what a batch reads past its blocking
+// word.
 //
 // Everything the workgroups exchange inside the launch (barrier words,
 // totals, the assigned sets) is accessed with device-scope atomics, which the
