@@ -107,6 +107,29 @@ def test_nds_duplicates_and_segments():
         np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
 
 
+@pytest.mark.parametrize("m", [3, 5, 8])
+def test_nds_wide_sorts_with_ties_constraints_and_nan(m):
+    """Sorts of 64 row blocks and more (the dominance kernel with its general
+    tiles out of line up to six objectives, row ends from four): f0 on few
+    levels so tie groups cross blocks, infeasible rows with equal and NaN cv,
+    NaN and infinite objectives."""
+    rng = np.random.default_rng(40 + m)
+    n = 6000
+    f = rng.random((m, n), dtype=np.float32)
+    f[0] = rng.integers(0, 30, n).astype(np.float32) / np.float32(30)
+    f[1, rng.random(n) < 0.2] = 0.5
+    f[:, rng.random(n) < 0.01] = np.nan
+    f[m - 1, rng.random(n) < 0.01] = np.inf
+    cv = np.where(rng.random(n) < 0.2, rng.integers(1, 5, n).astype(np.float32) * 0.25, 0.0).astype(np.float32)
+    cv[rng.random(n) < 0.01] = np.nan
+    for stop in (n, n // 3):
+        print(f"nds wide case n={n} m={m} stop={stop}", flush=True)
+        fr_o, mf_o = O.nds(f, cv, stop)
+        fr_g, mf_g = K.nds_rank(_f(f), _f(cv), stop)
+        assert mf_g == mf_o
+        np.testing.assert_array_equal(fr_g.cpu().numpy(), fr_o)
+
+
 def test_nds_larger_random():
     rng = np.random.default_rng(31)
     n, m = 20000, 3
