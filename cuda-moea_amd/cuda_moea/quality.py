@@ -151,12 +151,16 @@ def _hv2(pts: np.ndarray, ref: np.ndarray) -> float:
     return float(np.sum((nxt - x) * (ref[1] - y)))
 
 
-def hypervolume(points, reference_point) -> float:
-    """Exact volume dominated by `points` (minimisation) and bounded by
-    `reference_point`, for two or three objectives.  Rows not strictly better
-    than the reference point in every objective contribute nothing."""
+def hypervolume(points, reference_point, maximize: bool = False) -> float:
+    """Exact volume dominated by `points` (minimisation, or maximisation with
+    `maximize`) and bounded by `reference_point`, for two or three objectives
+    (the exact method of tests/evaluation/metrics.py's hypervolume).  Rows not
+    strictly better than the reference point in every objective contribute
+    nothing."""
     pts = np.asarray(points, np.float64)
     ref = np.asarray(reference_point, np.float64)
+    if maximize:
+        pts, ref = -pts, -ref
     if pts.ndim != 2 or ref.shape != (pts.shape[1],) or not np.isfinite(ref).all():
         raise ValueError("reference_point must hold one finite value per objective")
     pts = pts[np.isfinite(pts).all(axis=1)]

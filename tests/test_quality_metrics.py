@@ -67,6 +67,18 @@ def test_hypervolume_closed_forms():
         Q.hypervolume(np.zeros((2, 4)), np.ones(4))
 
 
+def test_hypervolume_reference_known_answers():
+    """The known answers of the reference's hypervolume tests
+    (tests/python/test_evaluation_metrics.py:26-58): one point, a union with a
+    dominated point, points outside the reference ignored, maximisation, and an
+    exact three-objective box."""
+    assert Q.hypervolume([[1.0, 2.0]], [3.0, 4.0]) == pytest.approx(4.0)
+    assert Q.hypervolume([[1.0, 2.0], [2.0, 1.0], [2.5, 2.5]], [3.0, 3.0]) == pytest.approx(3.0)
+    assert Q.hypervolume([[1.0, 1.0], [4.0, 0.0]], [3.0, 3.0]) == pytest.approx(4.0)
+    assert Q.hypervolume([[2.0, 1.0], [1.0, 2.0]], [0.0, 0.0], maximize=True) == pytest.approx(3.0)
+    assert Q.hypervolume([[1.0, 1.0, 1.0]], [3.0, 4.0, 5.0]) == pytest.approx(24.0)
+
+
 def test_igd_identities():
     ref = Q.pareto_front_points("DTLZ2", 3, 2000)
     assert Q.igd(ref, ref) == 0.0
