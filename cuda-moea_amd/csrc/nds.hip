@@ -49,6 +49,7 @@ namespace moea {
 
 namespace {
 
+constexpr int kRowEndMinM = 4;           // fewest objectives for which the sort computes row ends
 constexpr int kDomWaves = 4;             // waves (work units) per dominance workgroup
 constexpr int kDomThreads = 64 * kDomWaves;
 constexpr int kKeyBits = 34;             // segment (2 bits) above a 32-bit order key
@@ -804,6 +805,9 @@ __global__ void b_nds_block_info(const BInst* __restrict__ insts, BMap mp, int m
     const BInst& I = insts[i];
     const BRows R = brows(I, mode);
     block_info_one(keys_sorted + R.key_off, R.rows, local, I.kind, I.hi);
+    // row ends start at 0; the dominance kernel raises them (four or more objectives)
+    const int row = local * 64 + static_cast<int>(threadIdx.x);
+    if (I.m >= kRowEndMinM && row < R.rows) I.rend[row] = 0;
 }
 
 template <int M>
@@ -818,36 +822,7 @@ __global__ __launch_bounds__(kDomThreads) void b_nds_dominance(const BInst* __re
     const long long units = 4ll * Wc * (Wc + 1);
     const long long u = static_cast<long long>(local) * kDomWaves + wave_uniform(static_cast<int>(threadIdx.x >> 6));
     if (u >= units) return;
-    dom_unit<M>(I.fs, I.cvs, W * 64, R.rows, W, Wc, I.kind, I.hi, I.B, u);
-}
-
-// Row ends: for each sorted row, one past the last word of its dominator
-// mask that is non-zero (0 when nothing dominates it).  Bits beyond it are
-// all clear, so the peel checks and scans stop there; for many objectives
-// most rows are dominated by few rows or none.  One wave per row block, lane
-// = row, the block's chunks read in order.
-__global__ __launch_bounds__(64) void b_nds_rowend(const BInst* __restrict__ insts, BMap mp, int mode)
-{
-    int rb;
-    const int i = bmap_find(mp, static_cast<int>(blockIdx.x), rb);
-    const BInst& I = insts[i];
-    const BRows R = brows(I, mode);
-    const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
-    const int lane = static_cast<int>(threadIdx.x);
-    const int p = rb * 64 + lane;
-    const int end = I.hi[rb] + 1;
-    const uint64_t* rowbase = peel_rowbase(I.B, Wc, rb, lane);
-    int last = -1;
-    for (int c = 0; c * 8 < end; ++c) {
-        ulonglong2 v[4];
-        peel_load_chunk(rowbase, c, v);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int w = c * 8 + j;
-            if (w < end && chunk_word(v, j) != 0ull) last = w;
-        }
-    }
-    if (p < R.rows) I.rend[p] = last + 1;
+    dom_unit<M>(I.fs, I.cvs, W * 64, R.rows, W, Wc, I.kind, I.hi, I.B, u, M >= kRowEndMinM ? I.rend : nullptr);
 }
 
 // Where a batched peel leaves the index of its last front: the selection's
@@ -876,7 +851,8 @@ __global__ __launch_bounds__(kPeelThreads) void b_nds_peel_groups(const BInst* _
         const BRows R = brows(I, mode);
         const int W = (R.rows + 63) / 64, Wc = (W + 7) / 8;
         PeelArgs P{I.B, I.hi, I.perm, R.rows, W, Wc, mode == 1 ? I.n : R.rows, (W + G - 1) / G, R.fronts,
-                   I.A, I.A + W, I.pstate, peel_max_front(I, mode), I.rend, spin_limit, error};
+                   I.A, I.A + W, I.pstate, peel_max_front(I, mode), I.m >= kRowEndMinM ? I.rend : nullptr,
+                   spin_limit, error};
         __syncthreads();
         if (!peel_body(P, g, G, s_dyn)) return;
     }
@@ -895,7 +871,6 @@ __global__ void b_peel_reset(const BInst* __restrict__ insts, int mode, int stat
 
 
 constexpr int kMaxWords = 4096;   // n <= 262144: the assigned set fits in 32 KiB of LDS
-constexpr int kRowEndMinM = 4;    // fewest objectives for which the sort computes row ends
 constexpr int kWideBlocks = 64;   // row blocks (n >= 4033) from which dominance uses nds_dominance_wide
 
 // Polls before a peel barrier gives up (kSpinLimit unless a test lowers it
@@ -1199,8 +1174,6 @@ int batch_nds(const BatchLaunch& L, const BNdsGrids& G, const BNdsScratch& S, in
     }
     stage_end(kStDominance, st);
     stage_begin(kStPeel, st);
-    hipLaunchKernelGGL(b_nds_rowend, dim3(G.blocks64.blocks), dim3(64), 0, st, L.insts, G.blocks64.map, mode);
-    MOEA_HIP(hipGetLastError());
     hipLaunchKernelGGL(b_peel_reset, dim3(L.count), dim3(256), 0, st, L.insts, mode, state);
     MOEA_HIP(hipGetLastError());
     int mode_arg = mode;
