@@ -17,7 +17,7 @@
 //     infeasible one, a NaN objective compares false both ways, infeasible
 //     rows dominate by strictly smaller cv, and q <= p in every objective with
 //     f0(q) > f0(p) is impossible.
-//  2. nds_gather: objectives and cv in sorted order; per 64-row block b,
+//  2. nds_block_prep: objectives and cv in sorted order; per 64-row block b,
 //     kind[b] (all rows in segment 0?) and hi[b], the last block holding a row
 //     whose key equals the key of block b's last row (b itself unless an f0
 //     tie runs past the block).  Dominators of a row of block b all lie in
@@ -99,12 +99,12 @@ __global__ void nds_keys(const float* __restrict__ f, int ldf, const float* __re
     vals[s] = s;
 }
 
-// Sorted copies of the objectives (m rows of npad) and cv; padding rows are 0.
-__global__ void nds_gather(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
-                           int n, int m, int npad, const int32_t* __restrict__ perm,
-                           float* __restrict__ fs, float* __restrict__ cvs)
+// Sorted copies of the objectives (m rows of npad) and cv of row t; padding
+// rows are 0.
+__device__ __forceinline__ void gather_sorted_row(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
+                                                  int n, int m, int npad, const int32_t* __restrict__ perm,
+                                                  float* __restrict__ fs, float* __restrict__ cvs, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npad) return;
     if (t < n) {
         const int s = perm[t];
@@ -142,14 +142,21 @@ __device__ __forceinline__ void block_info_one(const uint64_t* __restrict__ keys
     hi[b] = end >> 6;
 }
 
-// zero (optional, `words` of them): cleared on the way, for a later launch;
-// mf_init (optional): set to -1, the value the peel replaces with its result;
-// rend (optional, one per padded row, 64-thread blocks): set to 0, the row
-// ends the dominance kernel raises.
-__global__ void nds_block_info(const uint64_t* __restrict__ keys, int n, uint8_t* __restrict__ kind,
-                               int32_t* __restrict__ hi, uint32_t* __restrict__ zero, int words,
-                               int32_t* __restrict__ mf_init, int32_t* __restrict__ rend)
+// One 64-thread block per 64-row block after the sort: the block's sorted
+// objectives and cv (gather_sorted_row) and its kind and tie bound
+// (block_info_one).  zero (optional, `words` of them): cleared on the way, for
+// a later launch; mf_init (optional): set to -1, the value the peel replaces
+// with its result; rend (optional, one per padded row): set to 0, the row ends
+// the dominance kernel raises.
+__global__ __launch_bounds__(64) void nds_block_prep(const float* __restrict__ f, int ldf,
+                                                     const float* __restrict__ cv, int m, int npad,
+                                                     const int32_t* __restrict__ perm, float* __restrict__ fs,
+                                                     float* __restrict__ cvs, const uint64_t* __restrict__ keys, int n,
+                                                     uint8_t* __restrict__ kind, int32_t* __restrict__ hi,
+                                                     uint32_t* __restrict__ zero, int words,
+                                                     int32_t* __restrict__ mf_init, int32_t* __restrict__ rend)
 {
+    gather_sorted_row(f, ldf, cv, n, m, npad, perm, fs, cvs, static_cast<int>(blockIdx.x * 64 + threadIdx.x));
     for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < words;
          j += static_cast<int>(gridDim.x * blockDim.x))
         zero[j] = 0u;
@@ -974,15 +981,12 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
     MOEA_HIP(hipGetLastError());
     MOEA_CHECK(sort_pairs(L.sort_temp, L.sort_bytes, L.keys_in, L.keys_out, L.vals_in, L.perm, n,
                           stream, kKeyBits));
-    hipLaunchKernelGGL(nds_gather, dim3(ceil_div(npad, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
-                       npad, L.perm, L.fs, L.cvs);
-    MOEA_HIP(hipGetLastError());
-    // also clears the peel's barrier state (not read before the peel)
     // row ends pay for their atomics when many rows have no dominator near
     // their own block, which is the case for many objectives
     int32_t* rend = m >= kRowEndMinM ? L.rend : nullptr;
-    hipLaunchKernelGGL(nds_block_info, dim3(W), dim3(64), 0, stream, L.keys_out, n, L.kind, L.hi, L.state,
-                       kPeelStateWords, max_front, rend);
+    // also clears the peel's barrier state (not read before the peel)
+    hipLaunchKernelGGL(nds_block_prep, dim3(W), dim3(64), 0, stream, f, ldf, cv, m, npad, L.perm, L.fs, L.cvs,
+                       L.keys_out, n, L.kind, L.hi, L.state, kPeelStateWords, max_front, rend);
     MOEA_HIP(hipGetLastError());
     stage_end(kStNdsSort, stream);
 
@@ -1022,7 +1026,7 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 // process ends; host code reaches it only through the address that
 // hipGetSymbolAddress reports for the current device (the array's name is a
 // host-side stand-in, not a device address).  Nothing in it needs clearing
-// between calls: nds_block_info zeroes the peel state and the peel
+// between calls: nds_block_prep zeroes the peel state and the peel
 // initializes cursors, fronts and the assigned sets.  Calls that overlap on
 // one GPU share the array and race.
 constexpr size_t kStaticScratchBytes = size_t{16} << 20;
