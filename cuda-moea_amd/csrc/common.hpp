@@ -559,6 +559,25 @@ __device__ inline bool coherent_barrier(uint32_t* state, uint32_t rank, uint32_t
 }
 
 // ----------------------------------------------------------------------------
+// Sort keys of the non-dominated sort
+// ----------------------------------------------------------------------------
+// Monotone map float -> u32 (numeric order for non-NaN values; -0 maps like +0).
+__device__ __forceinline__ uint32_t order_key(float x)
+{
+    const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// A row's 34-bit key from its first objective, whether any objective is NaN,
+// and its violation: segment 0 (feasible, no NaN) by f0, segment 1 (feasible
+// with a NaN objective) all equal, segment 2 (infeasible) by cv, NaN last.
+__device__ __forceinline__ uint64_t nds_key_from(float f0, bool any_nan, float c)
+{
+    if (c == 0.0f) return any_nan ? (1ull << 32) : order_key(f0);
+    return (2ull << 32) | (isnan(c) ? 0xFFFFFFFFu : order_key(c));
+}
+
+// ----------------------------------------------------------------------------
 // Wave helpers
 // ----------------------------------------------------------------------------
 __device__ inline uint64_t ballot(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }

@@ -15,8 +15,12 @@ size_t nds_workspace_bytes(int n);
 int nds_max_n();
 // max_front receives -2 when the peel gave up waiting at a grid barrier (the
 // fronts are then invalid); `error` (optional) is a sticky flag also set then.
+// keys_ready: the caller has already written each row's key (nds_key_from)
+// and index into the buffers nds_key_buffers reports for this workspace.
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
-            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error = nullptr);
+            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error = nullptr,
+            bool keys_ready = false);
+void nds_key_buffers(void* ws, int n, uint64_t** keys, int32_t** vals);
 int nds_run_static(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
                    int32_t* max_front, hipStream_t stream);
 int nds_static_max_n();

@@ -71,23 +71,14 @@ __device__ __forceinline__ T ld_uniform(const T* p, size_t i)
     return ((const_ptr)p)[i];   // address-space conversion (a C-style cast performs it)
 }
 
-// Monotone map float -> u32 (numeric order for non-NaN values; -0 maps like +0).
-__device__ __forceinline__ uint32_t order_key(float x)
-{
-    const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
 __device__ __forceinline__ uint64_t nds_key_of(const float* __restrict__ f, int ldf, const float* __restrict__ cv,
                                       int s, int m)
 {
     const float c = cv[s];
-    if (c == 0.0f) {
-        bool nan = false;
+    bool nan = false;
+    if (c == 0.0f)
         for (int k = 0; k < m; ++k) nan = nan || isnan(f[static_cast<size_t>(k) * ldf + s]);
-        return nan ? (1ull << 32) : order_key(f[s]);
-    }
-    return (2ull << 32) | (isnan(c) ? 0xFFFFFFFFu : order_key(c));
+    return nds_key_from(f[s], nan, c);
 }
 
 __global__ void nds_keys(const float* __restrict__ f, int ldf, const float* __restrict__ cv, int n,
@@ -961,8 +952,16 @@ size_t nds_workspace_bytes(int n)
 
 int nds_max_n() { return kMaxWords * 64; }
 
+void nds_key_buffers(void* ws, int n, uint64_t** keys, int32_t** vals)
+{
+    NdsLayout L;
+    carve_nds(ws, n, L);
+    *keys = L.keys_in;
+    *vals = L.vals_in;
+}
+
 int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at, int32_t* fronts,
-            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error)
+            int32_t* max_front, void* ws, size_t ws_bytes, hipStream_t stream, uint32_t* error, bool keys_ready)
 {
     if (n <= 0 || m <= 0 || m > kMaxM || !f || !cv || !fronts || !max_front || ldf < n)
         return fail(kInvalidArgument, "nds: invalid arguments");
@@ -976,9 +975,11 @@ int nds_run(const float* f, int ldf, const float* cv, int n, int m, int stop_at,
 
     stage_begin(kStNdsSort, stream);
     const int tb = 256;
-    hipLaunchKernelGGL(nds_keys, dim3(ceil_div(n, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
-                       L.keys_in, L.vals_in);
-    MOEA_HIP(hipGetLastError());
+    if (!keys_ready) {
+        hipLaunchKernelGGL(nds_keys, dim3(ceil_div(n, tb)), dim3(tb), 0, stream, f, ldf, cv, n, m,
+                           L.keys_in, L.vals_in);
+        MOEA_HIP(hipGetLastError());
+    }
     MOEA_CHECK(sort_pairs(L.sort_temp, L.sort_bytes, L.keys_in, L.keys_out, L.vals_in, L.perm, n,
                           stream, kKeyBits));
     // row ends pay for their atomics when many rows have no dominator near

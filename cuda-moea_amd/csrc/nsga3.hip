@@ -59,10 +59,13 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& tot
 // copies variables too; here only the O(M n) objective columns move).
 // omin (optional, 256-thread blocks): per block and objective, the fminf of
 // its offspring rows, for the running ideal point (norm_prep finishes it).
+// keys / vals (optional): the rows' sort keys and indices, for the sort of
+// the merged rows that follows.
 __global__ __launch_bounds__(256) void merge_objectives(const float* __restrict__ pf, const float* __restrict__ of,
                                                         const float* __restrict__ pcv, const float* __restrict__ ocv,
                                                         int n, int m, float* __restrict__ fm, float* __restrict__ cvm,
-                                                        float* __restrict__ omin)
+                                                        float* __restrict__ omin, uint64_t* __restrict__ keys,
+                                                        int32_t* __restrict__ vals)
 {
     __shared__ float s_min[4][kMaxM];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,18 +74,29 @@ __global__ __launch_bounds__(256) void merge_objectives(const float* __restrict_
     const bool par = i < n;
     const int s = par ? i : i - n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float f0 = 0.0f;
+    bool any_nan = false;
     for (int k = 0; k < m; ++k) {
         float v = INFINITY;
         if (valid) {
             v = par ? pf[static_cast<size_t>(k) * n + s] : of[static_cast<size_t>(k) * n + s];
             fm[static_cast<size_t>(k) * nm + i] = v;
+            if (k == 0) f0 = v;
+            any_nan = any_nan || isnan(v);
         }
         float z = (valid && !par) ? v : INFINITY;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) z = fminf(z, __shfl_xor(z, o));
         if (lane == 0) s_min[wave][k] = z;
     }
-    if (valid) cvm[i] = par ? pcv[s] : ocv[s];
+    if (valid) {
+        const float c = par ? pcv[s] : ocv[s];
+        cvm[i] = c;
+        if (keys) {
+            keys[i] = nds_key_from(f0, any_nan, c);
+            vals[i] = i;
+        }
+    }
     if (!omin) return;
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < m) {
@@ -2097,8 +2111,13 @@ int nsga3_select(const moea_nsga3_select_args& A)
     const uint32_t gen = static_cast<uint32_t>(A.generation);
     const moea_draws* dr = A.draws;
 
+    // without quantized violations the merge also writes the sort keys
+    const bool keys_in_merge = A.cv_quant.bins <= 0;
+    uint64_t* sort_keys = nullptr;
+    int32_t* sort_vals = nullptr;
+    if (keys_in_merge) nds_key_buffers(L.nds_ws, nm, &sort_keys, &sort_vals);
     hipLaunchKernelGGL(merge_objectives, dim3(nb), dim3(256), 0, st, A.parents.f, A.offspring.f,
-                       A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm, L.omin);
+                       A.parents.cv, A.offspring.cv, n, m, L.fm, L.cvm, L.omin, sort_keys, sort_vals);
     MOEA_HIP(hipGetLastError());
     const float* cv_sort = L.cvm;
     if (A.cv_quant.bins > 0) {
@@ -2107,7 +2126,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
         cv_sort = L.cvq;
     }
     MOEA_CHECK(nds_run(L.fm, nm, cv_sort, nm, m, n, L.fronts, L.meta + kMaxFront, L.nds_ws, L.nds_bytes, st,
-                       A.stats ? reinterpret_cast<uint32_t*>(A.stats + kDevError) : nullptr));
+                       A.stats ? reinterpret_cast<uint32_t*>(A.stats + kDevError) : nullptr, keys_in_merge));
     stage_begin(kStSelectPrep, st);
     const int nsb = ceil_div(nm, kScanThreads);
     hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);
