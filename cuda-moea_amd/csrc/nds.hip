@@ -569,7 +569,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
     const int nblk = P.W;
     const int bpg = P.blocks_per_group;
     uint64_t* s_A = s_dyn;                                          // (W) assigned before this round
-    uint64_t* s_next = s_A + nblk;                                  // (bpg) next masks of own blocks
+    uint64_t* s_next = s_A + nblk;                                  // (bpg) rows of own blocks assigned this round
     const int rows = bpg * 64;
     uint64_t* s_task = s_next + bpg;                                // (2 x 64 bpg) queued (cursor << 32 | row)
     uint64_t* s_rem = s_task + 2 * static_cast<size_t>(rows);       // (64 bpg) unassigned bits of the blocking word
@@ -587,6 +587,7 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         s_cur[j * 64 + lane] = 0;
         s_rem[j * 64 + lane] = 0ull;
         s_front[j * 64 + lane] = -1;
+        if (lane == 0) s_next[j] = 0ull;
         if (rb >= nblk) continue;
         const int p = rb * 64 + lane;
         s_end[j * 64 + lane] = (P.rend && p < P.n) ? P.rend[p] : P.hi[rb] + 1;
@@ -602,49 +603,152 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         if (threadIdx.x == 0) { s_count = 0; s_nq[0] = 0; s_nq[1] = 0; }
         __syncthreads();
 
-        // ---- phase A: lane per row
+        // ---- phase A
         int my_new = 0;
-        for (int j = lw; j < bpg; j += kPeelWaves) {
+        if (bpg * 4 <= kPeelWaves) {
+            // four lanes a row: wave lw takes rows 16 (lw & 3) .. 16 (lw & 3) + 15
+            // of row block lw / 4, and lane q = lane & 3 of a row reads words
+            // 4q .. 4q + 3 of the two chunks from the one holding its cursor
+            const int j = lw >> 2;
             const int rb = g + j * G;
-            if (rb >= nblk) break;
-            const uint64_t amask = s_A[rb];
-            const int p = rb * 64 + lane;
-            const int slot = j * 64 + lane;
-            const int end = s_end[slot];
-            const bool live = (p < P.n) && !((amask >> lane) & 1ull);
-            int cur = end;
-            bool ready = false, queued = false;
-            if (live) {
-                cur = s_cur[slot];
-                if (cur >= end) {
-                    ready = true;
-                } else {
-                    // still blocked while dominators of its blocking word are unassigned
-                    uint64_t rem = s_rem[slot] & ~s_A[cur];
-                    if (rem == 0) {
-                        peel_check(P.B, P.Wc, P.W, rb, lane, end, s_A, cur, ready, queued, rem);
-                        s_cur[slot] = cur;
+            if (j < bpg && rb < nblk) {
+                const int q = lane & 3;
+                const int r = (lw & 3) * 16 + (lane >> 2);
+                const uint64_t amask = s_A[rb];
+                const int p = rb * 64 + r;
+                const int slot = j * 64 + r;
+                const int end = s_end[slot];
+                const bool live = (p < P.n) && !((amask >> r) & 1ull);
+                int cur = end;
+                bool ready = false, queued = false, check = false;
+                uint64_t rem = 0;
+                if (live) {
+                    cur = s_cur[slot];
+                    if (cur >= end) {
+                        ready = true;
+                    } else {
+                        // still blocked while dominators of its blocking word are unassigned
+                        rem = s_rem[slot] & ~s_A[cur];
+                        check = (rem == 0);
                     }
+                }
+                if (ballot(check)) {
+                    const int c = cur >> 3;
+                    const int cc = c + (q >> 1);
+                    const int w0 = cc * 8 + (q & 1) * 4;
+                    ulonglong2 v0 = make_ulonglong2(0ull, 0ull), v1 = v0;
+                    if (check && cc * 8 < end) {
+                        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(
+                            peel_rowbase(P.B, P.Wc, rb, r) + static_cast<size_t>(cc) * 512 + (q & 1) * 4);
+                        v0 = src[0];
+                        v1 = src[1];
+                    }
+                    // the assigned-set words, read while the row's words are in flight
+                    uint64_t am[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) am[i] = s_A[min(w0 + i, P.W - 1)];
+                    const int stop = min((c + kCheckChunks) * 8, end);
+                    const uint64_t wv[4] = {v0.x, v0.y, v1.x, v1.y};
+                    int fw = -1;
+                    uint64_t fx = 0;
+#pragma unroll
+                    for (int i = 3; i >= 0; --i) {
+                        const int w = w0 + i;
+                        const uint64_t x = wv[i] & ~am[i];
+                        if (check && w >= cur && w < stop && x != 0) { fw = w; fx = x; }
+                    }
+                    // the row's first blocking word: from its lowest lane that has one
+                    const uint32_t gm = static_cast<uint32_t>(ballot(fw >= 0) >> (lane & ~3)) & 0xFu;
+                    const int src = (lane & ~3) + (gm ? __builtin_ctz(gm) : 0);
+                    const int bw = __shfl(fw, src);
+                    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(fx), src));
+                    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(fx >> 32), src));
+                    if (check) {
+                        if (gm) {
+                            cur = bw;
+                            rem = (static_cast<uint64_t>(hi) << 32) | lo;
+                        } else {
+                            cur = stop;
+                            ready = (cur >= end);
+                            queued = !ready;
+                        }
+                    }
+                }
+                const bool lead = (q == 0);
+                if (live && !ready && lead) {
+                    s_cur[slot] = cur;
                     s_rem[slot] = rem;
                 }
-            }
-            const uint64_t ready_m = ballot(ready);
-            const uint64_t task_m = ballot(queued);
-            if (ready) s_front[slot] = k;
-            if (task_m) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&s_nq[0], __builtin_popcountll(task_m));
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (queued) {
-                    const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                        static_cast<uint32_t>(task_m >> 32),
-                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
-                    s_task[base + below] = (static_cast<uint64_t>(cur) << 32) | static_cast<uint32_t>(p);
+                const uint64_t ready_m = ballot(ready && lead);
+                const uint64_t task_m = ballot(queued && lead);
+                if (ready && lead) s_front[slot] = k;
+                if (task_m) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&s_nq[0], __builtin_popcountll(task_m));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (queued && lead) {
+                        const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(task_m >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
+                        s_task[base + below] = (static_cast<uint64_t>(cur) << 32) | static_cast<uint32_t>(p);
+                    }
+                }
+                if (ready_m) {
+                    // the rows' bits sit at lanes 0, 4, ..., 60: bits 0 .. 15 of the wave's rows
+                    uint64_t rbits = 0;
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) rbits |= ((ready_m >> (4 * t)) & 1ull) << t;
+                    if (lane == 0) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(&s_next[j]),
+                                 static_cast<unsigned long long>(rbits << ((lw & 3) * 16)));
+                        my_new += __builtin_popcountll(rbits);
+                    }
                 }
             }
-            if (lane == 0) {
-                s_next[j] = amask | ready_m;
-                my_new += __builtin_popcountll(ready_m);
+        } else {
+            // lane per row
+            for (int j = lw; j < bpg; j += kPeelWaves) {
+                const int rb = g + j * G;
+                if (rb >= nblk) break;
+                const uint64_t amask = s_A[rb];
+                const int p = rb * 64 + lane;
+                const int slot = j * 64 + lane;
+                const int end = s_end[slot];
+                const bool live = (p < P.n) && !((amask >> lane) & 1ull);
+                int cur = end;
+                bool ready = false, queued = false;
+                if (live) {
+                    cur = s_cur[slot];
+                    if (cur >= end) {
+                        ready = true;
+                    } else {
+                        // still blocked while dominators of its blocking word are unassigned
+                        uint64_t rem = s_rem[slot] & ~s_A[cur];
+                        if (rem == 0) {
+                            peel_check(P.B, P.Wc, P.W, rb, lane, end, s_A, cur, ready, queued, rem);
+                            s_cur[slot] = cur;
+                        }
+                        s_rem[slot] = rem;
+                    }
+                }
+                const uint64_t ready_m = ballot(ready);
+                const uint64_t task_m = ballot(queued);
+                if (ready) s_front[slot] = k;
+                if (task_m) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&s_nq[0], __builtin_popcountll(task_m));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (queued) {
+                        const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(task_m >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(task_m), 0u)));
+                        s_task[base + below] = (static_cast<uint64_t>(cur) << 32) | static_cast<uint32_t>(p);
+                    }
+                }
+                if (lane == 0) {
+                    s_next[j] = ready_m;
+                    my_new += __builtin_popcountll(ready_m);
+                }
             }
         }
 
@@ -698,7 +802,8 @@ __device__ __forceinline__ bool peel_body(const PeelArgs& P, int g, int G, uint6
         __syncthreads();
         for (int j = threadIdx.x; j < bpg; j += kPeelThreads) {
             const int rb = g + j * G;
-            if (rb < nblk) st_relaxed_u64(&Anext[rb], s_next[j]);
+            if (rb < nblk) st_relaxed_u64(&Anext[rb], s_A[rb] | s_next[j]);
+            s_next[j] = 0ull;
         }
         if (threadIdx.x == 0 && s_count)
             __hip_atomic_fetch_add(&P.state[kPeelTotal], static_cast<uint32_t>(s_count), __ATOMIC_RELAXED,
