@@ -387,12 +387,13 @@ __global__ __launch_bounds__(kDomThreads) void nds_dominance_wide(
 // its blocking word c (the first word of its dominator row that held an
 // unassigned dominator when last looked at) and rem, the dominators of word c
 // that were still unassigned then.  Round k:
-//   A  lane per row: rem &= ~A[c] against the set assigned before round k.  A
-//      row with rem left stays blocked without touching HBM.  A row whose rem
-//      emptied reads on from c: kCheckChunks chunks from the one holding c, in
-//      one batch.  Rows whose dominator words run out are front k; rows still
-//      clear with words left go to the workgroup's task queue in LDS as
-//      (row, c).
+//   A  rem &= ~A[c] against the set assigned before round k.  A row with rem
+//      left stays blocked without touching HBM.  A row whose rem emptied reads
+//      on from c: kCheckChunks chunks from the one holding c, in one batch
+//      (four lanes a row over all 16 waves when the workgroup owns four row
+//      blocks or fewer, else a lane per row).  Rows whose dominator words run
+//      out are front k; rows still clear with words left go to the
+//      workgroup's task queue in LDS as (row, c).
 //   B  the workgroup's 16 waves scan the queued rows in sweeps of one round
 //      trip per row (8 lanes and 64 words a row in the first sweep, twice as
 //      many in each later one while lanes are free), each until it meets its
