@@ -228,6 +228,7 @@ int run_step(moea_run_args& a, int generations)
                                                 a.stream));
             } else if (a.algorithm == MOEA_ALGORITHM_NSGA3) {
                 MOEA_CHECK(parent_fronts(a, par, st));
+                a.directions_ready = 0;   // the sort used the workspace: the cone tables are rebuilt
             }
         }
         moea_reproduce_args r{};

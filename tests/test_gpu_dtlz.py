@@ -384,3 +384,21 @@ def test_dtlz1_run_matches_oracle(name, ratio, gens, seed):
     np.testing.assert_array_equal(res.objectives.cpu().numpy(), f.T)
     np.testing.assert_array_equal(res.constraints.cpu().numpy().reshape(-1), cv)
     np.testing.assert_array_equal(res.auxiliary_indices.cpu().numpy(), fr)
+
+
+def test_constraint_switch_with_cone_association_matches_oracle():
+    """A constraint switch mid-run re-sorts the parents inside the run's
+    workspace, and the next selection rebuilds its cone tables (2080
+    directions: the cone-pruned association).  C1-DTLZ1, ratio 0.5, ten
+    generations, against the oracle, bit for bit."""
+    import oracle as O
+    prob = cm.C1DTLZ1(dimension=12, objectives=3, constraint_activation_ratio=0.5)
+    alg = cm.NSGA3(population_size=2080, max_generations=10, problem=prob, seed=5, print_progress=False)
+    print("c1dtlz1 switch with cones n=2080", flush=True)
+    res = alg.run()
+    assert alg.k >= 2048
+    x, f, cv, fr, _ = O.nsga3_run_dtlz1(2080, 3, 12, 10, 5, constrained=True, ratio=0.5)
+    np.testing.assert_array_equal(res.variables.cpu().numpy(), x)
+    np.testing.assert_array_equal(res.objectives.cpu().numpy(), f.T)
+    np.testing.assert_array_equal(res.constraints.cpu().numpy().reshape(-1), cv)
+    np.testing.assert_array_equal(res.auxiliary_indices.cpu().numpy(), fr)
