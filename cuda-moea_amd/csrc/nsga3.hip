@@ -146,10 +146,13 @@ __global__ __launch_bounds__(kScanThreads) void front_counts(const int32_t* __re
     }
 }
 
+// It also writes the prior-front survivor flags (gather_selected_kernel's
+// prior copy, niching.cu:303-312) that the niching completes.
 __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __restrict__ fronts, int nm,
                                                               int n, const int32_t* __restrict__ counts,
                                                               int nblk, int32_t* __restrict__ gidx,
-                                                              int32_t* __restrict__ meta)
+                                                              int32_t* __restrict__ meta,
+                                                              uint8_t* __restrict__ flag)
 {
     __shared__ int s_wave[64];
     const int mf = meta[kMaxFront];
@@ -167,6 +170,8 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
     block_exclusive_scan(ap, s_wave, ap);
     block_exclusive_scan(al, s_wave, al);
     const int i = b * kScanThreads + threadIdx.x;
+    const int fr = i < nm ? fronts[i] : -1;
+    if (i < nm) flag[i] = prior_survivor(fr, mf, i, n) ? 1 : 0;
     if (mf < 0) {
         // the sort failed on the device (its peel gave up waiting): keep the
         // parents, so every later launch reads valid indices, and flag it
@@ -180,7 +185,6 @@ __global__ __launch_bounds__(kScanThreads) void front_scatter(const int32_t* __r
         }
         return;
     }
-    const int fr = i < nm ? fronts[i] : -1;
     const int fp = (fr >= 0 && fr < mf) ? 1 : 0, fl = (fr == mf) ? 1 : 0;
     const int op = block_exclusive_scan(fp, s_wave, t);
     const int ol = block_exclusive_scan(fl, s_wave, t);
@@ -1132,15 +1136,8 @@ __global__ void niche_keys3(const uint64_t* __restrict__ key2s, const int32_t* _
     if (t < nm) niche_keys3_body(key2s, meta, rho, k, nm, 0xFFFFFFFFu, 0ull, inj_b, seed, gen, key3, t);
 }
 
-// Survivor flags: all prior-front members plus the first N_rem niching picks
-// (gather_selected_kernel, niching.cu:192-206; prior copy at :303-312).
-__global__ void mark_chosen(const int32_t* __restrict__ fronts, const int32_t* __restrict__ meta,
-                            int nm, uint8_t* __restrict__ flag)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nm) return;
-    flag[s] = prior_survivor(fronts[s], meta[kMaxFront], s, nm / 2) ? 1 : 0;
-}
+// Survivor flags of the first N_rem niching picks, beside the prior-front
+// flags front_scatter wrote (gather_selected_kernel, niching.cu:192-206).
 __global__ void mark_selected(const int32_t* __restrict__ val3s, const int32_t* __restrict__ gidx,
                               const int32_t* __restrict__ meta, uint8_t* __restrict__ flag)
 {
@@ -1673,6 +1670,7 @@ __device__ __forceinline__ void niche_last(const NicheArgs& A, const int32_t* __
 // the survivors in ascending merged order (gather_selected_kernel,
 // niching.cu:192-206, prior copy :303-312; thrust::sort of the selection,
 // nsga3_population_update.cu:269), in one workgroup.
+// fronts == nullptr: the prior-front flags are already written (front_scatter).
 __device__ __forceinline__ void niche_select_body(const NicheArgs& A, int32_t* __restrict__ meta, int nm, int n,
                                                   const int32_t* __restrict__ fronts, int32_t* __restrict__ chosen,
                                                   uint64_t* gk0, uint64_t* gk1, int32_t* gv0, int32_t* gv1,
@@ -1685,7 +1683,7 @@ __device__ __forceinline__ void niche_select_body(const NicheArgs& A, int32_t* _
     const int nq = nm / 4;
     uint32_t* fw = reinterpret_cast<uint32_t*>(A.flag);
     constexpr int kRows = 16;          // loads in flight per thread
-    for (int q0 = tid; q0 < nq; q0 += kRows * kNicheThreads) {
+    for (int q0 = tid; fronts && q0 < nq; q0 += kRows * kNicheThreads) {
         int4 fr[kRows];
 #pragma unroll
         for (int u = 0; u < kRows; ++u) {
@@ -2131,7 +2129,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
     const int nsb = ceil_div(nm, kScanThreads);
     hipLaunchKernelGGL(front_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, L.meta, L.counts);
     hipLaunchKernelGGL(front_scatter, dim3(nsb), dim3(kScanThreads), 0, st, L.fronts, nm, n, L.counts, nsb,
-                       L.gidx, L.meta);
+                       L.gidx, L.meta, L.flag);
     const dim3 g_nb(nb), b256(256), g_assoc(nb, nchunks);
     MOEA_LAUNCH_M(m, norm_prep, g_nb, b256, 0, st, L.fm, nm, L.gidx, L.meta, L.fn, L.asf_val, L.asf_idx,
                   L.row_max, L.rho, k, L.omin, nb, A.ideal);
@@ -2156,7 +2154,7 @@ int nsga3_select(const moea_nsga3_select_args& A)
     if (niching_one_block(A.niching)) {
         NicheArgs na{L.ref, L.dist, L.rho, k, bits_ref, bits_rank, dr ? dr->niche_a : nullptr,
                      dr ? dr->niche_b : nullptr, A.seed, gen, L.gidx, L.flag, niching_buckets(A.niching)};
-        hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, n, L.fronts, L.chosen,
+        hipLaunchKernelGGL(niche_block, dim3(1), dim3(kNicheThreads), 0, st, na, L.meta, nm, n, nullptr, L.chosen,
                            L.k0, L.k1, L.v0, L.v1, niching_lds_cap(A.niching), A.stats);
     } else {
         hipLaunchKernelGGL(niche_keys1, dim3(nb), dim3(256), 0, st, L.ref, L.dist, L.meta, nm, k, L.k0, L.v0);
@@ -2170,7 +2168,6 @@ int nsga3_select(const moea_nsga3_select_args& A)
                            dr ? dr->niche_b : nullptr, A.seed, gen, L.k0);
         MOEA_HIP(hipGetLastError());
         MOEA_CHECK(sort_pairs(L.sort_tmp, L.sort_bytes, L.k0, L.k1, L.v0, L.v1, nm, st, bits_rank));
-        hipLaunchKernelGGL(mark_chosen, dim3(nb), dim3(256), 0, st, L.fronts, L.meta, nm, L.flag);
         hipLaunchKernelGGL(mark_selected, dim3(ceil_div(n, 256)), dim3(256), 0, st, L.v1, L.gidx, L.meta,
                            L.flag);
         hipLaunchKernelGGL(flag_counts, dim3(nsb), dim3(kScanThreads), 0, st, L.flag, nm, L.counts);
