@@ -71,7 +71,7 @@ struct BInst {
     // pruned association (K >= 2048): cone tables built at admission, else null
     float* wp;                    // (m, k) directions in block order
     int32_t* cperm;               // (k) block position -> direction index
-    double* cones;                // (ceil(k / 64), m + 3)
+    float* cones;                 // m + 3 rows of ceil(k / 64)
     void* cone_buf;               // the tables and their build scratch
     // NDS scratch (2n merged rows, or n for the initial parents)
     uint64_t* B;                  // (W * Wc * 512) dominator bitmask, sorted order

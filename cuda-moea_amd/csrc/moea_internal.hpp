@@ -35,7 +35,7 @@ int quantize_cv(const float* cv, float* out, int n, const moea_cv_quantization& 
 // Cone tables of the pruned association, built once per set of directions.
 bool assoc_prunable(int k, int m);
 size_t cone_bytes(int m, int k);
-void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, double** cones);
+void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, float** cones);
 int cones_build(const float* w, int k, int m, void* buf, hipStream_t st);
 int ideal_point(const float* f, int ldf, int n, int m, float* ideal, hipStream_t stream);
 size_t associate_workspace_bytes(int ld, int m, int k);

@@ -691,8 +691,12 @@ __device__ __forceinline__ void assoc_final_body(const float* __restrict__ part_
 //     |cos(theta_c)| < cos(alpha + asin S) = cos(alpha) sqrt(1 - S^2) - sin(alpha) S
 // (alpha + asin S < pi/2) all its angles lie in (asin S, pi - asin S) and the
 // block is skipped; the others are evaluated with the kernel's own arithmetic.
-// S is rounded up, sqrt(1 - S^2) down, cos(alpha) down and sin(alpha) up, and
-// the compare keeps a margin of 1e-6 over the float rounding of cos(theta_c).
+// S is rounded up, sqrt(1 - S^2) down, cos(alpha) down and sin(alpha) up (in
+// the float table too), and the compare keeps a margin of 2e-6 over the error
+// of cos(theta_c), a float fmaf chain over x / |x| (formed in double, rounded
+// to float: 2^-24 relative, or below 2^-149 absolute) and the table's float
+// center (within 2^-24 of the unit center): with the chain's gamma_M, at most
+// (M + 2) 2^-24 <= 7.2e-7 for M <= 10, with no overflow or underflow of |x|.
 // The result (smallest d2, then smallest index, non-finite d2 never chosen) is
 // therefore the brute-force result bit for bit.  One wave per individual: its
 // lanes compute cos(theta_c) for every block, evaluate the block with the
@@ -709,12 +713,14 @@ __device__ __forceinline__ double wave_sum_f64(double v)
     return v;
 }
 
-// Cone of direction block b: center c (M doubles), cos alpha, sin alpha, and
-// the largest |w|^2 of the block, both angle terms rounded outwards.
-// Layout: (ceil(k / 64), M + 3) doubles.
+// Cone of direction block b: center c (M components), cos alpha, sin alpha,
+// and the largest |w|^2 of the block, computed in double; the angle terms are
+// rounded outwards, then stored as floats rounded outwards again (cos alpha
+// down, sin alpha and |w|^2 up).  Layout: M + 3 rows of ceil(k / 64) floats,
+// row r of block b at r * ceil(k / 64) + b, so a wave's loads are contiguous.
 template <int M>
 __device__ __forceinline__ void assoc_cone_body(const float* __restrict__ w, int k, const int32_t* __restrict__ perm,
-                                                float* __restrict__ wp, double* __restrict__ cones, int b)
+                                                float* __restrict__ wp, float* __restrict__ cones, int b)
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const bool valid = b * 64 + lane < k;
@@ -762,18 +768,18 @@ __device__ __forceinline__ void assoc_cone_body(const float* __restrict__ w, int
     if (ca < -1.0) ca = -1.0;
     const double sa = sqrt(fmax(0.0, 1.0 - ca * ca)) + 1e-12;
     if (lane == 0) {
-        double* out = cones + static_cast<size_t>(b) * (M + 3);
+        const size_t nbk = static_cast<size_t>((k + 63) / 64);
 #pragma unroll
-        for (int c = 0; c < M; ++c) out[c] = cen[c];
-        out[M] = ca;
-        out[M + 1] = sa;
-        out[M + 2] = w2;
+        for (int c = 0; c < M; ++c) cones[c * nbk + b] = static_cast<float>(cen[c]);
+        cones[M * nbk + b] = __double2float_rd(ca);
+        cones[(M + 1) * nbk + b] = __double2float_ru(sa);
+        cones[(M + 2) * nbk + b] = __double2float_ru(w2);
     }
 }
 
 template <int M>
 __global__ __launch_bounds__(256) void assoc_cones(const float* __restrict__ w, int k, const int32_t* __restrict__ perm,
-                                                   float* __restrict__ wp, double* __restrict__ cones)
+                                                   float* __restrict__ wp, float* __restrict__ cones)
 {
     const int b = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     if (b >= (k + 63) / 64) return;
@@ -823,7 +829,7 @@ template <int M>
 __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, int nm, const float* __restrict__ a,
                                                   const float* __restrict__ wp, int k,
                                                   const int32_t* __restrict__ perm,
-                                                  const double* __restrict__ cones,
+                                                  const float* __restrict__ cones,
                                                   const int32_t* __restrict__ meta, float* __restrict__ dist,
                                                   int32_t* __restrict__ ref, int i, uint16_t* __restrict__ list,
                                                   int32_t* __restrict__ rho)
@@ -840,30 +846,31 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
     const double xn = sqrt(xx);
     const bool finite_x = xx < INFINITY;                    // false for an inf or nan coordinate
     const double inv_xn = (xn > 0.0 && finite_x) ? 1.0 / xn : 0.0;
+    float u[M];                                             // x / |x|
+#pragma unroll
+    for (int c = 0; c < M; ++c) u[c] = static_cast<float>(static_cast<double>(x[c]) * inv_xn);
     float cth[T];                                           // cos(theta_c) of blocks lane, lane + 64, ...
-    double best_cos = -3.0, w2max = 0.0;
+    float best_cos = -3.0f, w2max = 0.0f;
     int best_b = 0;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int b = lane + 64 * t;
         cth[t] = 0.0f;
         if (b < nbk) {
-            const double* cb = cones + static_cast<size_t>(b) * (M + 3);
-            double xc = 0.0;
+            float ct = 0.0f;
 #pragma unroll
-            for (int c = 0; c < M; ++c) xc += static_cast<double>(x[c]) * cb[c];
-            const double ct = xc * inv_xn;
-            cth[t] = static_cast<float>(ct);
-            w2max = fmax(w2max, cb[M + 2]);
+            for (int c = 0; c < M; ++c) ct = fmaf(u[c], cones[c * nbk + b], ct);
+            cth[t] = ct;
+            w2max = fmaxf(w2max, cones[(M + 2) * nbk + b]);
             if (ct > best_cos) { best_cos = ct; best_b = b; }
         }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
-        const double oc = __shfl_xor(best_cos, o);
+        const float oc = __shfl_xor(best_cos, o);
         const int ob = __shfl_xor(best_b, o);
         if (oc > best_cos || (oc == best_cos && ob < best_b)) { best_cos = oc; best_b = ob; }
-        w2max = fmax(w2max, __shfl_xor(w2max, o));
+        w2max = fmaxf(w2max, __shfl_xor(w2max, o));
     }
     const int first = wave_uniform(best_b);
     float bd = INFINITY;
@@ -900,7 +907,7 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
         // below 2^-80 the float d2 may carry subnormal (absolute) rounding
         const double floor_cur = fmax(static_cast<double>(cur), 0x1p-80);
         const double s0 = sqrt(floor_cur / ((1.0 - 1e-6) * xx)) * (1.0 + 1e-12);
-        S = s0 + 1e-6 * w2max * (1.0 + 1e-12);
+        S = s0 + 1e-6 * static_cast<double>(w2max) * (1.0 + 1e-12);
         prune = S < 1.0;
         if (prune) C = sqrt(1.0 - S * S) * (1.0 - 1e-12);
     }
@@ -913,9 +920,8 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
         const int b = lane + 64 * t;
         bool skip = false;
         if (prune && b < nbk) {
-            const double* cb = cones + static_cast<size_t>(b) * (M + 3);
-            const double ca = cb[M], sa = cb[M + 1];
-            skip = ca >= 0.0 && static_cast<double>(fabsf(cth[t])) < ca * C - sa * S - 1e-6;
+            const double ca = cones[M * nbk + b], sa = cones[(M + 1) * nbk + b];
+            skip = ca >= 0.0 && static_cast<double>(fabsf(cth[t])) < ca * C - sa * S - 2e-6;
         }
         const bool cand = b < nbk && b != first && !skip;
         const uint64_t mk = ballot(cand);
@@ -960,7 +966,7 @@ template <int M>
 __global__ __launch_bounds__(256) void associate_pruned(const float* __restrict__ fn, int nm,
                                                         const float* __restrict__ a, const float* __restrict__ wp,
                                                         int k, const int32_t* __restrict__ perm,
-                                                        const double* __restrict__ cones,
+                                                        const float* __restrict__ cones,
                                                         const int32_t* __restrict__ meta, float* __restrict__ dist,
                                                         int32_t* __restrict__ ref, int32_t* __restrict__ rho)
 {
@@ -984,7 +990,7 @@ __global__ void associate_final(const float* __restrict__ part_d2, const int32_t
 // Buffers of the cone preparation: direction keys and permutation, the sort
 // scratch and the cones.
 struct ConeBufs {
-    uint64_t* kin; uint64_t* kout; int32_t* vin; int32_t* perm; void* tmp; size_t tmp_bytes; double* cones;
+    uint64_t* kin; uint64_t* kout; int32_t* vin; int32_t* perm; void* tmp; size_t tmp_bytes; float* cones;
     float* wp;
 };
 
@@ -1004,7 +1010,7 @@ ConeBufs carve_cones(Carve& c, int m, int k)
     B.perm = c.take<int32_t>(k);
     B.tmp_bytes = sort_pairs_temp_bytes(k, morton_key_bits(m));
     B.tmp = c.take_bytes(B.tmp_bytes);
-    B.cones = c.take<double>(static_cast<size_t>(ceil_div(k, 64)) * (m + 3));
+    B.cones = c.take<float>(static_cast<size_t>(ceil_div(k, 64)) * (m + 3));
     B.wp = c.take<float>(static_cast<size_t>(m) * k);
     return B;
 }
@@ -1948,7 +1954,7 @@ size_t cone_bytes(int m, int k)
     return c.used;
 }
 
-void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, double** cones)
+void cone_views(void* buf, int m, int k, float** wp, int32_t** perm, float** cones)
 {
     Carve c(buf, ~size_t{0});
     const ConeBufs B = carve_cones(c, m, k);
