@@ -874,9 +874,9 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
     }
     const int first = wave_uniform(best_b);
     float bd = INFINITY;
-    int bj = -1;
-    // the kernel's arithmetic for one direction (original index j)
-    auto consider = [&](const float (&wj)[M], int j) {
+    int bq = -1;                 // block-order position of the lane's best; its original index decides ties
+    // the kernel's arithmetic for the direction at block-order position q
+    auto consider = [&](const float (&wj)[M], int q) {
         float dot = 0.0f;
 #pragma unroll
         for (int c = 0; c < M; ++c) dot = fmaf(x[c], wj[c], dot);
@@ -886,7 +886,12 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
             const float r = fmaf(-dot, wj[c], x[c]);
             d2 = fmaf(r, r, d2);
         }
-        if (d2 < bd || (d2 == bd && d2 < INFINITY && j < bj)) { bd = d2; bj = j; }
+        if (d2 < bd) {
+            bd = d2;
+            bq = q;
+        } else if (d2 == bd && d2 < INFINITY && perm[q] < perm[bq]) {
+            bq = q;
+        }
     };
     {
         const int q = first * 64 + lane;
@@ -894,7 +899,7 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
             float wj[M];
 #pragma unroll
             for (int c = 0; c < M; ++c) wj[c] = wp[static_cast<size_t>(c) * k + q];
-            consider(wj, perm[q]);
+            consider(wj, q);
         }
     }
     float cur = bd;
@@ -936,19 +941,19 @@ __device__ __forceinline__ void assoc_pruned_body(const float* __restrict__ fn, 
     constexpr int kBatch = 4;
     for (int base = 0; base < nc; base += kBatch) {
         float wv[kBatch][M];
-        int jv[kBatch];
+        int qv[kBatch];
 #pragma unroll
         for (int v = 0; v < kBatch; ++v) {
             const int q = (base + v < nc) ? static_cast<int>(list[base + v]) * 64 + lane : k;
-            const bool ok = q < k;
-            jv[v] = ok ? perm[q] : -1;
+            qv[v] = q;
 #pragma unroll
-            for (int c = 0; c < M; ++c) wv[v][c] = ok ? wp[static_cast<size_t>(c) * k + q] : 0.0f;
+            for (int c = 0; c < M; ++c) wv[v][c] = q < k ? wp[static_cast<size_t>(c) * k + q] : 0.0f;
         }
 #pragma unroll
         for (int v = 0; v < kBatch; ++v)
-            if (jv[v] >= 0) consider(wv[v], jv[v]);
+            if (qv[v] < k) consider(wv[v], qv[v]);
     }
+    int bj = bq >= 0 ? perm[bq] : -1;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         const float od = __shfl_xor(bd, o);
