@@ -54,11 +54,11 @@ __global__ void mating_kernel(int n, int active_host, const int32_t* __restrict_
 // SBX of one (pair, variable) element.  In midpoint +/- delta both operands are
 // products; the NVPTX combiner fuses the first (0.5f * (x1 + x2), an exact
 // product), so the sums evaluate unfused as written.
+// power = 1 / (1 + eta), formed once per launch (set_derived).
 __device__ __forceinline__ void sbx_one(float x1, float x2, float ub, float up, int sign, float uc, float lo,
-                               float hi, float eta, float prob, float copy_prob, float& ca, float& cb)
+                               float hi, float power, float prob, float copy_prob, float& ca, float& cb)
 {
     if (up >= prob || uc < copy_prob) { ca = x1; cb = x2; return; }
-    const float power = 1.0f / (1.0f + eta);
     // one powf_root on the selected base (a wave holds lanes of both branches)
     const float base = (ub < 0.5f) ? 2.0f * ub : 1.0f / (2.0f - 2.0f * ub);
     float beta = powf_root(base, power);
@@ -70,12 +70,13 @@ __device__ __forceinline__ void sbx_one(float x1, float x2, float ub, float up, 
 }
 
 // Deb's polynomial mutation of one element; value + delta * range is fused.
-__device__ __forceinline__ float pm_one(float value, float ug, float ud, float lo, float hi, float eta, float rate)
+// exponent = eta + 1 and inv_exponent = 1 / exponent, formed once per launch.
+__device__ __forceinline__ float pm_one(float value, float ug, float ud, float lo, float hi, float exponent,
+                                        float inv_exponent, float rate)
 {
     if (ug >= rate) return value;
     const float range = hi - lo;
     const float normalized = (value - lo) / range;
-    const float exponent = eta + 1.0f;
     // both branches' operands selected first, then one power and one root:
     //   ud <= 0.5: vm = 2 ud + (1 - 2 ud) (1 - x)^e,          delta = vm^(1/e) - 1
     //   else:      vm = 2 (1 - ud) + 2 (ud - 0.5) x^e,        delta = 1 - vm^(1/e)
@@ -85,7 +86,7 @@ __device__ __forceinline__ float pm_one(float value, float ug, float ud, float l
     const float a = lower ? 2.0f * ud : 2.0f * (1.0f - ud);
     const float b = lower ? 1.0f - 2.0f * ud : 2.0f * (ud - 0.5f);
     const float vm = a + b * powf_det(xy, exponent);
-    const float pw = powf_root(vm, 1.0f / exponent);
+    const float pw = powf_root(vm, inv_exponent);
     const float delta = lower ? pw - 1.0f : 1.0f - pw;
     return fminf(fmaxf(fmaf(delta, range, value), lo), hi);
 }
@@ -100,12 +101,22 @@ struct RepArgs {
     float eta_c, prob_c, copy_c;
     int mutate;
     float eta_m, rate_m;
+    float power_c, exp_m, inv_exp_m;   // 1 / (1 + eta_c), eta_m + 1, 1 / (eta_m + 1)
     const float* sbx_u; const float* sbx_p; const int32_t* sbx_sign; const float* sbx_copy;
     const float* pm_gate; const float* pm_delta;
     uint64_t seed;
     uint32_t gen;
     float* ox;
 };
+
+// The launch-wide terms of the SBX and mutation formulas, each one IEEE float
+// operation (correctly rounded on the host and on the device alike).
+__host__ __device__ __forceinline__ void set_derived(RepArgs& R)
+{
+    R.power_c = 1.0f / (1.0f + R.eta_c);
+    R.exp_m = R.eta_m + 1.0f;
+    R.inv_exp_m = 1.0f / R.exp_m;
+}
 
 // Parent row crossed into offspring row `row`.  kWinners 0: the winners as
 // the mating kernel wrote them; 1: a caller's winners, clamped into [0, n);
@@ -152,7 +163,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
             ub = u01(rs.x); up = u01(rs.y); uc = u01(rs.z); sg = sbx_direction(rs);
         }
         float ca, cb;
-        sbx_one(x1, x2, ub, up, sg, uc, lo, hi, R.eta_c, R.prob_c, R.copy_c, ca, cb);
+        sbx_one(x1, x2, ub, up, sg, uc, lo, hi, R.power_c, R.prob_c, R.copy_c, ca, cb);
         if (R.mutate) {
             const size_t ra = static_cast<size_t>(2 * pair) * R.d + j;
             const size_t rb = ra + R.d;
@@ -168,8 +179,8 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
                     db = u01(rp.y);
                 }
             }
-            ca = pm_one(ca, ga, da, lo, hi, R.eta_m, R.rate_m);
-            cb = pm_one(cb, gb, db, lo, hi, R.eta_m, R.rate_m);
+            ca = pm_one(ca, ga, da, lo, hi, R.exp_m, R.inv_exp_m, R.rate_m);
+            cb = pm_one(cb, gb, db, lo, hi, R.exp_m, R.inv_exp_m, R.rate_m);
         }
         R.ox[static_cast<size_t>(2 * pair) * R.d + j] = ca;
         R.ox[static_cast<size_t>(2 * pair + 1) * R.d + j] = cb;
@@ -188,7 +199,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
                 g = u01(rs.w);
                 if (g < R.rate_m) dd = u01(draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm).x);
             }
-            v = pm_one(v, g, dd, lo, hi, R.eta_m, R.rate_m);
+            v = pm_one(v, g, dd, lo, hi, R.exp_m, R.inv_exp_m, R.rate_m);
         }
         R.ox[static_cast<size_t>(row) * R.d + j] = v;
     }
@@ -271,6 +282,7 @@ __global__ __launch_bounds__(256) void b_sbx_pm_kernel(const BInst* __restrict__
     R.eta_m = annealed_hd(I.pm_eta0, I.pm_eta1, gen, I.max_gen);
     R.rate_m = I.pm_prob / static_cast<float>(I.d);
     R.seed = I.seed; R.gen = static_cast<uint32_t>(gen); R.ox = I.x[off];
+    set_derived(R);
     sbx_pm_elem<0>(R, static_cast<long long>(local) * blockDim.x + threadIdx.x);
 }
 
@@ -313,6 +325,7 @@ int mutate_rows(const float* x_in, float* x_out, int n, int d, const float* boun
     R.eta_c = 1.0f; R.prob_c = 0.0f; R.copy_c = 0.0f;
     R.mutate = 1; R.eta_m = pm_eta; R.rate_m = pm_prob / static_cast<float>(d);
     R.seed = seed; R.gen = static_cast<uint32_t>(generation); R.ox = x_out;
+    set_derived(R);
     const long long total = static_cast<long long>(R.pairs) * d + ((n & 1) ? d : 0);
     hipLaunchKernelGGL(sbx_pm_kernel<2>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, R);
     MOEA_HIP(hipGetLastError());
@@ -341,6 +354,7 @@ int reproduce(const moea_reproduce_args& A)
     R.eta_c = A.sbx_eta; R.prob_c = A.sbx_prob; R.copy_c = A.sbx_copy_prob;
     R.mutate = (A.mutation == MOEA_MUTATION_POLYNOMIAL) ? 1 : 0;
     R.eta_m = A.pm_eta; R.rate_m = A.pm_prob / static_cast<float>(A.d);
+    set_derived(R);
     if (dr) {
         R.sbx_u = dr->sbx_u; R.sbx_p = dr->sbx_p; R.sbx_sign = dr->sbx_sign; R.sbx_copy = dr->sbx_copy;
         R.pm_gate = dr->pm_gate; R.pm_delta = dr->pm_delta;
