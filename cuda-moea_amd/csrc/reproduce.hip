@@ -132,12 +132,15 @@ __device__ __forceinline__ int winner_of(const RepArgs& R, int row)
 
 // Element e of the (pair, variable) space; the odd tail row (n odd) copies its
 // parent (copy_last_parent_kernel, sbx_crossover.cu:72-83) and is then mutated.
-template <int kWinners>
+// kSmall: every element index is below 2^31 (R.small), so the draw counter's
+// high word is the constant 0 and the first Philox rounds are wave-uniform.
+template <int kWinners, bool kSmall = false>
 __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
 {
     const long long total_pair = static_cast<long long>(R.pairs) * R.d;
     const bool tail = (R.n & 1) != 0;
     if (e >= total_pair + (tail ? R.d : 0)) return;
+    const uint64_t di = kSmall ? static_cast<uint64_t>(static_cast<uint32_t>(e)) : static_cast<uint64_t>(e);
     if (e < total_pair) {
         int pair, j;
         if (R.small) {
@@ -156,7 +159,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
         int sg;
         U4 rs{0u, 0u, 0u, 0u};
         const bool draw_s = !R.sbx_u || (R.mutate && !R.pm_gate);
-        if (draw_s) rs = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
+        if (draw_s) rs = draw4(R.seed, di, R.gen, kStSbx);
         if (R.sbx_u) {
             ub = R.sbx_u[e]; up = R.sbx_p[e]; sg = R.sbx_sign[e]; uc = R.sbx_copy[e];
         } else {
@@ -174,7 +177,7 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
                 ga = u01(rs.w);
                 gb = gate_b(rs);
                 if (ga < R.rate_m || gb < R.rate_m) {
-                    const U4 rp = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm);
+                    const U4 rp = draw4(R.seed, di, R.gen, kStPm);
                     da = u01(rp.x);
                     db = u01(rp.y);
                 }
@@ -195,9 +198,9 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
             if (R.pm_gate) {
                 g = R.pm_gate[ra]; dd = R.pm_delta[ra];
             } else {
-                const U4 rs = draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStSbx);
+                const U4 rs = draw4(R.seed, di, R.gen, kStSbx);
                 g = u01(rs.w);
-                if (g < R.rate_m) dd = u01(draw4(R.seed, static_cast<uint64_t>(e), R.gen, kStPm).x);
+                if (g < R.rate_m) dd = u01(draw4(R.seed, di, R.gen, kStPm).x);
             }
             v = pm_one(v, g, dd, lo, hi, R.exp_m, R.inv_exp_m, R.rate_m);
         }
@@ -206,10 +209,10 @@ __device__ __forceinline__ void sbx_pm_elem(const RepArgs& R, long long e)
 }
 
 // One thread per (pair, variable).
-template <int kWinners>
+template <int kWinners, bool kSmall>
 __global__ __launch_bounds__(256) void sbx_pm_kernel(RepArgs R)
 {
-    sbx_pm_elem<kWinners>(R, static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+    sbx_pm_elem<kWinners, kSmall>(R, static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
 }
 
 // Population::initialize_random (core/population.cu:97-115):
@@ -327,7 +330,9 @@ int mutate_rows(const float* x_in, float* x_out, int n, int d, const float* boun
     R.seed = seed; R.gen = static_cast<uint32_t>(generation); R.ox = x_out;
     set_derived(R);
     const long long total = static_cast<long long>(R.pairs) * d + ((n & 1) ? d : 0);
-    hipLaunchKernelGGL(sbx_pm_kernel<2>, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, R);
+    const dim3 grid(static_cast<unsigned>((total + 255) / 256));
+    if (R.small) hipLaunchKernelGGL((sbx_pm_kernel<2, true>), grid, dim3(256), 0, st, R);
+    else hipLaunchKernelGGL((sbx_pm_kernel<2, false>), grid, dim3(256), 0, st, R);
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
@@ -365,8 +370,13 @@ int reproduce(const moea_reproduce_args& A)
     R.seed = A.seed; R.gen = gen; R.ox = A.offspring_x;
     const long long total = static_cast<long long>(R.pairs) * A.d + ((A.n & 1) ? A.d : 0);
     const dim3 grid(static_cast<unsigned>((total + 255) / 256));
-    if (A.mating == MOEA_MATING_GIVEN) hipLaunchKernelGGL(sbx_pm_kernel<1>, grid, dim3(256), 0, st, R);
-    else hipLaunchKernelGGL(sbx_pm_kernel<0>, grid, dim3(256), 0, st, R);
+    if (A.mating == MOEA_MATING_GIVEN) {
+        if (R.small) hipLaunchKernelGGL((sbx_pm_kernel<1, true>), grid, dim3(256), 0, st, R);
+        else hipLaunchKernelGGL((sbx_pm_kernel<1, false>), grid, dim3(256), 0, st, R);
+    } else {
+        if (R.small) hipLaunchKernelGGL((sbx_pm_kernel<0, true>), grid, dim3(256), 0, st, R);
+        else hipLaunchKernelGGL((sbx_pm_kernel<0, false>), grid, dim3(256), 0, st, R);
+    }
     MOEA_HIP(hipGetLastError());
     return kOk;
 }
