@@ -460,9 +460,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build runs each population on a single GPU "
-              "and has no multi-GPU path (see DESIGN.md for how it would shard); use --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: each population runs on a single GPU and there is "
+              "no multi-GPU path; use --gpus 1", file=sys.stderr)
         return 2
     import torch
     import cuda_moea as cm
