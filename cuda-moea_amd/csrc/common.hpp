@@ -239,7 +239,9 @@ __host__ __device__ inline float powf_det(float x, float y)
 //   x = 2^e m with m in [sqrt(1/2), sqrt(2)];
 //   log2 m = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1), by the odd series
 //     atanh s = s + s^3/3 + ... + s^11/11 (Abramowitz & Stegun 4.6.22), with
-//     s and the (2 / ln 2) product carried to double-float precision;
+//     s = (m - 1) r with r = 1 / (m + 1) one IEEE reciprocal, its low part
+//     from the fused remainder times r, and the (2 / ln 2) product carried
+//     to double-float precision;
 //   z = y (e + log2 m) as an unevaluated sum hi + lo;
 //   2^z = 2^k e^w, k = rint(hi), w = ((hi - k) + lo) ln 2, e^w by its Taylor
 //     polynomial to w^8 / 8! (A&S 4.2.1), |w| <= 0.35.
@@ -258,10 +260,11 @@ __host__ __device__ inline float powf_root(float x, float y)
     const float num = m - 1.0f;                           // exact
     const float den = m + 1.0f;
     const float den_lo = m - (den - 1.0f);                // den + den_lo = m + 1 exactly
-    const float s = num / den;
-    float rem = fmaf(-s, den, num);                       // exact remainder
+    const float rden = 1.0f / den;                        // one IEEE reciprocal for s and s_lo
+    const float s = num * rden;
+    float rem = fmaf(-s, den, num);                       // remainder, to a rounding of 2^-24 of itself
     rem = fmaf(-s, den_lo, rem);
-    const float s_lo = rem / den;                         // s + s_lo = (m - 1) / (m + 1)
+    const float s_lo = rem * rden;                        // s + s_lo = (m - 1) / (m + 1) to ~2^-45
     const float s2 = s * s;
     float p = 1.0f / 11.0f;
     p = fmaf(p, s2, 1.0f / 9.0f);

@@ -202,10 +202,11 @@ ORC_API float orc_powf_root(float x, float y)
     const float num = m - 1.0f;
     const float den = m + 1.0f;
     const float den_lo = m - (den - 1.0f);
-    const float s = num / den;
+    const float rden = 1.0f / den;
+    const float s = num * rden;
     float rem = fmaf(-s, den, num);
     rem = fmaf(-s, den_lo, rem);
-    const float s_lo = rem / den;
+    const float s_lo = rem * rden;
     const float s2 = s * s;
     float p = 1.0f / 11.0f;
     p = fmaf(p, s2, 1.0f / 9.0f);
